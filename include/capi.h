@@ -1,0 +1,126 @@
+/* C ABI of the InsPIRe PIR library (server half: HIP, one MI355X; client
+ * half: CPU only).
+ *
+ * Drop-in for the reference's src/capi.h: every declaration of that header
+ * (src/capi.h:27-182) is reproduced with the same name, signature, layout
+ * and return codes, so an FFI consumer of the reference links against this
+ * library unchanged. Flat layouts:
+ *   query    : [lwe_b_limb0 : db_rows][lwe_b_limb1 : db_rows]
+ *              [ksk5_b : blk][kskneg1_b : blk][rgsw : 4*blk], blk = d_eff*2*ring_n,
+ *              ksk [digit][limb][coeff] coefficient form,
+ *              rgsw [top_a|top_b|bot_a|bot_b][digit][limb][coeff] NTT form
+ *   response : [query][ct][a|b][limb][coeff], NTT form
+ * All values are residues in u64 words. Non-canonical residues (>= q) are
+ * reduced mod q by the server.
+ *
+ * Thread-safety: one ipir_server is externally serialized for
+ * ipir_answer_batch / ipir_answer_batch_device; the serving loop
+ * (ipir_loop_*) is the thread-safe, concurrent front end.
+ */
+#ifndef INSPIRE_CAPI_H
+#define INSPIRE_CAPI_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ipir_server ipir_server;
+
+typedef struct {
+    size_t n_entries;
+    size_t entry_bytes;
+    size_t db_rows;
+    size_t db_cols;
+    size_t n_packed;
+    size_t interp_d;
+    size_t num_cts;
+    size_t ring_n;
+    size_t d_eff;
+    uint8_t seed[64];
+} ipir_params;
+
+typedef struct {
+    size_t max_batch;
+    size_t resident_bytes;
+    size_t device_free_bytes;
+    size_t num_cts;
+} ipir_caps;
+
+typedef struct {
+    const uint64_t* lwe_b_limb0;
+    const uint64_t* lwe_b_limb1;
+    const uint64_t* ksk5_b;
+    const uint64_t* kskneg1_b;
+    const uint64_t* rgsw;
+} ipir_query_view;
+
+/* ---- server half (replaces src/capi.h:76-91) ---- */
+/* NULL on error (bad geometry, null db/out_params, max_batch == 0, device
+ * failure). db_rows_or_zero == 0 picks the default geometry (32768 rows). */
+ipir_server* ipir_server_create(size_t n_entries, size_t entry_bytes, size_t db_rows_or_zero,
+                                size_t max_batch, const uint16_t* db,
+                                const uint8_t* crs_seed_or_null, ipir_params* out_params);
+void ipir_server_caps(const ipir_server* srv, ipir_caps* out);
+/* 0 ok; 1 null argument; 2 count > max_batch; 3 null view field;
+ * 4 device failure. count == 0 returns 0 and writes nothing. */
+int ipir_answer_batch(ipir_server* srv, const ipir_query_view* queries, size_t count,
+                      uint64_t* out_resp);
+void ipir_server_destroy(ipir_server* srv);
+
+/* ---- client half (replaces src/capi.h:101-182; CPU only) ---- */
+typedef struct ipir_client_query ipir_client_query;
+size_t ipir_query_u64s(const ipir_params* params);
+size_t ipir_response_u64s(const ipir_params* params);
+size_t ipir_entry_slots(const ipir_params* params);
+int ipir_query_view_from_flat(const ipir_params* params, const uint64_t* flat,
+                              ipir_query_view* out_view);
+ipir_client_query* ipir_query_build(const ipir_params* params, uint64_t idx, uint64_t* out_query);
+int ipir_extract(const ipir_params* params, const ipir_client_query* q, const uint64_t* resp,
+                 uint16_t* out_slots);
+void ipir_client_query_destroy(ipir_client_query* q);
+size_t ipir_query_packed_bytes(const ipir_params* params);
+size_t ipir_response_packed_bytes(const ipir_params* params);
+int ipir_query_pack(const ipir_params* params, const uint64_t* flat, uint8_t* out);
+int ipir_response_pack(const ipir_params* params, const uint64_t* flat, uint8_t* out);
+int ipir_query_unpack(const ipir_params* params, const uint8_t* in, uint64_t* out_flat);
+int ipir_response_unpack(const ipir_params* params, const uint8_t* in, uint64_t* out_flat);
+size_t ipir_response_compressed_bytes(const ipir_params* params);
+int ipir_response_compress(const ipir_params* params, const uint64_t* flat, uint8_t* out);
+int ipir_extract_compressed(const ipir_params* params, const ipir_client_query* q,
+                            const uint8_t* in, uint16_t* out_slots);
+
+/* ---- additions without a counterpart in the reference ---- */
+
+/* Client query under a caller-seeded generator (reproducible tests and
+ * benchmarks); otherwise identical to ipir_query_build. */
+ipir_client_query* ipir_query_build_seeded(const ipir_params* params, uint64_t idx,
+                                           uint64_t rng_seed, uint64_t* out_query);
+
+/* Answer `count` queries whose flat layouts sit back to back in DEVICE
+ * memory (count * ipir_query_u64s u64s), writing the flat responses to
+ * DEVICE memory, on the given HIP stream (NULL = the server's own stream).
+ * Asynchronous: returns after enqueueing. Return codes as
+ * ipir_answer_batch. For callers that keep queries resident in HBM. */
+int ipir_answer_batch_device(ipir_server* srv, const uint64_t* d_queries, size_t count,
+                             uint64_t* d_out_resp, void* hip_stream);
+
+/* Continuous-batching serving loop over one server. Requests are admitted
+ * into free slots as they arrive, batches of different requests overlap on
+ * the device, and each request completes independently.
+ *   max_inflight: concurrent device batches (1..4); slots are shared.
+ * ipir_loop_submit copies the query, returns a ticket (> 0) or 0 on error.
+ * ipir_loop_wait blocks until the ticket's response is written to the
+ * out_resp given at submit; returns 0 and the request latency (submit to
+ * completion) in *latency_us. */
+typedef struct ipir_loop ipir_loop;
+ipir_loop* ipir_loop_start(ipir_server* srv, size_t max_inflight);
+uint64_t ipir_loop_submit(ipir_loop* loop, const ipir_query_view* q, uint64_t* out_resp);
+int ipir_loop_wait(ipir_loop* loop, uint64_t ticket, double* latency_us);
+void ipir_loop_stop(ipir_loop* loop);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* INSPIRE_CAPI_H */

@@ -1,0 +1,44 @@
+/* Inspection and stage-level entry points of the server library, used by
+ * the parity tests and the benchmark. No counterpart in the reference (its
+ * tests link the C++ internals directly); they expose intermediate device
+ * tensors through host buffers. All return 0 on success, nonzero on bad
+ * arguments or device failure, and synchronise before returning.
+ */
+#ifndef INSPIRE_DEBUG_H
+#define INSPIRE_DEBUG_H
+#include <stddef.h>
+#include <stdint.h>
+
+#include "capi.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* u32 words of one group's precomp block: [a: 2N][D+: 2*(N/2-1)*2*N]
+ * [D-: same][D_final: 2*2*N], each in [limb][..][coeff] order. */
+size_t ipir_dbg_precomp_group_words(void);
+int ipir_dbg_precomp_group(const ipir_server* srv, size_t group, uint32_t* out);
+/* rows [row0, row0+nrows) of the encoded row-major u16 database */
+int ipir_dbg_db_rows(const ipir_server* srv, size_t row0, size_t nrows, uint16_t* out);
+/* Answer one host flat query and return the intermediate tensors:
+ * interm [2][db_cols] (DB pass, coefficient form), packed_b
+ * [n_packed][2][N] (after the collapse), resp (flat response). Any output
+ * pointer may be NULL. */
+int ipir_dbg_answer_taps(ipir_server* srv, const uint64_t* query, uint32_t* interm, uint32_t* packed_b,
+                         uint64_t* resp);
+/* Standalone kernels on the current device. ntt: npoly RNS polynomials
+ * [npoly][2][N] u32, forward (inverse = 0) or inverse, in place on a copy.
+ * horner: cts [L][a|b][limb][N] NTT form, rgsw flat (4*d_eff*2*N u64,
+ * [part][digit][limb][coeff]); out [a|b][limb][N]. */
+int ipir_dbg_ntt(const uint32_t* in, uint32_t* out, size_t npoly, int inverse);
+int ipir_dbg_horner(const uint64_t* cts, size_t L, const uint64_t* rgsw, uint64_t* out);
+/* Per-stage device time (ms) of the last ipir_dbg_profile_batch run:
+ * [unmarshal+key NTT, DB pass, collapse pass, pack finalize, Horner]. */
+int ipir_dbg_profile_batch(ipir_server* srv, const uint64_t* d_queries, size_t count, uint64_t* d_out,
+                           double* stage_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* INSPIRE_DEBUG_H */

@@ -1,0 +1,53 @@
+// Protocol types and the client query/extract path.
+//
+// Drop-in for the reference's src/protocol.h (PreprocessData :12-24,
+// QueryState :30-35, QueryMessage :37-42, query :44, extract :53-55,
+// compress_response / extract_compressed :63-67).
+#pragma once
+#include <utility>
+
+#include "crypto.h"
+
+namespace inspire {
+
+// Device-resident preprocess output, produced by gpu_preprocess and adopted
+// (ownership transfer, source fields cleared) by gpu_setup_server.
+struct PreprocessData {
+    std::vector<uint32_t*> d_precomp_a;        // [n_packed] -> 2*N u32, NTT form
+    std::vector<uint32_t*> d_precomp_D_plus;   // [n_packed] -> 2*(N/2-1)*D_EFF*N u32
+    std::vector<uint32_t*> d_precomp_D_minus;  // same shape
+    std::vector<uint32_t*> d_precomp_D_final;  // [n_packed] -> 2*D_EFF*N u32
+    uint16_t* d_db_col = nullptr;              // encoded DB, row-major u16 [rows][cols]
+    // Single device allocation the per-group pointers above point into
+    // (this library allocates the precomp as one arena); null when the
+    // per-group pointers own their memory individually.
+    void* d_precomp_arena = nullptr;
+};
+
+struct QueryState {
+    RnsPoly s;        // secret, coefficient form
+    RnsPoly s_ntt;    // secret, NTT form
+    size_t k_star;    // response ciphertext index
+    size_t ell_star;  // slot offset within that ciphertext
+};
+
+struct QueryMessage {
+    Ksk ksk_5;
+    Ksk ksk_neg1;
+    LweQuery lwe;
+    RgswCt rgsw;
+};
+
+// Fresh secret per call (std::random_device).
+std::pair<QueryState, QueryMessage> query(const PublicParams& pp, size_t idx);
+// Same with a caller-seeded generator (no counterpart in the reference;
+// for reproducible tests and benchmarks).
+std::pair<QueryState, QueryMessage> query_seeded(const PublicParams& pp, size_t idx, uint64_t rng_seed);
+
+std::vector<uint16_t> extract(const PublicParams& pp, const QueryState& qst,
+                              const std::vector<RlweCt>& resp);
+std::vector<CompressedCt> compress_response(const std::vector<RlweCt>& resp);
+std::vector<uint16_t> extract_compressed(const PublicParams& pp, const QueryState& qst,
+                                         const std::vector<CompressedCt>& resp);
+
+}  // namespace inspire

@@ -1,0 +1,286 @@
+// Server half of the C ABI (reference src/capi.cu:51-132) plus the
+// device-resident batch entry point and the inspection entry points.
+// Exceptions never cross the boundary: failures become NULL / nonzero.
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "capi.h"
+#include "inspire_debug.h"
+#include "server_ctx.h"
+
+using namespace inspire;
+
+
+namespace {
+size_t blk() { return (size_t)D_EFF * 2 * N; }
+
+// copy `count` query views into the context's pinned flat staging
+void stage_views(GpuServerCtx* ctx, const ipir_query_view* qs, size_t count) {
+    const size_t R = ctx->pp.db_rows, b = blk();
+    for (size_t i = 0; i < count; i++) {
+        uint64_t* w = ctx->h_q + i * ctx->qlen;
+        std::memcpy(w, qs[i].lwe_b_limb0, R * 8);
+        std::memcpy(w + R, qs[i].lwe_b_limb1, R * 8);
+        std::memcpy(w + 2 * R, qs[i].ksk5_b, b * 8);
+        std::memcpy(w + 2 * R + b, qs[i].kskneg1_b, b * 8);
+        std::memcpy(w + 2 * R + 2 * b, qs[i].rgsw, 4 * b * 8);
+    }
+}
+}  // namespace
+
+extern "C" {
+
+ipir_server* ipir_server_create(size_t n_entries, size_t entry_bytes, size_t db_rows_or_zero, size_t max_batch,
+                                const uint16_t* db, const uint8_t* crs_seed_or_null, ipir_params* out) {
+    if (!db || !out || max_batch == 0) return nullptr;
+    try {
+        std::unique_ptr<ipir_server> srv(new ipir_server);
+        const size_t rows = db_rows_or_zero ? db_rows_or_zero : 32768;
+        srv->pp = crs_seed_or_null ? setup_seeded(n_entries, entry_bytes, rows, crs_seed_or_null)
+                                   : setup(n_entries, entry_bytes, rows);
+        PreprocessData pd = gpu_preprocess(srv->pp, db);
+        GpuServerConfig cfg;
+        cfg.max_batch = max_batch;
+        try {
+            srv->ctx = gpu_setup_server(srv->pp, pd, cfg);
+        } catch (...) {
+            dfree(pd.d_db_col);
+            dfree(pd.d_precomp_arena);
+            throw;
+        }
+        out->n_entries = srv->pp.N_entries;
+        out->entry_bytes = srv->pp.w;
+        out->db_rows = srv->pp.db_rows;
+        out->db_cols = srv->pp.db_cols;
+        out->n_packed = srv->pp.n_packed;
+        out->interp_d = srv->pp.D;
+        out->num_cts = srv->pp.num_cts;
+        out->ring_n = N;
+        out->d_eff = (size_t)D_EFF;
+        std::memcpy(out->seed, srv->pp.seed.data(), SEED_BYTES);
+        return srv.release();
+    } catch (...) {
+        return nullptr;
+    }
+}
+
+void ipir_server_caps(const ipir_server* srv, ipir_caps* out) {
+    if (!srv || !out) return;
+    const GpuServerCaps c = gpu_server_caps(srv->ctx);
+    out->max_batch = c.max_batch;
+    out->resident_bytes = c.resident_bytes;
+    out->device_free_bytes = c.device_free_bytes;
+    out->num_cts = c.num_cts;
+}
+
+int ipir_answer_batch(ipir_server* srv, const ipir_query_view* queries, size_t count, uint64_t* out_resp) {
+    if (!srv || !queries || !out_resp) return 1;
+    if (count == 0) return 0;
+    if (count > srv->ctx->cfg.max_batch) return 2;
+    for (size_t i = 0; i < count; i++) {
+        const ipir_query_view& v = queries[i];
+        if (!v.lwe_b_limb0 || !v.lwe_b_limb1 || !v.ksk5_b || !v.kskneg1_b || !v.rgsw) return 3;
+    }
+    try {
+        std::lock_guard<std::mutex> lk(srv->ctx->mu);
+        stage_views(srv->ctx, queries, count);
+        srv->ctx->answer_host(count, out_resp);
+        return 0;
+    } catch (...) {
+        return 4;
+    }
+}
+
+int ipir_answer_batch_device(ipir_server* srv, const uint64_t* d_queries, size_t count, uint64_t* d_out_resp,
+                             void* hip_stream) {
+    if (!srv || !d_queries || !d_out_resp) return 1;
+    if (count == 0) return 0;
+    if (count > srv->ctx->cfg.max_batch) return 2;
+    try {
+        hipStream_t s = hip_stream ? static_cast<hipStream_t>(hip_stream) : srv->ctx->stream;
+        srv->ctx->enqueue(d_queries, count, d_out_resp, s);
+        return 0;
+    } catch (...) {
+        return 4;
+    }
+}
+
+void ipir_server_destroy(ipir_server* srv) {
+    if (!srv) return;
+    gpu_free_server(srv->ctx);
+    delete srv;
+}
+
+// ---------------------------------------------------------------- inspection
+size_t ipir_dbg_precomp_group_words(void) { return ipx::PC_GROUP; }
+
+int ipir_dbg_precomp_group(const ipir_server* srv, size_t group, uint32_t* out) {
+    if (!srv || !out || group >= srv->ctx->G) return 1;
+    return hipMemcpy(out, srv->ctx->d_precomp + group * ipx::PC_GROUP, ipx::PC_GROUP * 4, hipMemcpyDeviceToHost) ==
+                   hipSuccess ? 0 : 4;
+}
+
+int ipir_dbg_db_rows(const ipir_server* srv, size_t row0, size_t nrows, uint16_t* out) {
+    if (!srv || !out || row0 + nrows > srv->pp.db_rows) return 1;
+    const size_t C = srv->pp.db_cols;
+    return hipMemcpy(out, srv->ctx->d_db + row0 * C, nrows * C * 2, hipMemcpyDeviceToHost) == hipSuccess ? 0 : 4;
+}
+
+int ipir_dbg_answer_taps(ipir_server* srv, const uint64_t* query, uint32_t* interm, uint32_t* packed_b,
+                         uint64_t* resp) {
+    if (!srv || !query) return 1;
+    GpuServerCtx* c = srv->ctx;
+    try {
+        std::lock_guard<std::mutex> lk(c->mu);
+        std::memcpy(c->h_q, query, c->qlen * 8);
+        std::vector<uint64_t> r(c->resp_u64s);
+        c->answer_host(1, r.data());
+        if (resp) std::memcpy(resp, r.data(), r.size() * 8);
+        if (interm) {
+            const size_t C = c->pp.db_cols, S = c->sh.slices;
+            std::vector<uint32_t> part(c->part_stride);
+            if (hipMemcpy(part.data(), c->d_part, part.size() * 4, hipMemcpyDeviceToHost) != hipSuccess) return 4;
+            for (int l = 0; l < 2; l++) {
+                const uint64_t q = l ? Q1 : Q0;
+                for (size_t j = 0; j < C; j++) {
+                    uint64_t a = 0;
+                    for (size_t s = 0; s < S; s++) a += part[(s * 2 + l) * C + j];
+                    interm[l * C + j] = (uint32_t)(a % q);
+                }
+            }
+        }
+        if (packed_b && hipMemcpy(packed_b, c->d_packed_b, c->packed_stride * 4, hipMemcpyDeviceToHost) != hipSuccess)
+            return 4;
+        return 0;
+    } catch (...) {
+        return 4;
+    }
+}
+
+namespace {
+struct Standalone {
+    uint32_t* tw = nullptr;
+    ipx::NttConst nc{};
+    ipx::CrtConst cc{};
+};
+Standalone* standalone() {
+    static Standalone* s = [] {
+        auto* x = new Standalone;
+        // reuse the context table builder through a throwaway context-free path
+        std::vector<uint32_t> h(2 * 4 * (size_t)N);
+        for (int l = 0; l < 2; l++) {
+            const NttTables& t = ntt_tables(l);
+            for (size_t i = 0; i < N; i++) {
+                h[(l * 4 + 0) * N + i] = (uint32_t)t.fwd_twiddles[i];
+                h[(l * 4 + 1) * N + i] = (uint32_t)t.fwd_primes[i];
+                h[(l * 4 + 2) * N + i] = (uint32_t)t.inv_twiddles[i];
+                h[(l * 4 + 3) * N + i] = (uint32_t)t.inv_primes[i];
+            }
+            x->nc.inv_n[l] = (uint32_t)t.inv_n;
+            x->nc.inv_n_p[l] = (uint32_t)t.inv_n_prime;
+        }
+        if (hipMalloc(&x->tw, h.size() * 4) != hipSuccess) return (Standalone*)nullptr;
+        if (hipMemcpy(x->tw, h.data(), h.size() * 4, hipMemcpyHostToDevice) != hipSuccess) return (Standalone*)nullptr;
+        x->nc.tw = x->tw;
+        x->cc.q1inv_q0 = (uint32_t)mod_inv(Q1 % Q0, Q0);
+        x->cc.q0inv_q1 = (uint32_t)mod_inv(Q0 % Q1, Q1);
+        return x;
+    }();
+    return s;
+}
+}  // namespace
+
+int ipir_dbg_ntt(const uint32_t* in, uint32_t* out, size_t npoly, int inverse) {
+    if (!in || !out || npoly == 0) return 1;
+    Standalone* st = standalone();
+    if (!st) return 4;
+    uint32_t* d = nullptr;
+    const size_t bytes = npoly * 2 * N * 4;
+    if (hipMalloc(&d, bytes) != hipSuccess) return 4;
+    int rc = 0;
+    if (hipMemcpy(d, in, bytes, hipMemcpyHostToDevice) != hipSuccess ||
+        ipx::launch_ntt_rns(d, npoly, st->nc, inverse != 0, nullptr) != hipSuccess ||
+        hipMemcpy(out, d, bytes, hipMemcpyDeviceToHost) != hipSuccess)
+        rc = 4;
+    dfree(d);
+    return rc;
+}
+
+int ipir_dbg_horner(const uint64_t* cts, size_t L, const uint64_t* rgsw, uint64_t* out) {
+    if (!cts || !rgsw || !out || L == 0) return 1;
+    Standalone* st = standalone();
+    if (!st) return 4;
+    const size_t b = blk();
+    std::vector<uint32_t> pa(L * 2 * N), pb(L * 2 * N), rg(4 * b);
+    for (size_t g = 0; g < L; g++)
+        for (size_t x = 0; x < 2 * N; x++) {
+            const uint64_t q = x < N ? Q0 : Q1;
+            pa[g * 2 * N + x] = (uint32_t)(cts[g * 4 * N + x] % q);
+            pb[g * 2 * N + x] = (uint32_t)(cts[g * 4 * N + 2 * N + x] % q);
+        }
+    for (size_t x = 0; x < 4 * b; x++) rg[x] = (uint32_t)(rgsw[x] % ((x / N) % 2 ? Q1 : Q0));
+    uint32_t *d_pa = nullptr, *d_pb = nullptr, *d_rg = nullptr;
+    uint64_t* d_o = nullptr;
+    int rc = 0;
+    if (hipMalloc(&d_pa, pa.size() * 4) || hipMalloc(&d_pb, pb.size() * 4) || hipMalloc(&d_rg, rg.size() * 4) ||
+        hipMalloc(&d_o, 4 * N * 8)) rc = 4;
+    if (!rc && (hipMemcpy(d_pa, pa.data(), pa.size() * 4, hipMemcpyHostToDevice) ||
+                hipMemcpy(d_pb, pb.data(), pb.size() * 4, hipMemcpyHostToDevice) ||
+                hipMemcpy(d_rg, rg.data(), rg.size() * 4, hipMemcpyHostToDevice) ||
+                ipx::launch_horner(d_pa, 2 * N, L, L, 1, d_pb, L * 2 * N, d_rg, 4 * b, d_o, 1, st->nc, st->cc,
+                                   nullptr) ||
+                hipMemcpy(out, d_o, 4 * N * 8, hipMemcpyDeviceToHost)))
+        rc = 4;
+    dfree(d_pa); dfree(d_pb); dfree(d_rg); dfree(d_o);
+    return rc;
+}
+
+int ipir_dbg_profile_batch(ipir_server* srv, const uint64_t* d_queries, size_t count, uint64_t* d_out,
+                           double* stage_ms) {
+    if (!srv || !d_queries || !d_out || !stage_ms || count == 0 || count > srv->ctx->cfg.max_batch) return 1;
+    GpuServerCtx* c = srv->ctx;
+    const size_t R = c->pp.db_rows, C = c->pp.db_cols;
+    hipEvent_t ev[6];
+    for (auto& e : ev) (void)hipEventCreate(&e);
+    hipStream_t s = c->stream;
+    int rc = 0;
+    try {
+        (void)hipEventRecord(ev[0], s);
+        hip_check(ipx::launch_unmarshal(d_queries, count, R, c->d_bq, c->d_keys, c->d_rgsw, s), "unmarshal");
+        hip_check(ipx::launch_ntt_rns(c->d_keys, count * 2 * D_EFF, c->nc, false, s), "key ntt");
+        (void)hipEventRecord(ev[1], s);
+        for (size_t q = 0; q < count; q++)
+            hip_check(ipx::launch_dbpass(c->d_db, c->d_bq + q * 2 * R, c->d_part + q * c->part_stride, R, C, c->sh, s),
+                      "dbpass");
+        (void)hipEventRecord(ev[2], s);
+        for (size_t q0 = 0; q0 < count; q0 += 4) {
+            const size_t nq = count - q0 < 4 ? count - q0 : 4;
+            hip_check(ipx::launch_collapse_partial(c->d_precomp, c->G, c->d_keys + q0 * c->keys_stride, c->keys_stride,
+                                                   nq, c->d_part2 + q0 * c->part2_stride, c->nch, s),
+                      "collapse");
+        }
+        (void)hipEventRecord(ev[3], s);
+        hip_check(ipx::launch_pack_finalize(c->d_part, c->part_stride, c->sh.slices, C, c->d_part2, c->part2_stride,
+                                            c->nch, c->d_precomp, c->G, c->d_keys, c->keys_stride, c->d_packed_b,
+                                            c->packed_stride, count, c->nc, s),
+                  "finalize");
+        (void)hipEventRecord(ev[4], s);
+        hip_check(ipx::launch_horner(c->d_precomp + ipx::PC_A, ipx::PC_GROUP, c->G, c->Da, c->T, c->d_packed_b,
+                                     c->packed_stride, c->d_rgsw, c->rgsw_stride, d_out, count, c->nc, c->cc, s),
+                  "horner");
+        (void)hipEventRecord(ev[5], s);
+        hip_check(hipEventSynchronize(ev[5]), "sync");
+        for (int i = 0; i < 5; i++) {
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, ev[i], ev[i + 1]);
+            stage_ms[i] = ms;
+        }
+    } catch (...) {
+        rc = 4;
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,77 @@
+// Host-side launchers of the gfx950 kernels. Every launcher enqueues on the
+// given stream and returns the launch status; none allocates or
+// synchronises.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "dev_common.h"
+
+namespace ipx {
+
+// ---------------------------------------------------------------- online
+// Per-query device layout produced by unmarshal (one "slot"):
+//   bq  : double [2][R]            (b residues as exact doubles, DB pass)
+//   keys: u32 [2 keys][digit][limb][N]  (ksk5, kskneg1; coeff -> NTT in place)
+//   rgsw: u32 [4 parts][digit][limb][N]  (NTT form)
+struct SlotLayout {
+    size_t R, C, G;
+    size_t keys_words() const { return 2 * (size_t)D_EFF * 2 * N; }
+    size_t rgsw_words() const { return 4 * (size_t)D_EFF * 2 * N; }
+    size_t query_u64s() const { return 2 * R + 6 * (size_t)D_EFF * 2 * N; }
+};
+
+hipError_t launch_unmarshal(const uint64_t* d_q, size_t count, size_t R, double* bq, uint32_t* keys,
+                            uint32_t* rgsw, hipStream_t s);
+// In-place forward NTT of `npoly` RNS polynomials laid out [npoly][2][N].
+hipError_t launch_ntt_rns(uint32_t* polys, size_t npoly, const NttConst& nc, bool inverse, hipStream_t s);
+
+// Plain database pass for one query: part[slice][limb][C] = sum over the
+// slice's rows of db[r][c]*b_limb[r] mod q_limb.
+struct DbPassShape {
+    uint32_t rows_per_slice;  // multiple of 4, divides R
+    uint32_t slices;          // R / rows_per_slice
+    uint32_t cols_per_thread; // 4 or 8
+};
+DbPassShape dbpass_shape(size_t R, size_t C, int target_blocks);
+hipError_t launch_dbpass(const uint16_t* db, const double* bq, uint32_t* part, size_t R, size_t C,
+                         const DbPassShape& sh, hipStream_t s);
+
+// Number of step chunks the collapse pass splits the 1023 steps into.
+uint32_t collapse_nch(size_t G);
+// Precomp-tensor pass: partial collapse sums for `nq` queries (<= 4) whose
+// NTT-form keys sit at keys + q*keys_stride. out part2 [nq][nch][G][2][N].
+hipError_t launch_collapse_partial(const uint32_t* precomp, size_t G, const uint32_t* keys,
+                                   size_t keys_stride, size_t nq, uint32_t* part2, uint32_t nch,
+                                   hipStream_t s);
+// packed_b[q][g] = NTT(sum of DB-pass partials) - collapse partials - Df*Kn.
+hipError_t launch_pack_finalize(const uint32_t* part, size_t part_stride, uint32_t slices, size_t C,
+                                const uint32_t* part2, size_t part2_stride, uint32_t nch,
+                                const uint32_t* precomp, size_t G, const uint32_t* keys,
+                                size_t keys_stride, uint32_t* packed_b, size_t packed_stride,
+                                size_t nq, const NttConst& nc, hipStream_t s);
+// Horner over each response ciphertext; writes the flat u64 response
+// [q][ct][a|b][limb][N]. Group g's packed a-part (shared by all queries) is
+// at pa_base + g*pa_stride; its b-part at packed_b + q*packed_stride + g*2N.
+hipError_t launch_horner(const uint32_t* pa_base, size_t pa_stride, size_t G, size_t Da, size_t T,
+                         const uint32_t* packed_b, size_t packed_stride, const uint32_t* rgsw,
+                         size_t rgsw_stride, uint64_t* out, size_t nq, const NttConst& nc,
+                         const CrtConst& cc, hipStream_t s);
+
+// ---------------------------------------------------------------- preprocess
+hipError_t launch_encode(uint16_t* db, size_t R, size_t C, size_t Da, hipStream_t s);
+// a32T[g][limb][p][j] for groups [g0, g0+ng): transposed a-side matmul
+// output. crs [nr][2][N] NTT form.
+hipError_t launch_aside(const uint16_t* db, size_t R, size_t C, size_t g0, size_t ng, const uint32_t* crs,
+                        uint32_t* a32T, const NttConst& nc, hipStream_t s);
+// F = NTT along j of a32T (in place), per (g, limb, p).
+hipError_t launch_ring_embed(uint32_t* a32T, size_t ng, const NttConst& nc, hipStream_t s);
+// CollapseA chains (forward and conjugate) for ng groups; writes D+/D- into
+// the precomp blocks and the final chain states into states [ng][2 dir][2][N].
+hipError_t launch_collapse_chain(const uint32_t* F, size_t ng, const uint32_t* ksk5a, uint32_t* precomp_g0,
+                                 uint32_t* states, const NttConst& nc, const CrtConst& cc, hipStream_t s);
+hipError_t launch_collapse_final(const uint32_t* states, size_t ng, const uint32_t* kneg1a,
+                                 uint32_t* precomp_g0, const NttConst& nc, const CrtConst& cc, hipStream_t s);
+
+}  // namespace ipx

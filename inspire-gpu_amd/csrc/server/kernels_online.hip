@@ -1,0 +1,383 @@
+// Online (per-query) kernels for gfx950: query unmarshal, batched NTT, the
+// plain database pass, the precomp-tensor (collapse) pass, pack finalize,
+// and the persistent Horner/external-product chain.
+//
+// Semantics follow the reference's online path (src/gpu_protocol.cu:
+// 1121-1339): mat-vec interm_c = DB^T b_c mod q_c (src/gpu_online.cu:23-41);
+// per group b = NTT(interm) minus the lazy collapse sums and the final
+// D_final x ksk_neg1 term (src/gpu_online.cu:547-632); Horner
+// acc = ext_prod(rgsw, acc) + packed[i] (src/gpu_online.cu:474-540). All
+// arithmetic is exact, so results equal the reference's bit for bit.
+#include "kernels.h"
+
+namespace ipx {
+
+namespace {
+
+constexpr size_t BLK = (size_t)D_EFF * 2 * N;  // one key / RGSW part, u32 words
+
+// ------------------------------------------------------------ unmarshal
+// Flat u64 wire query -> per-slot device layout, reducing every residue
+// mod its limb's prime (the reference narrows ksk/RGSW words without
+// reduction, src/gpu_protocol.cu:1066 and :50; canonical inputs agree).
+__global__ __launch_bounds__(256) void k_unmarshal(const uint64_t* __restrict__ q, size_t total, size_t R,
+                                                   double* __restrict__ bq, uint32_t* __restrict__ keys,
+                                                   uint32_t* __restrict__ rgsw) {
+    const size_t qlen = 2 * R + 6 * BLK;
+    for (size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x; x < total; x += (size_t)gridDim.x * blockDim.x) {
+        const size_t qi = x / qlen, off = x - qi * qlen;
+        const uint64_t v = q[x];
+        if (off < 2 * R) {
+            const int limb = off >= R;
+            bq[qi * 2 * R + off] = (double)red64(v, limb);
+        } else if (off < 2 * R + 2 * BLK) {
+            const size_t o = off - 2 * R;
+            keys[qi * 2 * BLK + o] = red64(v, (int)((o / N) & 1));
+        } else {
+            const size_t o = off - 2 * R - 2 * BLK;
+            rgsw[qi * 4 * BLK + o] = red64(v, (int)((o / N) & 1));
+        }
+    }
+}
+
+// ------------------------------------------------------------ batched NTT
+__global__ __launch_bounds__(256) void k_ntt_rns(uint32_t* __restrict__ polys, NttConst nc, int inverse) {
+    __shared__ uint32_t buf[POLY_LDS];
+    const int limb = blockIdx.x & 1;
+    uint32_t* p = polys + (size_t)blockIdx.x * N;
+    const uint32_t t = threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < 8; k++) buf[ls(t + 256 * k)] = p[t + 256 * k];
+    __syncthreads();
+    if (inverse) ntt_inv_lds(buf, t, nc, limb);
+    else ntt_fwd_lds(buf, t, nc, limb);
+#pragma unroll
+    for (int k = 0; k < 8; k++) p[t + 256 * k] = buf[ls(t + 256 * k)];
+}
+
+// ------------------------------------------------------------ database pass
+// One query. Lane owns CPT consecutive columns and walks the block's rows.
+// Products db*b (< 2^43) accumulate exactly in fp64 FMA (full rate on
+// CDNA) for up to 1024 rows (< 2^53), are then reduced mod q in fp64 and
+// folded into a canonical u32 accumulator.
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+template <int CPT> struct Vec;
+template <> struct Vec<8> { typedef u32x4 T; };
+template <> struct Vec<4> { typedef u32x2 T; };
+
+__device__ __forceinline__ uint32_t fmod_exact(double a, double q, double qinv) {
+    double t = floor(a * qinv);
+    double r = fma(-t, q, a);  // exact: |a - t*q| < 2q
+    if (r < 0) r += q;
+    if (r >= q) r -= q;
+    return (uint32_t)r;
+}
+
+template <int CPT>
+__global__ __launch_bounds__(256) void k_dbpass(const uint16_t* __restrict__ db, const double* __restrict__ bq,
+                                                uint32_t* __restrict__ part, uint32_t R, uint32_t C,
+                                                uint32_t rps) {
+    typedef typename Vec<CPT>::T VT;
+    const uint32_t col = (blockIdx.x * 256 + threadIdx.x) * CPT;
+    const uint32_t r0 = blockIdx.y * rps;
+    const double* b0 = bq + r0;
+    const double* b1 = bq + R + r0;
+    const VT* p = reinterpret_cast<const VT*>(db + (size_t)r0 * C + col);
+    const size_t rs = C / CPT;  // row stride in vectors
+    const double dq0 = (double)Q0, dq1 = (double)Q1, iq0 = 1.0 / dq0, iq1 = 1.0 / dq1;
+    uint32_t m0[CPT], m1[CPT];
+#pragma unroll
+    for (int e = 0; e < CPT; e++) m0[e] = m1[e] = 0;
+    for (uint32_t blk = 0; blk < rps; blk += 1024) {
+        const uint32_t nb = rps - blk < 1024 ? rps - blk : 1024;
+        double a0[CPT], a1[CPT];
+#pragma unroll
+        for (int e = 0; e < CPT; e++) a0[e] = a1[e] = 0.0;
+        for (uint32_t r = 0; r < nb; r += 4) {
+            VT v[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) v[u] = __builtin_nontemporal_load(p + (size_t)(blk + r + u) * rs);
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const double x0 = b0[blk + r + u], x1 = b1[blk + r + u];
+#pragma unroll
+                for (int e = 0; e < CPT; e++) {
+                    const uint32_t w = v[u][e >> 1];
+                    const double d = (double)((e & 1) ? (w >> 16) : (w & 0xffffu));
+                    a0[e] = fma(d, x0, a0[e]);
+                    a1[e] = fma(d, x1, a1[e]);
+                }
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < CPT; e++) {
+            m0[e] = add_mod(m0[e], fmod_exact(a0[e], dq0, iq0), Q0);
+            m1[e] = add_mod(m1[e], fmod_exact(a1[e], dq1, iq1), Q1);
+        }
+    }
+    uint32_t* o0 = part + (size_t)blockIdx.y * 2 * C + col;
+    uint32_t* o1 = o0 + C;
+#pragma unroll
+    for (int e = 0; e < CPT; e++) { o0[e] = m0[e]; o1[e] = m1[e]; }
+}
+
+// ------------------------------------------------------------ collapse pass
+// For queries q < nq, group g, limb c, coefficient i:
+//   sum_{s in chunk} sum_j D+[s][j][i]*K5[j][pf] + D-[s][j][i]*K5[j][pc]
+// with pf = perm_{5^(N/2-2-s)}(i), pc = N-1-pf (reference
+// src/gpu_online.cu:593-632 with the permutation of
+// src/gpu_protocol.cu:193-214 computed in closed form). Each D element is
+// loaded once and applied to every query of the launch.
+template <int BT>
+__global__ __launch_bounds__(256) void k_collapse_partial(const uint32_t* __restrict__ precomp,
+                                                          const uint32_t* __restrict__ keys, size_t kstride,
+                                                          uint32_t nq, uint32_t* __restrict__ part2, uint32_t G,
+                                                          uint32_t nch, uint32_t spc) {
+    __shared__ uint32_t K[BT][D_EFF][N];
+    const uint32_t itile = blockIdx.x & 7, ch = blockIdx.x >> 3, c = blockIdx.y, g = blockIdx.z;
+    for (uint32_t x = threadIdx.x; x < BT * D_EFF * N; x += 256) {
+        const uint32_t bt = x / (D_EFF * N), j = (x / N) % D_EFF, i = x % N;
+        K[bt][j][i] = bt < nq ? keys[bt * kstride + (j * 2 + c) * N + i] : 0u;
+    }
+    __syncthreads();
+    const uint32_t i = itile * 256 + threadIdx.x;
+    const uint32_t v = 2 * brev11(i) + 1;
+    const uint32_t s0 = ch * spc;
+    const uint32_t s1 = s0 + spc < NSTEPS ? s0 + spc : NSTEPS;
+    // t = 5^(N/2-2-s0) mod 2N; successive steps multiply by 5^-1 = 3277 mod 2N
+    uint32_t t = 1;
+    for (uint32_t e = HALF - 2 - s0, b = 5; e; e >>= 1, b = (b * b) & (2 * N - 1))
+        if (e & 1) t = (t * b) & (2 * N - 1);
+    const uint32_t* Dp = precomp + (size_t)g * PC_GROUP + PC_DP + (size_t)c * NSTEPS * D_EFF * N + i;
+    const uint32_t* Dm = precomp + (size_t)g * PC_GROUP + PC_DM + (size_t)c * NSTEPS * D_EFF * N + i;
+    uint64_t acc[BT];
+#pragma unroll
+    for (int b = 0; b < BT; b++) acc[b] = 0;
+    for (uint32_t s = s0; s < s1; s++) {
+        const uint32_t pf = perm_slot(t, v), pc = N - 1 - pf;
+        const size_t o = (size_t)s * D_EFF * N;
+        const uint64_t d0 = __builtin_nontemporal_load(Dp + o), d1 = __builtin_nontemporal_load(Dp + o + N);
+        const uint64_t e0 = __builtin_nontemporal_load(Dm + o), e1 = __builtin_nontemporal_load(Dm + o + N);
+#pragma unroll
+        for (int b = 0; b < BT; b++)
+            acc[b] += d0 * K[b][0][pf] + d1 * K[b][1][pf] + e0 * K[b][0][pc] + e1 * K[b][1][pc];
+        t = (t * 3277u) & (2 * N - 1);
+    }
+#pragma unroll
+    for (int b = 0; b < BT; b++)
+        if ((uint32_t)b < nq) part2[(((size_t)b * nch + ch) * G + g) * 2 * N + (size_t)c * N + i] = red64(acc[b], (int)c);
+}
+
+// ------------------------------------------------------------ pack finalize
+__global__ __launch_bounds__(256) void k_pack_finalize(const uint32_t* __restrict__ part, size_t part_stride,
+                                                       uint32_t S, uint32_t C, const uint32_t* __restrict__ part2,
+                                                       size_t p2_stride, uint32_t nch,
+                                                       const uint32_t* __restrict__ precomp, uint32_t G,
+                                                       const uint32_t* __restrict__ keys, size_t kstride,
+                                                       uint32_t* __restrict__ packed_b, size_t pstride,
+                                                       NttConst nc) {
+    __shared__ uint32_t buf[POLY_LDS];
+    const uint32_t g = blockIdx.x, c = blockIdx.y, qi = blockIdx.z, t = threadIdx.x;
+    const uint32_t q = qof((int)c);
+    const uint32_t* pp = part + qi * part_stride + (size_t)c * C + (size_t)g * N;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        const uint32_t i = t + 256 * k;
+        uint64_t a = 0;
+        for (uint32_t sl = 0; sl < S; sl++) a += pp[(size_t)sl * 2 * C + i];
+        buf[ls(i)] = red64(a, (int)c);
+    }
+    __syncthreads();
+    ntt_fwd_lds(buf, t, nc, (int)c);
+    const uint32_t* Df = precomp + (size_t)g * PC_GROUP + PC_DF + (size_t)c * D_EFF * N;
+    const uint32_t* Kn = keys + qi * kstride + BLK + (size_t)c * N;  // key 1 = ksk_neg1, [digit][limb][N]
+    const uint32_t* p2 = part2 + qi * p2_stride + (size_t)g * 2 * N + (size_t)c * N;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        const uint32_t i = t + 256 * k;
+        uint64_t sub = (uint64_t)Df[i] * Kn[i] + (uint64_t)Df[N + i] * Kn[2 * N + i];
+        for (uint32_t ch = 0; ch < nch; ch++) sub += p2[(size_t)ch * G * 2 * N + i];
+        packed_b[qi * pstride + (size_t)g * 2 * N + (size_t)c * N + i] = sub_mod(buf[ls(i)], red64(sub, (int)c), q);
+    }
+}
+
+// ------------------------------------------------------------ Horner chain
+// One workgroup per (response ciphertext, query); 4 teams of 256 lanes.
+// acc (a0, a1, b0, b1 polys, NTT form) lives in registers between steps;
+// per step: 4 INTTs, gadget split, 2 rounds of 4 NTTs, multiply-accumulate
+// against the query's RGSW rows, + packed[i]. L == 1 copies packed[start].
+__global__ __launch_bounds__(1024) void k_horner(const uint32_t* __restrict__ pa_base, size_t pa_stride, uint32_t G, uint32_t Da,
+                                                 uint32_t T, const uint32_t* __restrict__ packed_b,
+                                                 size_t pstride, const uint32_t* __restrict__ rgsw,
+                                                 size_t rstride, uint64_t* __restrict__ out, NttConst nc,
+                                                 CrtConst cc) {
+    __shared__ uint32_t buf[4][POLY_LDS];
+    const uint32_t ct = blockIdx.x, qi = blockIdx.y, tid = threadIdx.x;
+    const uint32_t team = tid >> 8, t = tid & 255;
+    const uint32_t start = ct * Da;
+    const uint32_t L = (G - start) < Da ? (G - start) : Da;
+    const uint32_t* pb = packed_b + qi * pstride;
+    const uint32_t* rg = rgsw + qi * rstride;
+    uint32_t acc[4][2];
+    {
+        const uint32_t gl = start + L - 1;
+        const uint32_t* pa = pa_base + (size_t)gl * pa_stride;
+#pragma unroll
+        for (int e = 0; e < 2; e++) {
+            const uint32_t i = tid + 1024 * e;
+            acc[0][e] = pa[i]; acc[1][e] = pa[N + i];
+            acc[2][e] = pb[(size_t)gl * 2 * N + i]; acc[3][e] = pb[(size_t)gl * 2 * N + N + i];
+        }
+    }
+    for (int st = (int)L - 2; st >= 0; st--) {
+#pragma unroll
+        for (int p = 0; p < 4; p++)
+#pragma unroll
+            for (int e = 0; e < 2; e++) buf[p][ls(tid + 1024 * e)] = acc[p][e];
+        __syncthreads();
+        ntt_inv_lds(buf[team], t, nc, (int)(team & 1));
+        uint32_t da0[2][2], da1[2][2], db0[2][2], db1[2][2];  // [e][digit], residue mod Q0 / Q1
+#pragma unroll
+        for (int e = 0; e < 2; e++) {
+            const uint32_t i = tid + 1024 * e;
+            gadget2(buf[0][ls(i)], buf[1][ls(i)], cc, da0[e], da1[e]);
+            gadget2(buf[2][ls(i)], buf[3][ls(i)], cc, db0[e], db1[e]);
+        }
+        uint64_t ra[2][2], rb[2][2];  // [limb][e]
+#pragma unroll
+        for (int round = 0; round < 2; round++) {
+            __syncthreads();
+#pragma unroll
+            for (int e = 0; e < 2; e++) {
+                const uint32_t i = tid + 1024 * e;
+#pragma unroll
+                for (int j = 0; j < 2; j++) {
+                    buf[j * 2 + 0][ls(i)] = round ? db0[e][j] : da0[e][j];
+                    buf[j * 2 + 1][ls(i)] = round ? db1[e][j] : da1[e][j];
+                }
+            }
+            __syncthreads();
+            ntt_fwd_lds(buf[team], t, nc, (int)(team & 1));
+            // RGSW parts: 0 top_a, 1 top_b, 2 bot_a, 3 bot_b; [part][digit][limb][N]
+            const uint32_t* pA = rg + (size_t)(round ? 2 : 0) * BLK;
+            const uint32_t* pB = rg + (size_t)(round ? 3 : 1) * BLK;
+#pragma unroll
+            for (int c = 0; c < 2; c++)
+#pragma unroll
+                for (int e = 0; e < 2; e++) {
+                    const uint32_t i = tid + 1024 * e;
+                    uint64_t sa = round ? ra[c][e] : 0, sb = round ? rb[c][e] : 0;
+#pragma unroll
+                    for (int j = 0; j < 2; j++) {
+                        const uint64_t d = buf[j * 2 + c][ls(i)];
+                        sa += d * pA[(size_t)(j * 2 + c) * N + i];
+                        sb += d * pB[(size_t)(j * 2 + c) * N + i];
+                    }
+                    ra[c][e] = sa; rb[c][e] = sb;
+                }
+        }
+        const uint32_t gi = start + (uint32_t)st;
+        const uint32_t* pa = pa_base + (size_t)gi * pa_stride;
+#pragma unroll
+        for (int c = 0; c < 2; c++)
+#pragma unroll
+            for (int e = 0; e < 2; e++) {
+                const uint32_t i = tid + 1024 * e;
+                const uint32_t q = qof(c);
+                acc[c][e] = add_mod(red64(ra[c][e], c), pa[(size_t)c * N + i], q);
+                acc[2 + c][e] = add_mod(red64(rb[c][e], c), pb[(size_t)gi * 2 * N + (size_t)c * N + i], q);
+            }
+        __syncthreads();
+    }
+    uint64_t* o = out + ((size_t)qi * T + ct) * 4 * N;
+#pragma unroll
+    for (int p = 0; p < 4; p++)
+#pragma unroll
+        for (int e = 0; e < 2; e++) o[(size_t)p * N + tid + 1024 * e] = acc[p][e];
+}
+
+}  // namespace
+
+hipError_t launch_unmarshal(const uint64_t* d_q, size_t count, size_t R, double* bq, uint32_t* keys,
+                            uint32_t* rgsw, hipStream_t s) {
+    const size_t total = count * (2 * R + 6 * BLK);
+    size_t blocks = (total + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(k_unmarshal, dim3((unsigned)blocks), dim3(256), 0, s, d_q, total, R, bq, keys, rgsw);
+    return hipGetLastError();
+}
+
+hipError_t launch_ntt_rns(uint32_t* polys, size_t npoly, const NttConst& nc, bool inverse, hipStream_t s) {
+    if (npoly == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_ntt_rns, dim3((unsigned)(2 * npoly)), dim3(256), 0, s, polys, nc, inverse ? 1 : 0);
+    return hipGetLastError();
+}
+
+DbPassShape dbpass_shape(size_t R, size_t C, int target_blocks) {
+    DbPassShape sh;
+    sh.cols_per_thread = (C >= 8 * 2048) ? 8 : 4;
+    const size_t col_blocks = C / (256 * sh.cols_per_thread);
+    size_t slices = 1;
+    while (col_blocks * slices < (size_t)target_blocks && R / (slices * 2) >= 64 && (R / (slices * 2)) % 4 == 0)
+        slices *= 2;
+    sh.slices = (uint32_t)slices;
+    sh.rows_per_slice = (uint32_t)(R / slices);
+    return sh;
+}
+
+hipError_t launch_dbpass(const uint16_t* db, const double* bq, uint32_t* part, size_t R, size_t C,
+                         const DbPassShape& sh, hipStream_t s) {
+    const unsigned cb = (unsigned)(C / (256 * sh.cols_per_thread));
+    if (sh.cols_per_thread == 8)
+        hipLaunchKernelGGL(k_dbpass<8>, dim3(cb, sh.slices), dim3(256), 0, s, db, bq, part, (uint32_t)R,
+                           (uint32_t)C, sh.rows_per_slice);
+    else
+        hipLaunchKernelGGL(k_dbpass<4>, dim3(cb, sh.slices), dim3(256), 0, s, db, bq, part, (uint32_t)R,
+                           (uint32_t)C, sh.rows_per_slice);
+    return hipGetLastError();
+}
+
+static uint32_t collapse_chunks(size_t G) {
+    uint32_t nch = 4;  // <= 256 steps per chunk keeps the u64 sums below 2^63
+    while (nch < 64 && (size_t)16 * nch * G < 4096) nch *= 2;
+    return nch;
+}
+
+hipError_t launch_collapse_partial(const uint32_t* precomp, size_t G, const uint32_t* keys, size_t kstride,
+                                   size_t nq, uint32_t* part2, uint32_t nch, hipStream_t s) {
+    const uint32_t spc = (NSTEPS + nch - 1) / nch;
+    dim3 grid(8 * nch, 2, (unsigned)G);
+    if (nq == 1)
+        hipLaunchKernelGGL(k_collapse_partial<1>, grid, dim3(256), 0, s, precomp, keys, kstride, (uint32_t)nq,
+                           part2, (uint32_t)G, nch, spc);
+    else if (nq == 2)
+        hipLaunchKernelGGL(k_collapse_partial<2>, grid, dim3(256), 0, s, precomp, keys, kstride, (uint32_t)nq,
+                           part2, (uint32_t)G, nch, spc);
+    else
+        hipLaunchKernelGGL(k_collapse_partial<4>, grid, dim3(256), 0, s, precomp, keys, kstride, (uint32_t)nq,
+                           part2, (uint32_t)G, nch, spc);
+    return hipGetLastError();
+}
+
+uint32_t collapse_nch(size_t G) { return collapse_chunks(G); }
+
+hipError_t launch_pack_finalize(const uint32_t* part, size_t part_stride, uint32_t slices, size_t C,
+                                const uint32_t* part2, size_t part2_stride, uint32_t nch, const uint32_t* precomp,
+                                size_t G, const uint32_t* keys, size_t keys_stride, uint32_t* packed_b,
+                                size_t packed_stride, size_t nq, const NttConst& nc, hipStream_t s) {
+    hipLaunchKernelGGL(k_pack_finalize, dim3((unsigned)G, 2, (unsigned)nq), dim3(256), 0, s, part, part_stride,
+                       slices, (uint32_t)C, part2, part2_stride, nch, precomp, (uint32_t)G, keys, keys_stride,
+                       packed_b, packed_stride, nc);
+    return hipGetLastError();
+}
+
+hipError_t launch_horner(const uint32_t* pa_base, size_t pa_stride, size_t G, size_t Da, size_t T,
+                         const uint32_t* packed_b, size_t packed_stride, const uint32_t* rgsw, size_t rgsw_stride,
+                         uint64_t* out, size_t nq, const NttConst& nc, const CrtConst& cc, hipStream_t s) {
+    hipLaunchKernelGGL(k_horner, dim3((unsigned)T, (unsigned)nq), dim3(1024), 0, s, pa_base, pa_stride, (uint32_t)G,
+                       (uint32_t)Da, (uint32_t)T, packed_b, packed_stride, rgsw, rgsw_stride, out, nc, cc);
+    return hipGetLastError();
+}
+
+}  // namespace ipx

@@ -1,0 +1,224 @@
+// Continuous-batching serving loop (capi.h ipir_loop_*). No counterpart in
+// the reference, whose server is a blocking, externally serialised call
+// (src/gpu_protocol.h:67, src/capi.h:12-14).
+//
+// The context's max_batch slots are split into `max_inflight` partitions,
+// each with its own HIP stream, pinned staging and completion thread. A
+// dispatcher thread admits waiting requests into the first free partition
+// (up to its slot count) as soon as one frees up, so a new batch starts
+// while earlier batches are still on the device and their kernels overlap
+// (e.g. one batch's latency-bound Horner chains beside the next batch's
+// HBM-bound database pass). Each request completes independently and
+// records its submit-to-completion latency. Results are bit-identical to
+// answering the request alone: the pipeline is exact and per-slot.
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "capi.h"
+#include "server_ctx.h"
+
+using namespace inspire;
+
+namespace {
+using Clock = std::chrono::steady_clock;
+
+struct Req {
+    uint64_t ticket = 0;
+    std::vector<uint64_t> q;  // flat query
+    uint64_t* out = nullptr;
+    Clock::time_point t0;
+    double lat_us = 0;
+    bool done = false;
+    bool failed = false;
+};
+
+struct Partition {
+    size_t slot0 = 0, nslots = 0;
+    hipStream_t stream = nullptr;
+    std::vector<Req*> batch;  // admitted requests; partition busy when non-empty
+    bool launched = false;    // batch's device work fully enqueued
+    std::thread completer;
+    std::condition_variable cv;  // wakes the completer
+};
+}  // namespace
+
+struct ipir_loop {
+    ipir_server* srv = nullptr;
+    GpuServerCtx* ctx = nullptr;
+    std::mutex m;
+    std::condition_variable cv_work;  // dispatcher: requests or free partitions
+    std::condition_variable cv_done;  // waiters
+    std::deque<Req*> pending;
+    std::unordered_map<uint64_t, Req*> reqs;
+    std::vector<Partition*> parts;
+    std::thread dispatcher;
+    bool stop = false;
+    uint64_t next_ticket = 1;
+    std::unique_lock<std::mutex> owner;  // holds ctx->mu while the loop runs
+
+    void dispatch_loop() {
+        std::unique_lock<std::mutex> lk(m);
+        for (;;) {
+            Partition* free_p = nullptr;
+            cv_work.wait(lk, [&] {
+                if (stop) return true;
+                if (pending.empty()) return false;
+                for (auto* p : parts)
+                    if (p->batch.empty()) { free_p = p; return true; }
+                return false;
+            });
+            if (stop) return;
+            const size_t n = pending.size() < free_p->nslots ? pending.size() : free_p->nslots;
+            for (size_t i = 0; i < n; i++) {
+                free_p->batch.push_back(pending.front());
+                pending.pop_front();
+            }
+            for (size_t i = 0; i < n; i++)
+                std::memcpy(ctx->h_q + (free_p->slot0 + i) * ctx->qlen, free_p->batch[i]->q.data(), ctx->qlen * 8);
+            lk.unlock();
+            bool ok = true;
+            try {
+                const size_t s0 = free_p->slot0;
+                hip_check(hipMemcpyAsync(ctx->d_q + s0 * ctx->qlen, ctx->h_q + s0 * ctx->qlen, n * ctx->qlen * 8,
+                                         hipMemcpyHostToDevice, free_p->stream), "loop h2d");
+                ctx->enqueue(ctx->d_q + s0 * ctx->qlen, n, ctx->d_resp + s0 * ctx->resp_u64s, free_p->stream, s0);
+                hip_check(hipMemcpyAsync(ctx->h_resp + s0 * ctx->resp_u64s, ctx->d_resp + s0 * ctx->resp_u64s,
+                                         n * ctx->resp_u64s * 8, hipMemcpyDeviceToHost, free_p->stream), "loop d2h");
+            } catch (...) {
+                ok = false;
+            }
+            lk.lock();
+            if (!ok)
+                for (Req* r : free_p->batch) r->failed = true;
+            free_p->launched = true;
+            free_p->cv.notify_one();
+        }
+    }
+
+    void complete_loop(Partition* p) {
+        std::unique_lock<std::mutex> lk(m);
+        for (;;) {
+            p->cv.wait(lk, [&] { return stop || p->launched; });
+            if (!p->launched) return;  // stopping with nothing in flight
+            lk.unlock();
+            const bool ok = hipStreamSynchronize(p->stream) == hipSuccess;
+            const auto now = Clock::now();
+            lk.lock();
+            for (size_t i = 0; i < p->batch.size(); i++) {
+                Req* r = p->batch[i];
+                if (ok && !r->failed)
+                    std::memcpy(r->out, ctx->h_resp + (p->slot0 + i) * ctx->resp_u64s, ctx->resp_u64s * 8);
+                else
+                    r->failed = true;
+                r->lat_us = std::chrono::duration<double, std::micro>(now - r->t0).count();
+                r->done = true;
+            }
+            p->batch.clear();
+            p->launched = false;
+            cv_done.notify_all();
+            cv_work.notify_one();
+        }
+    }
+};
+
+extern "C" {
+
+ipir_loop* ipir_loop_start(ipir_server* srv, size_t max_inflight) {
+    if (!srv || max_inflight == 0 || max_inflight > 4) return nullptr;
+    GpuServerCtx* ctx = srv->ctx;
+    if (max_inflight > ctx->cfg.max_batch) max_inflight = ctx->cfg.max_batch;
+    auto* L = new ipir_loop;
+    L->srv = srv;
+    L->ctx = ctx;
+    L->owner = std::unique_lock<std::mutex>(ctx->mu);
+    const size_t per = ctx->cfg.max_batch / max_inflight;
+    for (size_t i = 0; i < max_inflight; i++) {
+        auto* p = new Partition;
+        p->slot0 = i * per;
+        p->nslots = per;
+        if (hipStreamCreate(&p->stream) != hipSuccess) {
+            delete p;
+            break;
+        }
+        L->parts.push_back(p);
+    }
+    if (L->parts.empty()) {
+        delete L;
+        return nullptr;
+    }
+    for (auto* p : L->parts) p->completer = std::thread([L, p] { L->complete_loop(p); });
+    L->dispatcher = std::thread([L] { L->dispatch_loop(); });
+    return L;
+}
+
+uint64_t ipir_loop_submit(ipir_loop* L, const ipir_query_view* v, uint64_t* out_resp) {
+    if (!L || !v || !out_resp || !v->lwe_b_limb0 || !v->lwe_b_limb1 || !v->ksk5_b || !v->kskneg1_b || !v->rgsw)
+        return 0;
+    auto* r = new Req;
+    const size_t R = L->ctx->pp.db_rows, b = (size_t)D_EFF * 2 * N;
+    r->q.resize(L->ctx->qlen);
+    std::memcpy(r->q.data(), v->lwe_b_limb0, R * 8);
+    std::memcpy(r->q.data() + R, v->lwe_b_limb1, R * 8);
+    std::memcpy(r->q.data() + 2 * R, v->ksk5_b, b * 8);
+    std::memcpy(r->q.data() + 2 * R + b, v->kskneg1_b, b * 8);
+    std::memcpy(r->q.data() + 2 * R + 2 * b, v->rgsw, 4 * b * 8);
+    r->out = out_resp;
+    std::lock_guard<std::mutex> lk(L->m);
+    if (L->stop) {
+        delete r;
+        return 0;
+    }
+    r->ticket = L->next_ticket++;
+    r->t0 = Clock::now();
+    L->reqs[r->ticket] = r;
+    L->pending.push_back(r);
+    L->cv_work.notify_one();
+    return r->ticket;
+}
+
+int ipir_loop_wait(ipir_loop* L, uint64_t ticket, double* latency_us) {
+    if (!L) return 1;
+    std::unique_lock<std::mutex> lk(L->m);
+    auto it = L->reqs.find(ticket);
+    if (it == L->reqs.end()) return 1;
+    Req* r = it->second;
+    L->cv_done.wait(lk, [&] { return r->done; });
+    if (latency_us) *latency_us = r->lat_us;
+    const int rc = r->failed ? 4 : 0;
+    L->reqs.erase(it);
+    delete r;
+    return rc;
+}
+
+void ipir_loop_stop(ipir_loop* L) {
+    if (!L) return;
+    {
+        // let admitted work finish, then stop
+        std::unique_lock<std::mutex> lk(L->m);
+        L->cv_done.wait(lk, [&] {
+            if (!L->pending.empty()) return false;
+            for (auto* p : L->parts)
+                if (!p->batch.empty()) return false;
+            return true;
+        });
+        L->stop = true;
+    }
+    L->cv_work.notify_all();
+    for (auto* p : L->parts) p->cv.notify_all();
+    L->dispatcher.join();
+    for (auto* p : L->parts) {
+        p->completer.join();
+        (void)hipStreamDestroy(p->stream);
+        delete p;
+    }
+    for (auto& kv : L->reqs) delete kv.second;
+    L->owner.unlock();
+    delete L;
+}
+
+}  // extern "C"

@@ -1,0 +1,343 @@
+// Server orchestration on one MI355X: preprocess, context setup, the
+// batched answer pipeline and capability reporting.
+//
+// Lifecycle and ownership follow the reference (src/gpu_protocol.cu:535-1584):
+// gpu_preprocess returns device buffers that gpu_setup_server adopts; every
+// per-query buffer is allocated at setup for max_batch queries, so the
+// request path never allocates. Differences by design: the precomp of all
+// groups lives in one arena (one allocation, fixed group stride), the
+// answer pipeline is a fixed sequence of batched kernels on one stream (no
+// per-group launches), and errors surface as exceptions instead of exit().
+#include <cstring>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "gpu_protocol.h"
+#include "kernels.h"
+#include "server_ctx.h"
+
+using namespace ipx;
+
+namespace inspire {
+
+void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+#define HIPX(x) hip_check((x), #x)
+
+namespace {
+
+template <class T> T* dalloc(size_t count) {
+    void* p = nullptr;
+    if (count == 0) count = 1;
+    hip_check(hipMalloc(&p, count * sizeof(T)), "hipMalloc");
+    return static_cast<T*>(p);
+}
+
+// Twiddles + Shoup companions for both limbs in the device layout of
+// NttConst, built from the host NTT tables (the same tables the client uses).
+uint32_t* upload_tables(NttConst& nc, CrtConst& cc) {
+    std::vector<uint32_t> h(2 * 4 * (size_t)ipx::N);
+    for (int l = 0; l < 2; l++) {
+        const NttTables& t = ntt_tables(l);
+        for (size_t i = 0; i < ipx::N; i++) {
+            h[(l * 4 + 0) * ipx::N + i] = (uint32_t)t.fwd_twiddles[i];
+            h[(l * 4 + 1) * ipx::N + i] = (uint32_t)t.fwd_primes[i];
+            h[(l * 4 + 2) * ipx::N + i] = (uint32_t)t.inv_twiddles[i];
+            h[(l * 4 + 3) * ipx::N + i] = (uint32_t)t.inv_primes[i];
+        }
+        nc.inv_n[l] = (uint32_t)t.inv_n;
+        nc.inv_n_p[l] = (uint32_t)t.inv_n_prime;
+    }
+    uint32_t* d = dalloc<uint32_t>(h.size());
+    hip_check(hipMemcpy(d, h.data(), h.size() * 4, hipMemcpyHostToDevice), "upload twiddles");
+    nc.tw = d;
+    cc.q1inv_q0 = (uint32_t)mod_inv(Q1 % Q0, Q0);
+    cc.q0inv_q1 = (uint32_t)mod_inv(Q0 % Q1, Q1);
+    return d;
+}
+
+void poly_to_u32(const RnsPoly& p, uint32_t* dst) {
+    for (size_t i = 0; i < N; i++) { dst[i] = (uint32_t)p.limbs[0][i]; dst[N + i] = (uint32_t)p.limbs[1][i]; }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- preprocess
+PreprocessData gpu_preprocess(const PublicParams& pp, const uint16_t* db) {
+    if (!db) throw std::invalid_argument("gpu_preprocess: null db");
+    const size_t R = pp.db_rows, C = pp.db_cols, G = pp.n_packed;
+    if (R == 0 || R % N || C == 0 || C % N || G != C / N)
+        throw std::invalid_argument("gpu_preprocess: inconsistent geometry");
+    const size_t Da = pp.D < G ? pp.D : G;
+    const size_t nr = R / N;
+    hipStream_t s = nullptr;
+    NttConst nc{};
+    CrtConst cc{};
+    uint32_t* d_tw = nullptr;
+    uint16_t* d_db = nullptr;
+    uint32_t* arena = nullptr;
+    uint32_t *d_crs = nullptr, *d_k5 = nullptr, *d_kn = nullptr, *d_a32 = nullptr, *d_states = nullptr;
+    auto cleanup = [&](bool all) {
+        dfree(d_tw); dfree(d_crs); dfree(d_k5); dfree(d_kn); dfree(d_a32); dfree(d_states);
+        if (all) { dfree(d_db); dfree(arena); }
+        if (s) (void)hipStreamDestroy(s);
+    };
+    try {
+        HIPX(hipStreamCreate(&s));
+        d_tw = upload_tables(nc, cc);
+        d_db = dalloc<uint16_t>(R * C);
+        HIPX(hipMemcpy(d_db, db, R * C * sizeof(uint16_t), hipMemcpyHostToDevice));
+        HIPX(launch_encode(d_db, R, C, Da, s));
+
+        // CRS rows A_r = NTT(tau_{2N-1}(expand_seed("rlwe", r)))  (src/gpu_protocol.cu:567-583)
+        std::vector<uint32_t> h_crs(nr * 2 * N);
+        for (size_t r = 0; r < nr; r++) {
+            RnsPoly a = expand_seed(pp.seed.data(), "rlwe", r);
+            RnsPoly rev = automorphism(a, 2 * N - 1);
+            rev.to_ntt();
+            poly_to_u32(rev, &h_crs[r * 2 * N]);
+        }
+        d_crs = dalloc<uint32_t>(h_crs.size());
+        HIPX(hipMemcpy(d_crs, h_crs.data(), h_crs.size() * 4, hipMemcpyHostToDevice));
+        // Server-side key a-parts, NTT form, [digit][limb][N]  (src/gpu_protocol.cu:405-456)
+        std::vector<uint32_t> h_k5(D_EFF * 2 * N), h_kn(D_EFF * 2 * N);
+        for (int j = 0; j < D_EFF; j++) {
+            RnsPoly a5 = expand_seed(pp.seed.data(), "ksk", (uint64_t)AUTO_GEN * 100 + j + 1);
+            RnsPoly an = expand_seed(pp.seed.data(), "ksk", (uint64_t)(2 * N - 1) * 100 + j + 1);
+            a5.to_ntt();
+            an.to_ntt();
+            poly_to_u32(a5, &h_k5[j * 2 * N]);
+            poly_to_u32(an, &h_kn[j * 2 * N]);
+        }
+        d_k5 = dalloc<uint32_t>(h_k5.size());
+        d_kn = dalloc<uint32_t>(h_kn.size());
+        HIPX(hipMemcpy(d_k5, h_k5.data(), h_k5.size() * 4, hipMemcpyHostToDevice));
+        HIPX(hipMemcpy(d_kn, h_kn.data(), h_kn.size() * 4, hipMemcpyHostToDevice));
+
+        arena = dalloc<uint32_t>(G * PC_GROUP);
+        // groups are processed in chunks bounded by the transient ring-embed
+        // table (32 MiB per group)
+        const size_t chunk = G < 256 ? G : 256;
+        d_a32 = dalloc<uint32_t>(chunk * 2 * (size_t)N * N);
+        d_states = dalloc<uint32_t>(chunk * 4 * (size_t)N);
+        for (size_t g0 = 0; g0 < G; g0 += chunk) {
+            const size_t ng = G - g0 < chunk ? G - g0 : chunk;
+            uint32_t* pc = arena + g0 * PC_GROUP;
+            HIPX(launch_aside(d_db, R, C, g0, ng, d_crs, d_a32, nc, s));
+            HIPX(launch_ring_embed(d_a32, ng, nc, s));
+            HIPX(launch_collapse_chain(d_a32, ng, d_k5, pc, d_states, nc, cc, s));
+            HIPX(launch_collapse_final(d_states, ng, d_kn, pc, nc, cc, s));
+        }
+        HIPX(hipStreamSynchronize(s));
+        cleanup(false);
+    } catch (...) {
+        cleanup(true);
+        throw;
+    }
+    PreprocessData pd;
+    pd.d_db_col = d_db;
+    pd.d_precomp_arena = arena;
+    pd.d_precomp_a.resize(G);
+    pd.d_precomp_D_plus.resize(G);
+    pd.d_precomp_D_minus.resize(G);
+    pd.d_precomp_D_final.resize(G);
+    for (size_t g = 0; g < G; g++) {
+        uint32_t* b = arena + g * PC_GROUP;
+        pd.d_precomp_a[g] = b + PC_A;
+        pd.d_precomp_D_plus[g] = b + PC_DP;
+        pd.d_precomp_D_minus[g] = b + PC_DM;
+        pd.d_precomp_D_final[g] = b + PC_DF;
+    }
+    return pd;
+}
+
+// ---------------------------------------------------------------- setup
+GpuServerCtx* gpu_setup_server(const PublicParams& pp, const PreprocessData& precomp) {
+    return gpu_setup_server(pp, precomp, GpuServerConfig{});
+}
+
+GpuServerCtx* gpu_setup_server(const PublicParams& pp, const PreprocessData& precomp, const GpuServerConfig& cfg) {
+    if (cfg.max_batch == 0) throw std::invalid_argument("gpu_setup_server: max_batch must be >= 1");
+    const size_t G = pp.n_packed;
+    if (!precomp.d_db_col || precomp.d_precomp_a.size() != G || precomp.d_precomp_D_plus.size() != G ||
+        precomp.d_precomp_D_minus.size() != G || precomp.d_precomp_D_final.size() != G)
+        throw std::invalid_argument("gpu_setup_server: precomp does not match the geometry");
+    auto* ctx = new GpuServerCtx;
+    ctx->pp = pp;
+    ctx->cfg = cfg;
+    ctx->G = G;
+    ctx->Da = pp.D < G ? pp.D : G;
+    ctx->T = pp.num_cts;
+    try {
+        HIPX(hipGetDevice(&ctx->device));
+        HIPX(hipStreamCreate(&ctx->stream));
+        ctx->d_tw = upload_tables(ctx->nc, ctx->cc);
+        ctx->d_db = precomp.d_db_col;
+        if (precomp.d_precomp_arena) {
+            ctx->d_precomp = static_cast<uint32_t*>(precomp.d_precomp_arena);
+        } else {
+            // per-group buffers from elsewhere: gather them into one arena
+            ctx->d_precomp = dalloc<uint32_t>(G * PC_GROUP);
+            for (size_t g = 0; g < G; g++) {
+                uint32_t* b = ctx->d_precomp + g * PC_GROUP;
+                HIPX(hipMemcpy(b + PC_A, precomp.d_precomp_a[g], 2 * N * 4, hipMemcpyDeviceToDevice));
+                HIPX(hipMemcpy(b + PC_DP, precomp.d_precomp_D_plus[g], PC_DSZ * 4, hipMemcpyDeviceToDevice));
+                HIPX(hipMemcpy(b + PC_DM, precomp.d_precomp_D_minus[g], PC_DSZ * 4, hipMemcpyDeviceToDevice));
+                HIPX(hipMemcpy(b + PC_DF, precomp.d_precomp_D_final[g], 2 * D_EFF * N * 4, hipMemcpyDeviceToDevice));
+                dfree(precomp.d_precomp_a[g]); dfree(precomp.d_precomp_D_plus[g]);
+                dfree(precomp.d_precomp_D_minus[g]); dfree(precomp.d_precomp_D_final[g]);
+            }
+        }
+        // ownership transfer (src/gpu_protocol.cu:856-885)
+        auto& pd = const_cast<PreprocessData&>(precomp);
+        pd.d_db_col = nullptr;
+        pd.d_precomp_arena = nullptr;
+        pd.d_precomp_a.clear(); pd.d_precomp_D_plus.clear(); pd.d_precomp_D_minus.clear(); pd.d_precomp_D_final.clear();
+        ctx->alloc_slots();
+    } catch (...) {
+        gpu_free_server(ctx);
+        throw;
+    }
+    return ctx;
+}
+
+void GpuServerCtx::alloc_slots() {
+    const size_t R = pp.db_rows, C = pp.db_cols, B = cfg.max_batch;
+    sh = dbpass_shape(R, C, 2048);
+    nch = collapse_nch(G);
+    lay.R = R; lay.C = C; lay.G = G;
+    qlen = lay.query_u64s();
+    part_stride = (size_t)sh.slices * 2 * C;
+    part2_stride = (size_t)nch * G * 2 * N;
+    keys_stride = lay.keys_words();
+    rgsw_stride = lay.rgsw_words();
+    packed_stride = G * 2 * N;
+    resp_u64s = T * 4 * N;
+    d_q = dalloc<uint64_t>(B * qlen);
+    d_resp = dalloc<uint64_t>(B * resp_u64s);
+    d_bq = dalloc<double>(B * 2 * R);
+    d_keys = dalloc<uint32_t>(B * keys_stride);
+    d_rgsw = dalloc<uint32_t>(B * rgsw_stride);
+    d_part = dalloc<uint32_t>(B * part_stride);
+    d_part2 = dalloc<uint32_t>(B * part2_stride);
+    d_packed_b = dalloc<uint32_t>(B * packed_stride);
+    HIPX(hipHostMalloc((void**)&h_q, B * qlen * 8, hipHostMallocDefault));
+    HIPX(hipHostMalloc((void**)&h_resp, B * resp_u64s * 8, hipHostMallocDefault));
+    resident_bytes = R * C * 2 + G * PC_GROUP * 4 + 8 * N * 4 +
+                     B * (qlen * 8 + resp_u64s * 8 + 2 * R * 8 +
+                          (keys_stride + rgsw_stride + part_stride + part2_stride + packed_stride) * 4);
+}
+
+void GpuServerCtx::enqueue(const uint64_t* d_in, size_t count, uint64_t* d_out, hipStream_t s, size_t slot0) {
+    const size_t R = pp.db_rows, C = pp.db_cols;
+    double* bq = d_bq + slot0 * 2 * R;
+    uint32_t* keys = d_keys + slot0 * keys_stride;
+    uint32_t* rgsw = d_rgsw + slot0 * rgsw_stride;
+    uint32_t* part = d_part + slot0 * part_stride;
+    uint32_t* part2 = d_part2 + slot0 * part2_stride;
+    uint32_t* packed = d_packed_b + slot0 * packed_stride;
+    HIPX(launch_unmarshal(d_in, count, R, bq, keys, rgsw, s));
+    HIPX(launch_ntt_rns(keys, count * 2 * D_EFF, nc, false, s));
+    for (size_t q = 0; q < count; q++)
+        HIPX(launch_dbpass(d_db, bq + q * 2 * R, part + q * part_stride, R, C, sh, s));
+    for (size_t q0 = 0; q0 < count; q0 += 4) {
+        const size_t nq = count - q0 < 4 ? count - q0 : 4;
+        HIPX(launch_collapse_partial(d_precomp, G, keys + q0 * keys_stride, keys_stride, nq,
+                                     part2 + q0 * part2_stride, nch, s));
+    }
+    HIPX(launch_pack_finalize(part, part_stride, sh.slices, C, part2, part2_stride, nch, d_precomp, G, keys,
+                              keys_stride, packed, packed_stride, count, nc, s));
+    HIPX(launch_horner(d_precomp + PC_A, PC_GROUP, G, Da, T, packed, packed_stride, rgsw, rgsw_stride, d_out,
+                       count, nc, cc, s));
+}
+
+void GpuServerCtx::answer_host(size_t count, uint64_t* out, hipStream_t s, size_t slot0) {
+    if (!s) s = stream;
+    uint64_t* hq = h_q + slot0 * qlen;
+    uint64_t* hr = h_resp + slot0 * resp_u64s;
+    uint64_t* dq = d_q + slot0 * qlen;
+    uint64_t* dr = d_resp + slot0 * resp_u64s;
+    HIPX(hipMemcpyAsync(dq, hq, count * qlen * 8, hipMemcpyHostToDevice, s));
+    enqueue(dq, count, dr, s, slot0);
+    HIPX(hipMemcpyAsync(hr, dr, count * resp_u64s * 8, hipMemcpyDeviceToHost, s));
+    HIPX(hipStreamSynchronize(s));
+    if (out) std::memcpy(out, hr, count * resp_u64s * 8);
+}
+
+GpuServerCaps gpu_server_caps(const GpuServerCtx* ctx) {
+    GpuServerCaps c{};
+    c.max_batch = ctx->cfg.max_batch;
+    c.resident_bytes = ctx->resident_bytes;
+    c.num_cts = ctx->T;
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) == hipSuccess) c.device_free_bytes = fr;
+    return c;
+}
+
+namespace {
+void flatten_query(const QueryMessage& m, size_t R, uint64_t* w) {
+    if (m.lwe.b_limb0.size() != R || m.lwe.b_limb1.size() != R)
+        throw std::invalid_argument("gpu_answer_batch: query length differs from db_rows");
+    std::memcpy(w, m.lwe.b_limb0.data(), R * 8);
+    std::memcpy(w + R, m.lwe.b_limb1.data(), R * 8);
+    w += 2 * R;
+    auto parts = [&](const std::vector<RnsPoly>& v, bool want_ntt) {
+        if (v.size() != (size_t)D_EFF) throw std::invalid_argument("gpu_answer_batch: wrong digit count");
+        for (const RnsPoly& p0 : v) {
+            RnsPoly p = p0;
+            if (want_ntt) p.to_ntt(); else p.to_coeff();
+            std::memcpy(w, p.limbs[0].data(), N * 8);
+            std::memcpy(w + N, p.limbs[1].data(), N * 8);
+            w += 2 * N;
+        }
+    };
+    parts(m.ksk_5.b_parts, false);
+    parts(m.ksk_neg1.b_parts, false);
+    parts(m.rgsw.top.a_parts, true);
+    parts(m.rgsw.top.b_parts, true);
+    parts(m.rgsw.bottom.a_parts, true);
+    parts(m.rgsw.bottom.b_parts, true);
+}
+}  // namespace
+
+std::vector<std::vector<RlweCt>> gpu_answer_batch(GpuServerCtx* ctx, const QueryMessage* queries, size_t count) {
+    if (!ctx) throw std::invalid_argument("gpu_answer_batch: null ctx");
+    if (count == 0) return {};
+    if (count > ctx->cfg.max_batch) throw std::invalid_argument("gpu_answer_batch: count exceeds max_batch");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    for (size_t i = 0; i < count; i++) flatten_query(queries[i], ctx->pp.db_rows, ctx->h_q + i * ctx->qlen);
+    std::vector<uint64_t> flat(count * ctx->resp_u64s);
+    ctx->answer_host(count, flat.data());
+    std::vector<std::vector<RlweCt>> out(count, std::vector<RlweCt>(ctx->T));
+    const uint64_t* r = flat.data();
+    for (auto& resp : out)
+        for (auto& ct : resp)
+            for (RnsPoly* p : {&ct.a, &ct.b}) {
+                std::memcpy(p->limbs[0].data(), r, N * 8);
+                std::memcpy(p->limbs[1].data(), r + N, N * 8);
+                p->is_ntt = true;
+                r += 2 * N;
+            }
+    return out;
+}
+
+std::vector<RlweCt> gpu_answer(GpuServerCtx* ctx, const QueryMessage& qry) {
+    return gpu_answer_batch(ctx, &qry, 1)[0];
+}
+
+void gpu_free_server(GpuServerCtx* ctx) {
+    if (!ctx) return;
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    dfree(ctx->d_tw); dfree(ctx->d_db); dfree(ctx->d_precomp);
+    dfree(ctx->d_q); dfree(ctx->d_resp); dfree(ctx->d_bq); dfree(ctx->d_keys); dfree(ctx->d_rgsw);
+    dfree(ctx->d_part); dfree(ctx->d_part2); dfree(ctx->d_packed_b);
+    if (ctx->h_q) (void)hipHostFree(ctx->h_q);
+    if (ctx->h_resp) (void)hipHostFree(ctx->h_resp);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+}  // namespace inspire

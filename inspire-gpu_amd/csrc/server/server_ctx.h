@@ -1,0 +1,67 @@
+// Internal definition of the opaque GpuServerCtx (not installed).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+
+#include "gpu_protocol.h"
+#include "kernels.h"
+
+namespace inspire {
+
+void hip_check(hipError_t e, const char* what);
+inline void dfree(void* p) {
+    if (p) (void)hipFree(p);
+}
+
+struct GpuServerCtx {
+    PublicParams pp{};
+    GpuServerConfig cfg{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    size_t G = 0, Da = 0, T = 0;
+
+    // resident data
+    uint16_t* d_db = nullptr;       // encoded DB, row-major u16 [R][C]
+    uint32_t* d_precomp = nullptr;  // [G][PC_GROUP] arena
+    uint32_t* d_tw = nullptr;       // twiddles
+    ipx::NttConst nc{};
+    ipx::CrtConst cc{};
+
+    // per-query slot pools, max_batch each
+    ipx::SlotLayout lay{};
+    ipx::DbPassShape sh{};
+    uint32_t nch = 0;
+    size_t qlen = 0, resp_u64s = 0;
+    size_t part_stride = 0, part2_stride = 0, keys_stride = 0, rgsw_stride = 0, packed_stride = 0;
+    uint64_t* d_q = nullptr;        // flat queries (device staging)
+    uint64_t* d_resp = nullptr;     // flat responses (device staging)
+    double* d_bq = nullptr;
+    uint32_t* d_keys = nullptr;
+    uint32_t* d_rgsw = nullptr;
+    uint32_t* d_part = nullptr;
+    uint32_t* d_part2 = nullptr;
+    uint32_t* d_packed_b = nullptr;
+    uint64_t* h_q = nullptr;        // pinned host staging
+    uint64_t* h_resp = nullptr;
+
+    size_t resident_bytes = 0;
+    std::mutex mu;  // serialises host-staged answers on this context
+
+    void alloc_slots();
+    // Enqueue the answer of `count` flat queries at d_in (device) into d_out
+    // (device) on stream s, using slots [slot0, slot0 + count).
+    void enqueue(const uint64_t* d_in, size_t count, uint64_t* d_out, hipStream_t s, size_t slot0 = 0);
+    // Host-staged round trip through the pinned staging of slots
+    // [slot0, slot0+count): h_q -> device -> h_resp, then copy to out.
+    void answer_host(size_t count, uint64_t* out, hipStream_t s = nullptr, size_t slot0 = 0);
+};
+
+}  // namespace inspire
+
+// C ABI handle (capi.h): one database generation.
+struct ipir_server {
+    inspire::PublicParams pp;
+    inspire::GpuServerCtx* ctx;
+};
+
