@@ -38,6 +38,14 @@ int ipir_dbg_horner(const uint64_t* cts, size_t L, const uint64_t* rgsw, uint64_
 int ipir_dbg_profile_batch(ipir_server* srv, const uint64_t* d_queries, size_t count, uint64_t* d_out,
                            double* stage_ms);
 
+/* Device memory helpers for callers without their own HIP binding (tests,
+ * benchmark): allocate, copy (kind 1 host->device, 2 device->host,
+ * 3 device->device), free, and synchronise the device. */
+void* ipir_dev_alloc(size_t bytes);
+int ipir_dev_free(void* p);
+int ipir_dev_copy(void* dst, const void* src, size_t bytes, int kind);
+int ipir_dev_sync(void);
+
 #ifdef __cplusplus
 }
 #endif

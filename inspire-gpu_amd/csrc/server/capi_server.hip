@@ -283,4 +283,20 @@ int ipir_dbg_profile_batch(ipir_server* srv, const uint64_t* d_queries, size_t c
     return rc;
 }
 
+void* ipir_dev_alloc(size_t bytes) {
+    void* p = nullptr;
+    return hipMalloc(&p, bytes ? bytes : 1) == hipSuccess ? p : nullptr;
+}
+
+int ipir_dev_free(void* p) { return p && hipFree(p) == hipSuccess ? 0 : 1; }
+
+int ipir_dev_copy(void* dst, const void* src, size_t bytes, int kind) {
+    if (!dst || !src) return 1;
+    const hipMemcpyKind k = kind == 1 ? hipMemcpyHostToDevice : kind == 2 ? hipMemcpyDeviceToHost
+                          : kind == 3 ? hipMemcpyDeviceToDevice : hipMemcpyDefault;
+    return hipMemcpy(dst, src, bytes, k) == hipSuccess ? 0 : 4;
+}
+
+int ipir_dev_sync(void) { return hipDeviceSynchronize() == hipSuccess ? 0 : 4; }
+
 }  // extern "C"

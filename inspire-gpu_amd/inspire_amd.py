@@ -81,6 +81,10 @@ SERVER_FUNCS = [
     ("ipir_dbg_ntt", C.c_int, [VP, VP, SZ, C.c_int]),
     ("ipir_dbg_horner", C.c_int, [VP, SZ, VP, VP]),
     ("ipir_dbg_profile_batch", C.c_int, [VP, VP, SZ, VP, VP]),
+    ("ipir_dev_alloc", VP, [SZ]),
+    ("ipir_dev_free", C.c_int, [VP]),
+    ("ipir_dev_copy", C.c_int, [VP, VP, SZ, C.c_int]),
+    ("ipir_dev_sync", C.c_int, []),
 ]
 
 _libs = {}
@@ -105,6 +109,40 @@ def load(which="server"):
 
 def ptr(a):
     return C.c_void_p(a.ctypes.data)
+
+
+class DeviceBuffer:
+    """Device allocation through the library's own HIP runtime."""
+
+    def __init__(self, nbytes, lib=None):
+        self.lib = lib or load("server")
+        self.nbytes = nbytes
+        self.p = self.lib.ipir_dev_alloc(nbytes)
+        if not self.p:
+            raise MemoryError(f"ipir_dev_alloc({nbytes}) failed")
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr)
+        assert arr.nbytes <= self.nbytes
+        if self.lib.ipir_dev_copy(self.p, ptr(arr), arr.nbytes, 1):
+            raise RuntimeError("upload failed")
+
+    def download(self, arr):
+        assert arr.nbytes <= self.nbytes and arr.flags["C_CONTIGUOUS"]
+        if self.lib.ipir_dev_copy(ptr(arr), self.p, arr.nbytes, 2):
+            raise RuntimeError("download failed")
+        return arr
+
+    def free(self):
+        if self.p:
+            self.lib.ipir_dev_free(self.p)
+            self.p = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 class Params:

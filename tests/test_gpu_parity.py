@@ -1,0 +1,243 @@
+"""GPU parity: every device stage against the oracle, bit for bit, through
+the C ABI (and the inspection entry points of include/inspire_debug.h).
+
+Covers the reference's GPU test classes (tests/test_gpu_ntt.cu,
+test_gpu_extprod.cu / test_gpu_horner.cu, test_e2e_gpu.cu,
+test_gpu_batch.cu, test_capi.cu) plus direct checks of the stages the
+reference never pins (encode, precomp tensors, collapse).
+"""
+import ctypes as C
+import sys
+
+import numpy as np
+import pytest
+
+import inspire_amd as ia
+import oracle_lib as ol
+from oracle_lib import BLK, N, NSTEPS, P, Q0, Q1, QS
+
+pytestmark = pytest.mark.gpu
+SEED = np.array([0x42] + [0] * 63, np.uint8)
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+# ---------------------------------------------------------------- kernels
+def test_ntt_kernel_matches_oracle():
+    rng = np.random.default_rng(11)
+    npoly = 16
+    x = ol.rand_poly(rng, (npoly,))
+    lib = ia.load("server")
+    for inv in (0, 1):
+        xin = x.astype(np.uint32)
+        out = np.zeros_like(xin)
+        assert lib.ipir_dbg_ntt(ia.ptr(xin), ia.ptr(out), npoly, inv) == 0
+        want = x.copy()
+        for p in range(npoly):
+            for limb in (0, 1):
+                v = want[p, limb].copy()
+                (ol.oracle().orc_ntt_inv if inv else ol.oracle().orc_ntt_fwd)(v, limb)
+                want[p, limb] = v
+        assert np.array_equal(out.astype(np.uint64), want), f"inverse={inv}"
+
+
+@pytest.mark.parametrize("L", [1, 2, 5, 9])
+def test_horner_kernel_matches_oracle(L):
+    rng = np.random.default_rng(L)
+    rc, geo = ol.setup(262144, 120, 2048)
+    q, _ = ol.make_query(geo, SEED, 1000, 3)
+    rg = np.ascontiguousarray(q[2 * geo["R"] + 2 * BLK:])
+    cts = ol.rand_poly(rng, (L, 2)).reshape(-1)
+    out = np.zeros(4 * N, np.uint64)
+    assert ia.load("server").ipir_dbg_horner(ia.ptr(cts), L, ia.ptr(rg), ia.ptr(out)) == 0
+    want = np.zeros(4 * N, np.uint64)
+    ol.oracle().orc_horner(cts, L, rg, want)
+    assert np.array_equal(out, want)
+
+
+# ---------------------------------------------------------------- servers
+GEOS = {
+    "small": (256, 120, 2048, [0, 42, 100, 255]),             # G=1, no encode, no Horner
+    "medium": (262144, 120, 2048, [0, 42, 1000, 100000, 262143]),  # G=4, encode D=4, Horner
+}
+
+
+@pytest.fixture(scope="module", params=sorted(GEOS))
+def served(request):
+    n, w, rows, idxs = GEOS[request.param]
+    rc, geo = ol.setup(n, w, rows)
+    db = np.random.default_rng(0).integers(0, P, size=(geo["R"], geo["C"]), dtype=np.uint16)
+    say(f"[{request.param}] oracle preprocess G={geo['G']}")
+    pc = ol.OraclePrecomp(db, geo, SEED)
+    say(f"[{request.param}] GPU create")
+    srv = ia.Server(db, n, w, rows, max_batch=4, crs_seed=bytes(SEED))
+    yield request.param, geo, db, pc, srv, idxs
+    srv.close()
+
+
+def test_encoded_db_matches_oracle(served):
+    name, geo, db, pc, srv, _ = served
+    got = np.zeros((geo["R"], geo["C"]), np.uint16)
+    assert srv.lib.ipir_dbg_db_rows(srv.h, 0, geo["R"], ia.ptr(got)) == 0
+    assert np.array_equal(got, pc.db)
+
+
+def test_precomp_tensors_match_oracle(served):
+    name, geo, db, pc, srv, _ = served
+    words = srv.lib.ipir_dbg_precomp_group_words()
+    for g in range(geo["G"]):
+        blk = np.zeros(words, np.uint32)
+        assert srv.lib.ipir_dbg_precomp_group(srv.h, g, ia.ptr(blk)) == 0
+        a = blk[:2 * N].astype(np.uint64)
+        dsz = 2 * NSTEPS * 2 * N
+        Dp = blk[2 * N:2 * N + dsz].astype(np.uint64)
+        Dm = blk[2 * N + dsz:2 * N + 2 * dsz].astype(np.uint64)
+        Df = blk[2 * N + 2 * dsz:].astype(np.uint64)
+        assert np.array_equal(Dp, pc.Dp[g].reshape(-1)), f"D+ group {g}"
+        assert np.array_equal(Dm, pc.Dm[g].reshape(-1)), f"D- group {g}"
+        assert np.array_equal(Df, pc.Df[g].reshape(-1)), f"D_final group {g}"
+        assert np.array_equal(a, pc.a[g].reshape(-1)), f"a group {g}"
+
+
+def test_answer_stages_match_oracle_and_extract(served):
+    name, geo, db, pc, srv, idxs = served
+    for t, idx in enumerate(idxs):
+        say(f"[{name}] idx={idx}")
+        q, st = ol.make_query(geo, SEED, idx, 500 + t)
+        interm = np.zeros(2 * geo["C"], np.uint32)
+        packed = np.zeros(geo["G"] * 2 * N, np.uint32)
+        resp = np.zeros(geo["T"] * 4 * N, np.uint64)
+        assert srv.lib.ipir_dbg_answer_taps(srv.h, ia.ptr(q), ia.ptr(interm), ia.ptr(packed), ia.ptr(resp)) == 0
+        want, w_interm, w_packed = pc.answer(q, taps=True)
+        assert np.array_equal(interm.astype(np.uint64), w_interm), "DB pass"
+        assert np.array_equal(packed.astype(np.uint64), w_packed), "collapse / packed b"
+        assert np.array_equal(resp, want.reshape(-1)), "response"
+        assert np.array_equal(ol.extract(geo, st, resp.reshape(geo["T"], -1)), ol.expected_slots(geo, db, idx))
+
+
+def test_noncanonical_inputs_are_reduced(served):
+    """Residues >= q on the wire are reduced mod q (out of the reference's
+    contract; this library defines them)."""
+    name, geo, db, pc, srv, idxs = served
+    q, st = ol.make_query(geo, SEED, idxs[-1], 99)
+    R = geo["R"]
+    q2 = q.copy()
+    q2[:R] += np.uint64(Q0)            # lwe limb0
+    q2[R:2 * R] += np.uint64(5 * Q1)  # lwe limb1
+    body = q2[2 * R:].reshape(6 * 2, 2, N)
+    body[:, 0, :] += np.uint64(Q0)
+    body[:, 1, :] += np.uint64(2 * Q1)
+    a = srv.answer_batch([q])[0]
+    b = srv.answer_batch([q2])[0]
+    assert np.array_equal(a, b)
+
+
+# ---------------------------------------------------------------- batch contract
+@pytest.fixture(scope="module")
+def medium8():
+    n, w, rows = 524288, 120, 2048  # G=8, D=8 (reference tests/test_gpu_batch.cu:26-37)
+    rc, geo = ol.setup(n, w, rows)
+    db = np.random.default_rng(42).integers(0, P, size=(geo["R"], geo["C"]), dtype=np.uint16)
+    say("[medium8] GPU create")
+    srv = ia.Server(db, n, w, rows, max_batch=10, crs_seed=bytes(SEED))
+    yield geo, db, srv
+    srv.close()
+
+
+def test_batch_equals_singles_and_extracts(medium8):
+    geo, db, srv = medium8
+    idxs = [0, 42, 1000, 100000, 524287] * 2
+    qs, sts = [], []
+    for t, idx in enumerate(idxs):
+        q, st = ol.make_query(geo, SEED, idx, 7000 + t)
+        qs.append(q)
+        sts.append(st)
+    singles = [srv.answer_batch([q])[0] for q in qs]
+    batch = srv.answer_batch(qs)
+    for i in range(len(qs)):
+        assert np.array_equal(batch[i], singles[i]), f"batch[{i}] != single"
+        assert np.array_equal(ol.extract(geo, sts[i], batch[i].reshape(geo["T"], -1)),
+                              ol.expected_slots(geo, db, idxs[i]))
+    again = srv.answer_batch(qs)
+    assert np.array_equal(again, batch), "rerun not reproducible"
+    for k in (3, 2):  # partial batches
+        part = srv.answer_batch(qs[:k])
+        assert np.array_equal(part, batch[:k])
+
+
+def test_batch_validation_codes(medium8):
+    geo, db, srv = medium8
+    lib = srv.lib
+    q, _ = ol.make_query(geo, SEED, 1, 1)
+    views = srv.views([q] * 11)
+    out = np.zeros((11, geo["T"] * 4 * N), np.uint64)
+    assert lib.ipir_answer_batch(srv.h, views, 11, ia.ptr(out)) == 2   # over max_batch
+    assert lib.ipir_answer_batch(srv.h, views, 0, ia.ptr(out)) == 0    # empty batch
+    assert lib.ipir_answer_batch(None, views, 1, ia.ptr(out)) == 1
+    bad = srv.views([q])
+    bad[0].rgsw = None
+    assert lib.ipir_answer_batch(srv.h, bad, 1, ia.ptr(out)) == 3
+    assert not out.any(), "nothing written on validation failure"
+
+
+def test_capi_round_trip_with_product_client(medium8):
+    geo, db, srv = medium8
+    p = srv.params
+    assert p.seed == bytes(SEED), "pinned CRS seed honoured"
+    assert (p.ring_n, p.d_eff, p.db_rows, p.db_cols, p.n_packed) == (N, 2, geo["R"], geo["C"], geo["G"])
+    caps = srv.caps()
+    assert caps.max_batch == 10 and caps.num_cts == geo["T"] and caps.resident_bytes > 0
+    assert caps.device_free_bytes > 0
+    cl = ia.Client(p, ia.load("server"))
+    hs, qs, idxs = [], [], [5, 77777, 524000]
+    for idx in idxs:
+        h, q = cl.build(idx)  # fresh secret (random_device)
+        hs.append(h)
+        qs.append(q)
+    resp = srv.answer_batch(qs)
+    for h, r, idx in zip(hs, resp, idxs):
+        assert np.array_equal(cl.extract(h, r), ol.expected_slots(geo, db, idx))
+        cl.destroy(h)
+
+
+def test_device_resident_entry_matches_host_path(medium8):
+    geo, db, srv = medium8
+    qs = [ol.make_query(geo, SEED, idx, 40 + idx)[0] for idx in (3, 300000, 524287)]
+    host = srv.answer_batch(qs)
+    flat = np.ascontiguousarray(np.stack(qs))
+    dq = ia.DeviceBuffer(flat.nbytes)
+    dout = ia.DeviceBuffer(host.nbytes)
+    dq.upload(flat)
+    assert srv.lib.ipir_answer_batch_device(srv.h, C.c_void_p(dq.p), 3, C.c_void_p(dout.p), None) == 0
+    assert srv.lib.ipir_dev_sync() == 0
+    got = dout.download(np.zeros_like(host))
+    assert np.array_equal(got, host)
+    dq.free()
+    dout.free()
+
+
+def test_serving_loop_results_equal_alone(medium8):
+    geo, db, srv = medium8
+    lib = srv.lib
+    idxs = list(range(0, 524288, 524288 // 13))[:13]
+    qs = [ol.make_query(geo, SEED, idx, 900 + k)[0] for k, idx in enumerate(idxs)]
+    alone = [srv.answer_batch([q])[0] for q in qs]
+    loop = lib.ipir_loop_start(srv.h, 2)
+    assert loop
+    outs = [np.zeros(geo["T"] * 4 * N, np.uint64) for _ in qs]
+    tickets = []
+    for q, o in zip(qs, outs):
+        v = srv.views([q])
+        tickets.append(lib.ipir_loop_submit(loop, v, ia.ptr(o)))
+        tickets[-1] = (tickets[-1], v, q)  # keep the views alive
+    lat = C.c_double()
+    for (t, _, _) in tickets:
+        assert t > 0
+        assert lib.ipir_loop_wait(loop, t, C.byref(lat)) == 0
+        assert lat.value > 0
+    lib.ipir_loop_stop(loop)
+    for i in range(len(qs)):
+        assert np.array_equal(outs[i], alone[i]), f"loop result {i} differs from alone"
