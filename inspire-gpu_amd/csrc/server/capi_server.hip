@@ -137,19 +137,8 @@ int ipir_dbg_answer_taps(ipir_server* srv, const uint64_t* query, uint32_t* inte
         std::vector<uint64_t> r(c->resp_u64s);
         c->answer_host(1, r.data());
         if (resp) std::memcpy(resp, r.data(), r.size() * 8);
-        if (interm) {
-            const size_t C = c->pp.db_cols, S = c->sh.slices;
-            std::vector<uint32_t> part(c->part_stride);
-            if (hipMemcpy(part.data(), c->d_part, part.size() * 4, hipMemcpyDeviceToHost) != hipSuccess) return 4;
-            for (int l = 0; l < 2; l++) {
-                const uint64_t q = l ? Q1 : Q0;
-                for (size_t j = 0; j < C; j++) {
-                    uint64_t a = 0;
-                    for (size_t s = 0; s < S; s++) a += part[(s * 2 + l) * C + j];
-                    interm[l * C + j] = (uint32_t)(a % q);
-                }
-            }
-        }
+        if (interm && hipMemcpy(interm, c->d_interm, 2 * c->pp.db_cols * 4, hipMemcpyDeviceToHost) != hipSuccess)
+            return 4;
         if (packed_b && hipMemcpy(packed_b, c->d_packed_b, c->packed_stride * 4, hipMemcpyDeviceToHost) != hipSuccess)
             return 4;
         return 0;
@@ -240,35 +229,21 @@ int ipir_dbg_profile_batch(ipir_server* srv, const uint64_t* d_queries, size_t c
                            double* stage_ms) {
     if (!srv || !d_queries || !d_out || !stage_ms || count == 0 || count > srv->ctx->cfg.max_batch) return 1;
     GpuServerCtx* c = srv->ctx;
-    const size_t R = c->pp.db_rows, C = c->pp.db_cols;
     hipEvent_t ev[6];
     for (auto& e : ev) (void)hipEventCreate(&e);
     hipStream_t s = c->stream;
     int rc = 0;
     try {
         (void)hipEventRecord(ev[0], s);
-        hip_check(ipx::launch_unmarshal(d_queries, count, R, c->d_bq, c->d_keys, c->d_rgsw, s), "unmarshal");
-        hip_check(ipx::launch_ntt_rns(c->d_keys, count * 2 * D_EFF, c->nc, false, s), "key ntt");
+        c->stage_prep(d_queries, count, s, 0);
         (void)hipEventRecord(ev[1], s);
-        for (size_t q = 0; q < count; q++)
-            hip_check(ipx::launch_dbpass(c->d_db, c->d_bq + q * 2 * R, c->d_part + q * c->part_stride, R, C, c->sh, s),
-                      "dbpass");
+        c->stage_dbpass(count, s, 0);
         (void)hipEventRecord(ev[2], s);
-        for (size_t q0 = 0; q0 < count; q0 += 4) {
-            const size_t nq = count - q0 < 4 ? count - q0 : 4;
-            hip_check(ipx::launch_collapse_partial(c->d_precomp, c->G, c->d_keys + q0 * c->keys_stride, c->keys_stride,
-                                                   nq, c->d_part2 + q0 * c->part2_stride, c->nch, s),
-                      "collapse");
-        }
+        c->stage_collapse(count, s, 0);
         (void)hipEventRecord(ev[3], s);
-        hip_check(ipx::launch_pack_finalize(c->d_part, c->part_stride, c->sh.slices, C, c->d_part2, c->part2_stride,
-                                            c->nch, c->d_precomp, c->G, c->d_keys, c->keys_stride, c->d_packed_b,
-                                            c->packed_stride, count, c->nc, s),
-                  "finalize");
+        c->stage_finalize(count, s, 0);
         (void)hipEventRecord(ev[4], s);
-        hip_check(ipx::launch_horner(c->d_precomp + ipx::PC_A, ipx::PC_GROUP, c->G, c->Da, c->T, c->d_packed_b,
-                                     c->packed_stride, c->d_rgsw, c->rgsw_stride, d_out, count, c->nc, c->cc, s),
-                  "horner");
+        c->stage_horner(count, d_out, s, 0);
         (void)hipEventRecord(ev[5], s);
         hip_check(hipEventSynchronize(ev[5]), "sync");
         for (int i = 0; i < 5; i++) {

@@ -165,38 +165,56 @@ __device__ __forceinline__ void inv_phase(uint32_t* poly, uint32_t t, const uint
     for (int k = 0; k < 8; k++) poly[ls(base + S * k)] = x[k];
 }
 
+// Twiddle tables of one limb: forward, its Shoup companions, inverse, its
+// Shoup companions (N words each). Either the device tables of NttConst or
+// a copy staged in LDS by the kernel (stage_twiddles), which keeps the
+// per-butterfly address arithmetic out of the register file.
+struct Tw {
+    const uint32_t* w;
+    const uint32_t* wp;
+    const uint32_t* iw;
+    const uint32_t* iwp;
+};
+__device__ __forceinline__ Tw tw_global(const NttConst& nc, int limb) {
+    const uint32_t* b = nc.tw + (size_t)limb * 4 * N;
+    return Tw{b, b + N, b + 2 * N, b + 3 * N};
+}
+// Copy both limbs' tables (8*N words) into LDS; caller synchronises.
+__device__ __forceinline__ void stage_twiddles(uint32_t* lds_tw, const NttConst& nc, uint32_t tid, uint32_t nthreads) {
+    for (uint32_t x = tid; x < 8 * N; x += nthreads) lds_tw[x] = nc.tw[x];
+}
+__device__ __forceinline__ Tw tw_lds(const uint32_t* lds_tw, int limb) {
+    const uint32_t* b = lds_tw + (size_t)limb * 4 * N;
+    return Tw{b, b + N, b + 2 * N, b + 3 * N};
+}
+
 // Forward NTT of one LDS polynomial by a 256-lane team (t = lane in team).
 // Every lane of the whole workgroup must call this (it contains workgroup
 // barriers); lanes of a team with nothing to transform pass active=false
 // and only take part in the barriers. The caller synchronises before
 // (input visible); the routine ends with a barrier (output visible).
-__device__ __forceinline__ void ntt_fwd_lds(uint32_t* poly, uint32_t t, const NttConst& nc, int limb,
-                                            bool active = true) {
-    const uint32_t* w = nc.tw + (size_t)limb * 4 * N;
-    const uint32_t* wp = w + N;
+__device__ __forceinline__ void ntt_fwd_lds(uint32_t* poly, uint32_t t, const Tw& tw, int limb, bool active = true) {
     const uint32_t q = qof(limb);
-    if (active) fwd_phase<256, 3>(poly, t, w, wp, q);
+    if (active) fwd_phase<256, 3>(poly, t, tw.w, tw.wp, q);
     __syncthreads();
-    if (active) fwd_phase<32, 3>(poly, t, w, wp, q);
+    if (active) fwd_phase<32, 3>(poly, t, tw.w, tw.wp, q);
     __syncthreads();
-    if (active) fwd_phase<4, 3>(poly, t, w, wp, q);
+    if (active) fwd_phase<4, 3>(poly, t, tw.w, tw.wp, q);
     __syncthreads();
-    if (active) fwd_phase<1, 2>(poly, t, w, wp, q);
+    if (active) fwd_phase<1, 2>(poly, t, tw.w, tw.wp, q);
     __syncthreads();
 }
 
-__device__ __forceinline__ void ntt_inv_lds(uint32_t* poly, uint32_t t, const NttConst& nc, int limb,
-                                            bool active = true) {
-    const uint32_t* w = nc.tw + (size_t)limb * 4 * N + 2 * N;
-    const uint32_t* wp = w + N;
+__device__ __forceinline__ void ntt_inv_lds(uint32_t* poly, uint32_t t, const Tw& tw, int limb, uint32_t inv_n,
+                                            uint32_t inv_n_p, bool active = true) {
     const uint32_t q = qof(limb);
-    if (active) inv_phase<1, 2>(poly, t, w, wp, q, false, 0, 0);
+    if (active) inv_phase<1, 2>(poly, t, tw.iw, tw.iwp, q, false, 0, 0);
     __syncthreads();
-    if (active) inv_phase<4, 3>(poly, t, w, wp, q, false, 0, 0);
+    if (active) inv_phase<4, 3>(poly, t, tw.iw, tw.iwp, q, false, 0, 0);
     __syncthreads();
-    if (active) inv_phase<32, 3>(poly, t, w, wp, q, false, 0, 0);
+    if (active) inv_phase<32, 3>(poly, t, tw.iw, tw.iwp, q, false, 0, 0);
     __syncthreads();
-    if (active) inv_phase<256, 3>(poly, t, w, wp, q, true, nc.inv_n[limb], nc.inv_n_p[limb]);
+    if (active) inv_phase<256, 3>(poly, t, tw.iw, tw.iwp, q, true, inv_n, inv_n_p);
     __syncthreads();
 }
 

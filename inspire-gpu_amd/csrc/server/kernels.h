@@ -30,13 +30,17 @@ hipError_t launch_ntt_rns(uint32_t* polys, size_t npoly, const NttConst& nc, boo
 // Plain database pass for one query: part[slice][limb][C] = sum over the
 // slice's rows of db[r][c]*b_limb[r] mod q_limb.
 struct DbPassShape {
-    uint32_t rows_per_slice;  // multiple of 4, divides R
+    uint32_t rows_per_slice;  // multiple of unroll, divides R
     uint32_t slices;          // R / rows_per_slice
-    uint32_t cols_per_thread; // 4 or 8
+    uint32_t cols_per_thread; // 4 or 8 (8- or 16-byte loads)
+    uint32_t unroll;          // rows loaded per iteration: 4 or 8
 };
-DbPassShape dbpass_shape(size_t R, size_t C, int target_blocks);
+DbPassShape dbpass_shape(size_t R, size_t C, int target_blocks, int cpt, int unroll);
 hipError_t launch_dbpass(const uint16_t* db, const double* bq, uint32_t* part, size_t R, size_t C,
                          const DbPassShape& sh, hipStream_t s);
+// interm[q][limb][C] = sum over the S row slices of part (per query stride part_stride).
+hipError_t launch_reduce_slices(const uint32_t* part, size_t part_stride, uint32_t S, size_t C, uint32_t* interm,
+                                size_t nq, hipStream_t s);
 
 // Number of step chunks the collapse pass splits the 1023 steps into.
 uint32_t collapse_nch(size_t G);
@@ -45,12 +49,11 @@ uint32_t collapse_nch(size_t G);
 hipError_t launch_collapse_partial(const uint32_t* precomp, size_t G, const uint32_t* keys,
                                    size_t keys_stride, size_t nq, uint32_t* part2, uint32_t nch,
                                    hipStream_t s);
-// packed_b[q][g] = NTT(sum of DB-pass partials) - collapse partials - Df*Kn.
-hipError_t launch_pack_finalize(const uint32_t* part, size_t part_stride, uint32_t slices, size_t C,
-                                const uint32_t* part2, size_t part2_stride, uint32_t nch,
-                                const uint32_t* precomp, size_t G, const uint32_t* keys,
-                                size_t keys_stride, uint32_t* packed_b, size_t packed_stride,
-                                size_t nq, const NttConst& nc, hipStream_t s);
+// packed_b[q][g] = NTT(interm[q] group slice) - collapse partials - Df*Kn.
+hipError_t launch_pack_finalize(const uint32_t* interm, size_t C, const uint32_t* part2, size_t part2_stride,
+                                uint32_t nch, const uint32_t* precomp, size_t G, const uint32_t* keys,
+                                size_t keys_stride, uint32_t* packed_b, size_t packed_stride, size_t nq,
+                                const NttConst& nc, hipStream_t s);
 // Horner over each response ciphertext; writes the flat u64 response
 // [q][ct][a|b][limb][N]. Group g's packed a-part (shared by all queries) is
 // at pa_base + g*pa_stride; its b-part at packed_b + q*packed_stride + g*2N.

@@ -49,8 +49,8 @@ __global__ __launch_bounds__(256) void k_ntt_rns(uint32_t* __restrict__ polys, N
 #pragma unroll
     for (int k = 0; k < 8; k++) buf[ls(t + 256 * k)] = p[t + 256 * k];
     __syncthreads();
-    if (inverse) ntt_inv_lds(buf, t, nc, limb);
-    else ntt_fwd_lds(buf, t, nc, limb);
+    if (inverse) ntt_inv_lds(buf, t, tw_global(nc, limb), limb, nc.inv_n[limb], nc.inv_n_p[limb]);
+    else ntt_fwd_lds(buf, t, tw_global(nc, limb), limb);
 #pragma unroll
     for (int k = 0; k < 8; k++) p[t + 256 * k] = buf[ls(t + 256 * k)];
 }
@@ -74,7 +74,7 @@ __device__ __forceinline__ uint32_t fmod_exact(double a, double q, double qinv) 
     return (uint32_t)r;
 }
 
-template <int CPT>
+template <int CPT, int U>
 __global__ __launch_bounds__(256) void k_dbpass(const uint16_t* __restrict__ db, const double* __restrict__ bq,
                                                 uint32_t* __restrict__ part, uint32_t R, uint32_t C,
                                                 uint32_t rps) {
@@ -94,12 +94,12 @@ __global__ __launch_bounds__(256) void k_dbpass(const uint16_t* __restrict__ db,
         double a0[CPT], a1[CPT];
 #pragma unroll
         for (int e = 0; e < CPT; e++) a0[e] = a1[e] = 0.0;
-        for (uint32_t r = 0; r < nb; r += 4) {
-            VT v[4];
+        for (uint32_t r = 0; r < nb; r += U) {
+            VT v[U];
 #pragma unroll
-            for (int u = 0; u < 4; u++) v[u] = __builtin_nontemporal_load(p + (size_t)(blk + r + u) * rs);
+            for (int u = 0; u < U; u++) v[u] = __builtin_nontemporal_load(p + (size_t)(blk + r + u) * rs);
 #pragma unroll
-            for (int u = 0; u < 4; u++) {
+            for (int u = 0; u < U; u++) {
                 const double x0 = b0[blk + r + u], x1 = b1[blk + r + u];
 #pragma unroll
                 for (int e = 0; e < CPT; e++) {
@@ -120,6 +120,23 @@ __global__ __launch_bounds__(256) void k_dbpass(const uint16_t* __restrict__ db,
     uint32_t* o1 = o0 + C;
 #pragma unroll
     for (int e = 0; e < CPT; e++) { o0[e] = m0[e]; o1[e] = m1[e]; }
+}
+
+// interm[q][limb][col] = sum over row slices of part mod q_limb.
+__global__ __launch_bounds__(256) void k_reduce_slices(const uint32_t* __restrict__ part, size_t part_stride,
+                                                       uint32_t S, uint32_t C, uint32_t* __restrict__ interm) {
+    const uint32_t x = (blockIdx.x * 256 + threadIdx.x) * 4;  // over [2][C]
+    const uint32_t q = blockIdx.y;
+    const int limb = x >= C;
+    const u32x4* pp = reinterpret_cast<const u32x4*>(part + q * part_stride + x);
+    uint64_t a[4] = {0, 0, 0, 0};
+    for (uint32_t s = 0; s < S; s++) {
+        const u32x4 v = __builtin_nontemporal_load(pp + (size_t)s * (2 * C / 4));
+        a[0] += v[0]; a[1] += v[1]; a[2] += v[2]; a[3] += v[3];
+    }
+    u32x4 o;
+    o[0] = red64(a[0], limb); o[1] = red64(a[1], limb); o[2] = red64(a[2], limb); o[3] = red64(a[3], limb);
+    *reinterpret_cast<u32x4*>(interm + (size_t)q * 2 * C + x) = o;
 }
 
 // ------------------------------------------------------------ collapse pass
@@ -170,8 +187,8 @@ __global__ __launch_bounds__(256) void k_collapse_partial(const uint32_t* __rest
 }
 
 // ------------------------------------------------------------ pack finalize
-__global__ __launch_bounds__(256) void k_pack_finalize(const uint32_t* __restrict__ part, size_t part_stride,
-                                                       uint32_t S, uint32_t C, const uint32_t* __restrict__ part2,
+__global__ __launch_bounds__(256) void k_pack_finalize(const uint32_t* __restrict__ interm, uint32_t C,
+                                                       const uint32_t* __restrict__ part2,
                                                        size_t p2_stride, uint32_t nch,
                                                        const uint32_t* __restrict__ precomp, uint32_t G,
                                                        const uint32_t* __restrict__ keys, size_t kstride,
@@ -180,16 +197,11 @@ __global__ __launch_bounds__(256) void k_pack_finalize(const uint32_t* __restric
     __shared__ uint32_t buf[POLY_LDS];
     const uint32_t g = blockIdx.x, c = blockIdx.y, qi = blockIdx.z, t = threadIdx.x;
     const uint32_t q = qof((int)c);
-    const uint32_t* pp = part + qi * part_stride + (size_t)c * C + (size_t)g * N;
+    const uint32_t* pp = interm + (size_t)qi * 2 * C + (size_t)c * C + (size_t)g * N;
 #pragma unroll
-    for (int k = 0; k < 8; k++) {
-        const uint32_t i = t + 256 * k;
-        uint64_t a = 0;
-        for (uint32_t sl = 0; sl < S; sl++) a += pp[(size_t)sl * 2 * C + i];
-        buf[ls(i)] = red64(a, (int)c);
-    }
+    for (int k = 0; k < 8; k++) buf[ls(t + 256 * k)] = pp[t + 256 * k];
     __syncthreads();
-    ntt_fwd_lds(buf, t, nc, (int)c);
+    ntt_fwd_lds(buf, t, tw_global(nc, (int)c), (int)c);
     const uint32_t* Df = precomp + (size_t)g * PC_GROUP + PC_DF + (size_t)c * D_EFF * N;
     const uint32_t* Kn = keys + qi * kstride + BLK + (size_t)c * N;  // key 1 = ksk_neg1, [digit][limb][N]
     const uint32_t* p2 = part2 + qi * p2_stride + (size_t)g * 2 * N + (size_t)c * N;
@@ -213,8 +225,13 @@ __global__ __launch_bounds__(1024) void k_horner(const uint32_t* __restrict__ pa
                                                  size_t rstride, uint64_t* __restrict__ out, NttConst nc,
                                                  CrtConst cc) {
     __shared__ uint32_t buf[4][POLY_LDS];
+    __shared__ uint32_t twl[8 * N];
     const uint32_t ct = blockIdx.x, qi = blockIdx.y, tid = threadIdx.x;
     const uint32_t team = tid >> 8, t = tid & 255;
+    stage_twiddles(twl, nc, tid, 1024);
+    const int tlimb = (int)(team & 1);
+    const Tw tw = tw_lds(twl, tlimb);
+    const uint32_t inv_n = nc.inv_n[tlimb], inv_n_p = nc.inv_n_p[tlimb];
     const uint32_t start = ct * Da;
     const uint32_t L = (G - start) < Da ? (G - start) : Da;
     const uint32_t* pb = packed_b + qi * pstride;
@@ -236,7 +253,7 @@ __global__ __launch_bounds__(1024) void k_horner(const uint32_t* __restrict__ pa
 #pragma unroll
             for (int e = 0; e < 2; e++) buf[p][ls(tid + 1024 * e)] = acc[p][e];
         __syncthreads();
-        ntt_inv_lds(buf[team], t, nc, (int)(team & 1));
+        ntt_inv_lds(buf[team], t, tw, tlimb, inv_n, inv_n_p);
         uint32_t da0[2][2], da1[2][2], db0[2][2], db1[2][2];  // [e][digit], residue mod Q0 / Q1
 #pragma unroll
         for (int e = 0; e < 2; e++) {
@@ -258,7 +275,7 @@ __global__ __launch_bounds__(1024) void k_horner(const uint32_t* __restrict__ pa
                 }
             }
             __syncthreads();
-            ntt_fwd_lds(buf[team], t, nc, (int)(team & 1));
+            ntt_fwd_lds(buf[team], t, tw, tlimb);
             // RGSW parts: 0 top_a, 1 top_b, 2 bot_a, 3 bot_b; [part][digit][limb][N]
             const uint32_t* pA = rg + (size_t)(round ? 2 : 0) * BLK;
             const uint32_t* pB = rg + (size_t)(round ? 3 : 1) * BLK;
@@ -314,12 +331,15 @@ hipError_t launch_ntt_rns(uint32_t* polys, size_t npoly, const NttConst& nc, boo
     return hipGetLastError();
 }
 
-DbPassShape dbpass_shape(size_t R, size_t C, int target_blocks) {
+DbPassShape dbpass_shape(size_t R, size_t C, int target_blocks, int cpt, int unroll) {
     DbPassShape sh;
-    sh.cols_per_thread = (C >= 8 * 2048) ? 8 : 4;
+    sh.cols_per_thread = (uint32_t)cpt;
+    sh.unroll = (uint32_t)unroll;
+    if (C % (256 * (size_t)cpt)) sh.cols_per_thread = 4;  // C is a multiple of 2048 = 256*8 anyway
     const size_t col_blocks = C / (256 * sh.cols_per_thread);
     size_t slices = 1;
-    while (col_blocks * slices < (size_t)target_blocks && R / (slices * 2) >= 64 && (R / (slices * 2)) % 4 == 0)
+    while (col_blocks * slices < (size_t)target_blocks && R / (slices * 2) >= 64 &&
+           (R / (slices * 2)) % sh.unroll == 0)
         slices *= 2;
     sh.slices = (uint32_t)slices;
     sh.rows_per_slice = (uint32_t)(R / slices);
@@ -329,12 +349,23 @@ DbPassShape dbpass_shape(size_t R, size_t C, int target_blocks) {
 hipError_t launch_dbpass(const uint16_t* db, const double* bq, uint32_t* part, size_t R, size_t C,
                          const DbPassShape& sh, hipStream_t s) {
     const unsigned cb = (unsigned)(C / (256 * sh.cols_per_thread));
-    if (sh.cols_per_thread == 8)
-        hipLaunchKernelGGL(k_dbpass<8>, dim3(cb, sh.slices), dim3(256), 0, s, db, bq, part, (uint32_t)R,
-                           (uint32_t)C, sh.rows_per_slice);
-    else
-        hipLaunchKernelGGL(k_dbpass<4>, dim3(cb, sh.slices), dim3(256), 0, s, db, bq, part, (uint32_t)R,
-                           (uint32_t)C, sh.rows_per_slice);
+    const dim3 grid(cb, sh.slices);
+#define DBP(CPT, U)                                                                                         \
+    hipLaunchKernelGGL((k_dbpass<CPT, U>), grid, dim3(256), 0, s, db, bq, part, (uint32_t)R, (uint32_t)C, \
+                       sh.rows_per_slice)
+    if (sh.cols_per_thread == 8) {
+        if (sh.unroll == 8) DBP(8, 8); else DBP(8, 4);
+    } else {
+        if (sh.unroll == 8) DBP(4, 8); else DBP(4, 4);
+    }
+#undef DBP
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce_slices(const uint32_t* part, size_t part_stride, uint32_t S, size_t C, uint32_t* interm,
+                                size_t nq, hipStream_t s) {
+    hipLaunchKernelGGL(k_reduce_slices, dim3((unsigned)(2 * C / 1024), (unsigned)nq), dim3(256), 0, s, part,
+                       part_stride, S, (uint32_t)C, interm);
     return hipGetLastError();
 }
 
@@ -362,13 +393,12 @@ hipError_t launch_collapse_partial(const uint32_t* precomp, size_t G, const uint
 
 uint32_t collapse_nch(size_t G) { return collapse_chunks(G); }
 
-hipError_t launch_pack_finalize(const uint32_t* part, size_t part_stride, uint32_t slices, size_t C,
-                                const uint32_t* part2, size_t part2_stride, uint32_t nch, const uint32_t* precomp,
-                                size_t G, const uint32_t* keys, size_t keys_stride, uint32_t* packed_b,
-                                size_t packed_stride, size_t nq, const NttConst& nc, hipStream_t s) {
-    hipLaunchKernelGGL(k_pack_finalize, dim3((unsigned)G, 2, (unsigned)nq), dim3(256), 0, s, part, part_stride,
-                       slices, (uint32_t)C, part2, part2_stride, nch, precomp, (uint32_t)G, keys, keys_stride,
-                       packed_b, packed_stride, nc);
+hipError_t launch_pack_finalize(const uint32_t* interm, size_t C, const uint32_t* part2, size_t part2_stride,
+                                uint32_t nch, const uint32_t* precomp, size_t G, const uint32_t* keys,
+                                size_t keys_stride, uint32_t* packed_b, size_t packed_stride, size_t nq,
+                                const NttConst& nc, hipStream_t s) {
+    hipLaunchKernelGGL(k_pack_finalize, dim3((unsigned)G, 2, (unsigned)nq), dim3(256), 0, s, interm, (uint32_t)C,
+                       part2, part2_stride, nch, precomp, (uint32_t)G, keys, keys_stride, packed_b, packed_stride, nc);
     return hipGetLastError();
 }
 

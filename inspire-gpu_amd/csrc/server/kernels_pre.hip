@@ -89,43 +89,44 @@ __global__ __launch_bounds__(1024) void k_encode(uint16_t* __restrict__ db, uint
 }
 
 // ------------------------------------------------------------ a-side
-// Block (group, 8 columns): acc[col][c] = sum_r NTT(db[rN..rN+N-1][col]) *
+// Block (group, 4 columns): acc[col][c] = sum_r NTT(db[rN..rN+N-1][col]) *
 // crs[r][c]; a32T[g][c][k][col] = N^-1 * acc[col][c][N-1-k].
+// Polynomial p = col*2 + c; team tm transforms p = tm and p = tm + 4.
 __global__ __launch_bounds__(1024) void k_aside(const uint16_t* __restrict__ db, uint32_t C, uint32_t nr, uint32_t g0,
                                                 const uint32_t* __restrict__ crs, uint32_t* __restrict__ a32T,
                                                 NttConst nc) {
-    __shared__ uint32_t buf[16][POLY_LDS];
-    const uint32_t gl = blockIdx.x / (N / 8), j0 = (blockIdx.x % (N / 8)) * 8;
+    __shared__ uint32_t buf[8][POLY_LDS];
+    __shared__ uint32_t twl[8 * N];
+    const uint32_t gl = blockIdx.x / (N / 4), j0 = (blockIdx.x % (N / 4)) * 4;
     const uint32_t tid = threadIdx.x, team = tid >> 8, t = tid & 255;
+    stage_twiddles(twl, nc, tid, 1024);
+    const int c = (int)(team & 1);  // p = team + 4*rr has limb team&1
+    const Tw tw = tw_lds(twl, c);
     const size_t col0 = (size_t)(g0 + gl) * N + j0;
-    uint32_t acc[4][8];
+    uint32_t acc[2][8];
 #pragma unroll
-    for (int rr = 0; rr < 4; rr++)
+    for (int rr = 0; rr < 2; rr++)
 #pragma unroll
         for (int k = 0; k < 8; k++) acc[rr][k] = 0;
     for (uint32_t r = 0; r < nr; r++) {
 #pragma unroll
         for (int h = 0; h < 2; h++) {
             const uint32_t i = tid + 1024 * h;
-            const uint4 v = *reinterpret_cast<const uint4*>(db + ((size_t)r * N + i) * C + col0);
-            const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+            const uint2 v = *reinterpret_cast<const uint2*>(db + ((size_t)r * N + i) * C + col0);
+            const uint32_t w[2] = {v.x, v.y};
 #pragma unroll
-            for (int col = 0; col < 8; col++) {
+            for (int col = 0; col < 4; col++) {
                 const uint32_t x = (col & 1) ? (w[col >> 1] >> 16) : (w[col >> 1] & 0xffffu);
                 buf[col * 2][ls(i)] = x;
                 buf[col * 2 + 1][ls(i)] = x;
             }
         }
         __syncthreads();
+        ntt_fwd_lds(buf[team], t, tw, c);
+        ntt_fwd_lds(buf[team + 4], t, tw, c);
 #pragma unroll
-        for (int rr = 0; rr < 4; rr++) {
+        for (int rr = 0; rr < 2; rr++) {
             const uint32_t p = team + 4 * rr;
-            ntt_fwd_lds(buf[p], t, nc, (int)(p & 1));
-        }
-#pragma unroll
-        for (int rr = 0; rr < 4; rr++) {
-            const uint32_t p = team + 4 * rr;
-            const int c = (int)(p & 1);
             const uint32_t* A = crs + ((size_t)r * 2 + c) * N;
 #pragma unroll
             for (int k = 0; k < 8; k++) {
@@ -136,9 +137,8 @@ __global__ __launch_bounds__(1024) void k_aside(const uint16_t* __restrict__ db,
         __syncthreads();
     }
 #pragma unroll
-    for (int rr = 0; rr < 4; rr++) {
+    for (int rr = 0; rr < 2; rr++) {
         const uint32_t p = team + 4 * rr;
-        const int c = (int)(p & 1);
 #pragma unroll
         for (int k = 0; k < 8; k++) {
             const uint32_t e = t + 256 * k;
@@ -147,13 +147,13 @@ __global__ __launch_bounds__(1024) void k_aside(const uint16_t* __restrict__ db,
     }
     __syncthreads();
     for (uint32_t x = tid; x < 2 * N; x += 1024) {
-        const uint32_t c = x / N, k = x % N, e = N - 1 - k;
-        uint32_t o[8];
-#pragma unroll
-        for (int col = 0; col < 8; col++) o[col] = buf[col * 2 + c][ls(e)];
-        uint4* dst = reinterpret_cast<uint4*>(a32T + (((size_t)gl * 2 + c) * N + k) * N + j0);
-        dst[0] = make_uint4(o[0], o[1], o[2], o[3]);
-        dst[1] = make_uint4(o[4], o[5], o[6], o[7]);
+        const uint32_t cc = x / N, k = x % N, e = N - 1 - k;
+        uint4 o;
+        o.x = buf[0 * 2 + cc][ls(e)];
+        o.y = buf[1 * 2 + cc][ls(e)];
+        o.z = buf[2 * 2 + cc][ls(e)];
+        o.w = buf[3 * 2 + cc][ls(e)];
+        *reinterpret_cast<uint4*>(a32T + (((size_t)gl * 2 + cc) * N + k) * N + j0) = o;
     }
 }
 
@@ -167,7 +167,7 @@ __global__ __launch_bounds__(256) void k_ntt_rows(uint32_t* __restrict__ rows, N
 #pragma unroll
     for (int k = 0; k < 8; k++) buf[ls(t + 256 * k)] = p[t + 256 * k];
     __syncthreads();
-    ntt_fwd_lds(buf, t, nc, limb);
+    ntt_fwd_lds(buf, t, tw_global(nc, limb), limb);
 #pragma unroll
     for (int k = 0; k < 8; k++) p[t + 256 * k] = buf[ls(t + 256 * k)];
 }
@@ -184,8 +184,13 @@ __global__ __launch_bounds__(1024) void k_collapse_chain(const uint32_t* __restr
                                                          uint32_t* __restrict__ states, NttConst nc, CrtConst cc) {
     __shared__ uint32_t buf[4][POLY_LDS];
     __shared__ uint32_t K[4][N];  // [digit*2 + limb][N]
+    __shared__ uint32_t twl[8 * N];
     const uint32_t gl = blockIdx.x, dir = blockIdx.y, tid = threadIdx.x, team = tid >> 8, t = tid & 255;
     for (uint32_t x = tid; x < 4 * N; x += 1024) K[x / N][x % N] = ksk5a[x];
+    stage_twiddles(twl, nc, tid, 1024);
+    const int tlimb = (int)(team & 1);
+    const Tw tw = tw_lds(twl, tlimb);
+    const uint32_t inv_n = nc.inv_n[tlimb], inv_n_p = nc.inv_n_p[tlimb];
     const uint32_t* Fg = F + (size_t)gl * 2 * N * N;
     uint32_t* Dout = precomp_g0 + (size_t)gl * PC_GROUP + (dir ? PC_DM : PC_DP);
     uint32_t v[2], st[2][2];  // [e], [limb][e]
@@ -220,7 +225,7 @@ __global__ __launch_bounds__(1024) void k_collapse_chain(const uint32_t* __restr
             buf[1][ls(tid + 1024 * e)] = st[1][e];
         }
         __syncthreads();
-        ntt_inv_lds(buf[team], t, nc, (int)(team & 1), team < 2);
+        ntt_inv_lds(buf[team], t, tw, tlimb, inv_n, inv_n_p, team < 2);
         uint32_t d0[2][2], d1[2][2];
 #pragma unroll
         for (int e = 0; e < 2; e++) gadget2(buf[0][ls(tid + 1024 * e)], buf[1][ls(tid + 1024 * e)], cc, d0[e], d1[e]);
@@ -233,7 +238,7 @@ __global__ __launch_bounds__(1024) void k_collapse_chain(const uint32_t* __restr
                 buf[j * 2 + 1][ls(tid + 1024 * e)] = d1[e][j];
             }
         __syncthreads();
-        ntt_fwd_lds(buf[team], t, nc, (int)(team & 1));
+        ntt_fwd_lds(buf[team], t, tw, tlimb);
 #pragma unroll
         for (int c = 0; c < 2; c++)
 #pragma unroll
@@ -265,7 +270,11 @@ __global__ __launch_bounds__(1024) void k_collapse_final(const uint32_t* __restr
                                                          uint32_t* __restrict__ precomp_g0, NttConst nc,
                                                          CrtConst cc) {
     __shared__ uint32_t buf[4][POLY_LDS];
+    __shared__ uint32_t twl[8 * N];
     const uint32_t gl = blockIdx.x, tid = threadIdx.x, team = tid >> 8, t = tid & 255;
+    stage_twiddles(twl, nc, tid, 1024);
+    const int tlimb = (int)(team & 1);
+    const Tw tw = tw_lds(twl, tlimb);
     const uint32_t* sf = states + (size_t)gl * 4 * N;  // dir 0
     const uint32_t* sc = sf + 2 * N;                   // dir 1
     uint32_t* pc = precomp_g0 + (size_t)gl * PC_GROUP;
@@ -275,7 +284,7 @@ __global__ __launch_bounds__(1024) void k_collapse_final(const uint32_t* __restr
         buf[1][ls(tid + 1024 * e)] = sc[N + tid + 1024 * e];
     }
     __syncthreads();
-    ntt_inv_lds(buf[team], t, nc, (int)(team & 1), team < 2);
+    ntt_inv_lds(buf[team], t, tw, tlimb, nc.inv_n[tlimb], nc.inv_n_p[tlimb], team < 2);
     uint32_t d0[2][2], d1[2][2];
 #pragma unroll
     for (int e = 0; e < 2; e++) gadget2(buf[0][ls(tid + 1024 * e)], buf[1][ls(tid + 1024 * e)], cc, d0[e], d1[e]);
@@ -288,7 +297,7 @@ __global__ __launch_bounds__(1024) void k_collapse_final(const uint32_t* __restr
             buf[j * 2 + 1][ls(tid + 1024 * e)] = d1[e][j];
         }
     __syncthreads();
-    ntt_fwd_lds(buf[team], t, nc, (int)(team & 1));
+    ntt_fwd_lds(buf[team], t, tw, tlimb);
 #pragma unroll
     for (int c = 0; c < 2; c++)
 #pragma unroll
@@ -326,7 +335,7 @@ hipError_t launch_encode(uint16_t* db, size_t R, size_t C, size_t Da, hipStream_
 
 hipError_t launch_aside(const uint16_t* db, size_t R, size_t C, size_t g0, size_t ng, const uint32_t* crs,
                         uint32_t* a32T, const NttConst& nc, hipStream_t s) {
-    hipLaunchKernelGGL(k_aside, dim3((unsigned)(ng * (N / 8))), dim3(1024), 0, s, db, (uint32_t)C,
+    hipLaunchKernelGGL(k_aside, dim3((unsigned)(ng * (N / 4))), dim3(1024), 0, s, db, (uint32_t)C,
                        (uint32_t)(R / N), (uint32_t)g0, crs, a32T, nc);
     return hipGetLastError();
 }

@@ -8,6 +8,8 @@
 // groups lives in one arena (one allocation, fixed group stride), the
 // answer pipeline is a fixed sequence of batched kernels on one stream (no
 // per-group launches), and errors surface as exceptions instead of exit().
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <stdexcept>
@@ -204,13 +206,25 @@ GpuServerCtx* gpu_setup_server(const PublicParams& pp, const PreprocessData& pre
     return ctx;
 }
 
+namespace {
+struct DbCand {
+    int cpt, unroll, target;
+};
+const DbCand kDbCands[] = {{8, 4, 2048}, {8, 8, 2048}, {4, 8, 2048}, {8, 4, 1024}, {4, 8, 4096}, {8, 8, 4096}};
+}  // namespace
+
 void GpuServerCtx::alloc_slots() {
     const size_t R = pp.db_rows, C = pp.db_cols, B = cfg.max_batch;
-    sh = dbpass_shape(R, C, 2048);
+    sh = dbpass_shape(R, C, 2048, 8, 4);
+    size_t max_slices = sh.slices;
+    for (const DbCand& c : kDbCands) {
+        const DbPassShape x = dbpass_shape(R, C, c.target, c.cpt, c.unroll);
+        if (x.slices > max_slices) max_slices = x.slices;
+    }
     nch = collapse_nch(G);
     lay.R = R; lay.C = C; lay.G = G;
     qlen = lay.query_u64s();
-    part_stride = (size_t)sh.slices * 2 * C;
+    part_stride = max_slices * 2 * C;
     part2_stride = (size_t)nch * G * 2 * N;
     keys_stride = lay.keys_words();
     rgsw_stride = lay.rgsw_words();
@@ -222,36 +236,89 @@ void GpuServerCtx::alloc_slots() {
     d_keys = dalloc<uint32_t>(B * keys_stride);
     d_rgsw = dalloc<uint32_t>(B * rgsw_stride);
     d_part = dalloc<uint32_t>(B * part_stride);
+    d_interm = dalloc<uint32_t>(B * 2 * C);
     d_part2 = dalloc<uint32_t>(B * part2_stride);
     d_packed_b = dalloc<uint32_t>(B * packed_stride);
+    HIPX(hipMemset(d_bq, 0, B * 2 * R * sizeof(double)));
     HIPX(hipHostMalloc((void**)&h_q, B * qlen * 8, hipHostMallocDefault));
     HIPX(hipHostMalloc((void**)&h_resp, B * resp_u64s * 8, hipHostMallocDefault));
     resident_bytes = R * C * 2 + G * PC_GROUP * 4 + 8 * N * 4 +
                      B * (qlen * 8 + resp_u64s * 8 + 2 * R * 8 +
-                          (keys_stride + rgsw_stride + part_stride + part2_stride + packed_stride) * 4);
+                          (keys_stride + rgsw_stride + part_stride + 2 * C + part2_stride + packed_stride) * 4);
+    autotune_dbpass();
+}
+
+void GpuServerCtx::autotune_dbpass() {
+    const char* force = std::getenv("INSPIRE_DBPASS");
+    const size_t R = pp.db_rows, C = pp.db_cols;
+    if (force) {
+        int a = 8, b = 4, t = 2048;
+        if (std::sscanf(force, "%d,%d,%d", &a, &b, &t) == 3 && (a == 4 || a == 8) && (b == 4 || b == 8) && t > 0)
+            sh = dbpass_shape(R, C, t, a, b);
+        return;
+    }
+    const char* at = std::getenv("INSPIRE_AUTOTUNE");
+    if (at && at[0] == '0') return;
+    hipEvent_t e0, e1;
+    HIPX(hipEventCreate(&e0));
+    HIPX(hipEventCreate(&e1));
+    float best = 1e30f;
+    DbPassShape best_sh = sh;
+    for (const DbCand& c : kDbCands) {
+        const DbPassShape x = dbpass_shape(R, C, c.target, c.cpt, c.unroll);
+        HIPX(launch_dbpass(d_db, d_bq, d_part, R, C, x, stream));  // warm
+        HIPX(hipEventRecord(e0, stream));
+        for (int it = 0; it < 3; it++) HIPX(launch_dbpass(d_db, d_bq, d_part, R, C, x, stream));
+        HIPX(hipEventRecord(e1, stream));
+        HIPX(hipEventSynchronize(e1));
+        float ms = 0;
+        HIPX(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) { best = ms; best_sh = x; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    sh = best_sh;
+}
+
+void GpuServerCtx::stage_prep(const uint64_t* d_in, size_t count, hipStream_t s, size_t slot0) {
+    HIPX(launch_unmarshal(d_in, count, pp.db_rows, d_bq + slot0 * 2 * pp.db_rows, d_keys + slot0 * keys_stride,
+                          d_rgsw + slot0 * rgsw_stride, s));
+    HIPX(launch_ntt_rns(d_keys + slot0 * keys_stride, count * 2 * D_EFF, nc, false, s));
+}
+
+void GpuServerCtx::stage_dbpass(size_t count, hipStream_t s, size_t slot0) {
+    const size_t R = pp.db_rows, C = pp.db_cols;
+    for (size_t q = slot0; q < slot0 + count; q++)
+        HIPX(launch_dbpass(d_db, d_bq + q * 2 * R, d_part + q * part_stride, R, C, sh, s));
+    HIPX(launch_reduce_slices(d_part + slot0 * part_stride, part_stride, sh.slices, C, d_interm + slot0 * 2 * C,
+                              count, s));
+}
+
+void GpuServerCtx::stage_collapse(size_t count, hipStream_t s, size_t slot0) {
+    for (size_t q0 = slot0; q0 < slot0 + count; q0 += 4) {
+        const size_t nq = slot0 + count - q0 < 4 ? slot0 + count - q0 : 4;
+        HIPX(launch_collapse_partial(d_precomp, G, d_keys + q0 * keys_stride, keys_stride, nq,
+                                     d_part2 + q0 * part2_stride, nch, s));
+    }
+}
+
+void GpuServerCtx::stage_finalize(size_t count, hipStream_t s, size_t slot0) {
+    HIPX(launch_pack_finalize(d_interm + slot0 * 2 * pp.db_cols, pp.db_cols, d_part2 + slot0 * part2_stride,
+                              part2_stride, nch, d_precomp, G, d_keys + slot0 * keys_stride, keys_stride,
+                              d_packed_b + slot0 * packed_stride, packed_stride, count, nc, s));
+}
+
+void GpuServerCtx::stage_horner(size_t count, uint64_t* d_out, hipStream_t s, size_t slot0) {
+    HIPX(launch_horner(d_precomp + PC_A, PC_GROUP, G, Da, T, d_packed_b + slot0 * packed_stride, packed_stride,
+                       d_rgsw + slot0 * rgsw_stride, rgsw_stride, d_out, count, nc, cc, s));
 }
 
 void GpuServerCtx::enqueue(const uint64_t* d_in, size_t count, uint64_t* d_out, hipStream_t s, size_t slot0) {
-    const size_t R = pp.db_rows, C = pp.db_cols;
-    double* bq = d_bq + slot0 * 2 * R;
-    uint32_t* keys = d_keys + slot0 * keys_stride;
-    uint32_t* rgsw = d_rgsw + slot0 * rgsw_stride;
-    uint32_t* part = d_part + slot0 * part_stride;
-    uint32_t* part2 = d_part2 + slot0 * part2_stride;
-    uint32_t* packed = d_packed_b + slot0 * packed_stride;
-    HIPX(launch_unmarshal(d_in, count, R, bq, keys, rgsw, s));
-    HIPX(launch_ntt_rns(keys, count * 2 * D_EFF, nc, false, s));
-    for (size_t q = 0; q < count; q++)
-        HIPX(launch_dbpass(d_db, bq + q * 2 * R, part + q * part_stride, R, C, sh, s));
-    for (size_t q0 = 0; q0 < count; q0 += 4) {
-        const size_t nq = count - q0 < 4 ? count - q0 : 4;
-        HIPX(launch_collapse_partial(d_precomp, G, keys + q0 * keys_stride, keys_stride, nq,
-                                     part2 + q0 * part2_stride, nch, s));
-    }
-    HIPX(launch_pack_finalize(part, part_stride, sh.slices, C, part2, part2_stride, nch, d_precomp, G, keys,
-                              keys_stride, packed, packed_stride, count, nc, s));
-    HIPX(launch_horner(d_precomp + PC_A, PC_GROUP, G, Da, T, packed, packed_stride, rgsw, rgsw_stride, d_out,
-                       count, nc, cc, s));
+    stage_prep(d_in, count, s, slot0);
+    stage_dbpass(count, s, slot0);
+    stage_collapse(count, s, slot0);
+    stage_finalize(count, s, slot0);
+    stage_horner(count, d_out, s, slot0);
 }
 
 void GpuServerCtx::answer_host(size_t count, uint64_t* out, hipStream_t s, size_t slot0) {
@@ -333,7 +400,7 @@ void gpu_free_server(GpuServerCtx* ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     dfree(ctx->d_tw); dfree(ctx->d_db); dfree(ctx->d_precomp);
     dfree(ctx->d_q); dfree(ctx->d_resp); dfree(ctx->d_bq); dfree(ctx->d_keys); dfree(ctx->d_rgsw);
-    dfree(ctx->d_part); dfree(ctx->d_part2); dfree(ctx->d_packed_b);
+    dfree(ctx->d_part); dfree(ctx->d_interm); dfree(ctx->d_part2); dfree(ctx->d_packed_b);
     if (ctx->h_q) (void)hipHostFree(ctx->h_q);
     if (ctx->h_resp) (void)hipHostFree(ctx->h_resp);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);

@@ -39,7 +39,8 @@ struct GpuServerCtx {
     double* d_bq = nullptr;
     uint32_t* d_keys = nullptr;
     uint32_t* d_rgsw = nullptr;
-    uint32_t* d_part = nullptr;
+    uint32_t* d_part = nullptr;     // DB-pass row-slice partials [B][S][2][C]
+    uint32_t* d_interm = nullptr;   // DB-pass result [B][2][C]
     uint32_t* d_part2 = nullptr;
     uint32_t* d_packed_b = nullptr;
     uint64_t* h_q = nullptr;        // pinned host staging
@@ -49,6 +50,14 @@ struct GpuServerCtx {
     std::mutex mu;  // serialises host-staged answers on this context
 
     void alloc_slots();
+    // Pick the fastest database-pass variant on this device and geometry.
+    void autotune_dbpass();
+    // Pipeline stages over slots [slot0, slot0+count) on stream s.
+    void stage_prep(const uint64_t* d_in, size_t count, hipStream_t s, size_t slot0);
+    void stage_dbpass(size_t count, hipStream_t s, size_t slot0);
+    void stage_collapse(size_t count, hipStream_t s, size_t slot0);
+    void stage_finalize(size_t count, hipStream_t s, size_t slot0);
+    void stage_horner(size_t count, uint64_t* d_out, hipStream_t s, size_t slot0);
     // Enqueue the answer of `count` flat queries at d_in (device) into d_out
     // (device) on stream s, using slots [slot0, slot0 + count).
     void enqueue(const uint64_t* d_in, size_t count, uint64_t* d_out, hipStream_t s, size_t slot0 = 0);
