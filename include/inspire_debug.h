@@ -19,8 +19,13 @@ extern "C" {
  * [D-: same][D_final: 2*2*N], each in [limb][..][coeff] order. */
 size_t ipir_dbg_precomp_group_words(void);
 int ipir_dbg_precomp_group(const ipir_server* srv, size_t group, uint32_t* out);
-/* rows [row0, row0+nrows) of the encoded row-major u16 database */
+/* rows [row0, row0+nrows) of the encoded u16 database, returned row-major */
 int ipir_dbg_db_rows(const ipir_server* srv, size_t row0, size_t nrows, uint16_t* out);
+/* Database pass alone over nq (<= max_batch) query vectors b [nq][2][db_rows]
+ * (residues mod q0 | q1, u64): interm [nq][2][db_cols]. path -1 = the tuned
+ * plans of the answer pipeline, 0 = the fp64 pass per query, 1 = the int8
+ * MFMA pass. */
+int ipir_dbg_db_pass(ipir_server* srv, const uint64_t* b, size_t nq, int path, uint32_t* interm);
 /* Answer one host flat query and return the intermediate tensors:
  * interm [2][db_cols] (DB pass, coefficient form), packed_b
  * [n_packed][2][N] (after the collapse), resp (flat response). Any output

@@ -123,8 +123,36 @@ int ipir_dbg_precomp_group(const ipir_server* srv, size_t group, uint32_t* out) 
 
 int ipir_dbg_db_rows(const ipir_server* srv, size_t row0, size_t nrows, uint16_t* out) {
     if (!srv || !out || row0 + nrows > srv->pp.db_rows) return 1;
+    if (nrows == 0) return 0;
     const size_t C = srv->pp.db_cols;
-    return hipMemcpy(out, srv->ctx->d_db + row0 * C, nrows * C * 2, hipMemcpyDeviceToHost) == hipSuccess ? 0 : 4;
+    // resident layout: 16-row blocks, (r, c) at ((r/16)*C + c)*16 + r%16
+    const size_t kb0 = row0 / 16, kb1 = (row0 + nrows + 15) / 16;
+    std::vector<uint16_t> blk((kb1 - kb0) * C * 16);
+    if (hipMemcpy(blk.data(), srv->ctx->d_db + kb0 * C * 16, blk.size() * 2, hipMemcpyDeviceToHost) != hipSuccess)
+        return 4;
+    for (size_t r = row0; r < row0 + nrows; r++)
+        for (size_t c = 0; c < C; c++) out[(r - row0) * C + c] = blk[((r / 16 - kb0) * C + c) * 16 + r % 16];
+    return 0;
+}
+
+int ipir_dbg_db_pass(ipir_server* srv, const uint64_t* b, size_t nq, int path, uint32_t* interm) {
+    if (!srv || !b || !interm || nq == 0 || nq > srv->ctx->cfg.max_batch || path < -1 || path > 1) return 1;
+    GpuServerCtx* c = srv->ctx;
+    const size_t R = c->pp.db_rows, C = c->pp.db_cols;
+    for (size_t i = 0; i < nq * 2 * R; i++)
+        if (b[i] >= ((i / R) % 2 ? ipx::Q1 : ipx::Q0)) return 1;
+    try {
+        std::lock_guard<std::mutex> lk(c->mu);
+        std::vector<double> bd(nq * 2 * R);
+        for (size_t i = 0; i < bd.size(); i++) bd[i] = (double)b[i];
+        hip_check(hipMemcpy(c->d_bq, bd.data(), bd.size() * 8, hipMemcpyHostToDevice), "dbg db pass upload");
+        c->stage_dbpass(nq, c->stream, 0, 0, path);
+        hip_check(hipStreamSynchronize(c->stream), "dbg db pass");
+        hip_check(hipMemcpy(interm, c->d_interm, nq * 2 * C * 4, hipMemcpyDeviceToHost), "dbg db pass download");
+        return 0;
+    } catch (...) {
+        return 4;
+    }
 }
 
 int ipir_dbg_answer_taps(ipir_server* srv, const uint64_t* query, uint32_t* interm, uint32_t* packed_b,

@@ -27,17 +27,31 @@ hipError_t launch_unmarshal(const uint64_t* d_q, size_t count, size_t R, double*
 // In-place forward NTT of `npoly` RNS polynomials laid out [npoly][2][N].
 hipError_t launch_ntt_rns(uint32_t* polys, size_t npoly, const NttConst& nc, bool inverse, hipStream_t s);
 
-// Plain database pass for one query: part[slice][limb][C] = sum over the
-// slice's rows of db[r][c]*b_limb[r] mod q_limb.
-struct DbPassShape {
-    uint32_t rows_per_slice;  // multiple of unroll, divides R
-    uint32_t slices;          // R / rows_per_slice
-    uint32_t cols_per_thread; // 4 or 8 (8- or 16-byte loads)
-    uint32_t unroll;          // rows loaded per iteration: 4 or 8
+// Database pass over the resident blocked DB (element (r, c) at
+// ((r/16)*C + c)*16 + r%16): out + q*out_qstride + slice*2C holds
+// [limb][C] = sum over the slice's rows of db[r][c]*b_{q,limb}[r] mod q_limb.
+struct DbPlan {
+    enum Kind { F64, MFMA_DIRECT, MFMA_LDS } kind;
+    uint32_t param;   // F64: 16-row blocks per iteration (2, 4, 8); MFMA: M tiles per wave
+    uint32_t slices;  // row slices (power of two)
 };
-DbPassShape dbpass_shape(size_t R, size_t C, int target_blocks, int cpt, int unroll);
-hipError_t launch_dbpass(const uint16_t* db, const double* bq, uint32_t* part, size_t R, size_t C,
-                         const DbPassShape& sh, hipStream_t s);
+// Row-major staging rows -> blocked DB, blocks [kb0, kb0 + nkb).
+hipError_t launch_to_blocked(const uint16_t* src, uint16_t* dst, size_t C, size_t kb0, size_t nkb, hipStream_t s);
+DbPlan dbplan_f64(size_t R, size_t C, int unroll, size_t target_blocks);
+// Queries one batched (int8 MFMA) database pass serves.
+constexpr size_t kMfmaMaxQueries = 16;
+// int8-MFMA plan for nq (1..kMfmaMaxQueries) queries.
+DbPlan dbplan_mfma(size_t R, size_t C, size_t nq, size_t target_blocks);
+// Digit rows the MFMA plan stages per 16-row block (int8 scratch: R*Mp bytes).
+size_t dbplan_mp(const DbPlan& p);
+// One query, fp64 FMA on the vector ALUs.
+hipError_t launch_dbpass_f64(const uint16_t* db, const double* bq, uint32_t* out, size_t R, size_t C,
+                             const DbPlan& p, hipStream_t s);
+// nq queries (bq [q][limb][R] doubles) in one pass of the DB through int8
+// MFMA; dig (R*Mp bytes) and S (2*nq words) are scratch.
+hipError_t launch_dbpass_mfma(const uint16_t* db, const double* bq, size_t nq, int8_t* dig, uint32_t* S,
+                              uint32_t* out, size_t out_qstride, size_t R, size_t C, const DbPlan& p,
+                              hipStream_t s);
 // interm[q][limb][C] = sum over the S row slices of part (per query stride part_stride).
 hipError_t launch_reduce_slices(const uint32_t* part, size_t part_stride, uint32_t S, size_t C, uint32_t* interm,
                                 size_t nq, hipStream_t s);

@@ -1,0 +1,424 @@
+// Database-pass kernels for gfx950 over the resident DB layout.
+//
+// Resident layout ("16-row blocks"): element (row r, col c) of the encoded
+// u16 database sits at ((r/16)*C + c)*16 + r%16. One 32-byte load gives a
+// lane 16 consecutive rows of one column: one K-slice of an int8 MFMA
+// operand, and for the plain pass a unit-stride walk down a column with
+// wave-uniform query words. 16 lanes on consecutive columns read 512
+// contiguous bytes.
+//
+// Every kernel computes the reference's mat-vec (src/gpu_online.cu:23-41)
+//   interm[q][c][col] = sum_r db[r][col] * b_{q,c}[r] mod q_c
+// exactly, over a slice of the rows; slices are summed mod q afterwards.
+#include "kernels.h"
+
+namespace ipx {
+
+namespace {
+
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef int32_t i32x4 __attribute__((ext_vector_type(4)));
+
+// a mod q for an integer-valued double 0 <= a < 2^53.
+__device__ __forceinline__ uint32_t fmod_exact(double a, double q, double qinv) {
+    const double t = floor(a * qinv);
+    double r = fma(-t, q, a);  // exact: the true value is an integer below 2q in magnitude
+    if (r < 0) r += q;
+    if (r >= q) r -= q;
+    return (uint32_t)r;
+}
+
+// ------------------------------------------------------------ relayout
+// Row-major rows [16*kb0, 16*(kb0 + gridDim.y)) of `src` (whose first row is
+// row 16*kb0) -> blocked `dst`.
+__global__ __launch_bounds__(256) void k_to_blocked(const uint16_t* __restrict__ src, uint16_t* __restrict__ dst,
+                                                    uint32_t C, uint32_t kb0) {
+    const uint32_t col = blockIdx.x * 256 + threadIdx.x;
+    const uint32_t kbl = blockIdx.y;
+    uint32_t w[8];
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+        const uint32_t lo = src[((size_t)kbl * 16 + 2 * e) * C + col];
+        const uint32_t hi = src[((size_t)kbl * 16 + 2 * e + 1) * C + col];
+        w[e] = lo | (hi << 16);
+    }
+    u32x4* d = reinterpret_cast<u32x4*>(dst + ((size_t)(kb0 + kbl) * C + col) * 16);
+    d[0] = u32x4{w[0], w[1], w[2], w[3]};
+    d[1] = u32x4{w[4], w[5], w[6], w[7]};
+}
+
+// ------------------------------------------------------------ plain pass (fp64)
+// One query. Thread = one column, walking its slice 16 rows (32 bytes) at a
+// time with wave-uniform query words. Products db*b < 2^43 accumulate exactly
+// in fp64 FMA for 1024 rows (< 2^53), in four interleaved accumulators per
+// limb; every 1024 rows the sums are reduced mod q and folded into a
+// canonical u32. out[limb][C] for this slice at out + slice*2C.
+template <int U>
+__global__ __launch_bounds__(256) void k_dbpass_f64(const uint16_t* __restrict__ db, const double* __restrict__ bq,
+                                                    uint32_t* __restrict__ out, uint32_t R, uint32_t C,
+                                                    uint32_t kbps) {
+    const uint32_t col = blockIdx.x * 256 + threadIdx.x;
+    const uint32_t kb0 = blockIdx.y * kbps;
+    const u32x4* p = reinterpret_cast<const u32x4*>(db + ((size_t)kb0 * C + col) * 16);
+    const size_t kstep = (size_t)C * 2;  // one 16-row block, in u32x4
+    const double* b0 = bq + (size_t)kb0 * 16;
+    const double* b1 = b0 + R;
+    const double dq0 = (double)Q0, dq1 = (double)Q1, iq0 = 1.0 / dq0, iq1 = 1.0 / dq1;
+    uint32_t m0 = 0, m1 = 0;
+    double a0[4] = {0, 0, 0, 0}, a1[4] = {0, 0, 0, 0};
+    for (uint32_t kb = 0; kb < kbps; kb += U) {
+        u32x4 v[U][2];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            v[u][0] = __builtin_nontemporal_load(p + (size_t)(kb + u) * kstep);
+            v[u][1] = __builtin_nontemporal_load(p + (size_t)(kb + u) * kstep + 1);
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const double* x0 = b0 + (size_t)(kb + u) * 16;
+            const double* x1 = b1 + (size_t)(kb + u) * 16;
+#pragma unroll
+            for (int e = 0; e < 16; e++) {
+                const uint32_t w = v[u][e >> 3][(e & 7) >> 1];
+                const double d = (double)((e & 1) ? (w >> 16) : (w & 0xffffu));
+                a0[e & 3] = fma(d, x0[e], a0[e & 3]);
+                a1[e & 3] = fma(d, x1[e], a1[e & 3]);
+            }
+        }
+        if (((kb + U) & 63) == 0 || kb + U >= kbps) {  // every 64 blocks = 1024 rows
+            m0 = add_mod(m0, fmod_exact((a0[0] + a0[1]) + (a0[2] + a0[3]), dq0, iq0), Q0);
+            m1 = add_mod(m1, fmod_exact((a1[0] + a1[1]) + (a1[2] + a1[3]), dq1, iq1), Q1);
+#pragma unroll
+            for (int e = 0; e < 4; e++) a0[e] = a1[e] = 0.0;
+        }
+    }
+    uint32_t* o = out + (size_t)blockIdx.y * 2 * C;
+    o[col] = m0;
+    o[C + col] = m1;
+}
+
+// ------------------------------------------------------------ batched pass (int8 MFMA)
+// Query side: b = sum_k d_k 256^k with balanced base-256 digits d_0..d_2 in
+// [-128, 127] and d_3 in [0, 8] (b < 2^27). DB side: v = 256*h' + l' + 32896
+// with h' = hi byte - 128, l' = lo byte - 128 (a byte XOR 0x80 read as int8).
+// So   sum_r v_r b_r = sum_k 256^k (256*H_k + L_k) + 32896 * sum_r b_r,
+// with H_k = sum_r h'_r d_{k,r} and L_k = sum_r l'_r d_{k,r}: int8 x int8 dot
+// products over at most 2^15 rows, |H_k|, |L_k| <= 2^29, exact in i32.
+//
+// A operand rows m = (q*2 + c)*4 + k (pair pr = q*2 + c, digit k), stored as
+// dig[R/16][Mp][16 bytes] (byte e of row m in block kb = digit of row 16kb+e).
+// MFMA 16x16x64 i8 lane mapping used: A/B lane l holds 16 K-bytes of
+// M row / N col l%16 at K offset 16*(l/16) (the same K order on both sides,
+// so any K permutation the hardware applies cancels); the i32 output holds
+// rows 4*(l/16)+i, col l%16 in element i. A lane thus owns all four digits
+// of one pair for one column and recombines them locally.
+
+// dig for query pairs [0, 2*nq) and zero digits for padding pairs < Mp/4.
+__global__ __launch_bounds__(256) void k_digits(const double* __restrict__ bq, uint32_t R, uint32_t nq,
+                                                uint32_t Mp, int8_t* __restrict__ dig) {
+    const uint32_t nkb = R / 16;
+    const uint32_t x = blockIdx.x * 256 + threadIdx.x;
+    if (x >= nkb * (Mp / 4)) return;
+    const uint32_t pr = x / nkb, kb = x % nkb;
+    u32x4 o[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) o[k] = u32x4{0, 0, 0, 0};
+    if (pr < 2 * nq) {
+        const double* b = bq + (size_t)pr * R + (size_t)kb * 16;  // bq [q][c][R] = [pr][R]
+#pragma unroll
+        for (int e = 0; e < 16; e++) {
+            int32_t v = (int32_t)(uint32_t)b[e];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                int32_t d = v;
+                if (k < 3) {
+                    d = ((v & 255) ^ 128) - 128;  // balanced digit in [-128, 127]
+                    v = (v - d) >> 8;
+                }
+                o[k][e >> 2] |= ((uint32_t)d & 255u) << (8 * (e & 3));
+            }
+        }
+    }
+    u32x4* dst = reinterpret_cast<u32x4*>(dig + ((size_t)kb * Mp + (size_t)pr * 4) * 16);
+#pragma unroll
+    for (int k = 0; k < 4; k++) dst[k] = o[k];
+}
+
+// S[pr] = sum_r b_pr[r] mod q_c, the DB-offset correction.
+__global__ __launch_bounds__(256) void k_bsum(const double* __restrict__ bq, uint32_t R, uint32_t* __restrict__ S) {
+    __shared__ uint64_t red[256];
+    const uint32_t pr = blockIdx.x;
+    const double* b = bq + (size_t)pr * R;
+    uint64_t a = 0;
+    for (uint32_t i = threadIdx.x; i < R; i += 256) a += (uint64_t)b[i];
+    red[threadIdx.x] = a;
+    __syncthreads();
+    for (uint32_t s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) S[pr] = red64(red[0], (int)(pr & 1));
+}
+
+// 16 u16 (one column, 16 rows) -> int8 B operands of the low and high byte
+// planes, offset by -128.
+__device__ __forceinline__ void split_planes(const u32x4& lo, const u32x4& hi, i32x4& Bl, i32x4& Bh) {
+    const uint32_t wd[8] = {lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        Bl[j] = (int32_t)(__builtin_amdgcn_perm(wd[2 * j + 1], wd[2 * j], 0x06040200u) ^ 0x80808080u);
+        Bh[j] = (int32_t)(__builtin_amdgcn_perm(wd[2 * j + 1], wd[2 * j], 0x07050301u) ^ 0x80808080u);
+    }
+}
+
+// Recombine one lane's accumulators of pair pr and store the canonical residue.
+__device__ __forceinline__ void store_pair(const i32x4& al, const i32x4& ah, uint32_t pr, uint32_t nq, bool corr,
+                                           const uint32_t* __restrict__ S, uint32_t* __restrict__ out,
+                                           size_t out_qstride, uint32_t slice, uint32_t C, uint32_t col) {
+    const uint32_t q = pr >> 1;
+    const int c = (int)(pr & 1);
+    if (q >= nq) return;
+    int64_t s = corr ? (int64_t)32896 * S[pr] : 0;
+#pragma unroll
+    for (int i = 0; i < 4; i++) s += ((int64_t)ah[i] * 256 + (int64_t)al[i]) * ((int64_t)1 << (8 * i));
+    // |s| < 2^58.1 (|d_3| <= 8 bounds the top digit's term); add a multiple
+    // of q_c above that to make it non-negative
+    const uint64_t qc = c ? Q1 : Q0;
+    const uint64_t off = qc * ((1ull << 59) / qc);
+    out[(size_t)q * out_qstride + (size_t)slice * 2 * C + (size_t)c * C + col] = red64((uint64_t)(s + (int64_t)off), c);
+}
+
+// Few queries (Mp = 16*MT rows, <= 2*MT queries): the 4 waves of a block own
+// disjoint column ranges (NT 16-column tiles each) and load their DB operands
+// straight from HBM into registers, one 64-row step ahead.
+template <int MT, int NT>
+__global__ __launch_bounds__(256) void k_dbpass_mfma_direct(const uint16_t* __restrict__ db,
+                                                            const int8_t* __restrict__ dig,
+                                                            const uint32_t* __restrict__ S,
+                                                            uint32_t* __restrict__ out, size_t out_qstride,
+                                                            uint32_t C, uint32_t nq, uint32_t kbps) {
+    constexpr uint32_t Mp = 16 * MT;
+    const uint32_t w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const uint32_t slice = blockIdx.y;
+    const uint32_t col0 = (blockIdx.x * 4 + w) * NT * 16;
+    const uint32_t kb0 = slice * kbps;
+    const uint32_t nks = kbps / 4;
+    const u32x4* bsrc = reinterpret_cast<const u32x4*>(db + ((size_t)(kb0 + (l >> 4)) * C + col0 + (l & 15)) * 16);
+    const size_t bstep = (size_t)C * 8;  // 4 blocks, in u32x4
+    const i32x4* asrc = reinterpret_cast<const i32x4*>(dig) + (size_t)(kb0 + (l >> 4)) * Mp + (l & 15);
+    const size_t astep = (size_t)4 * Mp;
+    i32x4 acc[MT][NT][2];
+#pragma unroll
+    for (int a = 0; a < MT; a++)
+#pragma unroll
+        for (int n = 0; n < NT; n++) acc[a][n][0] = acc[a][n][1] = i32x4{0, 0, 0, 0};
+    u32x4 v[2][NT][2];
+    i32x4 A[2][MT];
+    auto load = [&](int bi, uint32_t ks) {
+#pragma unroll
+        for (int n = 0; n < NT; n++) {
+            v[bi][n][0] = __builtin_nontemporal_load(bsrc + ks * bstep + n * 32);
+            v[bi][n][1] = __builtin_nontemporal_load(bsrc + ks * bstep + n * 32 + 1);
+        }
+#pragma unroll
+        for (int a = 0; a < MT; a++) A[bi][a] = asrc[ks * astep + a * 16];
+    };
+    auto compute = [&](int bi) {
+#pragma unroll
+        for (int n = 0; n < NT; n++) {
+            i32x4 Bl, Bh;
+            split_planes(v[bi][n][0], v[bi][n][1], Bl, Bh);
+#pragma unroll
+            for (int a = 0; a < MT; a++) {
+                acc[a][n][0] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[bi][a], Bl, acc[a][n][0], 0, 0, 0);
+                acc[a][n][1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[bi][a], Bh, acc[a][n][1], 0, 0, 0);
+            }
+        }
+    };
+    load(0, 0);
+    for (uint32_t ks = 0; ks < nks; ks += 2) {  // nks is even
+        load(1, ks + 1);
+        compute(0);
+        if (ks + 2 < nks) load(0, ks + 2);
+        compute(1);
+    }
+#pragma unroll
+    for (int a = 0; a < MT; a++)
+#pragma unroll
+        for (int n = 0; n < NT; n++)
+            store_pair(acc[a][n][0], acc[a][n][1], (uint32_t)a * 4 + (l >> 4), nq, slice == 0, S, out, out_qstride,
+                       slice, C, col0 + n * 16 + (l & 15));
+}
+
+// Many queries (Mp = 64*MT rows, <= 8*MT queries): the 4 waves of a block
+// split the M rows (MT tiles each) and share one 128-column DB tile per
+// 64-row step, staged through LDS (double buffered, loaded a step ahead
+// through registers). The digits (L2-resident) are read per wave.
+template <int MT>
+__global__ __launch_bounds__(256) void k_dbpass_mfma_lds(const uint16_t* __restrict__ db,
+                                                         const int8_t* __restrict__ dig,
+                                                         const uint32_t* __restrict__ S,
+                                                         uint32_t* __restrict__ out, size_t out_qstride,
+                                                         uint32_t C, uint32_t nq, uint32_t kbps) {
+    constexpr int NT = 8;
+    constexpr uint32_t Mp = 64 * MT;
+    // [buf][block kb of the step][half][col] 16-byte units
+    __shared__ u32x4 tile[2][4 * 2 * 128];
+    const uint32_t tid = threadIdx.x, w = tid >> 6, l = tid & 63;
+    const uint32_t slice = blockIdx.y;
+    const uint32_t colbase = blockIdx.x * 128;
+    const uint32_t kb0 = slice * kbps;
+    const uint32_t nks = kbps / 4;
+    // this thread's 4 16-byte units of a step's tile: x = tid + 256j -> block x/256, unit x%256 = (col, half)
+    const u32x4* gsrc = reinterpret_cast<const u32x4*>(db + ((size_t)kb0 * C + colbase) * 16);
+    const size_t bstep = (size_t)C * 8;
+    auto gaddr = [&](uint32_t ks, int j) {
+        const uint32_t x = tid + 256 * j;
+        return gsrc + ks * bstep + (size_t)(x >> 8) * C * 2 + (x & 255);
+    };
+    auto laddr = [&](int j) {
+        const uint32_t x = tid + 256 * j, u = x & 255;
+        return ((x >> 8) * 2 + (u & 1)) * 128 + (u >> 1);
+    };
+    const i32x4* asrc = reinterpret_cast<const i32x4*>(dig) + (size_t)(kb0 + (l >> 4)) * Mp + w * MT * 16 + (l & 15);
+    const size_t astep = (size_t)4 * Mp;
+    i32x4 acc[MT][NT][2];
+#pragma unroll
+    for (int a = 0; a < MT; a++)
+#pragma unroll
+        for (int n = 0; n < NT; n++) acc[a][n][0] = acc[a][n][1] = i32x4{0, 0, 0, 0};
+    u32x4 pre[4];
+    i32x4 A[MT], An[MT];
+#pragma unroll
+    for (int j = 0; j < 4; j++) pre[j] = __builtin_nontemporal_load(gaddr(0, j));
+#pragma unroll
+    for (int a = 0; a < MT; a++) A[a] = asrc[a * 16];
+    for (uint32_t ks = 0; ks < nks; ks++) {
+        const int bi = ks & 1;
+#pragma unroll
+        for (int j = 0; j < 4; j++) tile[bi][laddr(j)] = pre[j];
+        __syncthreads();
+        if (ks + 1 < nks) {
+#pragma unroll
+            for (int j = 0; j < 4; j++) pre[j] = __builtin_nontemporal_load(gaddr(ks + 1, j));
+#pragma unroll
+            for (int a = 0; a < MT; a++) An[a] = asrc[(ks + 1) * astep + a * 16];
+        }
+        const uint32_t kbl = l >> 4;
+#pragma unroll
+        for (int n = 0; n < NT; n++) {
+            const uint32_t cl = n * 16 + (l & 15);
+            i32x4 Bl, Bh;
+            split_planes(tile[bi][(kbl * 2 + 0) * 128 + cl], tile[bi][(kbl * 2 + 1) * 128 + cl], Bl, Bh);
+#pragma unroll
+            for (int a = 0; a < MT; a++) {
+                acc[a][n][0] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[a], Bl, acc[a][n][0], 0, 0, 0);
+                acc[a][n][1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[a], Bh, acc[a][n][1], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int a = 0; a < MT; a++) A[a] = An[a];
+    }
+#pragma unroll
+    for (int a = 0; a < MT; a++)
+#pragma unroll
+        for (int n = 0; n < NT; n++)
+            store_pair(acc[a][n][0], acc[a][n][1], (w * MT + (uint32_t)a) * 4 + (l >> 4), nq, slice == 0, S, out,
+                       out_qstride, slice, C, colbase + n * 16 + (l & 15));
+}
+
+// Largest power-of-two slice count s with blocks*s <= target that keeps
+// every slice a whole number of `unit`-block chunks.
+uint32_t pick_slices(size_t nkb, size_t blocks, size_t target, size_t unit) {
+    uint32_t s = 1;
+    while (blocks * s * 2 <= target && nkb % (s * 2 * unit) == 0 && nkb / (s * 2) >= unit) s *= 2;
+    return s;
+}
+
+}  // namespace
+
+hipError_t launch_to_blocked(const uint16_t* src, uint16_t* dst, size_t C, size_t kb0, size_t nkb, hipStream_t s) {
+    if (C % 256) return hipErrorInvalidValue;
+    if (nkb == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_to_blocked, dim3((unsigned)(C / 256), (unsigned)nkb), dim3(256), 0, s, src, dst,
+                       (uint32_t)C, (uint32_t)kb0);
+    return hipGetLastError();
+}
+
+DbPlan dbplan_f64(size_t R, size_t C, int unroll, size_t target_blocks) {
+    DbPlan p;
+    p.kind = DbPlan::F64;
+    p.param = (uint32_t)unroll;
+    p.slices = pick_slices(R / 16, C / 256, target_blocks, (size_t)(unroll > 4 ? unroll : 4));
+    return p;
+}
+
+DbPlan dbplan_mfma(size_t R, size_t C, size_t nq, size_t target_blocks) {
+    DbPlan p;
+    // direct: 2 queries per 16-row M tile; LDS: 4 waves x MT tiles, 8 queries per MT
+    p.kind = nq <= 4 ? DbPlan::MFMA_DIRECT : DbPlan::MFMA_LDS;
+    p.param = nq <= 2 ? 1 : nq <= 4 ? 2 : nq <= 8 ? 1 : 2;
+    const size_t cols = p.kind == DbPlan::MFMA_DIRECT ? 4 * 4 * 16 : 128;
+    // slices: whole number of 128-row (8-block) chunks so the step count is even
+    p.slices = pick_slices(R / 16, C / cols, target_blocks, 8);
+    // at most 2^16 rows per slice keeps |H_k|, |L_k| <= 2^30 (i32-exact) and
+    // the recombined sum below 2^58.1
+    while (R / p.slices > 65536) p.slices *= 2;
+    return p;
+}
+
+size_t dbplan_mp(const DbPlan& p) { return p.kind == DbPlan::MFMA_DIRECT ? 16 * p.param : 64 * p.param; }
+
+hipError_t launch_dbpass_f64(const uint16_t* db, const double* bq, uint32_t* out, size_t R, size_t C,
+                             const DbPlan& p, hipStream_t s) {
+    const uint32_t kbps = (uint32_t)(R / 16 / p.slices);
+    const dim3 grid((unsigned)(C / 256), p.slices);
+    switch (p.param) {
+        case 2: hipLaunchKernelGGL(k_dbpass_f64<2>, grid, dim3(256), 0, s, db, bq, out, (uint32_t)R, (uint32_t)C, kbps); break;
+        case 8: hipLaunchKernelGGL(k_dbpass_f64<8>, grid, dim3(256), 0, s, db, bq, out, (uint32_t)R, (uint32_t)C, kbps); break;
+        default: hipLaunchKernelGGL(k_dbpass_f64<4>, grid, dim3(256), 0, s, db, bq, out, (uint32_t)R, (uint32_t)C, kbps); break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_dbpass_mfma(const uint16_t* db, const double* bq, size_t nq, int8_t* dig, uint32_t* S,
+                              uint32_t* out, size_t out_qstride, size_t R, size_t C, const DbPlan& p,
+                              hipStream_t s) {
+    if (p.kind == DbPlan::F64 || nq == 0 || nq > kMfmaMaxQueries) return hipErrorInvalidValue;
+    const uint32_t Mp = (uint32_t)dbplan_mp(p);
+    if (2 * nq > Mp / 4) return hipErrorInvalidValue;
+    const uint32_t nkb = (uint32_t)(R / 16);
+    const uint32_t kbps = nkb / p.slices;
+    if (kbps % 8) return hipErrorInvalidValue;
+    const size_t nthr = (size_t)nkb * (Mp / 4);
+    hipLaunchKernelGGL(k_digits, dim3((unsigned)((nthr + 255) / 256)), dim3(256), 0, s, bq, (uint32_t)R,
+                       (uint32_t)nq, Mp, dig);
+    hipLaunchKernelGGL(k_bsum, dim3((unsigned)(2 * nq)), dim3(256), 0, s, bq, (uint32_t)R, S);
+    const uint32_t Cc = (uint32_t)C, Q = (uint32_t)nq;
+    if (p.kind == DbPlan::MFMA_DIRECT) {
+        if (C % 256) return hipErrorInvalidValue;
+        const dim3 grid((unsigned)(C / 256), p.slices);
+#define DIRECT(MT) \
+    hipLaunchKernelGGL((k_dbpass_mfma_direct<MT, 4>), grid, dim3(256), 0, s, db, dig, S, out, out_qstride, Cc, Q, kbps)
+        switch (p.param) {
+            case 1: DIRECT(1); break;
+            case 2: DIRECT(2); break;
+            default: return hipErrorInvalidValue;
+        }
+#undef DIRECT
+    } else {
+        if (C % 128) return hipErrorInvalidValue;
+        const dim3 grid((unsigned)(C / 128), p.slices);
+#define LDS(MT) \
+    hipLaunchKernelGGL((k_dbpass_mfma_lds<MT>), grid, dim3(256), 0, s, db, dig, S, out, out_qstride, Cc, Q, kbps)
+        switch (p.param) {
+            case 1: LDS(1); break;
+            case 2: LDS(2); break;
+            default: return hipErrorInvalidValue;
+        }
+#undef LDS
+    }
+    return hipGetLastError();
+}
+
+}  // namespace ipx

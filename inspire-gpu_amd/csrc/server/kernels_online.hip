@@ -55,72 +55,8 @@ __global__ __launch_bounds__(256) void k_ntt_rns(uint32_t* __restrict__ polys, N
     for (int k = 0; k < 8; k++) p[t + 256 * k] = buf[ls(t + 256 * k)];
 }
 
-// ------------------------------------------------------------ database pass
-// One query. Lane owns CPT consecutive columns and walks the block's rows.
-// Products db*b (< 2^43) accumulate exactly in fp64 FMA (full rate on
-// CDNA) for up to 1024 rows (< 2^53), are then reduced mod q in fp64 and
-// folded into a canonical u32 accumulator.
+// ------------------------------------------------------------ slice reduction
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
-template <int CPT> struct Vec;
-template <> struct Vec<8> { typedef u32x4 T; };
-template <> struct Vec<4> { typedef u32x2 T; };
-
-__device__ __forceinline__ uint32_t fmod_exact(double a, double q, double qinv) {
-    double t = floor(a * qinv);
-    double r = fma(-t, q, a);  // exact: |a - t*q| < 2q
-    if (r < 0) r += q;
-    if (r >= q) r -= q;
-    return (uint32_t)r;
-}
-
-template <int CPT, int U>
-__global__ __launch_bounds__(256) void k_dbpass(const uint16_t* __restrict__ db, const double* __restrict__ bq,
-                                                uint32_t* __restrict__ part, uint32_t R, uint32_t C,
-                                                uint32_t rps) {
-    typedef typename Vec<CPT>::T VT;
-    const uint32_t col = (blockIdx.x * 256 + threadIdx.x) * CPT;
-    const uint32_t r0 = blockIdx.y * rps;
-    const double* b0 = bq + r0;
-    const double* b1 = bq + R + r0;
-    const VT* p = reinterpret_cast<const VT*>(db + (size_t)r0 * C + col);
-    const size_t rs = C / CPT;  // row stride in vectors
-    const double dq0 = (double)Q0, dq1 = (double)Q1, iq0 = 1.0 / dq0, iq1 = 1.0 / dq1;
-    uint32_t m0[CPT], m1[CPT];
-#pragma unroll
-    for (int e = 0; e < CPT; e++) m0[e] = m1[e] = 0;
-    for (uint32_t blk = 0; blk < rps; blk += 1024) {
-        const uint32_t nb = rps - blk < 1024 ? rps - blk : 1024;
-        double a0[CPT], a1[CPT];
-#pragma unroll
-        for (int e = 0; e < CPT; e++) a0[e] = a1[e] = 0.0;
-        for (uint32_t r = 0; r < nb; r += U) {
-            VT v[U];
-#pragma unroll
-            for (int u = 0; u < U; u++) v[u] = __builtin_nontemporal_load(p + (size_t)(blk + r + u) * rs);
-#pragma unroll
-            for (int u = 0; u < U; u++) {
-                const double x0 = b0[blk + r + u], x1 = b1[blk + r + u];
-#pragma unroll
-                for (int e = 0; e < CPT; e++) {
-                    const uint32_t w = v[u][e >> 1];
-                    const double d = (double)((e & 1) ? (w >> 16) : (w & 0xffffu));
-                    a0[e] = fma(d, x0, a0[e]);
-                    a1[e] = fma(d, x1, a1[e]);
-                }
-            }
-        }
-#pragma unroll
-        for (int e = 0; e < CPT; e++) {
-            m0[e] = add_mod(m0[e], fmod_exact(a0[e], dq0, iq0), Q0);
-            m1[e] = add_mod(m1[e], fmod_exact(a1[e], dq1, iq1), Q1);
-        }
-    }
-    uint32_t* o0 = part + (size_t)blockIdx.y * 2 * C + col;
-    uint32_t* o1 = o0 + C;
-#pragma unroll
-    for (int e = 0; e < CPT; e++) { o0[e] = m0[e]; o1[e] = m1[e]; }
-}
 
 // interm[q][limb][col] = sum over row slices of part mod q_limb.
 __global__ __launch_bounds__(256) void k_reduce_slices(const uint32_t* __restrict__ part, size_t part_stride,
@@ -328,37 +264,6 @@ hipError_t launch_unmarshal(const uint64_t* d_q, size_t count, size_t R, double*
 hipError_t launch_ntt_rns(uint32_t* polys, size_t npoly, const NttConst& nc, bool inverse, hipStream_t s) {
     if (npoly == 0) return hipSuccess;
     hipLaunchKernelGGL(k_ntt_rns, dim3((unsigned)(2 * npoly)), dim3(256), 0, s, polys, nc, inverse ? 1 : 0);
-    return hipGetLastError();
-}
-
-DbPassShape dbpass_shape(size_t R, size_t C, int target_blocks, int cpt, int unroll) {
-    DbPassShape sh;
-    sh.cols_per_thread = (uint32_t)cpt;
-    sh.unroll = (uint32_t)unroll;
-    if (C % (256 * (size_t)cpt)) sh.cols_per_thread = 4;  // C is a multiple of 2048 = 256*8 anyway
-    const size_t col_blocks = C / (256 * sh.cols_per_thread);
-    size_t slices = 1;
-    while (col_blocks * slices < (size_t)target_blocks && R / (slices * 2) >= 64 &&
-           (R / (slices * 2)) % sh.unroll == 0)
-        slices *= 2;
-    sh.slices = (uint32_t)slices;
-    sh.rows_per_slice = (uint32_t)(R / slices);
-    return sh;
-}
-
-hipError_t launch_dbpass(const uint16_t* db, const double* bq, uint32_t* part, size_t R, size_t C,
-                         const DbPassShape& sh, hipStream_t s) {
-    const unsigned cb = (unsigned)(C / (256 * sh.cols_per_thread));
-    const dim3 grid(cb, sh.slices);
-#define DBP(CPT, U)                                                                                         \
-    hipLaunchKernelGGL((k_dbpass<CPT, U>), grid, dim3(256), 0, s, db, bq, part, (uint32_t)R, (uint32_t)C, \
-                       sh.rows_per_slice)
-    if (sh.cols_per_thread == 8) {
-        if (sh.unroll == 8) DBP(8, 8); else DBP(8, 4);
-    } else {
-        if (sh.unroll == 8) DBP(4, 8); else DBP(4, 4);
-    }
-#undef DBP
     return hipGetLastError();
 }
 

@@ -89,7 +89,8 @@ __global__ __launch_bounds__(1024) void k_encode(uint16_t* __restrict__ db, uint
 }
 
 // ------------------------------------------------------------ a-side
-// Block (group, 4 columns): acc[col][c] = sum_r NTT(db[rN..rN+N-1][col]) *
+// Block (group, 4 columns) over the blocked DB (element (r, c) at
+// ((r/16)*C + c)*16 + r%16): acc[col][c] = sum_r NTT(db[rN..rN+N-1][col]) *
 // crs[r][c]; a32T[g][c][k][col] = N^-1 * acc[col][c][N-1-k].
 // Polynomial p = col*2 + c; team tm transforms p = tm and p = tm + 4.
 __global__ __launch_bounds__(1024) void k_aside(const uint16_t* __restrict__ db, uint32_t C, uint32_t nr, uint32_t g0,
@@ -112,11 +113,11 @@ __global__ __launch_bounds__(1024) void k_aside(const uint16_t* __restrict__ db,
 #pragma unroll
         for (int h = 0; h < 2; h++) {
             const uint32_t i = tid + 1024 * h;
-            const uint2 v = *reinterpret_cast<const uint2*>(db + ((size_t)r * N + i) * C + col0);
-            const uint32_t w[2] = {v.x, v.y};
+            const size_t row = (size_t)r * N + i;  // blocked DB: ((row/16)*C + col)*16 + row%16
+            const uint16_t* src = db + ((row >> 4) * C + col0) * 16 + (row & 15);
 #pragma unroll
             for (int col = 0; col < 4; col++) {
-                const uint32_t x = (col & 1) ? (w[col >> 1] >> 16) : (w[col >> 1] & 0xffffu);
+                const uint32_t x = src[col * 16];
                 buf[col * 2][ls(i)] = x;
                 buf[col * 2 + 1][ls(i)] = x;
             }

@@ -39,6 +39,7 @@ struct Req {
 
 struct Partition {
     size_t slot0 = 0, nslots = 0;
+    int lane = 0;  // scratch set of the context this partition uses
     hipStream_t stream = nullptr;
     std::vector<Req*> batch;  // admitted requests; partition busy when non-empty
     bool launched = false;    // batch's device work fully enqueued
@@ -86,7 +87,8 @@ struct ipir_loop {
                 const size_t s0 = free_p->slot0;
                 hip_check(hipMemcpyAsync(ctx->d_q + s0 * ctx->qlen, ctx->h_q + s0 * ctx->qlen, n * ctx->qlen * 8,
                                          hipMemcpyHostToDevice, free_p->stream), "loop h2d");
-                ctx->enqueue(ctx->d_q + s0 * ctx->qlen, n, ctx->d_resp + s0 * ctx->resp_u64s, free_p->stream, s0);
+                ctx->enqueue(ctx->d_q + s0 * ctx->qlen, n, ctx->d_resp + s0 * ctx->resp_u64s, free_p->stream, s0,
+                             free_p->lane);
                 hip_check(hipMemcpyAsync(ctx->h_resp + s0 * ctx->resp_u64s, ctx->d_resp + s0 * ctx->resp_u64s,
                                          n * ctx->resp_u64s * 8, hipMemcpyDeviceToHost, free_p->stream), "loop d2h");
             } catch (...) {
@@ -140,6 +142,7 @@ ipir_loop* ipir_loop_start(ipir_server* srv, size_t max_inflight) {
     for (size_t i = 0; i < max_inflight; i++) {
         auto* p = new Partition;
         p->slot0 = i * per;
+        p->lane = (int)i;
         p->nslots = per;
         if (hipStreamCreate(&p->stream) != hipSuccess) {
             delete p;

@@ -80,19 +80,35 @@ PreprocessData gpu_preprocess(const PublicParams& pp, const uint16_t* db) {
     CrtConst cc{};
     uint32_t* d_tw = nullptr;
     uint16_t* d_db = nullptr;
+    uint16_t* d_stage = nullptr;
     uint32_t* arena = nullptr;
     uint32_t *d_crs = nullptr, *d_k5 = nullptr, *d_kn = nullptr, *d_a32 = nullptr, *d_states = nullptr;
     auto cleanup = [&](bool all) {
-        dfree(d_tw); dfree(d_crs); dfree(d_k5); dfree(d_kn); dfree(d_a32); dfree(d_states);
+        dfree(d_tw); dfree(d_crs); dfree(d_k5); dfree(d_kn); dfree(d_a32); dfree(d_states); dfree(d_stage);
+        d_stage = nullptr;
         if (all) { dfree(d_db); dfree(arena); }
         if (s) (void)hipStreamDestroy(s);
     };
     try {
         HIPX(hipStreamCreate(&s));
         d_tw = upload_tables(nc, cc);
+        // The host DB is row-major. It goes up in slabs of whole rows
+        // (~256 MiB): copy, encode in place (row-local), then scatter into
+        // the resident 16-row-block layout.
         d_db = dalloc<uint16_t>(R * C);
-        HIPX(hipMemcpy(d_db, db, R * C * sizeof(uint16_t), hipMemcpyHostToDevice));
-        HIPX(launch_encode(d_db, R, C, Da, s));
+        size_t slab = ((size_t)1 << 27) / C / 16 * 16;
+        if (slab < 16) slab = 16;
+        if (slab > R) slab = R;
+        d_stage = dalloc<uint16_t>(slab * C);
+        for (size_t r0 = 0; r0 < R; r0 += slab) {
+            const size_t nrows = R - r0 < slab ? R - r0 : slab;
+            HIPX(hipMemcpyAsync(d_stage, db + r0 * C, nrows * C * sizeof(uint16_t), hipMemcpyHostToDevice, s));
+            HIPX(launch_encode(d_stage, nrows, C, Da, s));
+            HIPX(launch_to_blocked(d_stage, d_db, C, r0 / 16, nrows / 16, s));
+        }
+        HIPX(hipStreamSynchronize(s));
+        dfree(d_stage);
+        d_stage = nullptr;
 
         // CRS rows A_r = NTT(tau_{2N-1}(expand_seed("rlwe", r)))  (src/gpu_protocol.cu:567-583)
         std::vector<uint32_t> h_crs(nr * 2 * N);
@@ -207,20 +223,34 @@ GpuServerCtx* gpu_setup_server(const PublicParams& pp, const PreprocessData& pre
 }
 
 namespace {
+// Single-query database-pass candidates: (MFMA?, fp64 unroll, target workgroups).
 struct DbCand {
-    int cpt, unroll, target;
+    bool mfma;
+    int unroll;
+    size_t target;
 };
-const DbCand kDbCands[] = {{8, 4, 2048}, {8, 8, 2048}, {4, 8, 2048}, {8, 4, 1024}, {4, 8, 4096}, {8, 8, 4096}};
+const DbCand kDbCands[] = {{false, 4, 2048}, {false, 8, 2048}, {false, 2, 2048}, {false, 4, 4096},
+                           {false, 8, 1024}, {true, 0, 512},   {true, 0, 1024},  {true, 0, 2048}};
+const size_t kMfmaTargets[] = {256, 512, 1024, 2048};
+
+DbPlan cand_plan(const DbCand& c, size_t R, size_t C) {
+    return c.mfma ? dbplan_mfma(R, C, 1, c.target) : dbplan_f64(R, C, c.unroll, c.target);
+}
 }  // namespace
 
 void GpuServerCtx::alloc_slots() {
     const size_t R = pp.db_rows, C = pp.db_cols, B = cfg.max_batch;
-    sh = dbpass_shape(R, C, 2048, 8, 4);
-    size_t max_slices = sh.slices;
+    plan1 = dbplan_f64(R, C, 4, 2048);
+    size_t max_slices = 1;
     for (const DbCand& c : kDbCands) {
-        const DbPassShape x = dbpass_shape(R, C, c.target, c.cpt, c.unroll);
+        const DbPlan x = cand_plan(c, R, C);
         if (x.slices > max_slices) max_slices = x.slices;
     }
+    for (size_t t : kMfmaTargets)
+        for (size_t nq = 1; nq <= kMfmaMaxQueries; nq++) {
+            const DbPlan x = dbplan_mfma(R, C, nq, t);
+            if (x.slices > max_slices) max_slices = x.slices;
+        }
     nch = collapse_nch(G);
     lay.R = R; lay.C = C; lay.G = G;
     qlen = lay.query_u64s();
@@ -239,22 +269,35 @@ void GpuServerCtx::alloc_slots() {
     d_interm = dalloc<uint32_t>(B * 2 * C);
     d_part2 = dalloc<uint32_t>(B * part2_stride);
     d_packed_b = dalloc<uint32_t>(B * packed_stride);
+    for (int l = 0; l < kLanes; l++) {
+        d_dig[l] = dalloc<int8_t>(R * 256);
+        d_S[l] = dalloc<uint32_t>(64);
+    }
     HIPX(hipMemset(d_bq, 0, B * 2 * R * sizeof(double)));
     HIPX(hipHostMalloc((void**)&h_q, B * qlen * 8, hipHostMallocDefault));
     HIPX(hipHostMalloc((void**)&h_resp, B * resp_u64s * 8, hipHostMallocDefault));
     resident_bytes = R * C * 2 + G * PC_GROUP * 4 + 8 * N * 4 +
                      B * (qlen * 8 + resp_u64s * 8 + 2 * R * 8 +
-                          (keys_stride + rgsw_stride + part_stride + 2 * C + part2_stride + packed_stride) * 4);
+                          (keys_stride + rgsw_stride + part_stride + 2 * C + part2_stride + packed_stride) * 4) +
+                     kLanes * (R * 256 + 64 * 4);
     autotune_dbpass();
 }
 
 void GpuServerCtx::autotune_dbpass() {
-    const char* force = std::getenv("INSPIRE_DBPASS");
+    // INSPIRE_DBPASS forces the single-query plan: "f64,<unroll>,<target>" or
+    // "mfma,<target>"; INSPIRE_MFMA_TARGET sets the batched pass's target;
+    // INSPIRE_AUTOTUNE=0 keeps the defaults.
     const size_t R = pp.db_rows, C = pp.db_cols;
-    if (force) {
-        int a = 8, b = 4, t = 2048;
-        if (std::sscanf(force, "%d,%d,%d", &a, &b, &t) == 3 && (a == 4 || a == 8) && (b == 4 || b == 8) && t > 0)
-            sh = dbpass_shape(R, C, t, a, b);
+    if (const char* t = std::getenv("INSPIRE_MFMA_TARGET")) {
+        const long v = std::atol(t);
+        if (v > 0) mfma_target = (size_t)v;
+    }
+    if (const char* force = std::getenv("INSPIRE_DBPASS")) {
+        int u = 4, t = 2048;
+        if (std::sscanf(force, "f64,%d,%d", &u, &t) == 2 && (u == 2 || u == 4 || u == 8) && t > 0)
+            plan1 = dbplan_f64(R, C, u, (size_t)t);
+        else if (std::sscanf(force, "mfma,%d", &t) == 1 && t > 0)
+            plan1 = dbplan_mfma(R, C, 1, (size_t)t);
         return;
     }
     const char* at = std::getenv("INSPIRE_AUTOTUNE");
@@ -262,22 +305,39 @@ void GpuServerCtx::autotune_dbpass() {
     hipEvent_t e0, e1;
     HIPX(hipEventCreate(&e0));
     HIPX(hipEventCreate(&e1));
-    float best = 1e30f;
-    DbPassShape best_sh = sh;
-    for (const DbCand& c : kDbCands) {
-        const DbPassShape x = dbpass_shape(R, C, c.target, c.cpt, c.unroll);
-        HIPX(launch_dbpass(d_db, d_bq, d_part, R, C, x, stream));  // warm
+    auto time_plan = [&](const DbPlan& x, size_t nq) {
+        auto run = [&]() {
+            if (x.kind == DbPlan::F64)
+                HIPX(launch_dbpass_f64(d_db, d_bq, d_part, R, C, x, stream));
+            else
+                HIPX(launch_dbpass_mfma(d_db, d_bq, nq, d_dig[0], d_S[0], d_part, part_stride, R, C, x, stream));
+        };
+        run();  // warm
         HIPX(hipEventRecord(e0, stream));
-        for (int it = 0; it < 3; it++) HIPX(launch_dbpass(d_db, d_bq, d_part, R, C, x, stream));
+        for (int it = 0; it < 3; it++) run();
         HIPX(hipEventRecord(e1, stream));
         HIPX(hipEventSynchronize(e1));
         float ms = 0;
         HIPX(hipEventElapsedTime(&ms, e0, e1));
-        if (ms < best) { best = ms; best_sh = x; }
+        return ms;
+    };
+    float best = 1e30f;
+    for (const DbCand& c : kDbCands) {
+        const DbPlan x = cand_plan(c, R, C);
+        const float ms = time_plan(x, 1);
+        if (ms < best) { best = ms; plan1 = x; }
+    }
+    // batched pass: the workgroup target at the largest batch this context serves
+    const size_t nq = cfg.max_batch < kMfmaMaxQueries ? cfg.max_batch : kMfmaMaxQueries;
+    if (nq > 1) {
+        best = 1e30f;
+        for (size_t t : kMfmaTargets) {
+            const float ms = time_plan(dbplan_mfma(R, C, nq, t), nq);
+            if (ms < best) { best = ms; mfma_target = t; }
+        }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    sh = best_sh;
 }
 
 void GpuServerCtx::stage_prep(const uint64_t* d_in, size_t count, hipStream_t s, size_t slot0) {
@@ -286,12 +346,25 @@ void GpuServerCtx::stage_prep(const uint64_t* d_in, size_t count, hipStream_t s,
     HIPX(launch_ntt_rns(d_keys + slot0 * keys_stride, count * 2 * D_EFF, nc, false, s));
 }
 
-void GpuServerCtx::stage_dbpass(size_t count, hipStream_t s, size_t slot0) {
+void GpuServerCtx::stage_dbpass(size_t count, hipStream_t s, size_t slot0, int lane, int force) {
     const size_t R = pp.db_rows, C = pp.db_cols;
-    for (size_t q = slot0; q < slot0 + count; q++)
-        HIPX(launch_dbpass(d_db, d_bq + q * 2 * R, d_part + q * part_stride, R, C, sh, s));
-    HIPX(launch_reduce_slices(d_part + slot0 * part_stride, part_stride, sh.slices, C, d_interm + slot0 * 2 * C,
-                              count, s));
+    const size_t step = force == 0 ? 1 : kMfmaMaxQueries;
+    for (size_t q0 = slot0; q0 < slot0 + count; q0 += step) {
+        const size_t nq = slot0 + count - q0 < step ? slot0 + count - q0 : step;
+        DbPlan p = nq == 1 ? plan1 : dbplan_mfma(R, C, nq, mfma_target);
+        if (force == 0 && p.kind != DbPlan::F64) p = dbplan_f64(R, C, 4, 2048);
+        if (force == 1 && p.kind == DbPlan::F64) p = dbplan_mfma(R, C, nq, mfma_target);
+        // one slice: straight into interm; else row-slice partials, reduced below
+        uint32_t* out = p.slices == 1 ? d_interm + q0 * 2 * C : d_part + q0 * part_stride;
+        const size_t qstride = p.slices == 1 ? 2 * C : part_stride;
+        if (p.kind == DbPlan::F64)
+            HIPX(launch_dbpass_f64(d_db, d_bq + q0 * 2 * R, out, R, C, p, s));
+        else
+            HIPX(launch_dbpass_mfma(d_db, d_bq + q0 * 2 * R, nq, d_dig[lane], d_S[lane], out, qstride, R, C, p, s));
+        if (p.slices > 1)
+            HIPX(launch_reduce_slices(d_part + q0 * part_stride, part_stride, p.slices, C, d_interm + q0 * 2 * C, nq,
+                                      s));
+    }
 }
 
 void GpuServerCtx::stage_collapse(size_t count, hipStream_t s, size_t slot0) {
@@ -313,9 +386,11 @@ void GpuServerCtx::stage_horner(size_t count, uint64_t* d_out, hipStream_t s, si
                        d_rgsw + slot0 * rgsw_stride, rgsw_stride, d_out, count, nc, cc, s));
 }
 
-void GpuServerCtx::enqueue(const uint64_t* d_in, size_t count, uint64_t* d_out, hipStream_t s, size_t slot0) {
+void GpuServerCtx::enqueue(const uint64_t* d_in, size_t count, uint64_t* d_out, hipStream_t s, size_t slot0,
+                           int lane) {
+    if (lane < 0 || lane >= kLanes) throw std::invalid_argument("enqueue: scratch lane out of range");
     stage_prep(d_in, count, s, slot0);
-    stage_dbpass(count, s, slot0);
+    stage_dbpass(count, s, slot0, lane);
     stage_collapse(count, s, slot0);
     stage_finalize(count, s, slot0);
     stage_horner(count, d_out, s, slot0);
@@ -401,6 +476,7 @@ void gpu_free_server(GpuServerCtx* ctx) {
     dfree(ctx->d_tw); dfree(ctx->d_db); dfree(ctx->d_precomp);
     dfree(ctx->d_q); dfree(ctx->d_resp); dfree(ctx->d_bq); dfree(ctx->d_keys); dfree(ctx->d_rgsw);
     dfree(ctx->d_part); dfree(ctx->d_interm); dfree(ctx->d_part2); dfree(ctx->d_packed_b);
+    for (int l = 0; l < GpuServerCtx::kLanes; l++) { dfree(ctx->d_dig[l]); dfree(ctx->d_S[l]); }
     if (ctx->h_q) (void)hipHostFree(ctx->h_q);
     if (ctx->h_resp) (void)hipHostFree(ctx->h_resp);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);

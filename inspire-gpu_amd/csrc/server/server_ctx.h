@@ -22,7 +22,7 @@ struct GpuServerCtx {
     size_t G = 0, Da = 0, T = 0;
 
     // resident data
-    uint16_t* d_db = nullptr;       // encoded DB, row-major u16 [R][C]
+    uint16_t* d_db = nullptr;       // encoded DB, u16 in 16-row blocks: (r, c) at ((r/16)*C + c)*16 + r%16
     uint32_t* d_precomp = nullptr;  // [G][PC_GROUP] arena
     uint32_t* d_tw = nullptr;       // twiddles
     ipx::NttConst nc{};
@@ -30,7 +30,8 @@ struct GpuServerCtx {
 
     // per-query slot pools, max_batch each
     ipx::SlotLayout lay{};
-    ipx::DbPassShape sh{};
+    ipx::DbPlan plan1{};            // database pass of a single query (autotuned)
+    size_t mfma_target = 1024;      // workgroups the batched pass aims for
     uint32_t nch = 0;
     size_t qlen = 0, resp_u64s = 0;
     size_t part_stride = 0, part2_stride = 0, keys_stride = 0, rgsw_stride = 0, packed_stride = 0;
@@ -42,6 +43,10 @@ struct GpuServerCtx {
     uint32_t* d_part = nullptr;     // DB-pass row-slice partials [B][S][2][C]
     uint32_t* d_interm = nullptr;   // DB-pass result [B][2][C]
     uint32_t* d_part2 = nullptr;
+    // scratch of the batched database pass, one set per concurrent lane
+    static constexpr int kLanes = 4;
+    int8_t* d_dig[kLanes] = {};     // query digits [R/16][256][16]
+    uint32_t* d_S[kLanes] = {};     // digit-offset corrections [64]
     uint32_t* d_packed_b = nullptr;
     uint64_t* h_q = nullptr;        // pinned host staging
     uint64_t* h_resp = nullptr;
@@ -54,13 +59,16 @@ struct GpuServerCtx {
     void autotune_dbpass();
     // Pipeline stages over slots [slot0, slot0+count) on stream s.
     void stage_prep(const uint64_t* d_in, size_t count, hipStream_t s, size_t slot0);
-    void stage_dbpass(size_t count, hipStream_t s, size_t slot0);
+    // force: -1 the tuned plans, 0 the fp64 pass per query, 1 the MFMA pass
+    void stage_dbpass(size_t count, hipStream_t s, size_t slot0, int lane = 0, int force = -1);
     void stage_collapse(size_t count, hipStream_t s, size_t slot0);
     void stage_finalize(size_t count, hipStream_t s, size_t slot0);
     void stage_horner(size_t count, uint64_t* d_out, hipStream_t s, size_t slot0);
     // Enqueue the answer of `count` flat queries at d_in (device) into d_out
-    // (device) on stream s, using slots [slot0, slot0 + count).
-    void enqueue(const uint64_t* d_in, size_t count, uint64_t* d_out, hipStream_t s, size_t slot0 = 0);
+    // (device) on stream s, using slots [slot0, slot0 + count) and scratch
+    // set `lane` (work that may overlap on the device uses distinct lanes).
+    void enqueue(const uint64_t* d_in, size_t count, uint64_t* d_out, hipStream_t s, size_t slot0 = 0,
+                 int lane = 0);
     // Host-staged round trip through the pinned staging of slots
     // [slot0, slot0+count): h_q -> device -> h_resp, then copy to out.
     void answer_host(size_t count, uint64_t* out, hipStream_t s = nullptr, size_t slot0 = 0);

@@ -77,6 +77,7 @@ SERVER_FUNCS = [
     ("ipir_dbg_precomp_group_words", SZ, []),
     ("ipir_dbg_precomp_group", C.c_int, [VP, SZ, VP]),
     ("ipir_dbg_db_rows", C.c_int, [VP, SZ, SZ, VP]),
+    ("ipir_dbg_db_pass", C.c_int, [VP, VP, SZ, C.c_int, VP]),
     ("ipir_dbg_answer_taps", C.c_int, [VP, VP, VP, VP, VP]),
     ("ipir_dbg_ntt", C.c_int, [VP, VP, SZ, C.c_int]),
     ("ipir_dbg_horner", C.c_int, [VP, SZ, VP, VP]),

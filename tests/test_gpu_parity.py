@@ -147,6 +147,35 @@ def medium8():
     srv.close()
 
 
+@pytest.mark.parametrize("path", [0, 1, -1])
+def test_db_pass_paths_match_oracle(medium8, path):
+    """Both database-pass kernels (fp64 per query, int8 MFMA batched) and the
+    tuned dispatch, on the resident (encoded) DB, against the oracle's
+    128-bit mat-vec; extreme residues q-1 and 0 in the query vectors."""
+    geo, db, srv = medium8
+    R, Cc = geo["R"], geo["C"]
+    enc = np.zeros((R, Cc), np.uint16)
+    assert srv.lib.ipir_dbg_db_rows(srv.h, 0, R, ia.ptr(enc)) == 0
+    flat = enc.reshape(-1)
+    rng = np.random.default_rng(100 + path)
+    for nq in (1, 2, 3, 5, 8, 10):
+        b = np.zeros((nq, 2, R), np.uint64)
+        b[:, 0] = rng.integers(0, Q0, (nq, R), dtype=np.uint64)
+        b[:, 1] = rng.integers(0, Q1, (nq, R), dtype=np.uint64)
+        b[0, 0, :R // 4] = Q0 - 1
+        b[0, 1, :R // 4] = Q1 - 1
+        b[-1, :, R // 2:R // 2 + 64] = 0
+        got = np.zeros((nq, 2, Cc), np.uint32)
+        assert srv.lib.ipir_dbg_db_pass(srv.h, ia.ptr(b), nq, path, ia.ptr(got)) == 0
+        for k in range(nq):
+            want = np.zeros(2 * Cc, np.uint64)
+            ol.oracle().orc_matvec(flat, R, Cc, np.ascontiguousarray(b[k, 0]), np.ascontiguousarray(b[k, 1]),
+                                   0, Cc, want)
+            assert np.array_equal(got[k].reshape(-1).astype(np.uint64), want), f"path {path} nq {nq} query {k}"
+    bad = np.full((1, 2, R), Q0, np.uint64)  # non-canonical residues are rejected
+    assert srv.lib.ipir_dbg_db_pass(srv.h, ia.ptr(bad), 1, path, ia.ptr(np.zeros((1, 2, Cc), np.uint32))) == 1
+
+
 def test_batch_equals_singles_and_extracts(medium8):
     geo, db, srv = medium8
     idxs = [0, 42, 1000, 100000, 524287] * 2
