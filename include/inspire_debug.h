@@ -15,7 +15,8 @@
 extern "C" {
 #endif
 
-/* u32 words of one group's precomp block: [a: 2N][D+: 2*(N/2-1)*2*N]
+/* One group's precomp in the reference's layout (converted back from the
+ * resident dot-product form): u32 words [a: 2N][D+: 2*(N/2-1)*2*N]
  * [D-: same][D_final: 2*2*N], each in [limb][..][coeff] order. */
 size_t ipir_dbg_precomp_group_words(void);
 int ipir_dbg_precomp_group(const ipir_server* srv, size_t group, uint32_t* out);

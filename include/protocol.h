@@ -12,15 +12,21 @@ namespace inspire {
 
 // Device-resident preprocess output, produced by gpu_preprocess and adopted
 // (ownership transfer, source fields cleared) by gpu_setup_server.
+//
+// gpu_preprocess allocates the precomp as one arena (d_precomp_arena) in the
+// server's resident layout; the per-group pointers are then views into it:
+// a and D_final as in the reference, D_plus the group's collapse tensors in
+// dot-product form (D+ and D- together), D_minus null. When d_precomp_arena
+// is null the per-group pointers own their memory and hold the reference's
+// layout (D+ / D- each [limb][step][digit][coeff]); gpu_setup_server
+// converts them and frees them.
 struct PreprocessData {
     std::vector<uint32_t*> d_precomp_a;        // [n_packed] -> 2*N u32, NTT form
-    std::vector<uint32_t*> d_precomp_D_plus;   // [n_packed] -> 2*(N/2-1)*D_EFF*N u32
-    std::vector<uint32_t*> d_precomp_D_minus;  // same shape
+    std::vector<uint32_t*> d_precomp_D_plus;   // [n_packed] (see above)
+    std::vector<uint32_t*> d_precomp_D_minus;  // [n_packed] (see above)
     std::vector<uint32_t*> d_precomp_D_final;  // [n_packed] -> 2*D_EFF*N u32
-    uint16_t* d_db_col = nullptr;              // encoded DB, row-major u16 [rows][cols]
-    // Single device allocation the per-group pointers above point into
-    // (this library allocates the precomp as one arena); null when the
-    // per-group pointers own their memory individually.
+    // encoded DB, u16 in 16-row blocks: (r, c) at ((r/16)*cols + c)*16 + r%16
+    uint16_t* d_db_col = nullptr;
     void* d_precomp_arena = nullptr;
 };
 

@@ -113,12 +113,54 @@ void ipir_server_destroy(ipir_server* srv) {
 }
 
 // ---------------------------------------------------------------- inspection
-size_t ipir_dbg_precomp_group_words(void) { return ipx::PC_GROUP; }
+size_t ipir_dbg_precomp_group_words(void) { return ipx::REF_GROUP_WORDS; }
 
 int ipir_dbg_precomp_group(const ipir_server* srv, size_t group, uint32_t* out) {
     if (!srv || !out || group >= srv->ctx->G) return 1;
-    return hipMemcpy(out, srv->ctx->d_precomp + group * ipx::PC_GROUP, ipx::PC_GROUP * 4, hipMemcpyDeviceToHost) ==
-                   hipSuccess ? 0 : 4;
+    using ipx::DD_K; using ipx::DD_SZ; using ipx::HALF; using ipx::LOGN; using ipx::NSTEPS; using ipx::PC_A;
+    using ipx::PC_DD; using ipx::PC_DF; using ipx::PC_GROUP; using ipx::RD_SZ;
+    constexpr uint32_t N = ipx::N;
+    constexpr int D_EFF = ipx::D_EFF;
+    const uint32_t* blk_dev = srv->ctx->d_precomp + group * PC_GROUP;
+    std::vector<uint8_t> dd(DD_SZ * 4);
+    uint32_t* Dp = out + 2 * N;
+    uint32_t* Dm = Dp + RD_SZ;
+    uint32_t* Df = Dm + RD_SZ;
+    if (hipMemcpy(out, blk_dev + PC_A, 2 * N * 4, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(Df, blk_dev + PC_DF, 2 * D_EFF * N * 4, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(dd.data(), blk_dev + PC_DD, dd.size(), hipMemcpyDeviceToHost) != hipSuccess)
+        return 4;
+    // back from the dot-product form: D+/-[c][s][j][i] = Dd[c][j][p+/-(s, i)][i]
+    auto br = [](uint32_t x) {
+        uint32_t r = 0;
+        for (uint32_t b = 0; b < LOGN; b++) r |= ((x >> b) & 1u) << (LOGN - 1 - b);
+        return r;
+    };
+    auto dd_at = [&](uint32_t c, uint32_t j, uint32_t p, uint32_t i) {
+        const size_t k = (size_t)j * N + p;
+        uint32_t v = 0;
+        for (uint32_t a = 0; a < 4; a++) {
+            const uint32_t byte = dd[((((size_t)c * 4 + a) * (DD_K / 16) + k / 16) * N + i) * 16 + k % 16];
+            v |= (a < 3 ? byte ^ 0x80u : byte) << (8 * a);
+        }
+        return v;
+    };
+    uint32_t t = 1;  // 5^(N/2-2-s) mod 2N, starting at s = 0
+    for (uint32_t e = 0; e < HALF - 2; e++) t = (t * 5) & (2 * N - 1);
+    for (uint32_t s = 0; s < NSTEPS; s++) {
+        for (uint32_t i = 0; i < N; i++) {
+            const uint32_t u = (t * (2 * br(i) + 1)) & (2 * N - 1);
+            const uint32_t pf = br((u - 1) >> 1), pc = N - 1 - pf;
+            for (uint32_t c = 0; c < 2; c++)
+                for (uint32_t j = 0; j < (uint32_t)D_EFF; j++) {
+                    const size_t o = (((size_t)c * NSTEPS + s) * D_EFF + j) * N + i;
+                    Dp[o] = dd_at(c, j, pf, i);
+                    Dm[o] = dd_at(c, j, pc, i);
+                }
+        }
+        t = (t * 3277u) & (2 * N - 1);  // times 5^-1
+    }
+    return 0;
 }
 
 int ipir_dbg_db_rows(const ipir_server* srv, size_t row0, size_t nrows, uint16_t* out) {

@@ -33,14 +33,31 @@ struct CrtConst {
     uint32_t q0inv_q1;
 };
 
-// Per-group precomp block, u32 words: [a: 2N][D+: 2*NSTEPS*2*N][D-: same][Df: 2*2*N].
-// Inside D+/D-: [limb][step][digit][coeff]; inside Df: [limb][digit][coeff].
+// The collapse tensors D+/D- in the reference's per-group order, u32 words
+// [limb][step][digit][coeff] for each direction (RD_SZ words each); the
+// preprocess chain produces them in this order and the inspection entry
+// point returns them in it.
+constexpr size_t RD_SZ = 2 * (size_t)NSTEPS * D_EFF * N;
+// Words of one group in the reference order: [a: 2N][D+][D-][Df: 2*2*N].
+constexpr size_t REF_GROUP_WORDS = 2 * (size_t)N + 2 * RD_SZ + 2 * (size_t)D_EFF * N;
+
+// Resident per-group precomp block, u32 words: [a: 2N][Dd: DD_SZ][Df: 2*2*N].
+// Dd is D+/D- in dot-product form. For slot i the 2*NSTEPS collapse steps
+// of both directions gather the key at 2*NSTEPS distinct slots p, so
+//   sum_s sum_j D+[s][j][i] K[j][p+(s,i)] + D-[s][j][i] K[j][p-(s,i)]
+//     = sum_{j,p} Dd[j][p][i] K[j][p]
+// with Dd[j][p][i] = D+/-[s][j][i] where p = p+/-(s, i), and 0 at the two
+// slots no step reaches. Per limb this is a K = 2N (j, p) by N (i) matrix,
+// stored as four byte planes in 16-row blocks:
+//   byte (((limb*4 + a)*(2N/16) + k/16)*N + i)*16 + k%16, k = j*N + p,
+// plane a < 3 holding byte a of the residue XOR 0x80 (read as int8 it is
+// byte - 128) and plane 3 the top byte (< 8) as is.
+constexpr size_t DD_K = 2 * (size_t)N;          // K rows per limb
+constexpr size_t DD_SZ = 2 * DD_K * N;          // words (4 plane bytes per entry)
 constexpr size_t PC_A = 0;
-constexpr size_t PC_DP = 2 * (size_t)N;
-constexpr size_t PC_DSZ = 2 * (size_t)NSTEPS * D_EFF * N;
-constexpr size_t PC_DM = PC_DP + PC_DSZ;
-constexpr size_t PC_DF = PC_DM + PC_DSZ;
-constexpr size_t PC_GROUP = PC_DF + 2 * (size_t)D_EFF * N;  // 8190*N words = 67.09 MB
+constexpr size_t PC_DD = 2 * (size_t)N;
+constexpr size_t PC_DF = PC_DD + DD_SZ;
+constexpr size_t PC_GROUP = PC_DF + 2 * (size_t)D_EFF * N;  // 8198*N words = 67.16 MB
 
 // Twiddle tables in device memory: tw[limb*4*N + {0,1,2,3}*N + i] =
 // fwd, fwd Shoup companion, inv, inv Shoup companion.

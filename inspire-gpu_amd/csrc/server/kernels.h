@@ -56,16 +56,17 @@ hipError_t launch_dbpass_mfma(const uint16_t* db, const double* bq, size_t nq, i
 hipError_t launch_reduce_slices(const uint32_t* part, size_t part_stride, uint32_t S, size_t C, uint32_t* interm,
                                 size_t nq, hipStream_t s);
 
-// Number of step chunks the collapse pass splits the 1023 steps into.
-uint32_t collapse_nch(size_t G);
-// Precomp-tensor pass: partial collapse sums for `nq` queries (<= 4) whose
-// NTT-form keys sit at keys + q*keys_stride. out part2 [nq][nch][G][2][N].
-hipError_t launch_collapse_partial(const uint32_t* precomp, size_t G, const uint32_t* keys,
-                                   size_t keys_stride, size_t nq, uint32_t* part2, uint32_t nch,
-                                   hipStream_t s);
+// Queries one collapse pass serves.
+constexpr size_t kCollapseMaxQueries = 16;
+// Precomp-tensor pass in dot-product form (int8 MFMA, Dd read once) for nq
+// queries whose NTT-form keys sit at keys + q*kstride:
+// part2[q][g][limb][N] at q*p2_stride. kdig (512 KiB) and KS (2*nq words)
+// are scratch.
+hipError_t launch_collapse(const uint32_t* precomp, size_t G, const uint32_t* keys, size_t kstride, size_t nq,
+                           int8_t* kdig, uint32_t* KS, uint32_t* part2, size_t p2_stride, hipStream_t s);
 // packed_b[q][g] = NTT(interm[q] group slice) - collapse partials - Df*Kn.
 hipError_t launch_pack_finalize(const uint32_t* interm, size_t C, const uint32_t* part2, size_t part2_stride,
-                                uint32_t nch, const uint32_t* precomp, size_t G, const uint32_t* keys,
+                                const uint32_t* precomp, size_t G, const uint32_t* keys,
                                 size_t keys_stride, uint32_t* packed_b, size_t packed_stride, size_t nq,
                                 const NttConst& nc, hipStream_t s);
 // Horner over each response ciphertext; writes the flat u64 response
@@ -84,10 +85,15 @@ hipError_t launch_aside(const uint16_t* db, size_t R, size_t C, size_t g0, size_
                         uint32_t* a32T, const NttConst& nc, hipStream_t s);
 // F = NTT along j of a32T (in place), per (g, limb, p).
 hipError_t launch_ring_embed(uint32_t* a32T, size_t ng, const NttConst& nc, hipStream_t s);
-// CollapseA chains (forward and conjugate) for ng groups; writes D+/D- into
-// the precomp blocks and the final chain states into states [ng][2 dir][2][N].
-hipError_t launch_collapse_chain(const uint32_t* F, size_t ng, const uint32_t* ksk5a, uint32_t* precomp_g0,
+// CollapseA chains (forward and conjugate) for ng groups; writes D+/D- in
+// the reference order to dtmp [ng][dir][RD_SZ] and the final chain states
+// into states [ng][2 dir][2][N].
+hipError_t launch_collapse_chain(const uint32_t* F, size_t ng, const uint32_t* ksk5a, uint32_t* dtmp,
                                  uint32_t* states, const NttConst& nc, const CrtConst& cc, hipStream_t s);
+// D+/D- (reference order, group g at Dp/Dm + g*gstride) -> the Dd planes of
+// precomp blocks [0, ng) at precomp_g0.
+hipError_t launch_dd_relayout(const uint32_t* Dp, const uint32_t* Dm, size_t gstride, size_t ng, uint32_t* precomp_g0,
+                              hipStream_t s);
 hipError_t launch_collapse_final(const uint32_t* states, size_t ng, const uint32_t* kneg1a,
                                  uint32_t* precomp_g0, const NttConst& nc, const CrtConst& cc, hipStream_t s);
 

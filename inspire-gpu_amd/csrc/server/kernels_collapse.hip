@@ -1,0 +1,262 @@
+// The precomp-tensor ("collapse") pass in dot-product form, for gfx950.
+//
+// Reference semantics (src/gpu_online.cu:547-632, driven per group by
+// src/gpu_protocol.cu:1225-1283): for query q, group g, limb c, slot i
+//   part[q][g][c][i] = sum_s sum_j D+[s][j][i] K5[j][p+(s,i)] + D-[s][j][i] K5[j][p-(s,i)]
+// where K5 = NTT(ksk5_b) of the query and p+(s,i) = perm_{5^(N/2-2-s)}(i),
+// p-(s,i) = N-1-p+(s,i). Writing slot p as the odd residue u = 2BR(p)+1 mod
+// 2N, p+/- correspond to u = +/-5^(N/2-2-s) * (2BR(i)+1); since 5 has order
+// N/2 mod 2N and -1 is not a power of 5, the 2*(N/2-1) pairs (s, dir) hit
+// distinct slots. So the pass equals sum_{j,p} Dd[j][p][i] K5[j][p] with Dd
+// the D+/- entries scattered to their slots (two slots per i stay zero):
+// per limb a (2N x 2N-row) by N matrix-vector product per query, which a
+// batch turns into one int8 MFMA contraction that reads Dd once.
+#include "kernels.h"
+
+namespace ipx {
+
+namespace {
+
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef int32_t i32x4 __attribute__((ext_vector_type(4)));
+
+// e with 5^e = x mod 2N, for x = 1 mod 4 (bit by bit: 5^(2^k) = 1 + 2^(k+2)
+// mod 2^(k+3), so the k-th bit is set iff x*5^-e is not 1 mod 2^(k+3)).
+__device__ __forceinline__ uint32_t dlog5(uint32_t x) {
+    // 5^-(2^k) mod 2^12, k = 0..9
+    const uint32_t inv[10] = {3277u, 3113u, 3729u, 3617u, 65u, 129u, 257u, 513u, 1025u, 2049u};
+    uint32_t e = 0;
+#pragma unroll
+    for (int k = 0; k < 10; k++) {
+        if ((x & ((8u << k) - 1)) != 1u) {
+            x = (x * inv[k]) & (2 * N - 1);
+            e |= 1u << k;
+        }
+    }
+    return e;
+}
+
+// v^-1 mod 2N for odd v (Newton: each step doubles the correct low bits).
+__device__ __forceinline__ uint32_t inv_odd(uint32_t v) {
+    uint32_t x = v;  // correct mod 8
+#pragma unroll
+    for (int k = 0; k < 3; k++) x = x * (2u - v * x);
+    return x & (2 * N - 1);
+}
+
+// ------------------------------------------------------------ preprocess relayout
+// Block (group, limb, digit j, 16-slot tile of i): stage D+/-[.][j][i-tile]
+// (2 x 1023 steps x 16 slots) in LDS, then emit the Dd rows (j, p) of the
+// tile as the four byte planes. Sources in the reference order
+// [limb][step][digit][coeff] at Dp/Dm + group*gstride.
+__global__ __launch_bounds__(1024) void k_dd_relayout(const uint32_t* __restrict__ Dp, const uint32_t* __restrict__ Dm,
+                                                      size_t gstride, uint32_t* __restrict__ pc0) {
+    __shared__ uint32_t T[2][NSTEPS][16];
+    const uint32_t it = blockIdx.x % (N / 16), rest = blockIdx.x / (N / 16);
+    const uint32_t j = rest % D_EFF, c = (rest / D_EFF) % 2, gl = rest / (2 * D_EFF);
+    const uint32_t i0 = it * 16, tid = threadIdx.x;
+    for (uint32_t x = tid; x < 2 * NSTEPS * 4; x += 1024) {
+        const uint32_t row = x >> 2, qd = x & 3, dir = row / NSTEPS, s = row % NSTEPS;
+        const uint32_t* src = (dir ? Dm : Dp) + gl * gstride + (((size_t)c * NSTEPS + s) * D_EFF + j) * N + i0 + qd * 4;
+        const u32x4 v = *reinterpret_cast<const u32x4*>(src);
+        T[dir][s][qd * 4 + 0] = v[0];
+        T[dir][s][qd * 4 + 1] = v[1];
+        T[dir][s][qd * 4 + 2] = v[2];
+        T[dir][s][qd * 4 + 3] = v[3];
+    }
+    __syncthreads();
+    uint8_t* dd = reinterpret_cast<uint8_t*>(pc0 + (size_t)gl * PC_GROUP + PC_DD);
+    for (uint32_t x = tid; x < 16 * (N / 16); x += 1024) {
+        const uint32_t il = x & 15, pb = x >> 4, i = i0 + il;
+        const uint32_t vinv = inv_odd(2 * brev11(i) + 1);
+        uint32_t pl[4][4];
+#pragma unroll
+        for (int e = 0; e < 16; e++) {
+            const uint32_t p = pb * 16 + e;
+            const uint32_t w = ((2 * brev11(p) + 1) * vinv) & (2 * N - 1);
+            const uint32_t neg = (w & 3) == 3;
+            const uint32_t e5 = dlog5(neg ? 2 * N - w : w);
+            const uint32_t v = e5 == HALF - 1 ? 0u : T[neg][HALF - 2 - e5][il];
+            const uint32_t b[4] = {(v & 255u) ^ 128u, ((v >> 8) & 255u) ^ 128u, ((v >> 16) & 255u) ^ 128u, v >> 24};
+#pragma unroll
+            for (int a = 0; a < 4; a++) {
+                if ((e & 3) == 0) pl[a][e >> 2] = 0;
+                pl[a][e >> 2] |= b[a] << (8 * (e & 3));
+            }
+        }
+#pragma unroll
+        for (int a = 0; a < 4; a++) {
+            u32x4* dst = reinterpret_cast<u32x4*>(
+                dd + ((((size_t)c * 4 + a) * (DD_K / 16) + (size_t)j * (N / 16) + pb) * N + i) * 16);
+            *dst = u32x4{pl[a][0], pl[a][1], pl[a][2], pl[a][3]};
+        }
+    }
+}
+
+// ------------------------------------------------------------ online
+// Key digits: kdig[c][kb][Mp][16 bytes], row m = q*4 + b holding digit b
+// (balanced base 256, as for the database pass) of K5_q[j][c][p] at
+// k = j*N + p = 16*kb + byte. Padding rows (q >= nq) are zero.
+__global__ __launch_bounds__(256) void k_kdigits(const uint32_t* __restrict__ keys, size_t kstride, uint32_t nq,
+                                                 uint32_t Mp, int8_t* __restrict__ kdig) {
+    const uint32_t nkb = DD_K / 16, nqp = Mp / 4;
+    const uint32_t x = blockIdx.x * 256 + threadIdx.x;
+    if (x >= 2 * nkb * nqp) return;
+    const uint32_t kb = x % nkb, q = (x / nkb) % nqp, c = x / (nkb * nqp);
+    u32x4 o[4];
+#pragma unroll
+    for (int b = 0; b < 4; b++) o[b] = u32x4{0, 0, 0, 0};
+    if (q < nq) {
+        const uint32_t k0 = kb * 16, j = k0 / N, p0 = k0 % N;
+        const uint32_t* K = keys + (size_t)q * kstride + ((size_t)j * 2 + c) * N + p0;
+#pragma unroll
+        for (int e = 0; e < 16; e++) {
+            int32_t v = (int32_t)K[e];
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                int32_t d = v;
+                if (b < 3) {
+                    d = ((v & 255) ^ 128) - 128;
+                    v = (v - d) >> 8;
+                }
+                o[b][e >> 2] |= ((uint32_t)d & 255u) << (8 * (e & 3));
+            }
+        }
+    }
+    u32x4* dst = reinterpret_cast<u32x4*>(kdig + (((size_t)c * nkb + kb) * Mp + (size_t)q * 4) * 16);
+#pragma unroll
+    for (int b = 0; b < 4; b++) dst[b] = o[b];
+}
+
+// KS[q*2 + c] = sum_{j,p} K5_q[j][c][p] mod q_c (the plane-offset correction).
+__global__ __launch_bounds__(256) void k_ksum(const uint32_t* __restrict__ keys, size_t kstride,
+                                              uint32_t* __restrict__ KS) {
+    __shared__ uint64_t red[256];
+    const uint32_t q = blockIdx.x >> 1, c = blockIdx.x & 1;
+    uint64_t a = 0;
+    for (uint32_t x = threadIdx.x; x < DD_K; x += 256)
+        a += keys[(size_t)q * kstride + ((size_t)(x / N) * 2 + c) * N + x % N];
+    red[threadIdx.x] = a;
+    __syncthreads();
+    for (uint32_t s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) KS[blockIdx.x] = red64(red[0], (int)c);
+}
+
+// Block (128-slot tile of i, limb, group), 4 waves of 32 slots (2 N tiles)
+// each, all M rows (MT tiles of 4 queries x 4 digits). Per 64-row K step a
+// lane loads its 16-byte operand of each plane straight from HBM (a step
+// ahead) and the key digits (L2-resident). With Dd entry = sum_a 256^a x_a +
+// 128*65793 (x_a the signed planes, x_3 the top byte) and K = sum_b 256^b d_b:
+//   sum_k Dd_k K_k = sum_{a,b} 256^(a+b) A_ab + 128*65793*sum_k K_k,
+// A_ab = sum_k x_a d_b over 4096 rows: |A_ab| <= 2^26, exact in i32.
+template <int MT>
+__global__ __launch_bounds__(256) void k_collapse_mfma(const uint32_t* __restrict__ precomp,
+                                                       const int8_t* __restrict__ kdig,
+                                                       const uint32_t* __restrict__ KS,
+                                                       uint32_t* __restrict__ part2, size_t p2_stride,
+                                                       uint32_t nq) {
+    constexpr uint32_t Mp = 16 * MT;
+    constexpr uint32_t NKB = DD_K / 16;
+    const uint32_t w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const uint32_t it = blockIdx.x, c = blockIdx.y, g = blockIdx.z;
+    const uint32_t col0 = it * 128 + w * 32;
+    const i32x4* dd = reinterpret_cast<const i32x4*>(precomp + (size_t)g * PC_GROUP + PC_DD) +
+                      (size_t)c * 4 * NKB * N + (size_t)(l >> 4) * N + col0 + (l & 15);
+    const size_t pstride = (size_t)NKB * N;  // plane, in 16-byte units
+    const size_t bstep = (size_t)4 * N;      // 4 blocks of 16 rows
+    const i32x4* asrc = reinterpret_cast<const i32x4*>(kdig) + ((size_t)c * NKB + (l >> 4)) * Mp + (l & 15);
+    const size_t astep = (size_t)4 * Mp;
+    i32x4 acc[MT][2][4];
+#pragma unroll
+    for (int a = 0; a < MT; a++)
+#pragma unroll
+        for (int n = 0; n < 2; n++)
+#pragma unroll
+            for (int pl = 0; pl < 4; pl++) acc[a][n][pl] = i32x4{0, 0, 0, 0};
+    i32x4 B[2][2][4], A[2][MT];
+    auto load = [&](int bi, uint32_t ks) {
+#pragma unroll
+        for (int n = 0; n < 2; n++)
+#pragma unroll
+            for (int pl = 0; pl < 4; pl++)
+                B[bi][n][pl] = __builtin_nontemporal_load(dd + pl * pstride + ks * bstep + n * 16);
+#pragma unroll
+        for (int a = 0; a < MT; a++) A[bi][a] = asrc[ks * astep + a * 16];
+    };
+    auto compute = [&](int bi) {
+#pragma unroll
+        for (int n = 0; n < 2; n++)
+#pragma unroll
+            for (int pl = 0; pl < 4; pl++)
+#pragma unroll
+                for (int a = 0; a < MT; a++)
+                    acc[a][n][pl] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[bi][a], B[bi][n][pl], acc[a][n][pl], 0, 0, 0);
+    };
+    constexpr uint32_t nks = DD_K / 64;  // even
+    load(0, 0);
+    for (uint32_t ks = 0; ks < nks; ks += 2) {
+        load(1, ks + 1);
+        compute(0);
+        if (ks + 2 < nks) load(0, ks + 2);
+        compute(1);
+    }
+    const uint64_t qc = c ? Q1 : Q0;
+    uint64_t wk[7];  // 256^k mod q_c
+    wk[0] = 1;
+#pragma unroll
+    for (int k = 1; k < 7; k++) wk[k] = (wk[k - 1] * 256) % qc;
+    const uint64_t off = qc * ((1ull << 59) / qc);
+#pragma unroll
+    for (int a = 0; a < MT; a++) {
+        const uint32_t q = a * 4 + (l >> 4);
+        if (q >= nq) continue;
+        const int64_t corr = (int64_t)8421504 * KS[q * 2 + c];  // 128*65793
+#pragma unroll
+        for (int n = 0; n < 2; n++) {
+            int64_t lv[7] = {0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+            for (int pl = 0; pl < 4; pl++)
+#pragma unroll
+                for (int b = 0; b < 4; b++) lv[pl + b] += acc[a][n][pl][b];
+            int64_t s = corr;
+#pragma unroll
+            for (int k = 0; k < 7; k++) s += (int64_t)wk[k] * lv[k];  // |s| < 2^58
+            const uint32_t i = col0 + n * 16 + (l & 15);
+            part2[(size_t)q * p2_stride + (size_t)g * 2 * N + (size_t)c * N + i] = red64((uint64_t)(s + (int64_t)off), (int)c);
+        }
+    }
+}
+
+}  // namespace
+
+hipError_t launch_dd_relayout(const uint32_t* Dp, const uint32_t* Dm, size_t gstride, size_t ng, uint32_t* precomp_g0,
+                              hipStream_t s) {
+    if (ng == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_dd_relayout, dim3((unsigned)(ng * 2 * D_EFF * (N / 16))), dim3(1024), 0, s, Dp, Dm, gstride,
+                       precomp_g0);
+    return hipGetLastError();
+}
+
+hipError_t launch_collapse(const uint32_t* precomp, size_t G, const uint32_t* keys, size_t kstride, size_t nq,
+                           int8_t* kdig, uint32_t* KS, uint32_t* part2, size_t p2_stride, hipStream_t s) {
+    if (nq == 0 || nq > kCollapseMaxQueries) return hipErrorInvalidValue;
+    const uint32_t mt = nq <= 4 ? 1 : nq <= 8 ? 2 : 4;
+    const uint32_t Mp = 16 * mt;
+    const size_t nthr = 2 * (DD_K / 16) * (Mp / 4);
+    hipLaunchKernelGGL(k_kdigits, dim3((unsigned)((nthr + 255) / 256)), dim3(256), 0, s, keys, kstride, (uint32_t)nq,
+                       Mp, kdig);
+    hipLaunchKernelGGL(k_ksum, dim3((unsigned)(2 * nq)), dim3(256), 0, s, keys, kstride, KS);
+    const dim3 grid(N / 128, 2, (unsigned)G);
+    if (mt == 1)
+        hipLaunchKernelGGL(k_collapse_mfma<1>, grid, dim3(256), 0, s, precomp, kdig, KS, part2, p2_stride, (uint32_t)nq);
+    else if (mt == 2)
+        hipLaunchKernelGGL(k_collapse_mfma<2>, grid, dim3(256), 0, s, precomp, kdig, KS, part2, p2_stride, (uint32_t)nq);
+    else
+        hipLaunchKernelGGL(k_collapse_mfma<4>, grid, dim3(256), 0, s, precomp, kdig, KS, part2, p2_stride, (uint32_t)nq);
+    return hipGetLastError();
+}
+
+}  // namespace ipx

@@ -1,6 +1,7 @@
 // Online (per-query) kernels for gfx950: query unmarshal, batched NTT, the
-// plain database pass, the precomp-tensor (collapse) pass, pack finalize,
-// and the persistent Horner/external-product chain.
+// row-slice reduction of the database pass, pack finalize, and the
+// persistent Horner/external-product chain (the database and collapse
+// passes live in kernels_db.hip and kernels_collapse.hip).
 //
 // Semantics follow the reference's online path (src/gpu_protocol.cu:
 // 1121-1339): mat-vec interm_c = DB^T b_c mod q_c (src/gpu_online.cu:23-41);
@@ -75,58 +76,14 @@ __global__ __launch_bounds__(256) void k_reduce_slices(const uint32_t* __restric
     *reinterpret_cast<u32x4*>(interm + (size_t)q * 2 * C + x) = o;
 }
 
-// ------------------------------------------------------------ collapse pass
-// For queries q < nq, group g, limb c, coefficient i:
-//   sum_{s in chunk} sum_j D+[s][j][i]*K5[j][pf] + D-[s][j][i]*K5[j][pc]
-// with pf = perm_{5^(N/2-2-s)}(i), pc = N-1-pf (reference
-// src/gpu_online.cu:593-632 with the permutation of
-// src/gpu_protocol.cu:193-214 computed in closed form). Each D element is
-// loaded once and applied to every query of the launch.
-template <int BT>
-__global__ __launch_bounds__(256) void k_collapse_partial(const uint32_t* __restrict__ precomp,
-                                                          const uint32_t* __restrict__ keys, size_t kstride,
-                                                          uint32_t nq, uint32_t* __restrict__ part2, uint32_t G,
-                                                          uint32_t nch, uint32_t spc) {
-    __shared__ uint32_t K[BT][D_EFF][N];
-    const uint32_t itile = blockIdx.x & 7, ch = blockIdx.x >> 3, c = blockIdx.y, g = blockIdx.z;
-    for (uint32_t x = threadIdx.x; x < BT * D_EFF * N; x += 256) {
-        const uint32_t bt = x / (D_EFF * N), j = (x / N) % D_EFF, i = x % N;
-        K[bt][j][i] = bt < nq ? keys[bt * kstride + (j * 2 + c) * N + i] : 0u;
-    }
-    __syncthreads();
-    const uint32_t i = itile * 256 + threadIdx.x;
-    const uint32_t v = 2 * brev11(i) + 1;
-    const uint32_t s0 = ch * spc;
-    const uint32_t s1 = s0 + spc < NSTEPS ? s0 + spc : NSTEPS;
-    // t = 5^(N/2-2-s0) mod 2N; successive steps multiply by 5^-1 = 3277 mod 2N
-    uint32_t t = 1;
-    for (uint32_t e = HALF - 2 - s0, b = 5; e; e >>= 1, b = (b * b) & (2 * N - 1))
-        if (e & 1) t = (t * b) & (2 * N - 1);
-    const uint32_t* Dp = precomp + (size_t)g * PC_GROUP + PC_DP + (size_t)c * NSTEPS * D_EFF * N + i;
-    const uint32_t* Dm = precomp + (size_t)g * PC_GROUP + PC_DM + (size_t)c * NSTEPS * D_EFF * N + i;
-    uint64_t acc[BT];
-#pragma unroll
-    for (int b = 0; b < BT; b++) acc[b] = 0;
-    for (uint32_t s = s0; s < s1; s++) {
-        const uint32_t pf = perm_slot(t, v), pc = N - 1 - pf;
-        const size_t o = (size_t)s * D_EFF * N;
-        const uint64_t d0 = __builtin_nontemporal_load(Dp + o), d1 = __builtin_nontemporal_load(Dp + o + N);
-        const uint64_t e0 = __builtin_nontemporal_load(Dm + o), e1 = __builtin_nontemporal_load(Dm + o + N);
-#pragma unroll
-        for (int b = 0; b < BT; b++)
-            acc[b] += d0 * K[b][0][pf] + d1 * K[b][1][pf] + e0 * K[b][0][pc] + e1 * K[b][1][pc];
-        t = (t * 3277u) & (2 * N - 1);
-    }
-#pragma unroll
-    for (int b = 0; b < BT; b++)
-        if ((uint32_t)b < nq) part2[(((size_t)b * nch + ch) * G + g) * 2 * N + (size_t)c * N + i] = red64(acc[b], (int)c);
-}
-
 // ------------------------------------------------------------ pack finalize
+// packed_b[q][g][c] = NTT(interm[q][c][gN..gN+N-1]) - part2[q][g][c]
+//                     - sum_j Df[g][c][j] * Kn[j][c]  (reference
+// src/gpu_protocol.cu:1225-1283, src/gpu_online.cu:547-592).
 __global__ __launch_bounds__(256) void k_pack_finalize(const uint32_t* __restrict__ interm, uint32_t C,
                                                        const uint32_t* __restrict__ part2,
-                                                       size_t p2_stride, uint32_t nch,
-                                                       const uint32_t* __restrict__ precomp, uint32_t G,
+                                                       size_t p2_stride,
+                                                       const uint32_t* __restrict__ precomp,
                                                        const uint32_t* __restrict__ keys, size_t kstride,
                                                        uint32_t* __restrict__ packed_b, size_t pstride,
                                                        NttConst nc) {
@@ -144,8 +101,7 @@ __global__ __launch_bounds__(256) void k_pack_finalize(const uint32_t* __restric
 #pragma unroll
     for (int k = 0; k < 8; k++) {
         const uint32_t i = t + 256 * k;
-        uint64_t sub = (uint64_t)Df[i] * Kn[i] + (uint64_t)Df[N + i] * Kn[2 * N + i];
-        for (uint32_t ch = 0; ch < nch; ch++) sub += p2[(size_t)ch * G * 2 * N + i];
+        const uint64_t sub = (uint64_t)Df[i] * Kn[i] + (uint64_t)Df[N + i] * Kn[2 * N + i] + p2[i];
         packed_b[qi * pstride + (size_t)g * 2 * N + (size_t)c * N + i] = sub_mod(buf[ls(i)], red64(sub, (int)c), q);
     }
 }
@@ -274,36 +230,12 @@ hipError_t launch_reduce_slices(const uint32_t* part, size_t part_stride, uint32
     return hipGetLastError();
 }
 
-static uint32_t collapse_chunks(size_t G) {
-    uint32_t nch = 4;  // <= 256 steps per chunk keeps the u64 sums below 2^63
-    while (nch < 64 && (size_t)16 * nch * G < 4096) nch *= 2;
-    return nch;
-}
-
-hipError_t launch_collapse_partial(const uint32_t* precomp, size_t G, const uint32_t* keys, size_t kstride,
-                                   size_t nq, uint32_t* part2, uint32_t nch, hipStream_t s) {
-    const uint32_t spc = (NSTEPS + nch - 1) / nch;
-    dim3 grid(8 * nch, 2, (unsigned)G);
-    if (nq == 1)
-        hipLaunchKernelGGL(k_collapse_partial<1>, grid, dim3(256), 0, s, precomp, keys, kstride, (uint32_t)nq,
-                           part2, (uint32_t)G, nch, spc);
-    else if (nq == 2)
-        hipLaunchKernelGGL(k_collapse_partial<2>, grid, dim3(256), 0, s, precomp, keys, kstride, (uint32_t)nq,
-                           part2, (uint32_t)G, nch, spc);
-    else
-        hipLaunchKernelGGL(k_collapse_partial<4>, grid, dim3(256), 0, s, precomp, keys, kstride, (uint32_t)nq,
-                           part2, (uint32_t)G, nch, spc);
-    return hipGetLastError();
-}
-
-uint32_t collapse_nch(size_t G) { return collapse_chunks(G); }
-
 hipError_t launch_pack_finalize(const uint32_t* interm, size_t C, const uint32_t* part2, size_t part2_stride,
-                                uint32_t nch, const uint32_t* precomp, size_t G, const uint32_t* keys,
+                                const uint32_t* precomp, size_t G, const uint32_t* keys,
                                 size_t keys_stride, uint32_t* packed_b, size_t packed_stride, size_t nq,
                                 const NttConst& nc, hipStream_t s) {
     hipLaunchKernelGGL(k_pack_finalize, dim3((unsigned)G, 2, (unsigned)nq), dim3(256), 0, s, interm, (uint32_t)C,
-                       part2, part2_stride, nch, precomp, (uint32_t)G, keys, keys_stride, packed_b, packed_stride, nc);
+                       part2, part2_stride, precomp, keys, keys_stride, packed_b, packed_stride, nc);
     return hipGetLastError();
 }
 

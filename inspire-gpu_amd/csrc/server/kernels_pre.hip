@@ -181,7 +181,7 @@ __global__ __launch_bounds__(256) void k_ntt_rows(uint32_t* __restrict__ rows, N
 // are gathered through the same slot permutation p.
 __global__ __launch_bounds__(1024) void k_collapse_chain(const uint32_t* __restrict__ F,
                                                          const uint32_t* __restrict__ ksk5a,
-                                                         uint32_t* __restrict__ precomp_g0,
+                                                         uint32_t* __restrict__ dtmp,
                                                          uint32_t* __restrict__ states, NttConst nc, CrtConst cc) {
     __shared__ uint32_t buf[4][POLY_LDS];
     __shared__ uint32_t K[4][N];  // [digit*2 + limb][N]
@@ -193,7 +193,7 @@ __global__ __launch_bounds__(1024) void k_collapse_chain(const uint32_t* __restr
     const Tw tw = tw_lds(twl, tlimb);
     const uint32_t inv_n = nc.inv_n[tlimb], inv_n_p = nc.inv_n_p[tlimb];
     const uint32_t* Fg = F + (size_t)gl * 2 * N * N;
-    uint32_t* Dout = precomp_g0 + (size_t)gl * PC_GROUP + (dir ? PC_DM : PC_DP);
+    uint32_t* Dout = dtmp + ((size_t)gl * 2 + dir) * RD_SZ;
     uint32_t v[2], st[2][2];  // [e], [limb][e]
 #pragma unroll
     for (int e = 0; e < 2; e++) v[e] = 2 * brev11(tid + 1024 * e) + 1;
@@ -346,10 +346,9 @@ hipError_t launch_ring_embed(uint32_t* a32T, size_t ng, const NttConst& nc, hipS
     return hipGetLastError();
 }
 
-hipError_t launch_collapse_chain(const uint32_t* F, size_t ng, const uint32_t* ksk5a, uint32_t* precomp_g0,
+hipError_t launch_collapse_chain(const uint32_t* F, size_t ng, const uint32_t* ksk5a, uint32_t* dtmp,
                                  uint32_t* states, const NttConst& nc, const CrtConst& cc, hipStream_t s) {
-    hipLaunchKernelGGL(k_collapse_chain, dim3((unsigned)ng, 2), dim3(1024), 0, s, F, ksk5a, precomp_g0, states, nc,
-                       cc);
+    hipLaunchKernelGGL(k_collapse_chain, dim3((unsigned)ng, 2), dim3(1024), 0, s, F, ksk5a, dtmp, states, nc, cc);
     return hipGetLastError();
 }
 

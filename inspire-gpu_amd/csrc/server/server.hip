@@ -83,8 +83,11 @@ PreprocessData gpu_preprocess(const PublicParams& pp, const uint16_t* db) {
     uint16_t* d_stage = nullptr;
     uint32_t* arena = nullptr;
     uint32_t *d_crs = nullptr, *d_k5 = nullptr, *d_kn = nullptr, *d_a32 = nullptr, *d_states = nullptr;
+    uint32_t* d_dtmp = nullptr;
     auto cleanup = [&](bool all) {
         dfree(d_tw); dfree(d_crs); dfree(d_k5); dfree(d_kn); dfree(d_a32); dfree(d_states); dfree(d_stage);
+        dfree(d_dtmp);
+        d_dtmp = nullptr;
         d_stage = nullptr;
         if (all) { dfree(d_db); dfree(arena); }
         if (s) (void)hipStreamDestroy(s);
@@ -136,18 +139,22 @@ PreprocessData gpu_preprocess(const PublicParams& pp, const uint16_t* db) {
         HIPX(hipMemcpy(d_kn, h_kn.data(), h_kn.size() * 4, hipMemcpyHostToDevice));
 
         arena = dalloc<uint32_t>(G * PC_GROUP);
-        // groups are processed in chunks bounded by the transient ring-embed
-        // table (32 MiB per group)
-        const size_t chunk = G < 256 ? G : 256;
+        // Groups go in chunks of up to 128 (two chains per group fill the
+        // CUs), bounded by the transient ring-embed table (32 MiB per group)
+        // and the chains' D+/D- in the reference order (64 MiB per group),
+        // which the relayout then turns into the resident dot-product form.
+        const size_t chunk = G < 128 ? G : 128;
         d_a32 = dalloc<uint32_t>(chunk * 2 * (size_t)N * N);
         d_states = dalloc<uint32_t>(chunk * 4 * (size_t)N);
+        d_dtmp = dalloc<uint32_t>(chunk * 2 * RD_SZ);
         for (size_t g0 = 0; g0 < G; g0 += chunk) {
             const size_t ng = G - g0 < chunk ? G - g0 : chunk;
             uint32_t* pc = arena + g0 * PC_GROUP;
             HIPX(launch_aside(d_db, R, C, g0, ng, d_crs, d_a32, nc, s));
             HIPX(launch_ring_embed(d_a32, ng, nc, s));
-            HIPX(launch_collapse_chain(d_a32, ng, d_k5, pc, d_states, nc, cc, s));
+            HIPX(launch_collapse_chain(d_a32, ng, d_k5, d_dtmp, d_states, nc, cc, s));
             HIPX(launch_collapse_final(d_states, ng, d_kn, pc, nc, cc, s));
+            HIPX(launch_dd_relayout(d_dtmp, d_dtmp + RD_SZ, 2 * RD_SZ, ng, pc, s));
         }
         HIPX(hipStreamSynchronize(s));
         cleanup(false);
@@ -165,8 +172,8 @@ PreprocessData gpu_preprocess(const PublicParams& pp, const uint16_t* db) {
     for (size_t g = 0; g < G; g++) {
         uint32_t* b = arena + g * PC_GROUP;
         pd.d_precomp_a[g] = b + PC_A;
-        pd.d_precomp_D_plus[g] = b + PC_DP;
-        pd.d_precomp_D_minus[g] = b + PC_DM;
+        pd.d_precomp_D_plus[g] = b + PC_DD;
+        pd.d_precomp_D_minus[g] = nullptr;
         pd.d_precomp_D_final[g] = b + PC_DF;
     }
     return pd;
@@ -197,14 +204,22 @@ GpuServerCtx* gpu_setup_server(const PublicParams& pp, const PreprocessData& pre
         if (precomp.d_precomp_arena) {
             ctx->d_precomp = static_cast<uint32_t*>(precomp.d_precomp_arena);
         } else {
-            // per-group buffers from elsewhere: gather them into one arena
+            // per-group buffers in the reference layout: gather a / D_final
+            // into one arena and convert D+/D- into the dot-product form
             ctx->d_precomp = dalloc<uint32_t>(G * PC_GROUP);
+            for (size_t g = 0; g < G; g++)
+                if (!precomp.d_precomp_a[g] || !precomp.d_precomp_D_plus[g] || !precomp.d_precomp_D_minus[g] ||
+                    !precomp.d_precomp_D_final[g])
+                    throw std::invalid_argument("gpu_setup_server: missing per-group precomp buffer");
             for (size_t g = 0; g < G; g++) {
                 uint32_t* b = ctx->d_precomp + g * PC_GROUP;
                 HIPX(hipMemcpy(b + PC_A, precomp.d_precomp_a[g], 2 * N * 4, hipMemcpyDeviceToDevice));
-                HIPX(hipMemcpy(b + PC_DP, precomp.d_precomp_D_plus[g], PC_DSZ * 4, hipMemcpyDeviceToDevice));
-                HIPX(hipMemcpy(b + PC_DM, precomp.d_precomp_D_minus[g], PC_DSZ * 4, hipMemcpyDeviceToDevice));
                 HIPX(hipMemcpy(b + PC_DF, precomp.d_precomp_D_final[g], 2 * D_EFF * N * 4, hipMemcpyDeviceToDevice));
+                HIPX(launch_dd_relayout(precomp.d_precomp_D_plus[g], precomp.d_precomp_D_minus[g], 0, 1, b,
+                                        ctx->stream));
+            }
+            HIPX(hipStreamSynchronize(ctx->stream));
+            for (size_t g = 0; g < G; g++) {
                 dfree(precomp.d_precomp_a[g]); dfree(precomp.d_precomp_D_plus[g]);
                 dfree(precomp.d_precomp_D_minus[g]); dfree(precomp.d_precomp_D_final[g]);
             }
@@ -251,11 +266,10 @@ void GpuServerCtx::alloc_slots() {
             const DbPlan x = dbplan_mfma(R, C, nq, t);
             if (x.slices > max_slices) max_slices = x.slices;
         }
-    nch = collapse_nch(G);
     lay.R = R; lay.C = C; lay.G = G;
     qlen = lay.query_u64s();
     part_stride = max_slices * 2 * C;
-    part2_stride = (size_t)nch * G * 2 * N;
+    part2_stride = G * 2 * N;
     keys_stride = lay.keys_words();
     rgsw_stride = lay.rgsw_words();
     packed_stride = G * 2 * N;
@@ -272,6 +286,8 @@ void GpuServerCtx::alloc_slots() {
     for (int l = 0; l < kLanes; l++) {
         d_dig[l] = dalloc<int8_t>(R * 256);
         d_S[l] = dalloc<uint32_t>(64);
+        d_kdig[l] = dalloc<int8_t>(kKdigBytes);
+        d_KS[l] = dalloc<uint32_t>(64);
     }
     HIPX(hipMemset(d_bq, 0, B * 2 * R * sizeof(double)));
     HIPX(hipHostMalloc((void**)&h_q, B * qlen * 8, hipHostMallocDefault));
@@ -279,7 +295,7 @@ void GpuServerCtx::alloc_slots() {
     resident_bytes = R * C * 2 + G * PC_GROUP * 4 + 8 * N * 4 +
                      B * (qlen * 8 + resp_u64s * 8 + 2 * R * 8 +
                           (keys_stride + rgsw_stride + part_stride + 2 * C + part2_stride + packed_stride) * 4) +
-                     kLanes * (R * 256 + 64 * 4);
+                     kLanes * (R * 256 + kKdigBytes + 128 * 4);
     autotune_dbpass();
 }
 
@@ -367,17 +383,17 @@ void GpuServerCtx::stage_dbpass(size_t count, hipStream_t s, size_t slot0, int l
     }
 }
 
-void GpuServerCtx::stage_collapse(size_t count, hipStream_t s, size_t slot0) {
-    for (size_t q0 = slot0; q0 < slot0 + count; q0 += 4) {
-        const size_t nq = slot0 + count - q0 < 4 ? slot0 + count - q0 : 4;
-        HIPX(launch_collapse_partial(d_precomp, G, d_keys + q0 * keys_stride, keys_stride, nq,
-                                     d_part2 + q0 * part2_stride, nch, s));
+void GpuServerCtx::stage_collapse(size_t count, hipStream_t s, size_t slot0, int lane) {
+    for (size_t q0 = slot0; q0 < slot0 + count; q0 += kCollapseMaxQueries) {
+        const size_t nq = slot0 + count - q0 < kCollapseMaxQueries ? slot0 + count - q0 : kCollapseMaxQueries;
+        HIPX(launch_collapse(d_precomp, G, d_keys + q0 * keys_stride, keys_stride, nq, d_kdig[lane], d_KS[lane],
+                             d_part2 + q0 * part2_stride, part2_stride, s));
     }
 }
 
 void GpuServerCtx::stage_finalize(size_t count, hipStream_t s, size_t slot0) {
     HIPX(launch_pack_finalize(d_interm + slot0 * 2 * pp.db_cols, pp.db_cols, d_part2 + slot0 * part2_stride,
-                              part2_stride, nch, d_precomp, G, d_keys + slot0 * keys_stride, keys_stride,
+                              part2_stride, d_precomp, G, d_keys + slot0 * keys_stride, keys_stride,
                               d_packed_b + slot0 * packed_stride, packed_stride, count, nc, s));
 }
 
@@ -391,7 +407,7 @@ void GpuServerCtx::enqueue(const uint64_t* d_in, size_t count, uint64_t* d_out, 
     if (lane < 0 || lane >= kLanes) throw std::invalid_argument("enqueue: scratch lane out of range");
     stage_prep(d_in, count, s, slot0);
     stage_dbpass(count, s, slot0, lane);
-    stage_collapse(count, s, slot0);
+    stage_collapse(count, s, slot0, lane);
     stage_finalize(count, s, slot0);
     stage_horner(count, d_out, s, slot0);
 }
@@ -476,7 +492,9 @@ void gpu_free_server(GpuServerCtx* ctx) {
     dfree(ctx->d_tw); dfree(ctx->d_db); dfree(ctx->d_precomp);
     dfree(ctx->d_q); dfree(ctx->d_resp); dfree(ctx->d_bq); dfree(ctx->d_keys); dfree(ctx->d_rgsw);
     dfree(ctx->d_part); dfree(ctx->d_interm); dfree(ctx->d_part2); dfree(ctx->d_packed_b);
-    for (int l = 0; l < GpuServerCtx::kLanes; l++) { dfree(ctx->d_dig[l]); dfree(ctx->d_S[l]); }
+    for (int l = 0; l < GpuServerCtx::kLanes; l++) {
+        dfree(ctx->d_dig[l]); dfree(ctx->d_S[l]); dfree(ctx->d_kdig[l]); dfree(ctx->d_KS[l]);
+    }
     if (ctx->h_q) (void)hipHostFree(ctx->h_q);
     if (ctx->h_resp) (void)hipHostFree(ctx->h_resp);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);

@@ -32,7 +32,6 @@ struct GpuServerCtx {
     ipx::SlotLayout lay{};
     ipx::DbPlan plan1{};            // database pass of a single query (autotuned)
     size_t mfma_target = 1024;      // workgroups the batched pass aims for
-    uint32_t nch = 0;
     size_t qlen = 0, resp_u64s = 0;
     size_t part_stride = 0, part2_stride = 0, keys_stride = 0, rgsw_stride = 0, packed_stride = 0;
     uint64_t* d_q = nullptr;        // flat queries (device staging)
@@ -42,11 +41,15 @@ struct GpuServerCtx {
     uint32_t* d_rgsw = nullptr;
     uint32_t* d_part = nullptr;     // DB-pass row-slice partials [B][S][2][C]
     uint32_t* d_interm = nullptr;   // DB-pass result [B][2][C]
-    uint32_t* d_part2 = nullptr;
+    uint32_t* d_part2 = nullptr;    // collapse sums [B][G][2][N]
     // scratch of the batched database pass, one set per concurrent lane
     static constexpr int kLanes = 4;
     int8_t* d_dig[kLanes] = {};     // query digits [R/16][256][16]
     uint32_t* d_S[kLanes] = {};     // digit-offset corrections [64]
+    // scratch of the collapse pass: key digits [2][2N/16][64][16], key sums
+    static constexpr size_t kKdigBytes = 2 * (2 * ipx::N / 16) * 64 * 16;
+    int8_t* d_kdig[kLanes] = {};
+    uint32_t* d_KS[kLanes] = {};
     uint32_t* d_packed_b = nullptr;
     uint64_t* h_q = nullptr;        // pinned host staging
     uint64_t* h_resp = nullptr;
@@ -61,7 +64,7 @@ struct GpuServerCtx {
     void stage_prep(const uint64_t* d_in, size_t count, hipStream_t s, size_t slot0);
     // force: -1 the tuned plans, 0 the fp64 pass per query, 1 the MFMA pass
     void stage_dbpass(size_t count, hipStream_t s, size_t slot0, int lane = 0, int force = -1);
-    void stage_collapse(size_t count, hipStream_t s, size_t slot0);
+    void stage_collapse(size_t count, hipStream_t s, size_t slot0, int lane = 0);
     void stage_finalize(size_t count, hipStream_t s, size_t slot0);
     void stage_horner(size_t count, uint64_t* d_out, hipStream_t s, size_t slot0);
     // Enqueue the answer of `count` flat queries at d_in (device) into d_out
