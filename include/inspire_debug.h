@@ -39,6 +39,11 @@ int ipir_dbg_answer_taps(ipir_server* srv, const uint64_t* query, uint32_t* inte
  * [part][digit][limb][coeff]); out [a|b][limb][N]. */
 int ipir_dbg_ntt(const uint32_t* in, uint32_t* out, size_t npoly, int inverse);
 int ipir_dbg_horner(const uint64_t* cts, size_t L, const uint64_t* rgsw, uint64_t* out);
+/* horner with a chosen kernel: variant 0 = one workgroup per chain, 1 = a
+ * workgroup pair (one per limb). */
+int ipir_dbg_horner_variant(const uint64_t* cts, size_t L, const uint64_t* rgsw, uint64_t* out, int variant);
+/* Select the Horner kernel a server's answers use (mode as variant above). */
+int ipir_dbg_set_horner(ipir_server* srv, int mode);
 /* Per-stage device time (ms) of the last ipir_dbg_profile_batch run:
  * [unmarshal+key NTT, DB pass, collapse pass, pack finalize, Horner]. */
 int ipir_dbg_profile_batch(ipir_server* srv, const uint64_t* d_queries, size_t count, uint64_t* d_out,

@@ -267,7 +267,18 @@ int ipir_dbg_ntt(const uint32_t* in, uint32_t* out, size_t npoly, int inverse) {
 }
 
 int ipir_dbg_horner(const uint64_t* cts, size_t L, const uint64_t* rgsw, uint64_t* out) {
-    if (!cts || !rgsw || !out || L == 0) return 1;
+    return ipir_dbg_horner_variant(cts, L, rgsw, out, 0);
+}
+
+int ipir_dbg_set_horner(ipir_server* srv, int mode) {
+    if (!srv || mode < 0 || mode > 1) return 1;
+    std::lock_guard<std::mutex> lk(srv->ctx->mu);
+    srv->ctx->horner_mode = mode;
+    return 0;
+}
+
+int ipir_dbg_horner_variant(const uint64_t* cts, size_t L, const uint64_t* rgsw, uint64_t* out, int variant) {
+    if (!cts || !rgsw || !out || L == 0 || variant < 0 || variant > 1) return 1;
     Standalone* st = standalone();
     if (!st) return 4;
     const size_t b = blk();
@@ -279,19 +290,29 @@ int ipir_dbg_horner(const uint64_t* cts, size_t L, const uint64_t* rgsw, uint64_
             pb[g * 2 * N + x] = (uint32_t)(cts[g * 4 * N + 2 * N + x] % q);
         }
     for (size_t x = 0; x < 4 * b; x++) rg[x] = (uint32_t)(rgsw[x] % ((x / N) % 2 ? Q1 : Q0));
-    uint32_t *d_pa = nullptr, *d_pb = nullptr, *d_rg = nullptr;
+    uint32_t *d_pa = nullptr, *d_pb = nullptr, *d_rg = nullptr, *d_x = nullptr, *d_f = nullptr, *d_st = nullptr;
     uint64_t* d_o = nullptr;
     int rc = 0;
     if (hipMalloc(&d_pa, pa.size() * 4) || hipMalloc(&d_pb, pb.size() * 4) || hipMalloc(&d_rg, rg.size() * 4) ||
-        hipMalloc(&d_o, 4 * N * 8)) rc = 4;
+        hipMalloc(&d_o, 4 * N * 8) || hipMalloc(&d_x, ipx::horner_pair_xbuf_words(1) * 4) || hipMalloc(&d_f, 8) ||
+        hipMalloc(&d_st, 4) || hipMemset(d_st, 0, 4))
+        rc = 4;
     if (!rc && (hipMemcpy(d_pa, pa.data(), pa.size() * 4, hipMemcpyHostToDevice) ||
                 hipMemcpy(d_pb, pb.data(), pb.size() * 4, hipMemcpyHostToDevice) ||
-                hipMemcpy(d_rg, rg.data(), rg.size() * 4, hipMemcpyHostToDevice) ||
-                ipx::launch_horner(d_pa, 2 * N, L, L, 1, d_pb, L * 2 * N, d_rg, 4 * b, d_o, 1, st->nc, st->cc,
-                                   nullptr) ||
-                hipMemcpy(out, d_o, 4 * N * 8, hipMemcpyDeviceToHost)))
+                hipMemcpy(d_rg, rg.data(), rg.size() * 4, hipMemcpyHostToDevice)))
         rc = 4;
-    dfree(d_pa); dfree(d_pb); dfree(d_rg); dfree(d_o);
+    if (!rc) {
+        const hipError_t e =
+            variant == 1 ? ipx::launch_horner_pair(d_pa, 2 * N, L, L, 1, d_pb, L * 2 * N, d_rg, 4 * b, d_o, 1, d_x, d_f,
+                                                   d_st, GpuServerCtx::kHornerSpinLimit, st->nc, st->cc, nullptr)
+                         : ipx::launch_horner(d_pa, 2 * N, L, L, 1, d_pb, L * 2 * N, d_rg, 4 * b, d_o, 1, st->nc,
+                                              st->cc, nullptr);
+        uint32_t status = 0;
+        if (e || hipMemcpy(out, d_o, 4 * N * 8, hipMemcpyDeviceToHost) ||
+            hipMemcpy(&status, d_st, 4, hipMemcpyDeviceToHost) || status)
+            rc = 4;
+    }
+    dfree(d_pa); dfree(d_pb); dfree(d_rg); dfree(d_o); dfree(d_x); dfree(d_f); dfree(d_st);
     return rc;
 }
 

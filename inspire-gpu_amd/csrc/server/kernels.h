@@ -77,6 +77,18 @@ hipError_t launch_horner(const uint32_t* pa_base, size_t pa_stride, size_t G, si
                          size_t rgsw_stride, uint64_t* out, size_t nq, const NttConst& nc,
                          const CrtConst& cc, hipStream_t s);
 
+// The same chains with two workgroups per chain (one per limb) exchanging
+// coefficient-form values once per step. xbuf holds
+// horner_pair_xbuf_words(nq*T) words and flags 2*nq*T words (reset by the
+// launch); a partner wait longer than spin_limit sleeps sets *status to 1
+// and the chain's output is then undefined.
+size_t horner_pair_xbuf_words(size_t nchains);
+hipError_t launch_horner_pair(const uint32_t* pa_base, size_t pa_stride, size_t G, size_t Da, size_t T,
+                              const uint32_t* packed_b, size_t packed_stride, const uint32_t* rgsw,
+                              size_t rgsw_stride, uint64_t* out, size_t nq, uint32_t* xbuf, uint32_t* flags,
+                              uint32_t* status, uint32_t spin_limit, const NttConst& nc, const CrtConst& cc,
+                              hipStream_t s);
+
 // ---------------------------------------------------------------- preprocess
 hipError_t launch_encode(uint16_t* db, size_t R, size_t C, size_t Da, hipStream_t s);
 // a32T[g][limb][p][j] for groups [g0, g0+ng): transposed a-side matmul

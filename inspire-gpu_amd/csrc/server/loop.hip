@@ -108,7 +108,14 @@ struct ipir_loop {
             p->cv.wait(lk, [&] { return stop || p->launched; });
             if (!p->launched) return;  // stopping with nothing in flight
             lk.unlock();
-            const bool ok = hipStreamSynchronize(p->stream) == hipSuccess;
+            bool ok = hipStreamSynchronize(p->stream) == hipSuccess;
+            if (ok && ctx->horner_mode == 1) {
+                try {
+                    ctx->check_status(p->stream);
+                } catch (...) {
+                    ok = false;
+                }
+            }
             const auto now = Clock::now();
             lk.lock();
             for (size_t i = 0; i < p->batch.size(); i++) {

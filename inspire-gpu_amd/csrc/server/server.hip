@@ -289,13 +289,18 @@ void GpuServerCtx::alloc_slots() {
         d_kdig[l] = dalloc<int8_t>(kKdigBytes);
         d_KS[l] = dalloc<uint32_t>(64);
     }
+    d_hxbuf = dalloc<uint32_t>(horner_pair_xbuf_words(B * T));
+    d_hflags = dalloc<uint32_t>(2 * B * T);
+    d_status = dalloc<uint32_t>(1);
+    HIPX(hipMemset(d_status, 0, 4));
+    if (const char* h = std::getenv("INSPIRE_HORNER")) horner_mode = std::strcmp(h, "pair") == 0 ? 1 : 0;
     HIPX(hipMemset(d_bq, 0, B * 2 * R * sizeof(double)));
     HIPX(hipHostMalloc((void**)&h_q, B * qlen * 8, hipHostMallocDefault));
     HIPX(hipHostMalloc((void**)&h_resp, B * resp_u64s * 8, hipHostMallocDefault));
     resident_bytes = R * C * 2 + G * PC_GROUP * 4 + 8 * N * 4 +
                      B * (qlen * 8 + resp_u64s * 8 + 2 * R * 8 +
                           (keys_stride + rgsw_stride + part_stride + 2 * C + part2_stride + packed_stride) * 4) +
-                     kLanes * (R * 256 + kKdigBytes + 128 * 4);
+                     kLanes * (R * 256 + kKdigBytes + 128 * 4) + (horner_pair_xbuf_words(B * T) + 2 * B * T) * 4;
     autotune_dbpass();
 }
 
@@ -398,8 +403,25 @@ void GpuServerCtx::stage_finalize(size_t count, hipStream_t s, size_t slot0) {
 }
 
 void GpuServerCtx::stage_horner(size_t count, uint64_t* d_out, hipStream_t s, size_t slot0) {
-    HIPX(launch_horner(d_precomp + PC_A, PC_GROUP, G, Da, T, d_packed_b + slot0 * packed_stride, packed_stride,
-                       d_rgsw + slot0 * rgsw_stride, rgsw_stride, d_out, count, nc, cc, s));
+    if (horner_mode == 1)
+        HIPX(launch_horner_pair(d_precomp + PC_A, PC_GROUP, G, Da, T, d_packed_b + slot0 * packed_stride,
+                                packed_stride, d_rgsw + slot0 * rgsw_stride, rgsw_stride, d_out, count,
+                                d_hxbuf + horner_pair_xbuf_words(slot0 * T), d_hflags + 2 * slot0 * T, d_status,
+                                kHornerSpinLimit, nc, cc, s));
+    else
+        HIPX(launch_horner(d_precomp + PC_A, PC_GROUP, G, Da, T, d_packed_b + slot0 * packed_stride, packed_stride,
+                           d_rgsw + slot0 * rgsw_stride, rgsw_stride, d_out, count, nc, cc, s));
+}
+
+void GpuServerCtx::check_status(hipStream_t s) {
+    uint32_t st = 0;
+    HIPX(hipMemcpyAsync(&st, d_status, 4, hipMemcpyDeviceToHost, s));
+    HIPX(hipStreamSynchronize(s));
+    if (st) {
+        HIPX(hipMemsetAsync(d_status, 0, 4, s));
+        HIPX(hipStreamSynchronize(s));
+        throw std::runtime_error("Horner pair exchange exceeded its wait bound; responses of this call are invalid");
+    }
 }
 
 void GpuServerCtx::enqueue(const uint64_t* d_in, size_t count, uint64_t* d_out, hipStream_t s, size_t slot0,
@@ -422,6 +444,7 @@ void GpuServerCtx::answer_host(size_t count, uint64_t* out, hipStream_t s, size_
     enqueue(dq, count, dr, s, slot0);
     HIPX(hipMemcpyAsync(hr, dr, count * resp_u64s * 8, hipMemcpyDeviceToHost, s));
     HIPX(hipStreamSynchronize(s));
+    if (horner_mode == 1) check_status(s);
     if (out) std::memcpy(out, hr, count * resp_u64s * 8);
 }
 
@@ -492,6 +515,7 @@ void gpu_free_server(GpuServerCtx* ctx) {
     dfree(ctx->d_tw); dfree(ctx->d_db); dfree(ctx->d_precomp);
     dfree(ctx->d_q); dfree(ctx->d_resp); dfree(ctx->d_bq); dfree(ctx->d_keys); dfree(ctx->d_rgsw);
     dfree(ctx->d_part); dfree(ctx->d_interm); dfree(ctx->d_part2); dfree(ctx->d_packed_b);
+    dfree(ctx->d_hxbuf); dfree(ctx->d_hflags); dfree(ctx->d_status);
     for (int l = 0; l < GpuServerCtx::kLanes; l++) {
         dfree(ctx->d_dig[l]); dfree(ctx->d_S[l]); dfree(ctx->d_kdig[l]); dfree(ctx->d_KS[l]);
     }

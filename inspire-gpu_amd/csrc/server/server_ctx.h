@@ -51,6 +51,11 @@ struct GpuServerCtx {
     int8_t* d_kdig[kLanes] = {};
     uint32_t* d_KS[kLanes] = {};
     uint32_t* d_packed_b = nullptr;
+    // Horner: 0 = one workgroup per chain, 1 = a workgroup pair (one per limb)
+    int horner_mode = 0;
+    uint32_t* d_hxbuf = nullptr;    // pair exchange buffers [B*T chains]
+    uint32_t* d_hflags = nullptr;   // pair step flags [B*T][2]
+    uint32_t* d_status = nullptr;   // sticky device fault word (pair wait bound exceeded)
     uint64_t* h_q = nullptr;        // pinned host staging
     uint64_t* h_resp = nullptr;
 
@@ -67,6 +72,11 @@ struct GpuServerCtx {
     void stage_collapse(size_t count, hipStream_t s, size_t slot0, int lane = 0);
     void stage_finalize(size_t count, hipStream_t s, size_t slot0);
     void stage_horner(size_t count, uint64_t* d_out, hipStream_t s, size_t slot0);
+    // Throw (and clear) if a kernel reported a fault through d_status.
+    void check_status(hipStream_t s);
+    // ~2 s of 64-cycle sleeps: far above any partner skew, bounded so a
+    // chain whose partner never runs still ends.
+    static constexpr uint32_t kHornerSpinLimit = 1u << 26;
     // Enqueue the answer of `count` flat queries at d_in (device) into d_out
     // (device) on stream s, using slots [slot0, slot0 + count) and scratch
     // set `lane` (work that may overlap on the device uses distinct lanes).

@@ -44,15 +44,17 @@ def test_ntt_kernel_matches_oracle():
         assert np.array_equal(out.astype(np.uint64), want), f"inverse={inv}"
 
 
+@pytest.mark.parametrize("variant", [0, 1])
 @pytest.mark.parametrize("L", [1, 2, 5, 9])
-def test_horner_kernel_matches_oracle(L):
+def test_horner_kernel_matches_oracle(L, variant):
     rng = np.random.default_rng(L)
     rc, geo = ol.setup(262144, 120, 2048)
     q, _ = ol.make_query(geo, SEED, 1000, 3)
     rg = np.ascontiguousarray(q[2 * geo["R"] + 2 * BLK:])
     cts = ol.rand_poly(rng, (L, 2)).reshape(-1)
     out = np.zeros(4 * N, np.uint64)
-    assert ia.load("server").ipir_dbg_horner(ia.ptr(cts), L, ia.ptr(rg), ia.ptr(out)) == 0
+    say(f"horner L={L} variant={variant}")
+    assert ia.load("server").ipir_dbg_horner_variant(ia.ptr(cts), L, ia.ptr(rg), ia.ptr(out), variant) == 0
     want = np.zeros(4 * N, np.uint64)
     ol.oracle().orc_horner(cts, L, rg, want)
     assert np.array_equal(out, want)
@@ -195,6 +197,22 @@ def test_batch_equals_singles_and_extracts(medium8):
     for k in (3, 2):  # partial batches
         part = srv.answer_batch(qs[:k])
         assert np.array_equal(part, batch[:k])
+
+
+def test_pair_horner_answers_equal_single(medium8):
+    """The workgroup-pair Horner gives the same responses as the one-workgroup
+    kernel, alone and batched (Horner chain of 7 steps, 10 chains)."""
+    geo, db, srv = medium8
+    qs = [ol.make_query(geo, SEED, idx, 300 + k)[0] for k, idx in enumerate(range(0, 524288, 52429))]
+    base = srv.answer_batch(qs)
+    assert srv.lib.ipir_dbg_set_horner(srv.h, 1) == 0
+    try:
+        pair = srv.answer_batch(qs)
+        one = srv.answer_batch(qs[:1])
+    finally:
+        assert srv.lib.ipir_dbg_set_horner(srv.h, 0) == 0
+    assert np.array_equal(pair, base)
+    assert np.array_equal(one[0], base[0])
 
 
 def test_batch_validation_codes(medium8):
