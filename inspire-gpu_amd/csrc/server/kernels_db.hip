@@ -192,7 +192,7 @@ __device__ __forceinline__ void store_pair(const i32x4& al, const i32x4& ah, uin
 // disjoint column ranges (NT 16-column tiles each) and load their DB operands
 // straight from HBM into registers, one 64-row step ahead.
 template <int MT, int NT>
-__global__ __launch_bounds__(256) void k_dbpass_mfma_direct(const uint16_t* __restrict__ db,
+__global__ __launch_bounds__(256, 2) void k_dbpass_mfma_direct(const uint16_t* __restrict__ db,
                                                             const int8_t* __restrict__ dig,
                                                             const uint32_t* __restrict__ S,
                                                             uint32_t* __restrict__ out, size_t out_qstride,
@@ -353,12 +353,15 @@ DbPlan dbplan_f64(size_t R, size_t C, int unroll, size_t target_blocks) {
     return p;
 }
 
+// Columns one block of a direct-variant kernel covers: 4 waves x NT tiles of 16.
+uint32_t direct_nt(uint32_t mt) { return mt <= 2 ? 4 : mt <= 4 ? 4 : 2; }
+
 DbPlan dbplan_mfma(size_t R, size_t C, size_t nq, size_t target_blocks) {
     DbPlan p;
-    // direct: 2 queries per 16-row M tile; LDS: 4 waves x MT tiles, 8 queries per MT
-    p.kind = nq <= 4 ? DbPlan::MFMA_DIRECT : DbPlan::MFMA_LDS;
-    p.param = nq <= 2 ? 1 : nq <= 4 ? 2 : nq <= 8 ? 1 : 2;
-    const size_t cols = p.kind == DbPlan::MFMA_DIRECT ? 4 * 4 * 16 : 128;
+    // direct: 2 queries per 16-row M tile, every wave all M tiles
+    p.kind = DbPlan::MFMA_DIRECT;
+    p.param = nq <= 2 ? 1 : nq <= 4 ? 2 : nq <= 8 ? 4 : 8;
+    const size_t cols = p.kind == DbPlan::MFMA_DIRECT ? 4 * 16 * direct_nt(p.param) : 128;
     // slices: whole number of 128-row (8-block) chunks so the step count is even
     p.slices = pick_slices(R / 16, C / cols, target_blocks, 8);
     // at most 2^16 rows per slice keeps |H_k|, |L_k| <= 2^30 (i32-exact) and
@@ -396,13 +399,16 @@ hipError_t launch_dbpass_mfma(const uint16_t* db, const double* bq, size_t nq, i
     hipLaunchKernelGGL(k_bsum, dim3((unsigned)(2 * nq)), dim3(256), 0, s, bq, (uint32_t)R, S);
     const uint32_t Cc = (uint32_t)C, Q = (uint32_t)nq;
     if (p.kind == DbPlan::MFMA_DIRECT) {
-        if (C % 256) return hipErrorInvalidValue;
-        const dim3 grid((unsigned)(C / 256), p.slices);
-#define DIRECT(MT) \
-    hipLaunchKernelGGL((k_dbpass_mfma_direct<MT, 4>), grid, dim3(256), 0, s, db, dig, S, out, out_qstride, Cc, Q, kbps)
+        const uint32_t cols = 64 * direct_nt(p.param);
+        if (C % cols) return hipErrorInvalidValue;
+        const dim3 grid((unsigned)(C / cols), p.slices);
+#define DIRECT(MT, NT) \
+    hipLaunchKernelGGL((k_dbpass_mfma_direct<MT, NT>), grid, dim3(256), 0, s, db, dig, S, out, out_qstride, Cc, Q, kbps)
         switch (p.param) {
-            case 1: DIRECT(1); break;
-            case 2: DIRECT(2); break;
+            case 1: DIRECT(1, 4); break;
+            case 2: DIRECT(2, 4); break;
+            case 4: DIRECT(4, 4); break;
+            case 8: DIRECT(8, 2); break;
             default: return hipErrorInvalidValue;
         }
 #undef DIRECT
