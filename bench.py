@@ -93,6 +93,64 @@ def cpu_baseline(geo, n_groups_sample=8, col_frac=4, horner_steps=16):
     }
 
 
+def serving(lib, srv, qs, capacity_qps, resp_u64s, rates=(0.25, 0.5, 0.9), seconds=3.0, max_inflight=2):
+    """Poisson arrivals through the continuous-batching loop (ipir_loop_*):
+    at each rate (a fraction of the measured batched capacity) submit
+    requests at exponential inter-arrival times for `seconds`, wait for
+    each, and report throughput and latency percentiles (submit to
+    response written)."""
+    import threading
+    import inspire_amd as ia
+    views = [srv.views([q]) for q in qs]
+    ring = 1024
+    outs = np.zeros((ring, resp_u64s), np.uint64)
+    results = []
+    rng = np.random.default_rng(3)
+    for rate in rates:
+        lam = rate * capacity_qps
+        n = max(50, int(lam * seconds))
+        gaps = rng.exponential(1.0 / lam, n)
+        loop = lib.ipir_loop_start(srv.h, max_inflight)
+        assert loop
+        tickets = [None] * n
+        lat = np.zeros(n)
+        done_t = [0.0]
+        ready = threading.Semaphore(0)
+
+        def waiter():
+            for k in range(n):
+                ready.acquire()
+                l = C.c_double()
+                rc = lib.ipir_loop_wait(loop, tickets[k], C.byref(l))
+                assert rc == 0
+                lat[k] = l.value * 1e-3
+            done_t[0] = time.perf_counter()
+
+        th = threading.Thread(target=waiter)
+        th.start()
+        t0 = time.perf_counter()
+        t_next = t0
+        for k in range(n):
+            t_next += gaps[k]
+            while True:
+                d = t_next - time.perf_counter()
+                if d <= 0:
+                    break
+                time.sleep(min(d, 0.0005) if d > 0.0002 else 0)
+            tickets[k] = lib.ipir_loop_submit(loop, views[k % len(views)], ia.ptr(outs[k % ring]))
+            assert tickets[k]
+            ready.release()
+        th.join()
+        lib.ipir_loop_stop(loop)
+        wall = done_t[0] - t0
+        results.append({"offered_qps": round(lam, 1), "achieved_qps": round(n / wall, 1), "requests": n,
+                        "p50_ms": round(float(np.percentile(lat, 50)), 3),
+                        "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                        "p999_ms": round(float(np.percentile(lat, 99.9)), 3)})
+        log(f"[bench] serving at {rate:.2f}x capacity: {results[-1]}")
+    return results
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -101,6 +159,8 @@ def main():
     ap.add_argument("--config", default="c3", choices=sorted(CONFIGS))
     ap.add_argument("--batch", type=int, default=32)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--serving", action="store_true",
+                    help="also run Poisson arrivals through the serving loop at 0.25/0.5/0.9x capacity")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus > 1 is not supported "
@@ -108,12 +168,11 @@ def main():
         sys.exit(2)
 
     import inspire_amd as ia
-    import oracle_lib as ol
 
     n, w, rows, label = CONFIGS[args.config]
-    rc, geo = ol.setup(n, w, rows)
-    assert rc == 0
+    geo = ia.geometry(n, w, rows)
     R, Cc, G, T = geo["R"], geo["C"], geo["G"], geo["T"]
+    geo["Da"] = min(geo["D"], G)
     B = args.batch
     log(f"[bench] {label}: R={R} C={Cc} G={G} T={T}; generating DB ({R * Cc * 2 / 1e9:.2f} GB)")
     t0 = time.time()
@@ -134,7 +193,7 @@ def main():
         hs.append(h)
     flat = np.ascontiguousarray(np.stack(qs))
     qlen = flat.shape[1]
-    resp_u64s = T * 4 * ol.N
+    resp_u64s = T * 4 * ia.N
     dq = ia.DeviceBuffer(flat.nbytes, lib)
     dq.upload(flat)
     dout = ia.DeviceBuffer(B * resp_u64s * 8, lib)
@@ -147,7 +206,11 @@ def main():
     step(B)
     lib.ipir_dev_sync()
     resp = dout.download(np.zeros((B, resp_u64s), np.uint64))
-    ok = all(np.array_equal(cl.extract(hs[k], resp[k]), ol.expected_slots(geo, db, int(idx[k]))) for k in range(B))
+    def entry(i):  # the u16 slots of entry i in the row-major DB (src/protocol.cpp index math)
+        col = (i // R) * geo["cpe"]
+        return db[i % R, col:col + geo["cpe"]]
+
+    ok = all(np.array_equal(cl.extract(hs[k], resp[k]), entry(int(idx[k]))) for k in range(B))
     ref1 = srv.answer_batch([qs[0]])[0]
     ok = ok and np.array_equal(ref1, resp[0])
     log(f"[bench] extract check ({B} queries) and batch==single: {'ok' if ok else 'FAILED'}")
@@ -188,6 +251,10 @@ def main():
     dbpass_ms = st1[1]  # one plain database pass (batch 1)
     achieved = db_bytes / (dbpass_ms * 1e-3) / 1e9
 
+    serve = None
+    if args.serving:
+        serve = serving(lib, srv, qs, qps32, resp_u64s)
+
     cpu = None
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline (oracle, 1 core, bounded sample)")
@@ -224,6 +291,9 @@ def main():
                      "kernel": "k_dbpass (plain database pass, batch 1)"},
         "cpu_baseline": cpu,
     }
+    if serve is not None:
+        out["serving"] = {"loop": "ipir_loop (continuous batching, 2 partitions of max_batch/2 slots)",
+                          "rates": serve}
     for h in hs:
         cl.destroy(h)
     dq.free()

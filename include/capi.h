@@ -93,6 +93,11 @@ int ipir_extract_compressed(const ipir_params* params, const ipir_client_query* 
 
 /* ---- additions without a counterpart in the reference ---- */
 
+/* The geometry setup() computes for a database (params with a zero seed),
+ * for callers that lay out the row-major db_rows x db_cols u16 database
+ * before ipir_server_create. 0 on success, 1 if setup() rejects it. */
+int ipir_geometry(size_t n_entries, size_t entry_bytes, size_t db_rows_or_zero, ipir_params* out);
+
 /* Client query under a caller-seeded generator (reproducible tests and
  * benchmarks); otherwise identical to ipir_query_build. */
 ipir_client_query* ipir_query_build_seeded(const ipir_params* params, uint64_t idx,

@@ -42,7 +42,9 @@ int ipir_dbg_horner(const uint64_t* cts, size_t L, const uint64_t* rgsw, uint64_
 /* horner with a chosen kernel: variant 0 = one workgroup per chain, 1 = a
  * workgroup pair (one per limb). */
 int ipir_dbg_horner_variant(const uint64_t* cts, size_t L, const uint64_t* rgsw, uint64_t* out, int variant);
-/* Select the Horner kernel a server's answers use (mode as variant above). */
+/* Select the Horner kernel a server's answers use: mode as variant above,
+ * or -1 (the default) to use the pair only for calls with at most two
+ * chains. */
 int ipir_dbg_set_horner(ipir_server* srv, int mode);
 /* Per-stage device time (ms) of the last ipir_dbg_profile_batch run:
  * [unmarshal+key NTT, DB pass, collapse pass, pack finalize, Horner]. */

@@ -7,6 +7,7 @@
 // combine (:174-179), MSB-first bit streams with a zero-padded final byte
 // (:184-222), compressed responses of N x 28-bit then N x 20-bit values
 // flushed per ciphertext (:310-385).
+#include <array>
 #include <cstring>
 #include <new>
 
@@ -144,6 +145,27 @@ ipir_client_query* build(const ipir_params* params, uint64_t idx, uint64_t* out,
 }  // namespace
 
 extern "C" {
+
+int ipir_geometry(size_t n_entries, size_t entry_bytes, size_t db_rows_or_zero, ipir_params* out) {
+    if (!out) return 1;
+    try {
+        const PublicParams pp = setup_seeded(n_entries, entry_bytes, db_rows_or_zero ? db_rows_or_zero : 32768,
+                                             std::array<uint8_t, SEED_BYTES>{}.data());
+        out->n_entries = pp.N_entries;
+        out->entry_bytes = pp.w;
+        out->db_rows = pp.db_rows;
+        out->db_cols = pp.db_cols;
+        out->n_packed = pp.n_packed;
+        out->interp_d = pp.D;
+        out->num_cts = pp.num_cts;
+        out->ring_n = N;
+        out->d_eff = (size_t)D_EFF;
+        std::memset(out->seed, 0, SEED_BYTES);
+        return 0;
+    } catch (...) {
+        return 1;
+    }
+}
 
 size_t ipir_query_u64s(const ipir_params* p) { return params_ok(p) ? 2 * p->db_rows + 6 * p->d_eff * 2 * p->ring_n : 0; }
 size_t ipir_response_u64s(const ipir_params* p) { return params_ok(p) ? p->num_cts * 4 * p->ring_n : 0; }

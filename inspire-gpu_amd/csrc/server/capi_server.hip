@@ -167,13 +167,17 @@ int ipir_dbg_db_rows(const ipir_server* srv, size_t row0, size_t nrows, uint16_t
     if (!srv || !out || row0 + nrows > srv->pp.db_rows) return 1;
     if (nrows == 0) return 0;
     const size_t C = srv->pp.db_cols;
-    // resident layout: 16-row blocks, (r, c) at ((r/16)*C + c)*16 + r%16
+    // resident layout: plane p of (r, c) at byte ((r/16)*2 + p)*C*16 + c*16 + r%16
     const size_t kb0 = row0 / 16, kb1 = (row0 + nrows + 15) / 16;
-    std::vector<uint16_t> blk((kb1 - kb0) * C * 16);
-    if (hipMemcpy(blk.data(), srv->ctx->d_db + kb0 * C * 16, blk.size() * 2, hipMemcpyDeviceToHost) != hipSuccess)
+    std::vector<uint8_t> blk((kb1 - kb0) * 2 * C * 16);
+    if (hipMemcpy(blk.data(), srv->ctx->db_planes() + kb0 * 2 * C * 16, blk.size(), hipMemcpyDeviceToHost) !=
+        hipSuccess)
         return 4;
     for (size_t r = row0; r < row0 + nrows; r++)
-        for (size_t c = 0; c < C; c++) out[(r - row0) * C + c] = blk[((r / 16 - kb0) * C + c) * 16 + r % 16];
+        for (size_t c = 0; c < C; c++) {
+            const size_t base = ((r / 16 - kb0) * 2 * C + c) * 16 + r % 16;
+            out[(r - row0) * C + c] = (uint16_t)(blk[base] | (blk[base + C * 16] << 8));
+        }
     return 0;
 }
 
@@ -271,7 +275,7 @@ int ipir_dbg_horner(const uint64_t* cts, size_t L, const uint64_t* rgsw, uint64_
 }
 
 int ipir_dbg_set_horner(ipir_server* srv, int mode) {
-    if (!srv || mode < 0 || mode > 1) return 1;
+    if (!srv || mode < -1 || mode > 1) return 1;
     std::lock_guard<std::mutex> lk(srv->ctx->mu);
     srv->ctx->horner_mode = mode;
     return 0;

@@ -27,29 +27,30 @@ hipError_t launch_unmarshal(const uint64_t* d_q, size_t count, size_t R, double*
 // In-place forward NTT of `npoly` RNS polynomials laid out [npoly][2][N].
 hipError_t launch_ntt_rns(uint32_t* polys, size_t npoly, const NttConst& nc, bool inverse, hipStream_t s);
 
-// Database pass over the resident blocked DB (element (r, c) at
-// ((r/16)*C + c)*16 + r%16): out + q*out_qstride + slice*2C holds
-// [limb][C] = sum over the slice's rows of db[r][c]*b_{q,limb}[r] mod q_limb.
+// Database pass over the resident DB, byte planes in 16-row blocks: the
+// low (p = 0) and high (p = 1) byte of (r, c) at ((r/16)*2 + p)*C*16 +
+// c*16 + r%16. out + q*out_qstride + slice*2C holds [limb][C] = sum over the
+// slice's rows of db[r][c]*b_{q,limb}[r] mod q_limb.
 struct DbPlan {
-    enum Kind { F64, MFMA_DIRECT, MFMA_LDS } kind;
-    uint32_t param;   // F64: 16-row blocks per iteration (2, 4, 8); MFMA: M tiles per wave
+    enum Kind { F64, MFMA } kind;
+    uint32_t param;   // F64: 16-row blocks per iteration (2, 4, 8); MFMA: 16-row digit tiles (1, 2, 4, 8)
     uint32_t slices;  // row slices (power of two)
 };
-// Row-major staging rows -> blocked DB, blocks [kb0, kb0 + nkb).
-hipError_t launch_to_blocked(const uint16_t* src, uint16_t* dst, size_t C, size_t kb0, size_t nkb, hipStream_t s);
+// Row-major staging rows (16*nkb of them) -> blocks [kb0, kb0 + nkb) of the planes.
+hipError_t launch_to_planes(const uint16_t* src, uint8_t* dst, size_t C, size_t kb0, size_t nkb, hipStream_t s);
 DbPlan dbplan_f64(size_t R, size_t C, int unroll, size_t target_blocks);
 // Queries one batched (int8 MFMA) database pass serves.
 constexpr size_t kMfmaMaxQueries = 16;
 // int8-MFMA plan for nq (1..kMfmaMaxQueries) queries.
 DbPlan dbplan_mfma(size_t R, size_t C, size_t nq, size_t target_blocks);
-// Digit rows the MFMA plan stages per 16-row block (int8 scratch: R*Mp bytes).
+// Digit rows the MFMA plan stages per 16-row block (int8 scratch: R*Mp bytes, Mp <= 128).
 size_t dbplan_mp(const DbPlan& p);
 // One query, fp64 FMA on the vector ALUs.
-hipError_t launch_dbpass_f64(const uint16_t* db, const double* bq, uint32_t* out, size_t R, size_t C,
+hipError_t launch_dbpass_f64(const uint8_t* db, const double* bq, uint32_t* out, size_t R, size_t C,
                              const DbPlan& p, hipStream_t s);
 // nq queries (bq [q][limb][R] doubles) in one pass of the DB through int8
 // MFMA; dig (R*Mp bytes) and S (2*nq words) are scratch.
-hipError_t launch_dbpass_mfma(const uint16_t* db, const double* bq, size_t nq, int8_t* dig, uint32_t* S,
+hipError_t launch_dbpass_mfma(const uint8_t* db, const double* bq, size_t nq, int8_t* dig, uint32_t* S,
                               uint32_t* out, size_t out_qstride, size_t R, size_t C, const DbPlan& p,
                               hipStream_t s);
 // interm[q][limb][C] = sum over the S row slices of part (per query stride part_stride).
@@ -93,7 +94,7 @@ hipError_t launch_horner_pair(const uint32_t* pa_base, size_t pa_stride, size_t 
 hipError_t launch_encode(uint16_t* db, size_t R, size_t C, size_t Da, hipStream_t s);
 // a32T[g][limb][p][j] for groups [g0, g0+ng): transposed a-side matmul
 // output. crs [nr][2][N] NTT form.
-hipError_t launch_aside(const uint16_t* db, size_t R, size_t C, size_t g0, size_t ng, const uint32_t* crs,
+hipError_t launch_aside(const uint8_t* db, size_t R, size_t C, size_t g0, size_t ng, const uint32_t* crs,
                         uint32_t* a32T, const NttConst& nc, hipStream_t s);
 // F = NTT along j of a32T (in place), per (g, limb, p).
 hipError_t launch_ring_embed(uint32_t* a32T, size_t ng, const NttConst& nc, hipStream_t s);

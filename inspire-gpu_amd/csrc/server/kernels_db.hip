@@ -1,11 +1,13 @@
 // Database-pass kernels for gfx950 over the resident DB layout.
 //
-// Resident layout ("16-row blocks"): element (row r, col c) of the encoded
-// u16 database sits at ((r/16)*C + c)*16 + r%16. One 32-byte load gives a
-// lane 16 consecutive rows of one column: one K-slice of an int8 MFMA
-// operand, and for the plain pass a unit-stride walk down a column with
-// wave-uniform query words. 16 lanes on consecutive columns read 512
-// contiguous bytes.
+// Resident layout ("byte planes in 16-row blocks"): the encoded u16 value
+// of (row r, col c) is split into its low and high byte; byte plane p of
+// block kb = r/16 holds, for every column, the 16 bytes of rows 16kb..16kb+15:
+//   plane byte of (r, c) at ((r/16)*2 + p)*C*16 + c*16 + r%16.
+// A lane's 16-byte load is one column's 16 rows of one plane: exactly the
+// K-slice of an int8 MFMA B operand (after a sign flip, no byte shuffles),
+// and consecutive lanes read consecutive 16-byte units, so every load
+// instruction of a wavefront is fully coalesced.
 //
 // Every kernel computes the reference's mat-vec (src/gpu_online.cu:23-41)
 //   interm[q][c][col] = sum_r db[r][col] * b_{q,c}[r] mod q_c
@@ -29,60 +31,66 @@ __device__ __forceinline__ uint32_t fmod_exact(double a, double q, double qinv) 
 }
 
 // ------------------------------------------------------------ relayout
-// Row-major rows [16*kb0, 16*(kb0 + gridDim.y)) of `src` (whose first row is
-// row 16*kb0) -> blocked `dst`.
-__global__ __launch_bounds__(256) void k_to_blocked(const uint16_t* __restrict__ src, uint16_t* __restrict__ dst,
-                                                    uint32_t C, uint32_t kb0) {
+// Row-major rows [0, 16*gridDim.y) of `src` -> blocks [kb0, kb0 + gridDim.y) of the planes.
+__global__ __launch_bounds__(256) void k_to_planes(const uint16_t* __restrict__ src, uint8_t* __restrict__ dst,
+                                                   uint32_t C, uint32_t kb0) {
     const uint32_t col = blockIdx.x * 256 + threadIdx.x;
     const uint32_t kbl = blockIdx.y;
-    uint32_t w[8];
+    uint32_t lo[4] = {0, 0, 0, 0}, hi[4] = {0, 0, 0, 0};
 #pragma unroll
-    for (int e = 0; e < 8; e++) {
-        const uint32_t lo = src[((size_t)kbl * 16 + 2 * e) * C + col];
-        const uint32_t hi = src[((size_t)kbl * 16 + 2 * e + 1) * C + col];
-        w[e] = lo | (hi << 16);
+    for (int e = 0; e < 16; e++) {
+        const uint32_t v = src[((size_t)kbl * 16 + e) * C + col];
+        lo[e >> 2] |= (v & 255u) << (8 * (e & 3));
+        hi[e >> 2] |= (v >> 8) << (8 * (e & 3));
     }
-    u32x4* d = reinterpret_cast<u32x4*>(dst + ((size_t)(kb0 + kbl) * C + col) * 16);
-    d[0] = u32x4{w[0], w[1], w[2], w[3]};
-    d[1] = u32x4{w[4], w[5], w[6], w[7]};
+    const size_t kb = (size_t)kb0 + kbl;
+    *reinterpret_cast<u32x4*>(dst + ((kb * 2 + 0) * C + col) * 16) = u32x4{lo[0], lo[1], lo[2], lo[3]};
+    *reinterpret_cast<u32x4*>(dst + ((kb * 2 + 1) * C + col) * 16) = u32x4{hi[0], hi[1], hi[2], hi[3]};
 }
 
 // ------------------------------------------------------------ plain pass (fp64)
-// One query. Thread = one column, walking its slice 16 rows (32 bytes) at a
-// time with wave-uniform query words. Products db*b < 2^43 accumulate exactly
-// in fp64 FMA for 1024 rows (< 2^53), in four interleaved accumulators per
-// limb; every 1024 rows the sums are reduced mod q and folded into a
-// canonical u32. out[limb][C] for this slice at out + slice*2C.
+// One query. Thread = one column; per 16-row block it loads the column's low
+// and high plane (16 bytes each) and uses wave-uniform query words.
+// Products db*b < 2^43 accumulate exactly in fp64 FMA for 1024 rows
+// (< 2^53), in four interleaved accumulators per limb; every 1024 rows the
+// sums are reduced mod q and folded into a canonical u32.
+// out[limb][C] for this slice at out + slice*2C.
 template <int U>
-__global__ __launch_bounds__(256) void k_dbpass_f64(const uint16_t* __restrict__ db, const double* __restrict__ bq,
+__global__ __launch_bounds__(256) void k_dbpass_f64(const uint8_t* __restrict__ db, const double* __restrict__ bq,
                                                     uint32_t* __restrict__ out, uint32_t R, uint32_t C,
                                                     uint32_t kbps) {
     const uint32_t col = blockIdx.x * 256 + threadIdx.x;
     const uint32_t kb0 = blockIdx.y * kbps;
-    const u32x4* p = reinterpret_cast<const u32x4*>(db + ((size_t)kb0 * C + col) * 16);
-    const size_t kstep = (size_t)C * 2;  // one 16-row block, in u32x4
+    const u32x4* p = reinterpret_cast<const u32x4*>(db) + (size_t)kb0 * 2 * C + col;
+    const size_t kstep = (size_t)2 * C;  // one block (both planes), in 16-byte units
     const double* b0 = bq + (size_t)kb0 * 16;
     const double* b1 = b0 + R;
     const double dq0 = (double)Q0, dq1 = (double)Q1, iq0 = 1.0 / dq0, iq1 = 1.0 / dq1;
     uint32_t m0 = 0, m1 = 0;
     double a0[4] = {0, 0, 0, 0}, a1[4] = {0, 0, 0, 0};
     for (uint32_t kb = 0; kb < kbps; kb += U) {
-        u32x4 v[U][2];
+        u32x4 lo[U], hi[U];
 #pragma unroll
         for (int u = 0; u < U; u++) {
-            v[u][0] = __builtin_nontemporal_load(p + (size_t)(kb + u) * kstep);
-            v[u][1] = __builtin_nontemporal_load(p + (size_t)(kb + u) * kstep + 1);
+            lo[u] = __builtin_nontemporal_load(p + (size_t)(kb + u) * kstep);
+            hi[u] = __builtin_nontemporal_load(p + (size_t)(kb + u) * kstep + C);
         }
 #pragma unroll
         for (int u = 0; u < U; u++) {
             const double* x0 = b0 + (size_t)(kb + u) * 16;
             const double* x1 = b1 + (size_t)(kb + u) * 16;
 #pragma unroll
-            for (int e = 0; e < 16; e++) {
-                const uint32_t w = v[u][e >> 3][(e & 7) >> 1];
-                const double d = (double)((e & 1) ? (w >> 16) : (w & 0xffffu));
-                a0[e & 3] = fma(d, x0[e], a0[e & 3]);
-                a1[e & 3] = fma(d, x1[e], a1[e & 3]);
+            for (int j = 0; j < 4; j++) {
+                // rows 4j..4j+3: interleave the low and high bytes into two u16 pairs
+                const uint32_t w01 = __builtin_amdgcn_perm(hi[u][j], lo[u][j], 0x05010400u);
+                const uint32_t w23 = __builtin_amdgcn_perm(hi[u][j], lo[u][j], 0x07030602u);
+                const double d[4] = {(double)(w01 & 0xffffu), (double)(w01 >> 16), (double)(w23 & 0xffffu),
+                                     (double)(w23 >> 16)};
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    a0[e] = fma(d[e], x0[4 * j + e], a0[e]);
+                    a1[e] = fma(d[e], x1[4 * j + e], a1[e]);
+                }
             }
         }
         if (((kb + U) & 63) == 0 || kb + U >= kbps) {  // every 64 blocks = 1024 rows
@@ -103,13 +111,13 @@ __global__ __launch_bounds__(256) void k_dbpass_f64(const uint16_t* __restrict__
 // with h' = hi byte - 128, l' = lo byte - 128 (a byte XOR 0x80 read as int8).
 // So   sum_r v_r b_r = sum_k 256^k (256*H_k + L_k) + 32896 * sum_r b_r,
 // with H_k = sum_r h'_r d_{k,r} and L_k = sum_r l'_r d_{k,r}: int8 x int8 dot
-// products over at most 2^15 rows, |H_k|, |L_k| <= 2^29, exact in i32.
+// products; with at most 2^16 rows per slice |H_k|, |L_k| <= 2^30, exact in
+// i32.
 //
 // A operand rows m = (q*2 + c)*4 + k (pair pr = q*2 + c, digit k), stored as
 // dig[R/16][Mp][16 bytes] (byte e of row m in block kb = digit of row 16kb+e).
-// MFMA 16x16x64 i8 lane mapping used: A/B lane l holds 16 K-bytes of
-// M row / N col l%16 at K offset 16*(l/16) (the same K order on both sides,
-// so any K permutation the hardware applies cancels); the i32 output holds
+// MFMA 16x16x64 i8 lane mapping (checked on the device): A/B lane l holds 16
+// K-bytes of M row / N col l%16 at K offset 16*(l/16); the i32 output holds
 // rows 4*(l/16)+i, col l%16 in element i. A lane thus owns all four digits
 // of one pair for one column and recombines them locally.
 
@@ -144,13 +152,17 @@ __global__ __launch_bounds__(256) void k_digits(const double* __restrict__ bq, u
     for (int k = 0; k < 4; k++) dst[k] = o[k];
 }
 
-// S[pr] = sum_r b_pr[r] mod q_c, the DB-offset correction.
+// S[pr] = sum_r b_pr[r] mod q_c, the DB-offset correction. One block per
+// pair; 8 doubles per thread per pass.
 __global__ __launch_bounds__(256) void k_bsum(const double* __restrict__ bq, uint32_t R, uint32_t* __restrict__ S) {
     __shared__ uint64_t red[256];
     const uint32_t pr = blockIdx.x;
     const double* b = bq + (size_t)pr * R;
     uint64_t a = 0;
-    for (uint32_t i = threadIdx.x; i < R; i += 256) a += (uint64_t)b[i];
+    for (uint32_t i = threadIdx.x * 8; i < R; i += 256 * 8) {
+#pragma unroll
+        for (int e = 0; e < 8; e++) a += (uint64_t)b[i + e];
+    }
     red[threadIdx.x] = a;
     __syncthreads();
     for (uint32_t s = 128; s > 0; s >>= 1) {
@@ -158,17 +170,6 @@ __global__ __launch_bounds__(256) void k_bsum(const double* __restrict__ bq, uin
         __syncthreads();
     }
     if (threadIdx.x == 0) S[pr] = red64(red[0], (int)(pr & 1));
-}
-
-// 16 u16 (one column, 16 rows) -> int8 B operands of the low and high byte
-// planes, offset by -128.
-__device__ __forceinline__ void split_planes(const u32x4& lo, const u32x4& hi, i32x4& Bl, i32x4& Bh) {
-    const uint32_t wd[8] = {lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-        Bl[j] = (int32_t)(__builtin_amdgcn_perm(wd[2 * j + 1], wd[2 * j], 0x06040200u) ^ 0x80808080u);
-        Bh[j] = (int32_t)(__builtin_amdgcn_perm(wd[2 * j + 1], wd[2 * j], 0x07050301u) ^ 0x80808080u);
-    }
 }
 
 // Recombine one lane's accumulators of pair pr and store the canonical residue.
@@ -188,37 +189,40 @@ __device__ __forceinline__ void store_pair(const i32x4& al, const i32x4& ah, uin
     out[(size_t)q * out_qstride + (size_t)slice * 2 * C + (size_t)c * C + col] = red64((uint64_t)(s + (int64_t)off), c);
 }
 
-// Few queries (Mp = 16*MT rows, <= 2*MT queries): the 4 waves of a block own
+// Up to 2*MT queries (Mp = 16*MT digit rows): the 4 waves of a block own
 // disjoint column ranges (NT 16-column tiles each) and load their DB operands
-// straight from HBM into registers, one 64-row step ahead.
+// straight from HBM into registers, one 64-row step ahead; every wave reads
+// all MT digit tiles (L2/L1-resident). Two waves per SIMD.
 template <int MT, int NT>
-__global__ __launch_bounds__(256, 2) void k_dbpass_mfma_direct(const uint16_t* __restrict__ db,
-                                                            const int8_t* __restrict__ dig,
-                                                            const uint32_t* __restrict__ S,
-                                                            uint32_t* __restrict__ out, size_t out_qstride,
-                                                            uint32_t C, uint32_t nq, uint32_t kbps) {
+__global__ __launch_bounds__(256, 2) void k_dbpass_mfma(const uint8_t* __restrict__ db,
+                                                        const int8_t* __restrict__ dig,
+                                                        const uint32_t* __restrict__ S,
+                                                        uint32_t* __restrict__ out, size_t out_qstride,
+                                                        uint32_t C, uint32_t nq, uint32_t kbps) {
     constexpr uint32_t Mp = 16 * MT;
     const uint32_t w = threadIdx.x >> 6, l = threadIdx.x & 63;
     const uint32_t slice = blockIdx.y;
     const uint32_t col0 = (blockIdx.x * 4 + w) * NT * 16;
     const uint32_t kb0 = slice * kbps;
     const uint32_t nks = kbps / 4;
-    const u32x4* bsrc = reinterpret_cast<const u32x4*>(db + ((size_t)(kb0 + (l >> 4)) * C + col0 + (l & 15)) * 16);
-    const size_t bstep = (size_t)C * 8;  // 4 blocks, in u32x4
+    // lane's low-plane unit of block kb0 + l/16, column col0 + l%16; the high plane is C units later
+    const i32x4* bsrc = reinterpret_cast<const i32x4*>(db) + (size_t)(kb0 + (l >> 4)) * 2 * C + col0 + (l & 15);
+    const size_t bstep = (size_t)8 * C;  // 4 blocks x 2 planes, in 16-byte units
     const i32x4* asrc = reinterpret_cast<const i32x4*>(dig) + (size_t)(kb0 + (l >> 4)) * Mp + (l & 15);
     const size_t astep = (size_t)4 * Mp;
+    const i32x4 flip = {(int32_t)0x80808080, (int32_t)0x80808080, (int32_t)0x80808080, (int32_t)0x80808080};
     i32x4 acc[MT][NT][2];
 #pragma unroll
     for (int a = 0; a < MT; a++)
 #pragma unroll
         for (int n = 0; n < NT; n++) acc[a][n][0] = acc[a][n][1] = i32x4{0, 0, 0, 0};
-    u32x4 v[2][NT][2];
+    i32x4 v[2][NT][2];
     i32x4 A[2][MT];
     auto load = [&](int bi, uint32_t ks) {
 #pragma unroll
         for (int n = 0; n < NT; n++) {
-            v[bi][n][0] = __builtin_nontemporal_load(bsrc + ks * bstep + n * 32);
-            v[bi][n][1] = __builtin_nontemporal_load(bsrc + ks * bstep + n * 32 + 1);
+            v[bi][n][0] = __builtin_nontemporal_load(bsrc + ks * bstep + n * 16);
+            v[bi][n][1] = __builtin_nontemporal_load(bsrc + ks * bstep + n * 16 + C);
         }
 #pragma unroll
         for (int a = 0; a < MT; a++) A[bi][a] = asrc[ks * astep + a * 16];
@@ -226,8 +230,7 @@ __global__ __launch_bounds__(256, 2) void k_dbpass_mfma_direct(const uint16_t* _
     auto compute = [&](int bi) {
 #pragma unroll
         for (int n = 0; n < NT; n++) {
-            i32x4 Bl, Bh;
-            split_planes(v[bi][n][0], v[bi][n][1], Bl, Bh);
+            const i32x4 Bl = v[bi][n][0] ^ flip, Bh = v[bi][n][1] ^ flip;
 #pragma unroll
             for (int a = 0; a < MT; a++) {
                 acc[a][n][0] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[bi][a], Bl, acc[a][n][0], 0, 0, 0);
@@ -250,83 +253,6 @@ __global__ __launch_bounds__(256, 2) void k_dbpass_mfma_direct(const uint16_t* _
                        slice, C, col0 + n * 16 + (l & 15));
 }
 
-// Many queries (Mp = 64*MT rows, <= 8*MT queries): the 4 waves of a block
-// split the M rows (MT tiles each) and share one 128-column DB tile per
-// 64-row step, staged through LDS (double buffered, loaded a step ahead
-// through registers). The digits (L2-resident) are read per wave.
-template <int MT>
-__global__ __launch_bounds__(256) void k_dbpass_mfma_lds(const uint16_t* __restrict__ db,
-                                                         const int8_t* __restrict__ dig,
-                                                         const uint32_t* __restrict__ S,
-                                                         uint32_t* __restrict__ out, size_t out_qstride,
-                                                         uint32_t C, uint32_t nq, uint32_t kbps) {
-    constexpr int NT = 8;
-    constexpr uint32_t Mp = 64 * MT;
-    // [buf][block kb of the step][half][col] 16-byte units
-    __shared__ u32x4 tile[2][4 * 2 * 128];
-    const uint32_t tid = threadIdx.x, w = tid >> 6, l = tid & 63;
-    const uint32_t slice = blockIdx.y;
-    const uint32_t colbase = blockIdx.x * 128;
-    const uint32_t kb0 = slice * kbps;
-    const uint32_t nks = kbps / 4;
-    // this thread's 4 16-byte units of a step's tile: x = tid + 256j -> block x/256, unit x%256 = (col, half)
-    const u32x4* gsrc = reinterpret_cast<const u32x4*>(db + ((size_t)kb0 * C + colbase) * 16);
-    const size_t bstep = (size_t)C * 8;
-    auto gaddr = [&](uint32_t ks, int j) {
-        const uint32_t x = tid + 256 * j;
-        return gsrc + ks * bstep + (size_t)(x >> 8) * C * 2 + (x & 255);
-    };
-    auto laddr = [&](int j) {
-        const uint32_t x = tid + 256 * j, u = x & 255;
-        return ((x >> 8) * 2 + (u & 1)) * 128 + (u >> 1);
-    };
-    const i32x4* asrc = reinterpret_cast<const i32x4*>(dig) + (size_t)(kb0 + (l >> 4)) * Mp + w * MT * 16 + (l & 15);
-    const size_t astep = (size_t)4 * Mp;
-    i32x4 acc[MT][NT][2];
-#pragma unroll
-    for (int a = 0; a < MT; a++)
-#pragma unroll
-        for (int n = 0; n < NT; n++) acc[a][n][0] = acc[a][n][1] = i32x4{0, 0, 0, 0};
-    u32x4 pre[4];
-    i32x4 A[MT], An[MT];
-#pragma unroll
-    for (int j = 0; j < 4; j++) pre[j] = __builtin_nontemporal_load(gaddr(0, j));
-#pragma unroll
-    for (int a = 0; a < MT; a++) A[a] = asrc[a * 16];
-    for (uint32_t ks = 0; ks < nks; ks++) {
-        const int bi = ks & 1;
-#pragma unroll
-        for (int j = 0; j < 4; j++) tile[bi][laddr(j)] = pre[j];
-        __syncthreads();
-        if (ks + 1 < nks) {
-#pragma unroll
-            for (int j = 0; j < 4; j++) pre[j] = __builtin_nontemporal_load(gaddr(ks + 1, j));
-#pragma unroll
-            for (int a = 0; a < MT; a++) An[a] = asrc[(ks + 1) * astep + a * 16];
-        }
-        const uint32_t kbl = l >> 4;
-#pragma unroll
-        for (int n = 0; n < NT; n++) {
-            const uint32_t cl = n * 16 + (l & 15);
-            i32x4 Bl, Bh;
-            split_planes(tile[bi][(kbl * 2 + 0) * 128 + cl], tile[bi][(kbl * 2 + 1) * 128 + cl], Bl, Bh);
-#pragma unroll
-            for (int a = 0; a < MT; a++) {
-                acc[a][n][0] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[a], Bl, acc[a][n][0], 0, 0, 0);
-                acc[a][n][1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[a], Bh, acc[a][n][1], 0, 0, 0);
-            }
-        }
-#pragma unroll
-        for (int a = 0; a < MT; a++) A[a] = An[a];
-    }
-#pragma unroll
-    for (int a = 0; a < MT; a++)
-#pragma unroll
-        for (int n = 0; n < NT; n++)
-            store_pair(acc[a][n][0], acc[a][n][1], (w * MT + (uint32_t)a) * 4 + (l >> 4), nq, slice == 0, S, out,
-                       out_qstride, slice, C, colbase + n * 16 + (l & 15));
-}
-
 // Largest power-of-two slice count s with blocks*s <= target that keeps
 // every slice a whole number of `unit`-block chunks.
 uint32_t pick_slices(size_t nkb, size_t blocks, size_t target, size_t unit) {
@@ -335,13 +261,16 @@ uint32_t pick_slices(size_t nkb, size_t blocks, size_t target, size_t unit) {
     return s;
 }
 
+// 16-column tiles per wave for MT digit tiles (accumulators MT*NT*8 registers).
+uint32_t mfma_nt(uint32_t mt) { return mt <= 4 ? 4 : 2; }
+
 }  // namespace
 
-hipError_t launch_to_blocked(const uint16_t* src, uint16_t* dst, size_t C, size_t kb0, size_t nkb, hipStream_t s) {
+hipError_t launch_to_planes(const uint16_t* src, uint8_t* dst, size_t C, size_t kb0, size_t nkb, hipStream_t s) {
     if (C % 256) return hipErrorInvalidValue;
     if (nkb == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_to_blocked, dim3((unsigned)(C / 256), (unsigned)nkb), dim3(256), 0, s, src, dst,
-                       (uint32_t)C, (uint32_t)kb0);
+    hipLaunchKernelGGL(k_to_planes, dim3((unsigned)(C / 256), (unsigned)nkb), dim3(256), 0, s, src, dst, (uint32_t)C,
+                       (uint32_t)kb0);
     return hipGetLastError();
 }
 
@@ -353,15 +282,11 @@ DbPlan dbplan_f64(size_t R, size_t C, int unroll, size_t target_blocks) {
     return p;
 }
 
-// Columns one block of a direct-variant kernel covers: 4 waves x NT tiles of 16.
-uint32_t direct_nt(uint32_t mt) { return mt <= 2 ? 4 : mt <= 4 ? 4 : 2; }
-
 DbPlan dbplan_mfma(size_t R, size_t C, size_t nq, size_t target_blocks) {
     DbPlan p;
-    // direct: 2 queries per 16-row M tile, every wave all M tiles
-    p.kind = DbPlan::MFMA_DIRECT;
-    p.param = nq <= 2 ? 1 : nq <= 4 ? 2 : nq <= 8 ? 4 : 8;
-    const size_t cols = p.kind == DbPlan::MFMA_DIRECT ? 4 * 16 * direct_nt(p.param) : 128;
+    p.kind = DbPlan::MFMA;
+    p.param = nq <= 2 ? 1 : nq <= 4 ? 2 : nq <= 8 ? 4 : 8;  // 2 queries per 16-row digit tile
+    const size_t cols = 64 * mfma_nt(p.param);
     // slices: whole number of 128-row (8-block) chunks so the step count is even
     p.slices = pick_slices(R / 16, C / cols, target_blocks, 8);
     // at most 2^16 rows per slice keeps |H_k|, |L_k| <= 2^30 (i32-exact) and
@@ -370,10 +295,11 @@ DbPlan dbplan_mfma(size_t R, size_t C, size_t nq, size_t target_blocks) {
     return p;
 }
 
-size_t dbplan_mp(const DbPlan& p) { return p.kind == DbPlan::MFMA_DIRECT ? 16 * p.param : 64 * p.param; }
+size_t dbplan_mp(const DbPlan& p) { return p.kind == DbPlan::MFMA ? 16 * p.param : 0; }
 
-hipError_t launch_dbpass_f64(const uint16_t* db, const double* bq, uint32_t* out, size_t R, size_t C,
+hipError_t launch_dbpass_f64(const uint8_t* db, const double* bq, uint32_t* out, size_t R, size_t C,
                              const DbPlan& p, hipStream_t s) {
+    if (p.kind != DbPlan::F64 || C % 256) return hipErrorInvalidValue;
     const uint32_t kbps = (uint32_t)(R / 16 / p.slices);
     const dim3 grid((unsigned)(C / 256), p.slices);
     switch (p.param) {
@@ -384,46 +310,32 @@ hipError_t launch_dbpass_f64(const uint16_t* db, const double* bq, uint32_t* out
     return hipGetLastError();
 }
 
-hipError_t launch_dbpass_mfma(const uint16_t* db, const double* bq, size_t nq, int8_t* dig, uint32_t* S,
+hipError_t launch_dbpass_mfma(const uint8_t* db, const double* bq, size_t nq, int8_t* dig, uint32_t* S,
                               uint32_t* out, size_t out_qstride, size_t R, size_t C, const DbPlan& p,
                               hipStream_t s) {
-    if (p.kind == DbPlan::F64 || nq == 0 || nq > kMfmaMaxQueries) return hipErrorInvalidValue;
+    if (p.kind != DbPlan::MFMA || nq == 0 || nq > kMfmaMaxQueries) return hipErrorInvalidValue;
     const uint32_t Mp = (uint32_t)dbplan_mp(p);
     if (2 * nq > Mp / 4) return hipErrorInvalidValue;
     const uint32_t nkb = (uint32_t)(R / 16);
     const uint32_t kbps = nkb / p.slices;
-    if (kbps % 8) return hipErrorInvalidValue;
+    const uint32_t cols = 64 * mfma_nt(p.param);
+    if (kbps % 8 || C % cols) return hipErrorInvalidValue;
     const size_t nthr = (size_t)nkb * (Mp / 4);
     hipLaunchKernelGGL(k_digits, dim3((unsigned)((nthr + 255) / 256)), dim3(256), 0, s, bq, (uint32_t)R,
                        (uint32_t)nq, Mp, dig);
     hipLaunchKernelGGL(k_bsum, dim3((unsigned)(2 * nq)), dim3(256), 0, s, bq, (uint32_t)R, S);
     const uint32_t Cc = (uint32_t)C, Q = (uint32_t)nq;
-    if (p.kind == DbPlan::MFMA_DIRECT) {
-        const uint32_t cols = 64 * direct_nt(p.param);
-        if (C % cols) return hipErrorInvalidValue;
-        const dim3 grid((unsigned)(C / cols), p.slices);
-#define DIRECT(MT, NT) \
-    hipLaunchKernelGGL((k_dbpass_mfma_direct<MT, NT>), grid, dim3(256), 0, s, db, dig, S, out, out_qstride, Cc, Q, kbps)
-        switch (p.param) {
-            case 1: DIRECT(1, 4); break;
-            case 2: DIRECT(2, 4); break;
-            case 4: DIRECT(4, 4); break;
-            case 8: DIRECT(8, 2); break;
-            default: return hipErrorInvalidValue;
-        }
-#undef DIRECT
-    } else {
-        if (C % 128) return hipErrorInvalidValue;
-        const dim3 grid((unsigned)(C / 128), p.slices);
-#define LDS(MT) \
-    hipLaunchKernelGGL((k_dbpass_mfma_lds<MT>), grid, dim3(256), 0, s, db, dig, S, out, out_qstride, Cc, Q, kbps)
-        switch (p.param) {
-            case 1: LDS(1); break;
-            case 2: LDS(2); break;
-            default: return hipErrorInvalidValue;
-        }
-#undef LDS
+    const dim3 grid((unsigned)(C / cols), p.slices);
+#define MFMA(MT, NT) \
+    hipLaunchKernelGGL((k_dbpass_mfma<MT, NT>), grid, dim3(256), 0, s, db, dig, S, out, out_qstride, Cc, Q, kbps)
+    switch (p.param) {
+        case 1: MFMA(1, 4); break;
+        case 2: MFMA(2, 4); break;
+        case 4: MFMA(4, 4); break;
+        case 8: MFMA(8, 2); break;
+        default: return hipErrorInvalidValue;
     }
+#undef MFMA
     return hipGetLastError();
 }
 

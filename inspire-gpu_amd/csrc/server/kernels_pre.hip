@@ -89,11 +89,10 @@ __global__ __launch_bounds__(1024) void k_encode(uint16_t* __restrict__ db, uint
 }
 
 // ------------------------------------------------------------ a-side
-// Block (group, 4 columns) over the blocked DB (element (r, c) at
-// ((r/16)*C + c)*16 + r%16): acc[col][c] = sum_r NTT(db[rN..rN+N-1][col]) *
-// crs[r][c]; a32T[g][c][k][col] = N^-1 * acc[col][c][N-1-k].
+// Block (group, 4 columns) over the resident DB (byte planes in 16-row
+// blocks): acc[col][c] = sum_r NTT(db[rN..rN+N-1][col]) * crs[r][c]; a32T[g][c][k][col] = N^-1 * acc[col][c][N-1-k].
 // Polynomial p = col*2 + c; team tm transforms p = tm and p = tm + 4.
-__global__ __launch_bounds__(1024) void k_aside(const uint16_t* __restrict__ db, uint32_t C, uint32_t nr, uint32_t g0,
+__global__ __launch_bounds__(1024) void k_aside(const uint8_t* __restrict__ db, uint32_t C, uint32_t nr, uint32_t g0,
                                                 const uint32_t* __restrict__ crs, uint32_t* __restrict__ a32T,
                                                 NttConst nc) {
     __shared__ uint32_t buf[8][POLY_LDS];
@@ -113,11 +112,11 @@ __global__ __launch_bounds__(1024) void k_aside(const uint16_t* __restrict__ db,
 #pragma unroll
         for (int h = 0; h < 2; h++) {
             const uint32_t i = tid + 1024 * h;
-            const size_t row = (size_t)r * N + i;  // blocked DB: ((row/16)*C + col)*16 + row%16
-            const uint16_t* src = db + ((row >> 4) * C + col0) * 16 + (row & 15);
+            const size_t row = (size_t)r * N + i;  // plane p of (row, col) at ((row/16)*2 + p)*C*16 + col*16 + row%16
+            const uint8_t* src = db + ((row >> 4) * 2 * C + col0) * 16 + (row & 15);
 #pragma unroll
             for (int col = 0; col < 4; col++) {
-                const uint32_t x = src[col * 16];
+                const uint32_t x = (uint32_t)src[col * 16] | ((uint32_t)src[(size_t)C * 16 + col * 16] << 8);
                 buf[col * 2][ls(i)] = x;
                 buf[col * 2 + 1][ls(i)] = x;
             }
@@ -334,7 +333,7 @@ hipError_t launch_encode(uint16_t* db, size_t R, size_t C, size_t Da, hipStream_
     return hipGetLastError();
 }
 
-hipError_t launch_aside(const uint16_t* db, size_t R, size_t C, size_t g0, size_t ng, const uint32_t* crs,
+hipError_t launch_aside(const uint8_t* db, size_t R, size_t C, size_t g0, size_t ng, const uint32_t* crs,
                         uint32_t* a32T, const NttConst& nc, hipStream_t s) {
     hipLaunchKernelGGL(k_aside, dim3((unsigned)(ng * (N / 4))), dim3(1024), 0, s, db, (uint32_t)C,
                        (uint32_t)(R / N), (uint32_t)g0, crs, a32T, nc);

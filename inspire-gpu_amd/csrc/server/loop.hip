@@ -109,7 +109,7 @@ struct ipir_loop {
             if (!p->launched) return;  // stopping with nothing in flight
             lk.unlock();
             bool ok = hipStreamSynchronize(p->stream) == hipSuccess;
-            if (ok && ctx->horner_mode == 1) {
+            if (ok && ctx->horner_mode != 0) {
                 try {
                     ctx->check_status(p->stream);
                 } catch (...) {

@@ -97,7 +97,7 @@ PreprocessData gpu_preprocess(const PublicParams& pp, const uint16_t* db) {
         d_tw = upload_tables(nc, cc);
         // The host DB is row-major. It goes up in slabs of whole rows
         // (~256 MiB): copy, encode in place (row-local), then scatter into
-        // the resident 16-row-block layout.
+        // the resident layout (byte planes in 16-row blocks).
         d_db = dalloc<uint16_t>(R * C);
         size_t slab = ((size_t)1 << 27) / C / 16 * 16;
         if (slab < 16) slab = 16;
@@ -107,7 +107,7 @@ PreprocessData gpu_preprocess(const PublicParams& pp, const uint16_t* db) {
             const size_t nrows = R - r0 < slab ? R - r0 : slab;
             HIPX(hipMemcpyAsync(d_stage, db + r0 * C, nrows * C * sizeof(uint16_t), hipMemcpyHostToDevice, s));
             HIPX(launch_encode(d_stage, nrows, C, Da, s));
-            HIPX(launch_to_blocked(d_stage, d_db, C, r0 / 16, nrows / 16, s));
+            HIPX(launch_to_planes(d_stage, reinterpret_cast<uint8_t*>(d_db), C, r0 / 16, nrows / 16, s));
         }
         HIPX(hipStreamSynchronize(s));
         dfree(d_stage);
@@ -150,7 +150,7 @@ PreprocessData gpu_preprocess(const PublicParams& pp, const uint16_t* db) {
         for (size_t g0 = 0; g0 < G; g0 += chunk) {
             const size_t ng = G - g0 < chunk ? G - g0 : chunk;
             uint32_t* pc = arena + g0 * PC_GROUP;
-            HIPX(launch_aside(d_db, R, C, g0, ng, d_crs, d_a32, nc, s));
+            HIPX(launch_aside(reinterpret_cast<const uint8_t*>(d_db), R, C, g0, ng, d_crs, d_a32, nc, s));
             HIPX(launch_ring_embed(d_a32, ng, nc, s));
             HIPX(launch_collapse_chain(d_a32, ng, d_k5, d_dtmp, d_states, nc, cc, s));
             HIPX(launch_collapse_final(d_states, ng, d_kn, pc, nc, cc, s));
@@ -284,7 +284,7 @@ void GpuServerCtx::alloc_slots() {
     d_part2 = dalloc<uint32_t>(B * part2_stride);
     d_packed_b = dalloc<uint32_t>(B * packed_stride);
     for (int l = 0; l < kLanes; l++) {
-        d_dig[l] = dalloc<int8_t>(R * 256);
+        d_dig[l] = dalloc<int8_t>(R * 128);
         d_S[l] = dalloc<uint32_t>(64);
         d_kdig[l] = dalloc<int8_t>(kKdigBytes);
         d_KS[l] = dalloc<uint32_t>(64);
@@ -293,14 +293,15 @@ void GpuServerCtx::alloc_slots() {
     d_hflags = dalloc<uint32_t>(2 * B * T);
     d_status = dalloc<uint32_t>(1);
     HIPX(hipMemset(d_status, 0, 4));
-    if (const char* h = std::getenv("INSPIRE_HORNER")) horner_mode = std::strcmp(h, "pair") == 0 ? 1 : 0;
+    if (const char* h = std::getenv("INSPIRE_HORNER"))
+        horner_mode = std::strcmp(h, "pair") == 0 ? 1 : std::strcmp(h, "single") == 0 ? 0 : -1;
     HIPX(hipMemset(d_bq, 0, B * 2 * R * sizeof(double)));
     HIPX(hipHostMalloc((void**)&h_q, B * qlen * 8, hipHostMallocDefault));
     HIPX(hipHostMalloc((void**)&h_resp, B * resp_u64s * 8, hipHostMallocDefault));
     resident_bytes = R * C * 2 + G * PC_GROUP * 4 + 8 * N * 4 +
                      B * (qlen * 8 + resp_u64s * 8 + 2 * R * 8 +
                           (keys_stride + rgsw_stride + part_stride + 2 * C + part2_stride + packed_stride) * 4) +
-                     kLanes * (R * 256 + kKdigBytes + 128 * 4) + (horner_pair_xbuf_words(B * T) + 2 * B * T) * 4;
+                     kLanes * (R * 128 + kKdigBytes + 128 * 4) + (horner_pair_xbuf_words(B * T) + 2 * B * T) * 4;
     autotune_dbpass();
 }
 
@@ -329,9 +330,9 @@ void GpuServerCtx::autotune_dbpass() {
     auto time_plan = [&](const DbPlan& x, size_t nq) {
         auto run = [&]() {
             if (x.kind == DbPlan::F64)
-                HIPX(launch_dbpass_f64(d_db, d_bq, d_part, R, C, x, stream));
+                HIPX(launch_dbpass_f64(db_planes(), d_bq, d_part, R, C, x, stream));
             else
-                HIPX(launch_dbpass_mfma(d_db, d_bq, nq, d_dig[0], d_S[0], d_part, part_stride, R, C, x, stream));
+                HIPX(launch_dbpass_mfma(db_planes(), d_bq, nq, d_dig[0], d_S[0], d_part, part_stride, R, C, x, stream));
         };
         run();  // warm
         HIPX(hipEventRecord(e0, stream));
@@ -379,9 +380,10 @@ void GpuServerCtx::stage_dbpass(size_t count, hipStream_t s, size_t slot0, int l
         uint32_t* out = p.slices == 1 ? d_interm + q0 * 2 * C : d_part + q0 * part_stride;
         const size_t qstride = p.slices == 1 ? 2 * C : part_stride;
         if (p.kind == DbPlan::F64)
-            HIPX(launch_dbpass_f64(d_db, d_bq + q0 * 2 * R, out, R, C, p, s));
+            HIPX(launch_dbpass_f64(db_planes(), d_bq + q0 * 2 * R, out, R, C, p, s));
         else
-            HIPX(launch_dbpass_mfma(d_db, d_bq + q0 * 2 * R, nq, d_dig[lane], d_S[lane], out, qstride, R, C, p, s));
+            HIPX(launch_dbpass_mfma(db_planes(), d_bq + q0 * 2 * R, nq, d_dig[lane], d_S[lane], out, qstride, R, C, p,
+                                    s));
         if (p.slices > 1)
             HIPX(launch_reduce_slices(d_part + q0 * part_stride, part_stride, p.slices, C, d_interm + q0 * 2 * C, nq,
                                       s));
@@ -403,7 +405,10 @@ void GpuServerCtx::stage_finalize(size_t count, hipStream_t s, size_t slot0) {
 }
 
 void GpuServerCtx::stage_horner(size_t count, uint64_t* d_out, hipStream_t s, size_t slot0) {
-    if (horner_mode == 1)
+    // auto: the pair kernel shortens a lone chain (its per-step exchange costs
+    // more than it saves once many chains share the L2s)
+    const bool pair = horner_mode == 1 || (horner_mode < 0 && count * T <= kPairMaxChains);
+    if (pair)
         HIPX(launch_horner_pair(d_precomp + PC_A, PC_GROUP, G, Da, T, d_packed_b + slot0 * packed_stride,
                                 packed_stride, d_rgsw + slot0 * rgsw_stride, rgsw_stride, d_out, count,
                                 d_hxbuf + horner_pair_xbuf_words(slot0 * T), d_hflags + 2 * slot0 * T, d_status,
@@ -444,7 +449,7 @@ void GpuServerCtx::answer_host(size_t count, uint64_t* out, hipStream_t s, size_
     enqueue(dq, count, dr, s, slot0);
     HIPX(hipMemcpyAsync(hr, dr, count * resp_u64s * 8, hipMemcpyDeviceToHost, s));
     HIPX(hipStreamSynchronize(s));
-    if (horner_mode == 1) check_status(s);
+    if (horner_mode != 0) check_status(s);
     if (out) std::memcpy(out, hr, count * resp_u64s * 8);
 }
 

@@ -22,7 +22,10 @@ struct GpuServerCtx {
     size_t G = 0, Da = 0, T = 0;
 
     // resident data
-    uint16_t* d_db = nullptr;       // encoded DB, u16 in 16-row blocks: (r, c) at ((r/16)*C + c)*16 + r%16
+    // encoded DB, byte planes in 16-row blocks: plane p (low/high byte) of
+    // (r, c) at byte ((r/16)*2 + p)*C*16 + c*16 + r%16
+    uint16_t* d_db = nullptr;
+    const uint8_t* db_planes() const { return reinterpret_cast<const uint8_t*>(d_db); }
     uint32_t* d_precomp = nullptr;  // [G][PC_GROUP] arena
     uint32_t* d_tw = nullptr;       // twiddles
     ipx::NttConst nc{};
@@ -44,15 +47,17 @@ struct GpuServerCtx {
     uint32_t* d_part2 = nullptr;    // collapse sums [B][G][2][N]
     // scratch of the batched database pass, one set per concurrent lane
     static constexpr int kLanes = 4;
-    int8_t* d_dig[kLanes] = {};     // query digits [R/16][256][16]
+    int8_t* d_dig[kLanes] = {};     // query digits [R/16][<=128][16]
     uint32_t* d_S[kLanes] = {};     // digit-offset corrections [64]
     // scratch of the collapse pass: key digits [2][2N/16][64][16], key sums
     static constexpr size_t kKdigBytes = 2 * (2 * ipx::N / 16) * 64 * 16;
     int8_t* d_kdig[kLanes] = {};
     uint32_t* d_KS[kLanes] = {};
     uint32_t* d_packed_b = nullptr;
-    // Horner: 0 = one workgroup per chain, 1 = a workgroup pair (one per limb)
-    int horner_mode = 0;
+    // Horner: 0 = one workgroup per chain, 1 = a workgroup pair (one per
+    // limb), -1 = pair when a call has at most kPairMaxChains chains
+    int horner_mode = -1;
+    static constexpr size_t kPairMaxChains = 2;
     uint32_t* d_hxbuf = nullptr;    // pair exchange buffers [B*T chains]
     uint32_t* d_hflags = nullptr;   // pair step flags [B*T][2]
     uint32_t* d_status = nullptr;   // sticky device fault word (pair wait bound exceeded)

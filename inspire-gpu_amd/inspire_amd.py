@@ -46,6 +46,7 @@ U64 = C.c_uint64
 
 # (name, restype, argtypes) of every function the headers declare
 CLIENT_FUNCS = [
+    ("ipir_geometry", C.c_int, [SZ, SZ, SZ, P(ipir_params)]),
     ("ipir_query_u64s", SZ, [P(ipir_params)]),
     ("ipir_response_u64s", SZ, [P(ipir_params)]),
     ("ipir_entry_slots", SZ, [P(ipir_params)]),
@@ -200,6 +201,15 @@ class Client:
 
     def destroy(self, h):
         self.lib.ipir_client_query_destroy(h)
+
+
+def geometry(n_entries, entry_bytes, db_rows=0, lib=None):
+    """setup()'s geometry as a dict (R, C, G, D, T, cpe), or ValueError."""
+    raw = ipir_params()
+    if (lib or load("client")).ipir_geometry(n_entries, entry_bytes, db_rows, C.byref(raw)):
+        raise ValueError("setup rejects this geometry")
+    return dict(R=raw.db_rows, C=raw.db_cols, G=raw.n_packed, D=raw.interp_d, T=raw.num_cts,
+                cpe=(entry_bytes * 8 + 14) // 15)
 
 
 def make_params(n_entries, entry_bytes, db_rows, seed, R, Cc, G, D, T):

@@ -204,15 +204,17 @@ def test_pair_horner_answers_equal_single(medium8):
     kernel, alone and batched (Horner chain of 7 steps, 10 chains)."""
     geo, db, srv = medium8
     qs = [ol.make_query(geo, SEED, idx, 300 + k)[0] for k, idx in enumerate(range(0, 524288, 52429))]
-    base = srv.answer_batch(qs)
-    assert srv.lib.ipir_dbg_set_horner(srv.h, 1) == 0
+    assert srv.lib.ipir_dbg_set_horner(srv.h, 0) == 0
     try:
+        base = srv.answer_batch(qs)
+        one_single = srv.answer_batch(qs[:1])
+        assert srv.lib.ipir_dbg_set_horner(srv.h, 1) == 0
         pair = srv.answer_batch(qs)
         one = srv.answer_batch(qs[:1])
     finally:
-        assert srv.lib.ipir_dbg_set_horner(srv.h, 0) == 0
+        assert srv.lib.ipir_dbg_set_horner(srv.h, -1) == 0
     assert np.array_equal(pair, base)
-    assert np.array_equal(one[0], base[0])
+    assert np.array_equal(one[0], base[0]) and np.array_equal(one_single[0], base[0])
 
 
 def test_batch_validation_codes(medium8):
