@@ -33,17 +33,17 @@ hipError_t launch_ntt_rns(uint32_t* polys, size_t npoly, const NttConst& nc, boo
 // slice's rows of db[r][c]*b_{q,limb}[r] mod q_limb.
 struct DbPlan {
     enum Kind { F64, MFMA } kind;
-    uint32_t param;   // F64: 16-row blocks per iteration (2, 4, 8); MFMA: 16-row digit tiles (1, 2, 4, 8)
+    uint32_t param;   // F64: 16-row blocks per iteration (2, 4, 8); MFMA: 16-row digit tiles (1, 2, 4, 8, 16)
     uint32_t slices;  // row slices (power of two)
 };
 // Row-major staging rows (16*nkb of them) -> blocks [kb0, kb0 + nkb) of the planes.
 hipError_t launch_to_planes(const uint16_t* src, uint8_t* dst, size_t C, size_t kb0, size_t nkb, hipStream_t s);
 DbPlan dbplan_f64(size_t R, size_t C, int unroll, size_t target_blocks);
 // Queries one batched (int8 MFMA) database pass serves.
-constexpr size_t kMfmaMaxQueries = 16;
+constexpr size_t kMfmaMaxQueries = 32;
 // int8-MFMA plan for nq (1..kMfmaMaxQueries) queries.
 DbPlan dbplan_mfma(size_t R, size_t C, size_t nq, size_t target_blocks);
-// Digit rows the MFMA plan stages per 16-row block (int8 scratch: R*Mp bytes, Mp <= 128).
+// Digit rows the MFMA plan stages per 16-row block (int8 scratch: R*Mp bytes, Mp <= 256).
 size_t dbplan_mp(const DbPlan& p);
 // One query, fp64 FMA on the vector ALUs.
 hipError_t launch_dbpass_f64(const uint8_t* db, const double* bq, uint32_t* out, size_t R, size_t C,

@@ -253,6 +253,93 @@ __global__ __launch_bounds__(256, 2) void k_dbpass_mfma(const uint8_t* __restric
                        slice, C, col0 + n * 16 + (l & 15));
 }
 
+// Many queries (MT = 8 or 16 digit tiles, up to 16 or 32 queries): as above,
+// but the block's 4 waves share the step's digit tile (Mp x 64 bytes)
+// through LDS instead of each reading it from L2: the tile for step ks+1
+// is loaded into registers during step ks and written to the other half of
+// a double buffer after it, one barrier per step.
+template <int MT, int NT>
+__global__ __launch_bounds__(256, 2) void k_dbpass_mfma_sa(const uint8_t* __restrict__ db,
+                                                           const int8_t* __restrict__ dig,
+                                                           const uint32_t* __restrict__ S,
+                                                           uint32_t* __restrict__ out, size_t out_qstride,
+                                                           uint32_t C, uint32_t nq, uint32_t kbps) {
+    constexpr uint32_t Mp = 16 * MT;
+    constexpr uint32_t AU = Mp * 4 / 256;  // 16-byte units of the step's digit tile per thread
+    __shared__ i32x4 atile[2][4 * Mp];      // [buffer][block of the step][row]
+    const uint32_t tid = threadIdx.x, w = tid >> 6, l = tid & 63;
+    const uint32_t slice = blockIdx.y;
+    const uint32_t col0 = (blockIdx.x * 4 + w) * NT * 16;
+    const uint32_t kb0 = slice * kbps;
+    const uint32_t nks = kbps / 4;
+    const i32x4* bsrc = reinterpret_cast<const i32x4*>(db) + (size_t)(kb0 + (l >> 4)) * 2 * C + col0 + (l & 15);
+    const size_t bstep = (size_t)8 * C;
+    // the step's digit tile is 4*Mp contiguous 16-byte units starting at block 4*ks
+    const i32x4* atile_src = reinterpret_cast<const i32x4*>(dig) + (size_t)kb0 * Mp;
+    const i32x4 flip = {(int32_t)0x80808080, (int32_t)0x80808080, (int32_t)0x80808080, (int32_t)0x80808080};
+    i32x4 acc[MT][NT][2];
+#pragma unroll
+    for (int a = 0; a < MT; a++)
+#pragma unroll
+        for (int n = 0; n < NT; n++) acc[a][n][0] = acc[a][n][1] = i32x4{0, 0, 0, 0};
+    i32x4 v[2][NT][2];
+    i32x4 apre[AU];
+    auto load_b = [&](int bi, uint32_t ks) {
+#pragma unroll
+        for (int n = 0; n < NT; n++) {
+            v[bi][n][0] = __builtin_nontemporal_load(bsrc + ks * bstep + n * 16);
+            v[bi][n][1] = __builtin_nontemporal_load(bsrc + ks * bstep + n * 16 + C);
+        }
+    };
+    auto load_a = [&](uint32_t ks) {
+#pragma unroll
+        for (int u = 0; u < (int)AU; u++) apre[u] = atile_src[(size_t)ks * 4 * Mp + tid + 256 * u];
+    };
+    auto store_a = [&](int bi) {
+#pragma unroll
+        for (int u = 0; u < (int)AU; u++) atile[bi][tid + 256 * u] = apre[u];
+    };
+    auto compute = [&](int bi, int abuf) {
+#pragma unroll
+        for (int n = 0; n < NT; n++) {
+            const i32x4 Bl = v[bi][n][0] ^ flip, Bh = v[bi][n][1] ^ flip;
+#pragma unroll
+            for (int a = 0; a < MT; a++) {
+                const i32x4 A = atile[abuf][(l >> 4) * Mp + a * 16 + (l & 15)];
+                acc[a][n][0] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A, Bl, acc[a][n][0], 0, 0, 0);
+                acc[a][n][1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A, Bh, acc[a][n][1], 0, 0, 0);
+            }
+        }
+    };
+    load_a(0);
+    store_a(0);
+    load_b(0, 0);
+    if (nks > 1) load_a(1);
+    __syncthreads();
+    for (uint32_t ks = 0; ks < nks; ks += 2) {  // nks is even
+        load_b(1, ks + 1);
+        compute(0, 0);
+        store_a(1);  // digits of step ks+1
+        __syncthreads();
+        if (ks + 2 < nks) {
+            load_b(0, ks + 2);
+            load_a(ks + 2);
+        }
+        compute(1, 1);
+        if (ks + 2 < nks) {
+            store_a(0);  // digits of step ks+2
+            if (ks + 3 < nks) load_a(ks + 3);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int a = 0; a < MT; a++)
+#pragma unroll
+        for (int n = 0; n < NT; n++)
+            store_pair(acc[a][n][0], acc[a][n][1], (uint32_t)a * 4 + (l >> 4), nq, slice == 0, S, out, out_qstride,
+                       slice, C, col0 + n * 16 + (l & 15));
+}
+
 // Largest power-of-two slice count s with blocks*s <= target that keeps
 // every slice a whole number of `unit`-block chunks.
 uint32_t pick_slices(size_t nkb, size_t blocks, size_t target, size_t unit) {
@@ -262,7 +349,7 @@ uint32_t pick_slices(size_t nkb, size_t blocks, size_t target, size_t unit) {
 }
 
 // 16-column tiles per wave for MT digit tiles (accumulators MT*NT*8 registers).
-uint32_t mfma_nt(uint32_t mt) { return mt <= 4 ? 4 : 2; }
+uint32_t mfma_nt(uint32_t mt) { return mt <= 4 ? 4 : mt == 8 ? 2 : 1; }
 
 }  // namespace
 
@@ -285,7 +372,7 @@ DbPlan dbplan_f64(size_t R, size_t C, int unroll, size_t target_blocks) {
 DbPlan dbplan_mfma(size_t R, size_t C, size_t nq, size_t target_blocks) {
     DbPlan p;
     p.kind = DbPlan::MFMA;
-    p.param = nq <= 2 ? 1 : nq <= 4 ? 2 : nq <= 8 ? 4 : 8;  // 2 queries per 16-row digit tile
+    p.param = nq <= 2 ? 1 : nq <= 4 ? 2 : nq <= 8 ? 4 : nq <= 16 ? 8 : 16;  // 2 queries per 16-row digit tile
     const size_t cols = 64 * mfma_nt(p.param);
     // slices: whole number of 128-row (8-block) chunks so the step count is even
     p.slices = pick_slices(R / 16, C / cols, target_blocks, 8);
@@ -326,13 +413,14 @@ hipError_t launch_dbpass_mfma(const uint8_t* db, const double* bq, size_t nq, in
     hipLaunchKernelGGL(k_bsum, dim3((unsigned)(2 * nq)), dim3(256), 0, s, bq, (uint32_t)R, S);
     const uint32_t Cc = (uint32_t)C, Q = (uint32_t)nq;
     const dim3 grid((unsigned)(C / cols), p.slices);
-#define MFMA(MT, NT) \
-    hipLaunchKernelGGL((k_dbpass_mfma<MT, NT>), grid, dim3(256), 0, s, db, dig, S, out, out_qstride, Cc, Q, kbps)
+#define MFMA(K, MT, NT) \
+    hipLaunchKernelGGL((K<MT, NT>), grid, dim3(256), 0, s, db, dig, S, out, out_qstride, Cc, Q, kbps)
     switch (p.param) {
-        case 1: MFMA(1, 4); break;
-        case 2: MFMA(2, 4); break;
-        case 4: MFMA(4, 4); break;
-        case 8: MFMA(8, 2); break;
+        case 1: MFMA(k_dbpass_mfma, 1, 4); break;
+        case 2: MFMA(k_dbpass_mfma, 2, 4); break;
+        case 4: MFMA(k_dbpass_mfma, 4, 4); break;
+        case 8: MFMA(k_dbpass_mfma_sa, 8, 2); break;
+        case 16: MFMA(k_dbpass_mfma_sa, 16, 1); break;
         default: return hipErrorInvalidValue;
     }
 #undef MFMA

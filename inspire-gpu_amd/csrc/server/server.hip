@@ -284,7 +284,7 @@ void GpuServerCtx::alloc_slots() {
     d_part2 = dalloc<uint32_t>(B * part2_stride);
     d_packed_b = dalloc<uint32_t>(B * packed_stride);
     for (int l = 0; l < kLanes; l++) {
-        d_dig[l] = dalloc<int8_t>(R * 128);
+        d_dig[l] = dalloc<int8_t>(R * 256);
         d_S[l] = dalloc<uint32_t>(64);
         d_kdig[l] = dalloc<int8_t>(kKdigBytes);
         d_KS[l] = dalloc<uint32_t>(64);
@@ -301,7 +301,7 @@ void GpuServerCtx::alloc_slots() {
     resident_bytes = R * C * 2 + G * PC_GROUP * 4 + 8 * N * 4 +
                      B * (qlen * 8 + resp_u64s * 8 + 2 * R * 8 +
                           (keys_stride + rgsw_stride + part_stride + 2 * C + part2_stride + packed_stride) * 4) +
-                     kLanes * (R * 128 + kKdigBytes + 128 * 4) + (horner_pair_xbuf_words(B * T) + 2 * B * T) * 4;
+                     kLanes * (R * 256 + kKdigBytes + 128 * 4) + (horner_pair_xbuf_words(B * T) + 2 * B * T) * 4;
     autotune_dbpass();
 }
 

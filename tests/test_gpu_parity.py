@@ -138,13 +138,16 @@ def test_noncanonical_inputs_are_reduced(served):
 
 
 # ---------------------------------------------------------------- batch contract
+MB8 = 24  # max_batch of the medium8 server (covers the 17..32-query MFMA variant)
+
+
 @pytest.fixture(scope="module")
 def medium8():
     n, w, rows = 524288, 120, 2048  # G=8, D=8 (reference tests/test_gpu_batch.cu:26-37)
     rc, geo = ol.setup(n, w, rows)
     db = np.random.default_rng(42).integers(0, P, size=(geo["R"], geo["C"]), dtype=np.uint16)
     say("[medium8] GPU create")
-    srv = ia.Server(db, n, w, rows, max_batch=10, crs_seed=bytes(SEED))
+    srv = ia.Server(db, n, w, rows, max_batch=MB8, crs_seed=bytes(SEED))
     yield geo, db, srv
     srv.close()
 
@@ -160,7 +163,7 @@ def test_db_pass_paths_match_oracle(medium8, path):
     assert srv.lib.ipir_dbg_db_rows(srv.h, 0, R, ia.ptr(enc)) == 0
     flat = enc.reshape(-1)
     rng = np.random.default_rng(100 + path)
-    for nq in (1, 2, 3, 5, 8, 10):
+    for nq in (1, 2, 3, 5, 8, 10, 17, 24):
         b = np.zeros((nq, 2, R), np.uint64)
         b[:, 0] = rng.integers(0, Q0, (nq, R), dtype=np.uint64)
         b[:, 1] = rng.integers(0, Q1, (nq, R), dtype=np.uint64)
@@ -180,7 +183,7 @@ def test_db_pass_paths_match_oracle(medium8, path):
 
 def test_batch_equals_singles_and_extracts(medium8):
     geo, db, srv = medium8
-    idxs = [0, 42, 1000, 100000, 524287] * 2
+    idxs = [0, 42, 1000, 100000, 524287] * 4  # 20: a 17..32-query DB pass, two collapse passes
     qs, sts = [], []
     for t, idx in enumerate(idxs):
         q, st = ol.make_query(geo, SEED, idx, 7000 + t)
@@ -221,9 +224,9 @@ def test_batch_validation_codes(medium8):
     geo, db, srv = medium8
     lib = srv.lib
     q, _ = ol.make_query(geo, SEED, 1, 1)
-    views = srv.views([q] * 11)
-    out = np.zeros((11, geo["T"] * 4 * N), np.uint64)
-    assert lib.ipir_answer_batch(srv.h, views, 11, ia.ptr(out)) == 2   # over max_batch
+    views = srv.views([q] * (MB8 + 1))
+    out = np.zeros((MB8 + 1, geo["T"] * 4 * N), np.uint64)
+    assert lib.ipir_answer_batch(srv.h, views, MB8 + 1, ia.ptr(out)) == 2   # over max_batch
     assert lib.ipir_answer_batch(srv.h, views, 0, ia.ptr(out)) == 0    # empty batch
     assert lib.ipir_answer_batch(None, views, 1, ia.ptr(out)) == 1
     bad = srv.views([q])
@@ -238,7 +241,7 @@ def test_capi_round_trip_with_product_client(medium8):
     assert p.seed == bytes(SEED), "pinned CRS seed honoured"
     assert (p.ring_n, p.d_eff, p.db_rows, p.db_cols, p.n_packed) == (N, 2, geo["R"], geo["C"], geo["G"])
     caps = srv.caps()
-    assert caps.max_batch == 10 and caps.num_cts == geo["T"] and caps.resident_bytes > 0
+    assert caps.max_batch == MB8 and caps.num_cts == geo["T"] and caps.resident_bytes > 0
     assert caps.device_free_bytes > 0
     cl = ia.Client(p, ia.load("server"))
     hs, qs, idxs = [], [], [5, 77777, 524000]
