@@ -58,10 +58,10 @@ hipError_t launch_reduce_slices(const uint32_t* part, size_t part_stride, uint32
                                 size_t nq, hipStream_t s);
 
 // Queries one collapse pass serves.
-constexpr size_t kCollapseMaxQueries = 16;
+constexpr size_t kCollapseMaxQueries = 32;
 // Precomp-tensor pass in dot-product form (int8 MFMA, Dd read once) for nq
 // queries whose NTT-form keys sit at keys + q*kstride:
-// part2[q][g][limb][N] at q*p2_stride. kdig (512 KiB) and KS (2*nq words)
+// part2[q][g][limb][N] at q*p2_stride. kdig (1 MiB) and KS (2*nq words)
 // are scratch.
 hipError_t launch_collapse(const uint32_t* precomp, size_t G, const uint32_t* keys, size_t kstride, size_t nq,
                            int8_t* kdig, uint32_t* KS, uint32_t* part2, size_t p2_stride, hipStream_t s);

@@ -145,88 +145,131 @@ __global__ __launch_bounds__(256) void k_ksum(const uint32_t* __restrict__ keys,
     if (threadIdx.x == 0) KS[blockIdx.x] = red64(red[0], (int)c);
 }
 
-// Block (128-slot tile of i, limb, group), 4 waves of 32 slots (2 N tiles)
-// each, all M rows (MT tiles of 4 queries x 4 digits). Per 64-row K step a
-// lane loads its 16-byte operand of each plane straight from HBM (a step
-// ahead) and the key digits (L2-resident). With Dd entry = sum_a 256^a x_a +
-// 128*65793 (x_a the signed planes, x_3 the top byte) and K = sum_b 256^b d_b:
+// Recombine one lane's 16 accumulators (4 Dd planes x 4 key digits) of
+// query q for one slot and store the canonical residue. With Dd entry =
+// sum_a 256^a x_a + 128*65793 (x_a the signed planes, x_3 the top byte) and
+// K = sum_b 256^b d_b:
 //   sum_k Dd_k K_k = sum_{a,b} 256^(a+b) A_ab + 128*65793*sum_k K_k,
-// A_ab = sum_k x_a d_b over 4096 rows: |A_ab| <= 2^26, exact in i32.
-template <int MT>
-__global__ __launch_bounds__(256) void k_collapse_mfma(const uint32_t* __restrict__ precomp,
-                                                       const int8_t* __restrict__ kdig,
-                                                       const uint32_t* __restrict__ KS,
-                                                       uint32_t* __restrict__ part2, size_t p2_stride,
-                                                       uint32_t nq) {
+// A_ab = sum_k x_a d_b over 4096 rows: |A_ab| <= 2^26, exact in i32; the
+// 16 products fold into 7 levels weighted by 256^k mod q, |sum| < 2^58.
+__device__ __forceinline__ void collapse_store(const i32x4 (&acc)[4], uint32_t q, int c, uint32_t g, uint32_t i,
+                                               const uint32_t* __restrict__ KS, uint32_t* __restrict__ part2,
+                                               size_t p2_stride) {
+    const uint64_t qc = c ? Q1 : Q0;
+    int64_t lv[7] = {0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int pl = 0; pl < 4; pl++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) lv[pl + b] += acc[pl][b];
+    int64_t s = (int64_t)8421504 * KS[q * 2 + c];  // 128*65793 * sum(K)
+    uint64_t wk = 1;                                // 256^k mod q_c
+#pragma unroll
+    for (int k = 0; k < 7; k++) {
+        s += (int64_t)wk * lv[k];
+        wk = (wk * 256) % qc;
+    }
+    const uint64_t off = qc * ((1ull << 59) / qc);
+    part2[(size_t)q * p2_stride + (size_t)g * 2 * N + (size_t)c * N + i] = red64((uint64_t)(s + (int64_t)off), c);
+}
+
+// Block (4*NT*16-slot tile of i, limb, group): 4 waves of NT 16-slot tiles,
+// all M rows (MT tiles of 4 queries x 4 digits). Per 64-row K step a lane
+// loads its 16-byte operand of each Dd plane straight from HBM (a step
+// ahead). The key digits of the step (Mp x 64 bytes) are read by each wave
+// from L2 (SHARED = false) or staged once per block in an LDS double
+// buffer (SHARED = true, for large MT).
+template <int MT, int NT, bool SHARED>
+__global__ __launch_bounds__(256, 2) void k_collapse_mfma(const uint32_t* __restrict__ precomp,
+                                                          const int8_t* __restrict__ kdig,
+                                                          const uint32_t* __restrict__ KS,
+                                                          uint32_t* __restrict__ part2, size_t p2_stride,
+                                                          uint32_t nq) {
     constexpr uint32_t Mp = 16 * MT;
     constexpr uint32_t NKB = DD_K / 16;
-    const uint32_t w = threadIdx.x >> 6, l = threadIdx.x & 63;
-    const uint32_t it = blockIdx.x, c = blockIdx.y, g = blockIdx.z;
-    const uint32_t col0 = it * 128 + w * 32;
+    constexpr uint32_t AU = SHARED ? Mp * 4 / 256 : 1;  // 16-byte units of a step's digit tile per thread
+    __shared__ i32x4 atile[SHARED ? 2 : 1][SHARED ? 4 * Mp : 1];
+    const uint32_t tid = threadIdx.x, w = tid >> 6, l = tid & 63;
+    const uint32_t it = blockIdx.x, g = blockIdx.z;
+    const int c = (int)blockIdx.y;
+    const uint32_t col0 = (it * 4 + w) * NT * 16;
     const i32x4* dd = reinterpret_cast<const i32x4*>(precomp + (size_t)g * PC_GROUP + PC_DD) +
                       (size_t)c * 4 * NKB * N + (size_t)(l >> 4) * N + col0 + (l & 15);
     const size_t pstride = (size_t)NKB * N;  // plane, in 16-byte units
     const size_t bstep = (size_t)4 * N;      // 4 blocks of 16 rows
-    const i32x4* asrc = reinterpret_cast<const i32x4*>(kdig) + ((size_t)c * NKB + (l >> 4)) * Mp + (l & 15);
-    const size_t astep = (size_t)4 * Mp;
-    i32x4 acc[MT][2][4];
+    const i32x4* kd = reinterpret_cast<const i32x4*>(kdig) + (size_t)c * NKB * Mp;  // this limb's digits
+    i32x4 acc[MT][NT][4];
 #pragma unroll
     for (int a = 0; a < MT; a++)
 #pragma unroll
-        for (int n = 0; n < 2; n++)
+        for (int n = 0; n < NT; n++)
 #pragma unroll
             for (int pl = 0; pl < 4; pl++) acc[a][n][pl] = i32x4{0, 0, 0, 0};
-    i32x4 B[2][2][4], A[2][MT];
+    i32x4 B[2][NT][4];
+    i32x4 A[SHARED ? 1 : 2][SHARED ? 1 : MT];
+    i32x4 apre[AU];
     auto load = [&](int bi, uint32_t ks) {
 #pragma unroll
-        for (int n = 0; n < 2; n++)
+        for (int n = 0; n < NT; n++)
 #pragma unroll
             for (int pl = 0; pl < 4; pl++)
                 B[bi][n][pl] = __builtin_nontemporal_load(dd + pl * pstride + ks * bstep + n * 16);
+        if (!SHARED) {
 #pragma unroll
-        for (int a = 0; a < MT; a++) A[bi][a] = asrc[ks * astep + a * 16];
+            for (int a = 0; a < MT; a++) A[SHARED ? 0 : bi][SHARED ? 0 : a] = kd[((size_t)ks * 4 + (l >> 4)) * Mp + a * 16 + (l & 15)];
+        }
+    };
+    auto load_a = [&](uint32_t ks) {
+#pragma unroll
+        for (int u = 0; u < (int)AU; u++) apre[u] = kd[(size_t)ks * 4 * Mp + tid + 256 * u];
+    };
+    auto store_a = [&](int abuf) {
+#pragma unroll
+        for (int u = 0; u < (int)AU; u++) atile[SHARED ? abuf : 0][SHARED ? tid + 256 * u : 0] = apre[u];
     };
     auto compute = [&](int bi) {
 #pragma unroll
-        for (int n = 0; n < 2; n++)
+        for (int a = 0; a < MT; a++) {
+            const i32x4 Aop = SHARED ? atile[SHARED ? bi : 0][SHARED ? (l >> 4) * Mp + a * 16 + (l & 15) : 0]
+                                     : A[SHARED ? 0 : bi][SHARED ? 0 : a];
 #pragma unroll
-            for (int pl = 0; pl < 4; pl++)
+            for (int n = 0; n < NT; n++)
 #pragma unroll
-                for (int a = 0; a < MT; a++)
-                    acc[a][n][pl] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[bi][a], B[bi][n][pl], acc[a][n][pl], 0, 0, 0);
+                for (int pl = 0; pl < 4; pl++)
+                    acc[a][n][pl] = __builtin_amdgcn_mfma_i32_16x16x64_i8(Aop, B[bi][n][pl], acc[a][n][pl], 0, 0, 0);
+        }
     };
     constexpr uint32_t nks = DD_K / 64;  // even
+    if (SHARED) {
+        load_a(0);
+        store_a(0);
+        load_a(1);
+        __syncthreads();
+    }
     load(0, 0);
     for (uint32_t ks = 0; ks < nks; ks += 2) {
         load(1, ks + 1);
         compute(0);
+        if (SHARED) {
+            store_a(1);  // digits of step ks+1
+            __syncthreads();
+            if (ks + 2 < nks) load_a(ks + 2);
+        }
         if (ks + 2 < nks) load(0, ks + 2);
         compute(1);
+        if (SHARED) {
+            if (ks + 2 < nks) {
+                store_a(0);  // digits of step ks+2
+                if (ks + 3 < nks) load_a(ks + 3);
+            }
+            __syncthreads();
+        }
     }
-    const uint64_t qc = c ? Q1 : Q0;
-    uint64_t wk[7];  // 256^k mod q_c
-    wk[0] = 1;
-#pragma unroll
-    for (int k = 1; k < 7; k++) wk[k] = (wk[k - 1] * 256) % qc;
-    const uint64_t off = qc * ((1ull << 59) / qc);
 #pragma unroll
     for (int a = 0; a < MT; a++) {
         const uint32_t q = a * 4 + (l >> 4);
         if (q >= nq) continue;
-        const int64_t corr = (int64_t)8421504 * KS[q * 2 + c];  // 128*65793
 #pragma unroll
-        for (int n = 0; n < 2; n++) {
-            int64_t lv[7] = {0, 0, 0, 0, 0, 0, 0};
-#pragma unroll
-            for (int pl = 0; pl < 4; pl++)
-#pragma unroll
-                for (int b = 0; b < 4; b++) lv[pl + b] += acc[a][n][pl][b];
-            int64_t s = corr;
-#pragma unroll
-            for (int k = 0; k < 7; k++) s += (int64_t)wk[k] * lv[k];  // |s| < 2^58
-            const uint32_t i = col0 + n * 16 + (l & 15);
-            part2[(size_t)q * p2_stride + (size_t)g * 2 * N + (size_t)c * N + i] = red64((uint64_t)(s + (int64_t)off), (int)c);
-        }
+        for (int n = 0; n < NT; n++) collapse_store(acc[a][n], q, c, g, col0 + n * 16 + (l & 15), KS, part2, p2_stride);
     }
 }
 
@@ -243,19 +286,23 @@ hipError_t launch_dd_relayout(const uint32_t* Dp, const uint32_t* Dm, size_t gst
 hipError_t launch_collapse(const uint32_t* precomp, size_t G, const uint32_t* keys, size_t kstride, size_t nq,
                            int8_t* kdig, uint32_t* KS, uint32_t* part2, size_t p2_stride, hipStream_t s) {
     if (nq == 0 || nq > kCollapseMaxQueries) return hipErrorInvalidValue;
-    const uint32_t mt = nq <= 4 ? 1 : nq <= 8 ? 2 : 4;
+    const uint32_t mt = nq <= 4 ? 1 : nq <= 8 ? 2 : nq <= 16 ? 4 : 8;  // 4 queries per 16-row digit tile
     const uint32_t Mp = 16 * mt;
     const size_t nthr = 2 * (DD_K / 16) * (Mp / 4);
     hipLaunchKernelGGL(k_kdigits, dim3((unsigned)((nthr + 255) / 256)), dim3(256), 0, s, keys, kstride, (uint32_t)nq,
                        Mp, kdig);
     hipLaunchKernelGGL(k_ksum, dim3((unsigned)(2 * nq)), dim3(256), 0, s, keys, kstride, KS);
-    const dim3 grid(N / 128, 2, (unsigned)G);
-    if (mt == 1)
-        hipLaunchKernelGGL(k_collapse_mfma<1>, grid, dim3(256), 0, s, precomp, kdig, KS, part2, p2_stride, (uint32_t)nq);
-    else if (mt == 2)
-        hipLaunchKernelGGL(k_collapse_mfma<2>, grid, dim3(256), 0, s, precomp, kdig, KS, part2, p2_stride, (uint32_t)nq);
-    else
-        hipLaunchKernelGGL(k_collapse_mfma<4>, grid, dim3(256), 0, s, precomp, kdig, KS, part2, p2_stride, (uint32_t)nq);
+    const uint32_t Q = (uint32_t)nq;
+#define COLLAPSE(MT, NT, SH)                                                                                   \
+    hipLaunchKernelGGL((k_collapse_mfma<MT, NT, SH>), dim3(N / (64 * NT), 2, (unsigned)G), dim3(256), 0, s, \
+                       precomp, kdig, KS, part2, p2_stride, Q)
+    switch (mt) {
+        case 1: COLLAPSE(1, 2, false); break;
+        case 2: COLLAPSE(2, 2, false); break;
+        case 4: COLLAPSE(4, 2, false); break;
+        default: COLLAPSE(8, 1, true); break;
+    }
+#undef COLLAPSE
     return hipGetLastError();
 }
 

@@ -49,8 +49,8 @@ struct GpuServerCtx {
     static constexpr int kLanes = 4;
     int8_t* d_dig[kLanes] = {};     // query digits [R/16][<=256][16]
     uint32_t* d_S[kLanes] = {};     // digit-offset corrections [64]
-    // scratch of the collapse pass: key digits [2][2N/16][64][16], key sums
-    static constexpr size_t kKdigBytes = 2 * (2 * ipx::N / 16) * 64 * 16;
+    // scratch of the collapse pass: key digits [2][2N/16][<=128][16], key sums
+    static constexpr size_t kKdigBytes = 2 * (2 * ipx::N / 16) * 128 * 16;
     int8_t* d_kdig[kLanes] = {};
     uint32_t* d_KS[kLanes] = {};
     uint32_t* d_packed_b = nullptr;
