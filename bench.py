@@ -93,6 +93,14 @@ def cpu_baseline(geo, n_groups_sample=8, col_frac=4, horner_steps=16):
     }
 
 
+def int8_macs(nq, R, Cc):
+    """int8 MACs the MFMA database pass executes for nq queries: 16-row digit
+    tiles of 2 queries (4 digits x 2 limbs each) padded to the kernel's tile
+    count, times the DB's two byte planes, per row and column."""
+    tiles = 1 if nq <= 2 else 2 if nq <= 4 else 4 if nq <= 8 else 8 if nq <= 16 else 16
+    return 16.0 * tiles * 2 * R * Cc
+
+
 def serving(lib, srv, qs, capacity_qps, resp_u64s, rates=(0.25, 0.5, 0.9), seconds=3.0, max_inflight=2):
     """Poisson arrivals through the continuous-batching loop (ipir_loop_*):
     at each rate (a fraction of the measured batched capacity) submit
@@ -247,9 +255,34 @@ def main():
     stages32 = {k: round(v, 4) for k, v in zip(names, st32)}
     log(f"[bench] stages B=1 (ms): {stages1}")
     log(f"[bench] stages B={B} (ms): {stages32}")
+    # the dominant kernel: the batch-1 database pass, timed alone over 20
+    # launches with HIP events on the server stream (algorithmic bytes 2RC)
+    def dbpass(nq):
+        ms = C.c_double()
+        assert lib.ipir_dbg_time_dbpass(srv.h, nq, 20, C.byref(ms)) == 0
+        name = C.create_string_buffer(128)
+        lib.ipir_dbg_dbpass_kernel(srv.h, nq, name, 128)
+        return ms.value, name.value.decode()
     db_bytes = 2.0 * R * Cc
-    dbpass_ms = st1[1]  # one plain database pass (batch 1)
+    dbpass_ms, kname = dbpass(1)
     achieved = db_bytes / (dbpass_ms * 1e-3) / 1e9
+    dbB_ms, knameB = dbpass(min(B, 32))
+    log(f"[bench] DB pass: B=1 {kname} {dbpass_ms:.3f} ms ({achieved:.0f} GB/s); "
+        f"B={min(B, 32)} {knameB} {dbB_ms:.3f} ms")
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "dbpass_traffic.json")
+    if os.path.exists(tfile):
+        t = json.load(open(tfile))
+        if t.get("workload") == label and t.get("kernel") == kname.split(" ")[0]:
+            traffic = t["traffic_bytes"]
+    # NTT throughput at N = 2048, both limbs per polynomial, 1024 polynomials per launch
+    ntt = {}
+    for inv in (0, 1):
+        ms = C.c_double()
+        assert lib.ipir_dbg_ntt_bench(1024, inv, 20, C.byref(ms)) == 0
+        ntt["intt" if inv else "ntt"] = {"polys_per_s": round(1024 * 2 / (ms.value * 1e-3)),
+                                         "us_per_launch_1024x2": round(ms.value * 1e3, 2)}
+    log(f"[bench] NTT throughput (2048-point, per limb): {ntt}")
 
     serve = None
     if args.serving:
@@ -287,8 +320,16 @@ def main():
         "preprocess_s": round(t_pre, 2),
         "bit_exact_check": bool(ok),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_dbpass (plain database pass, batch 1)"},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": f"{kname} (database pass, batch 1; {dbpass_ms:.3f} ms per launch)"},
+        "batched_db_pass": {"kernel": knameB, "queries": min(B, 32), "ms": round(dbB_ms, 3),
+                            "hbm_GBs": round(db_bytes / (dbB_ms * 1e-3) / 1e9, 1),
+                            "int8_mfma_TOPs": round(2 * int8_macs(min(B, 32), R, Cc) / (dbB_ms * 1e-3) / 1e12, 1),
+                            "int8_mfma_frac_of_5000": round(2 * int8_macs(min(B, 32), R, Cc) / (dbB_ms * 1e-3)
+                                                            / 5.0e15, 4)},
+        "horner_step_us": {"batch1": round(st1[4] * 1e3 / max(1, geo["Da"] - 1), 2),
+                           f"batch{B}": round(st32[4] * 1e3 / max(1, geo["Da"] - 1), 2)},
+        "ntt_throughput": ntt,
         "cpu_baseline": cpu,
     }
     if serve is not None:

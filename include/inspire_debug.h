@@ -27,6 +27,13 @@ int ipir_dbg_db_rows(const ipir_server* srv, size_t row0, size_t nrows, uint16_t
  * plans of the answer pipeline, 0 = the fp64 pass per query, 1 = the int8
  * MFMA pass. */
 int ipir_dbg_db_pass(ipir_server* srv, const uint64_t* b, size_t nq, int path, uint32_t* interm);
+/* Device time of the database pass alone (the kernels answer_batch
+ * launches for it, on the server stream) over the first nq query slots as
+ * last staged, averaged over iters passes. */
+int ipir_dbg_time_dbpass(ipir_server* srv, size_t nq, int iters, double* ms_per_pass);
+/* The database-pass kernel (profiler name) and row-slice count the server
+ * uses for a batch of nq queries, NUL-terminated into out[cap]. */
+int ipir_dbg_dbpass_kernel(const ipir_server* srv, size_t nq, char* out, size_t cap);
 /* Answer one host flat query and return the intermediate tensors:
  * interm [2][db_cols] (DB pass, coefficient form), packed_b
  * [n_packed][2][N] (after the collapse), resp (flat response). Any output
@@ -38,6 +45,9 @@ int ipir_dbg_answer_taps(ipir_server* srv, const uint64_t* query, uint32_t* inte
  * horner: cts [L][a|b][limb][N] NTT form, rgsw flat (4*d_eff*2*N u64,
  * [part][digit][limb][coeff]); out [a|b][limb][N]. */
 int ipir_dbg_ntt(const uint32_t* in, uint32_t* out, size_t npoly, int inverse);
+/* Device time of one launch transforming npoly RNS polynomials (both limbs)
+ * resident in HBM, averaged over iters launches. */
+int ipir_dbg_ntt_bench(size_t npoly, int inverse, int iters, double* ms_per_call);
 int ipir_dbg_horner(const uint64_t* cts, size_t L, const uint64_t* rgsw, uint64_t* out);
 /* horner with a chosen kernel: variant 0 = one workgroup per chain, 1 = a
  * workgroup pair (one per limb). */

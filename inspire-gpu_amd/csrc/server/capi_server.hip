@@ -1,7 +1,9 @@
 // Server half of the C ABI (reference src/capi.cu:51-132) plus the
 // device-resident batch entry point and the inspection entry points.
 // Exceptions never cross the boundary: failures become NULL / nonzero.
+#include <cstdio>
 #include <cstring>
+#include <string>
 #include <memory>
 #include <vector>
 
@@ -201,6 +203,40 @@ int ipir_dbg_db_pass(ipir_server* srv, const uint64_t* b, size_t nq, int path, u
     }
 }
 
+int ipir_dbg_time_dbpass(ipir_server* srv, size_t nq, int iters, double* ms_per_pass) {
+    if (!srv || nq == 0 || nq > srv->ctx->cfg.max_batch || iters <= 0 || !ms_per_pass) return 1;
+    GpuServerCtx* c = srv->ctx;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = 0;
+    try {
+        std::lock_guard<std::mutex> lk(c->mu);
+        hip_check(hipEventCreate(&e0), "event");
+        hip_check(hipEventCreate(&e1), "event");
+        c->stage_dbpass(nq, c->stream, 0);  // warm-up
+        hip_check(hipEventRecord(e0, c->stream), "record");
+        for (int i = 0; i < iters; i++) c->stage_dbpass(nq, c->stream, 0);
+        hip_check(hipEventRecord(e1, c->stream), "record");
+        hip_check(hipEventSynchronize(e1), "sync");
+        float ms = 0;
+        hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+        *ms_per_pass = ms / iters;
+    } catch (...) {
+        rc = 4;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    return rc;
+}
+
+int ipir_dbg_dbpass_kernel(const ipir_server* srv, size_t nq, char* out, size_t cap) {
+    if (!srv || !out || cap == 0 || nq == 0 || nq > ipx::kMfmaMaxQueries) return 1;
+    const GpuServerCtx* c = srv->ctx;
+    const ipx::DbPlan p = nq == 1 ? c->plan1 : ipx::dbplan_mfma(c->pp.db_rows, c->pp.db_cols, nq, c->mfma_target);
+    const std::string s = ipx::dbplan_kernel(p) + " slices=" + std::to_string(p.slices);
+    std::snprintf(out, cap, "%s", s.c_str());
+    return 0;
+}
+
 int ipir_dbg_answer_taps(ipir_server* srv, const uint64_t* query, uint32_t* interm, uint32_t* packed_b,
                          uint64_t* resp) {
     if (!srv || !query) return 1;
@@ -266,6 +302,32 @@ int ipir_dbg_ntt(const uint32_t* in, uint32_t* out, size_t npoly, int inverse) {
         ipx::launch_ntt_rns(d, npoly, st->nc, inverse != 0, nullptr) != hipSuccess ||
         hipMemcpy(out, d, bytes, hipMemcpyDeviceToHost) != hipSuccess)
         rc = 4;
+    dfree(d);
+    return rc;
+}
+
+int ipir_dbg_ntt_bench(size_t npoly, int inverse, int iters, double* ms_per_call) {
+    if (npoly == 0 || iters <= 0 || !ms_per_call) return 1;
+    Standalone* st = standalone();
+    if (!st) return 4;
+    uint32_t* d = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = 0;
+    if (hipMalloc(&d, npoly * 2 * N * 4) || hipMemset(d, 0, npoly * 2 * N * 4) || hipEventCreate(&e0) ||
+        hipEventCreate(&e1))
+        rc = 4;
+    if (!rc && ipx::launch_ntt_rns(d, npoly, st->nc, inverse != 0, nullptr)) rc = 4;  // warm-up
+    if (!rc) {
+        (void)hipEventRecord(e0, nullptr);
+        for (int i = 0; i < iters && !rc; i++)
+            if (ipx::launch_ntt_rns(d, npoly, st->nc, inverse != 0, nullptr)) rc = 4;
+        (void)hipEventRecord(e1, nullptr);
+        float ms = 0;
+        if (!rc && (hipEventSynchronize(e1) || hipEventElapsedTime(&ms, e0, e1))) rc = 4;
+        *ms_per_call = ms / iters;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
     dfree(d);
     return rc;
 }

@@ -6,6 +6,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <string>
+
 #include "dev_common.h"
 
 namespace ipx {
@@ -45,6 +47,8 @@ constexpr size_t kMfmaMaxQueries = 32;
 DbPlan dbplan_mfma(size_t R, size_t C, size_t nq, size_t target_blocks);
 // Digit rows the MFMA plan stages per 16-row block (int8 scratch: R*Mp bytes, Mp <= 256).
 size_t dbplan_mp(const DbPlan& p);
+// Name of the kernel a plan launches (as profilers report it, without arguments).
+std::string dbplan_kernel(const DbPlan& p);
 // One query, fp64 FMA on the vector ALUs.
 hipError_t launch_dbpass_f64(const uint8_t* db, const double* bq, uint32_t* out, size_t R, size_t C,
                              const DbPlan& p, hipStream_t s);

@@ -12,6 +12,8 @@
 // Every kernel computes the reference's mat-vec (src/gpu_online.cu:23-41)
 //   interm[q][c][col] = sum_r db[r][col] * b_{q,c}[r] mod q_c
 // exactly, over a slice of the rows; slices are summed mod q afterwards.
+#include <string>
+
 #include "kernels.h"
 
 namespace ipx {
@@ -383,6 +385,13 @@ DbPlan dbplan_mfma(size_t R, size_t C, size_t nq, size_t target_blocks) {
 }
 
 size_t dbplan_mp(const DbPlan& p) { return p.kind == DbPlan::MFMA ? 16 * p.param : 0; }
+
+std::string dbplan_kernel(const DbPlan& p) {
+    if (p.kind == DbPlan::F64) return "k_dbpass_f64<" + std::to_string(p.param) + ">";
+    const uint32_t nt = mfma_nt(p.param);
+    return std::string(p.param >= 8 ? "k_dbpass_mfma_sa<" : "k_dbpass_mfma<") + std::to_string(p.param) + ", " +
+           std::to_string(nt) + ">";
+}
 
 hipError_t launch_dbpass_f64(const uint8_t* db, const double* bq, uint32_t* out, size_t R, size_t C,
                              const DbPlan& p, hipStream_t s) {

@@ -293,3 +293,37 @@ def test_serving_loop_results_equal_alone(medium8):
     lib.ipir_loop_stop(loop)
     for i in range(len(qs)):
         assert np.array_equal(outs[i], alone[i]), f"loop result {i} differs from alone"
+
+
+# ---------------------------------------------------------------- several response ciphertexts
+def test_two_ciphertext_responses_extract():
+    """n_packed = 512 > D = 256: two response ciphertexts per query (the
+    16 GB-and-up geometry's Horner split, reference src/gpu_protocol.cu:
+    1300-1339), 4 GiB DB at db_rows 2048. Every response extracts to its
+    entry, in a batch and alone, and the two Horner kernels agree."""
+    n, w, rows = 2 ** 25, 120, 2048
+    rc, geo = ol.setup(n, w, rows)
+    assert geo["G"] == 512 and geo["T"] == 2
+    db = np.random.default_rng(9).integers(0, P, size=(geo["R"], geo["C"]), dtype=np.uint16)
+    say("[wide] GPU create (G=512, T=2)")
+    srv = ia.Server(db, n, w, rows, max_batch=4, crs_seed=bytes(SEED))
+    try:
+        idxs = [5, 2 ** 24 + 77, n - 1]  # first half, second half (other ciphertext), last entry
+        qs, sts = [], []
+        for k, idx in enumerate(idxs):
+            q, st = ol.make_query(geo, SEED, idx, 60 + k)
+            qs.append(q)
+            sts.append(st)
+        batch = srv.answer_batch(qs)
+        for k, idx in enumerate(idxs):
+            got = ol.extract(geo, sts[k], batch[k].reshape(geo["T"], -1))
+            assert np.array_equal(got, ol.expected_slots(geo, db, idx)), f"entry {idx}"
+        single = srv.answer_batch(qs[:1])
+        assert np.array_equal(single[0], batch[0])
+        assert srv.lib.ipir_dbg_set_horner(srv.h, 1) == 0
+        pair = srv.answer_batch(qs)
+        assert srv.lib.ipir_dbg_set_horner(srv.h, 0) == 0
+        one = srv.answer_batch(qs)
+        assert np.array_equal(pair, batch) and np.array_equal(one, batch)
+    finally:
+        srv.close()
