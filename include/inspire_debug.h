@@ -52,6 +52,9 @@ int ipir_dbg_horner(const uint64_t* cts, size_t L, const uint64_t* rgsw, uint64_
 /* horner with a chosen kernel: variant 0 = one workgroup per chain, 1 = a
  * workgroup pair (one per limb). */
 int ipir_dbg_horner_variant(const uint64_t* cts, size_t L, const uint64_t* rgsw, uint64_t* out, int variant);
+/* Set the groups per chunk of the pipelined answer (a divisor of n_packed;
+ * 1 if the partial-sum buffer cannot hold the resulting plans). */
+int ipir_dbg_set_group_chunk(ipir_server* srv, size_t groups);
 /* Select the Horner kernel a server's answers use: mode as variant above,
  * or -1 (the default) to use the pair only for calls with at most two
  * chains. */

@@ -336,6 +336,20 @@ int ipir_dbg_horner(const uint64_t* cts, size_t L, const uint64_t* rgsw, uint64_
     return ipir_dbg_horner_variant(cts, L, rgsw, out, 0);
 }
 
+int ipir_dbg_set_group_chunk(ipir_server* srv, size_t groups) {
+    if (!srv || groups == 0 || groups > srv->ctx->G || srv->ctx->G % groups) return 1;
+    GpuServerCtx* c = srv->ctx;
+    std::lock_guard<std::mutex> lk(c->mu);
+    // the partial-sum buffer was sized for the setup-time chunk width
+    const size_t R = c->pp.db_rows, ncols = groups * N;
+    for (size_t nq = 1; nq <= ipx::kMfmaMaxQueries; nq++) {
+        const ipx::DbPlan x = nq == 1 ? c->plan_for(1, ncols) : ipx::dbplan_mfma(R, ncols, nq, c->mfma_target);
+        if (x.slices > 1 && x.slices * 2 * c->pp.db_cols > c->part_stride) return 1;
+    }
+    c->gchunk = groups;
+    return 0;
+}
+
 int ipir_dbg_set_horner(ipir_server* srv, int mode) {
     if (!srv || mode < -1 || mode > 1) return 1;
     std::lock_guard<std::mutex> lk(srv->ctx->mu);
@@ -357,11 +371,12 @@ int ipir_dbg_horner_variant(const uint64_t* cts, size_t L, const uint64_t* rgsw,
         }
     for (size_t x = 0; x < 4 * b; x++) rg[x] = (uint32_t)(rgsw[x] % ((x / N) % 2 ? Q1 : Q0));
     uint32_t *d_pa = nullptr, *d_pb = nullptr, *d_rg = nullptr, *d_x = nullptr, *d_f = nullptr, *d_st = nullptr;
+    uint32_t* d_hs = nullptr;
     uint64_t* d_o = nullptr;
     int rc = 0;
     if (hipMalloc(&d_pa, pa.size() * 4) || hipMalloc(&d_pb, pb.size() * 4) || hipMalloc(&d_rg, rg.size() * 4) ||
         hipMalloc(&d_o, 4 * N * 8) || hipMalloc(&d_x, ipx::horner_pair_xbuf_words(1) * 4) || hipMalloc(&d_f, 8) ||
-        hipMalloc(&d_st, 4) || hipMemset(d_st, 0, 4))
+        hipMalloc(&d_st, 4) || hipMemset(d_st, 0, 4) || hipMalloc(&d_hs, 4 * N * 4))
         rc = 4;
     if (!rc && (hipMemcpy(d_pa, pa.data(), pa.size() * 4, hipMemcpyHostToDevice) ||
                 hipMemcpy(d_pb, pb.data(), pb.size() * 4, hipMemcpyHostToDevice) ||
@@ -369,16 +384,17 @@ int ipir_dbg_horner_variant(const uint64_t* cts, size_t L, const uint64_t* rgsw,
         rc = 4;
     if (!rc) {
         const hipError_t e =
-            variant == 1 ? ipx::launch_horner_pair(d_pa, 2 * N, L, L, 1, d_pb, L * 2 * N, d_rg, 4 * b, d_o, 1, d_x, d_f,
-                                                   d_st, GpuServerCtx::kHornerSpinLimit, st->nc, st->cc, nullptr)
-                         : ipx::launch_horner(d_pa, 2 * N, L, L, 1, d_pb, L * 2 * N, d_rg, 4 * b, d_o, 1, st->nc,
-                                              st->cc, nullptr);
+            variant == 1 ? ipx::launch_horner_pair(d_pa, 2 * N, L, L, 1, d_pb, L * 2 * N, d_rg, 4 * b, d_o, d_hs, 0, L,
+                                                   1, d_x, d_f, d_st, GpuServerCtx::kHornerSpinLimit, st->nc, st->cc,
+                                                   nullptr)
+                         : ipx::launch_horner(d_pa, 2 * N, L, L, 1, d_pb, L * 2 * N, d_rg, 4 * b, d_o, d_hs, 0, L, 1,
+                                              st->nc, st->cc, nullptr);
         uint32_t status = 0;
         if (e || hipMemcpy(out, d_o, 4 * N * 8, hipMemcpyDeviceToHost) ||
             hipMemcpy(&status, d_st, 4, hipMemcpyDeviceToHost) || status)
             rc = 4;
     }
-    dfree(d_pa); dfree(d_pb); dfree(d_rg); dfree(d_o); dfree(d_x); dfree(d_f); dfree(d_st);
+    dfree(d_pa); dfree(d_pb); dfree(d_rg); dfree(d_o); dfree(d_x); dfree(d_f); dfree(d_st); dfree(d_hs);
     return rc;
 }
 

@@ -40,46 +40,63 @@ struct DbPlan {
 };
 // Row-major staging rows (16*nkb of them) -> blocks [kb0, kb0 + nkb) of the planes.
 hipError_t launch_to_planes(const uint16_t* src, uint8_t* dst, size_t C, size_t kb0, size_t nkb, hipStream_t s);
-DbPlan dbplan_f64(size_t R, size_t C, int unroll, size_t target_blocks);
+// Plans for a pass over ncols columns (the slice count fills the device).
+DbPlan dbplan_f64(size_t R, size_t ncols, int unroll, size_t target_blocks);
 // Queries one batched (int8 MFMA) database pass serves.
 constexpr size_t kMfmaMaxQueries = 32;
 // int8-MFMA plan for nq (1..kMfmaMaxQueries) queries.
-DbPlan dbplan_mfma(size_t R, size_t C, size_t nq, size_t target_blocks);
+DbPlan dbplan_mfma(size_t R, size_t ncols, size_t nq, size_t target_blocks);
 // Digit rows the MFMA plan stages per 16-row block (int8 scratch: R*Mp bytes, Mp <= 256).
 size_t dbplan_mp(const DbPlan& p);
+// Column granularity of a plan (one block's columns): col0 and ncols are multiples of it.
+size_t dbplan_cols(const DbPlan& p);
 // Name of the kernel a plan launches (as profilers report it, without arguments).
 std::string dbplan_kernel(const DbPlan& p);
-// One query, fp64 FMA on the vector ALUs.
-hipError_t launch_dbpass_f64(const uint8_t* db, const double* bq, uint32_t* out, size_t R, size_t C,
-                             const DbPlan& p, hipStream_t s);
-// nq queries (bq [q][limb][R] doubles) in one pass of the DB through int8
-// MFMA; dig (R*Mp bytes) and S (2*nq words) are scratch.
-hipError_t launch_dbpass_mfma(const uint8_t* db, const double* bq, size_t nq, int8_t* dig, uint32_t* S,
-                              uint32_t* out, size_t out_qstride, size_t R, size_t C, const DbPlan& p,
+// One query, fp64 FMA on the vector ALUs, over columns [col0, col0 + ncols).
+hipError_t launch_dbpass_f64(const uint8_t* db, const double* bq, uint32_t* out, size_t R, size_t C, size_t col0,
+                             size_t ncols, const DbPlan& p, hipStream_t s);
+// Query digits and offset sums of nq queries (bq [q][limb][R] doubles) for
+// an MFMA plan: dig (R*Mp bytes) and S (2*nq words).
+hipError_t launch_mfma_digits(const double* bq, size_t nq, size_t R, const DbPlan& p, int8_t* dig, uint32_t* S,
                               hipStream_t s);
-// interm[q][limb][C] = sum over the S row slices of part (per query stride part_stride).
+// The int8-MFMA pass of nq queries over columns [col0, col0 + ncols), from
+// launch_mfma_digits' output for a plan with the same digit tile count.
+hipError_t launch_dbpass_mfma(const uint8_t* db, const int8_t* dig, const uint32_t* S, size_t nq, uint32_t* out,
+                              size_t out_qstride, size_t R, size_t C, size_t col0, size_t ncols, const DbPlan& p,
+                              hipStream_t s);
+// interm[q][limb][col] = sum over the S row slices of part (per query stride
+// part_stride, slice stride 2C), for columns [col0, col0 + ncols).
 hipError_t launch_reduce_slices(const uint32_t* part, size_t part_stride, uint32_t S, size_t C, uint32_t* interm,
-                                size_t nq, hipStream_t s);
+                                size_t nq, size_t col0, size_t ncols, hipStream_t s);
 
 // Queries one collapse pass serves.
 constexpr size_t kCollapseMaxQueries = 32;
-// Precomp-tensor pass in dot-product form (int8 MFMA, Dd read once) for nq
-// queries whose NTT-form keys sit at keys + q*kstride:
-// part2[q][g][limb][N] at q*p2_stride. kdig (1 MiB) and KS (2*nq words)
-// are scratch.
-hipError_t launch_collapse(const uint32_t* precomp, size_t G, const uint32_t* keys, size_t kstride, size_t nq,
-                           int8_t* kdig, uint32_t* KS, uint32_t* part2, size_t p2_stride, hipStream_t s);
-// packed_b[q][g] = NTT(interm[q] group slice) - collapse partials - Df*Kn.
+// Key digits and key sums of nq queries whose NTT-form keys sit at
+// keys + q*kstride, for launch_collapse: kdig (1 MiB) and KS (2*nq words).
+hipError_t launch_collapse_keys(const uint32_t* keys, size_t kstride, size_t nq, int8_t* kdig, uint32_t* KS,
+                                hipStream_t s);
+// Precomp-tensor pass in dot-product form (int8 MFMA, Dd read once) of ng
+// groups whose precomp blocks start at precomp_g0, for nq queries:
+// part2_g0 + q*p2_stride + g*2N holds [limb][N] of group g (relative).
+hipError_t launch_collapse(const uint32_t* precomp_g0, size_t ng, const int8_t* kdig, const uint32_t* KS, size_t nq,
+                           uint32_t* part2_g0, size_t p2_stride, hipStream_t s);
+// packed_b[q][g] = NTT(interm[q] group slice) - collapse sums - Df*Kn, for
+// groups [g0, g0 + ng) (absolute indexes into every buffer).
 hipError_t launch_pack_finalize(const uint32_t* interm, size_t C, const uint32_t* part2, size_t part2_stride,
-                                const uint32_t* precomp, size_t G, const uint32_t* keys,
+                                const uint32_t* precomp, size_t g0, size_t ng, const uint32_t* keys,
                                 size_t keys_stride, uint32_t* packed_b, size_t packed_stride, size_t nq,
                                 const NttConst& nc, hipStream_t s);
-// Horner over each response ciphertext; writes the flat u64 response
-// [q][ct][a|b][limb][N]. Group g's packed a-part (shared by all queries) is
-// at pa_base + g*pa_stride; its b-part at packed_b + q*packed_stride + g*2N.
+// Horner over each response ciphertext, restricted to the groups in
+// [g_lo, g_hi): a chain whose top group lies in the range starts there,
+// one that started in an earlier launch resumes from hstate
+// ([q][ct][a|b][limb][N] u32), one that continues below g_lo saves its state
+// there, and a finished chain writes the flat u64 response
+// [q][ct][a|b][limb][N] (launches go from the highest group range down).
+// Group g's packed a-part (shared by all queries) is at pa_base +
+// g*pa_stride; its b-part at packed_b + q*packed_stride + g*2N.
 hipError_t launch_horner(const uint32_t* pa_base, size_t pa_stride, size_t G, size_t Da, size_t T,
-                         const uint32_t* packed_b, size_t packed_stride, const uint32_t* rgsw,
-                         size_t rgsw_stride, uint64_t* out, size_t nq, const NttConst& nc,
+                         const uint32_t* packed_b, size_t packed_stride, const uint32_t* rgsw, size_t rgsw_stride,
+                         uint64_t* out, uint32_t* hstate, size_t g_lo, size_t g_hi, size_t nq, const NttConst& nc,
                          const CrtConst& cc, hipStream_t s);
 
 // The same chains with two workgroups per chain (one per limb) exchanging
@@ -90,9 +107,9 @@ hipError_t launch_horner(const uint32_t* pa_base, size_t pa_stride, size_t G, si
 size_t horner_pair_xbuf_words(size_t nchains);
 hipError_t launch_horner_pair(const uint32_t* pa_base, size_t pa_stride, size_t G, size_t Da, size_t T,
                               const uint32_t* packed_b, size_t packed_stride, const uint32_t* rgsw,
-                              size_t rgsw_stride, uint64_t* out, size_t nq, uint32_t* xbuf, uint32_t* flags,
-                              uint32_t* status, uint32_t spin_limit, const NttConst& nc, const CrtConst& cc,
-                              hipStream_t s);
+                              size_t rgsw_stride, uint64_t* out, uint32_t* hstate, size_t g_lo, size_t g_hi,
+                              size_t nq, uint32_t* xbuf, uint32_t* flags, uint32_t* status, uint32_t spin_limit,
+                              const NttConst& nc, const CrtConst& cc, hipStream_t s);
 
 // ---------------------------------------------------------------- preprocess
 hipError_t launch_encode(uint16_t* db, size_t R, size_t C, size_t Da, hipStream_t s);

@@ -283,20 +283,28 @@ hipError_t launch_dd_relayout(const uint32_t* Dp, const uint32_t* Dm, size_t gst
     return hipGetLastError();
 }
 
-hipError_t launch_collapse(const uint32_t* precomp, size_t G, const uint32_t* keys, size_t kstride, size_t nq,
-                           int8_t* kdig, uint32_t* KS, uint32_t* part2, size_t p2_stride, hipStream_t s) {
+static uint32_t collapse_mt(size_t nq) { return nq <= 4 ? 1 : nq <= 8 ? 2 : nq <= 16 ? 4 : 8; }  // 4 queries per tile
+
+hipError_t launch_collapse_keys(const uint32_t* keys, size_t kstride, size_t nq, int8_t* kdig, uint32_t* KS,
+                                hipStream_t s) {
     if (nq == 0 || nq > kCollapseMaxQueries) return hipErrorInvalidValue;
-    const uint32_t mt = nq <= 4 ? 1 : nq <= 8 ? 2 : nq <= 16 ? 4 : 8;  // 4 queries per 16-row digit tile
-    const uint32_t Mp = 16 * mt;
+    const uint32_t Mp = 16 * collapse_mt(nq);
     const size_t nthr = 2 * (DD_K / 16) * (Mp / 4);
     hipLaunchKernelGGL(k_kdigits, dim3((unsigned)((nthr + 255) / 256)), dim3(256), 0, s, keys, kstride, (uint32_t)nq,
                        Mp, kdig);
     hipLaunchKernelGGL(k_ksum, dim3((unsigned)(2 * nq)), dim3(256), 0, s, keys, kstride, KS);
+    return hipGetLastError();
+}
+
+hipError_t launch_collapse(const uint32_t* precomp_g0, size_t ng, const int8_t* kdig, const uint32_t* KS, size_t nq,
+                           uint32_t* part2_g0, size_t p2_stride, hipStream_t s) {
+    if (nq == 0 || nq > kCollapseMaxQueries) return hipErrorInvalidValue;
+    if (ng == 0) return hipSuccess;
     const uint32_t Q = (uint32_t)nq;
-#define COLLAPSE(MT, NT, SH)                                                                                   \
-    hipLaunchKernelGGL((k_collapse_mfma<MT, NT, SH>), dim3(N / (64 * NT), 2, (unsigned)G), dim3(256), 0, s, \
-                       precomp, kdig, KS, part2, p2_stride, Q)
-    switch (mt) {
+#define COLLAPSE(MT, NT, SH)                                                                                    \
+    hipLaunchKernelGGL((k_collapse_mfma<MT, NT, SH>), dim3(N / (64 * NT), 2, (unsigned)ng), dim3(256), 0, s, \
+                       precomp_g0, kdig, KS, part2_g0, p2_stride, Q)
+    switch (collapse_mt(nq)) {
         case 1: COLLAPSE(1, 2, false); break;
         case 2: COLLAPSE(2, 2, false); break;
         case 4: COLLAPSE(4, 2, false); break;

@@ -363,21 +363,21 @@ hipError_t launch_to_planes(const uint16_t* src, uint8_t* dst, size_t C, size_t 
     return hipGetLastError();
 }
 
-DbPlan dbplan_f64(size_t R, size_t C, int unroll, size_t target_blocks) {
+DbPlan dbplan_f64(size_t R, size_t ncols, int unroll, size_t target_blocks) {
     DbPlan p;
     p.kind = DbPlan::F64;
     p.param = (uint32_t)unroll;
-    p.slices = pick_slices(R / 16, C / 256, target_blocks, (size_t)(unroll > 4 ? unroll : 4));
+    p.slices = pick_slices(R / 16, ncols / 256, target_blocks, (size_t)(unroll > 4 ? unroll : 4));
     return p;
 }
 
-DbPlan dbplan_mfma(size_t R, size_t C, size_t nq, size_t target_blocks) {
+DbPlan dbplan_mfma(size_t R, size_t ncols, size_t nq, size_t target_blocks) {
     DbPlan p;
     p.kind = DbPlan::MFMA;
     p.param = nq <= 2 ? 1 : nq <= 4 ? 2 : nq <= 8 ? 4 : nq <= 16 ? 8 : 16;  // 2 queries per 16-row digit tile
     const size_t cols = 64 * mfma_nt(p.param);
     // slices: whole number of 128-row (8-block) chunks so the step count is even
-    p.slices = pick_slices(R / 16, C / cols, target_blocks, 8);
+    p.slices = pick_slices(R / 16, ncols / cols, target_blocks, 8);
     // at most 2^16 rows per slice keeps |H_k|, |L_k| <= 2^30 (i32-exact) and
     // the recombined sum below 2^58.1
     while (R / p.slices > 65536) p.slices *= 2;
@@ -386,6 +386,8 @@ DbPlan dbplan_mfma(size_t R, size_t C, size_t nq, size_t target_blocks) {
 
 size_t dbplan_mp(const DbPlan& p) { return p.kind == DbPlan::MFMA ? 16 * p.param : 0; }
 
+size_t dbplan_cols(const DbPlan& p) { return p.kind == DbPlan::MFMA ? 64 * mfma_nt(p.param) : 256; }
+
 std::string dbplan_kernel(const DbPlan& p) {
     if (p.kind == DbPlan::F64) return "k_dbpass_f64<" + std::to_string(p.param) + ">";
     const uint32_t nt = mfma_nt(p.param);
@@ -393,37 +395,46 @@ std::string dbplan_kernel(const DbPlan& p) {
            std::to_string(nt) + ">";
 }
 
-hipError_t launch_dbpass_f64(const uint8_t* db, const double* bq, uint32_t* out, size_t R, size_t C,
-                             const DbPlan& p, hipStream_t s) {
-    if (p.kind != DbPlan::F64 || C % 256) return hipErrorInvalidValue;
+hipError_t launch_dbpass_f64(const uint8_t* db, const double* bq, uint32_t* out, size_t R, size_t C, size_t col0,
+                             size_t ncols, const DbPlan& p, hipStream_t s) {
+    if (p.kind != DbPlan::F64 || ncols % 256 || col0 % 256 || col0 + ncols > C) return hipErrorInvalidValue;
     const uint32_t kbps = (uint32_t)(R / 16 / p.slices);
-    const dim3 grid((unsigned)(C / 256), p.slices);
+    const dim3 grid((unsigned)(ncols / 256), p.slices);
+    const uint8_t* d = db + col0 * 16;
+    uint32_t* o = out + col0;
     switch (p.param) {
-        case 2: hipLaunchKernelGGL(k_dbpass_f64<2>, grid, dim3(256), 0, s, db, bq, out, (uint32_t)R, (uint32_t)C, kbps); break;
-        case 8: hipLaunchKernelGGL(k_dbpass_f64<8>, grid, dim3(256), 0, s, db, bq, out, (uint32_t)R, (uint32_t)C, kbps); break;
-        default: hipLaunchKernelGGL(k_dbpass_f64<4>, grid, dim3(256), 0, s, db, bq, out, (uint32_t)R, (uint32_t)C, kbps); break;
+        case 2: hipLaunchKernelGGL(k_dbpass_f64<2>, grid, dim3(256), 0, s, d, bq, o, (uint32_t)R, (uint32_t)C, kbps); break;
+        case 8: hipLaunchKernelGGL(k_dbpass_f64<8>, grid, dim3(256), 0, s, d, bq, o, (uint32_t)R, (uint32_t)C, kbps); break;
+        default: hipLaunchKernelGGL(k_dbpass_f64<4>, grid, dim3(256), 0, s, d, bq, o, (uint32_t)R, (uint32_t)C, kbps); break;
     }
     return hipGetLastError();
 }
 
-hipError_t launch_dbpass_mfma(const uint8_t* db, const double* bq, size_t nq, int8_t* dig, uint32_t* S,
-                              uint32_t* out, size_t out_qstride, size_t R, size_t C, const DbPlan& p,
+hipError_t launch_mfma_digits(const double* bq, size_t nq, size_t R, const DbPlan& p, int8_t* dig, uint32_t* S,
                               hipStream_t s) {
     if (p.kind != DbPlan::MFMA || nq == 0 || nq > kMfmaMaxQueries) return hipErrorInvalidValue;
     const uint32_t Mp = (uint32_t)dbplan_mp(p);
     if (2 * nq > Mp / 4) return hipErrorInvalidValue;
-    const uint32_t nkb = (uint32_t)(R / 16);
-    const uint32_t kbps = nkb / p.slices;
-    const uint32_t cols = 64 * mfma_nt(p.param);
-    if (kbps % 8 || C % cols) return hipErrorInvalidValue;
-    const size_t nthr = (size_t)nkb * (Mp / 4);
+    const size_t nthr = (R / 16) * (Mp / 4);
     hipLaunchKernelGGL(k_digits, dim3((unsigned)((nthr + 255) / 256)), dim3(256), 0, s, bq, (uint32_t)R,
                        (uint32_t)nq, Mp, dig);
     hipLaunchKernelGGL(k_bsum, dim3((unsigned)(2 * nq)), dim3(256), 0, s, bq, (uint32_t)R, S);
+    return hipGetLastError();
+}
+
+hipError_t launch_dbpass_mfma(const uint8_t* db, const int8_t* dig, const uint32_t* S, size_t nq, uint32_t* out,
+                              size_t out_qstride, size_t R, size_t C, size_t col0, size_t ncols, const DbPlan& p,
+                              hipStream_t s) {
+    if (p.kind != DbPlan::MFMA || nq == 0 || nq > kMfmaMaxQueries) return hipErrorInvalidValue;
+    const uint32_t Mp = (uint32_t)dbplan_mp(p);
+    const uint32_t kbps = (uint32_t)(R / 16) / p.slices;
+    const size_t cols = dbplan_cols(p);
+    if (2 * nq > Mp / 4 || kbps % 8 || ncols % cols || col0 % cols || col0 + ncols > C) return hipErrorInvalidValue;
     const uint32_t Cc = (uint32_t)C, Q = (uint32_t)nq;
-    const dim3 grid((unsigned)(C / cols), p.slices);
-#define MFMA(K, MT, NT) \
-    hipLaunchKernelGGL((K<MT, NT>), grid, dim3(256), 0, s, db, dig, S, out, out_qstride, Cc, Q, kbps)
+    const dim3 grid((unsigned)(ncols / cols), p.slices);
+    const uint8_t* d = db + col0 * 16;
+    uint32_t* o = out + col0;
+#define MFMA(K, MT, NT) hipLaunchKernelGGL((K<MT, NT>), grid, dim3(256), 0, s, d, dig, S, o, out_qstride, Cc, Q, kbps)
     switch (p.param) {
         case 1: MFMA(k_dbpass_mfma, 1, 4); break;
         case 2: MFMA(k_dbpass_mfma, 2, 4); break;

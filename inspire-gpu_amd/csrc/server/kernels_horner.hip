@@ -33,7 +33,8 @@ __global__ __launch_bounds__(1024) void k_horner_pair(const uint32_t* __restrict
                                                       uint32_t G, uint32_t Da, uint32_t T,
                                                       const uint32_t* __restrict__ packed_b, size_t pstride,
                                                       const uint32_t* __restrict__ rgsw, size_t rstride,
-                                                      uint64_t* __restrict__ out, uint32_t* __restrict__ xbuf,
+                                                      uint64_t* __restrict__ out, uint32_t* __restrict__ hstate,
+                                                      uint32_t g_lo, uint32_t g_hi, uint32_t* __restrict__ xbuf,
                                                       uint32_t* __restrict__ flags, uint32_t* __restrict__ status,
                                                       uint32_t nchains, uint32_t spin_limit, NttConst nc,
                                                       CrtConst cc) {
@@ -44,36 +45,50 @@ __global__ __launch_bounds__(1024) void k_horner_pair(const uint32_t* __restrict
     const uint32_t b = blockIdx.x, chain = (b >> 4) * 8 + (b & 7);
     const int c = (int)((b >> 3) & 1);
     if (chain >= nchains) return;  // both partners of a padding chain leave
+    const uint32_t ct = chain % T, qi = chain / T;
+    const uint32_t start = ct * Da;
+    const uint32_t L = (G - start) < Da ? (G - start) : Da;
+    const uint32_t top = start + L - 1;
+    // this launch covers the chain's groups in [lo, hi), highest first
+    const uint32_t lo = start > g_lo ? start : g_lo, hi = start + L < g_hi ? start + L : g_hi;
+    if (lo >= hi) return;  // both partners leave
     const uint32_t tid = threadIdx.x, team = tid >> 8, t = tid & 255;
     const uint32_t q = qof(c);
     for (uint32_t x = tid; x < 4 * N; x += 1024) twl[x] = nc.tw[(size_t)c * 4 * N + x];
     if (tid == 0) abort_flag = 0;
     const Tw tw{twl, twl + N, twl + 2 * N, twl + 3 * N};
-    const uint32_t ct = chain % T, qi = chain / T;
-    const uint32_t start = ct * Da;
-    const uint32_t L = (G - start) < Da ? (G - start) : Da;
     const uint32_t* pb = packed_b + qi * pstride;
     const uint32_t* rg = rgsw + qi * rstride;
     for (uint32_t x = tid; x < 8 * N; x += 1024) {
         const uint32_t part = x / (2 * N), j = (x / N) & 1, i = x % N;
         rgl[part][j][i] = rg[(size_t)part * BLK + ((size_t)j * 2 + c) * N + i];
     }
+    uint32_t* hs = hstate + ((size_t)qi * T + ct) * 4 * N;  // chain state between launches, [a|b][limb][N]
     uint32_t acc_a[2], acc_b[2];
-    {
-        const uint32_t gl = start + L - 1;
+    int first;
+    if (top < g_hi) {  // the chain starts here: acc = packed[top]
 #pragma unroll
         for (int e = 0; e < 2; e++) {
             const uint32_t i = tid + 1024 * e;
-            acc_a[e] = pa_base[(size_t)gl * pa_stride + (size_t)c * N + i];
-            acc_b[e] = pb[(size_t)gl * 2 * N + (size_t)c * N + i];
+            acc_a[e] = pa_base[(size_t)top * pa_stride + (size_t)c * N + i];
+            acc_b[e] = pb[(size_t)top * 2 * N + (size_t)c * N + i];
         }
+        first = (int)top - 1;
+    } else {
+#pragma unroll
+        for (int e = 0; e < 2; e++) {
+            const uint32_t i = tid + 1024 * e;
+            acc_a[e] = hs[(size_t)c * N + i];
+            acc_b[e] = hs[2 * (size_t)N + (size_t)c * N + i];
+        }
+        first = (int)hi - 1;
     }
     uint32_t* xb = xbuf + (size_t)chain * 2 * 2 * 2 * N;  // [parity][limb][a|b][N]
     uint32_t* my_flag = flags + (size_t)chain * 2 + c;
     uint32_t* other_flag = flags + (size_t)chain * 2 + (1 - c);
     __syncthreads();
     uint32_t k = 0;
-    for (int st = (int)L - 2; st >= 0; st--) {
+    for (int gs = first; gs >= (int)lo; gs--) {
         k++;
 #pragma unroll
         for (int e = 0; e < 2; e++) {
@@ -138,7 +153,7 @@ __global__ __launch_bounds__(1024) void k_horner_pair(const uint32_t* __restrict
         __syncthreads();
         ntt_fwd_lds(buf[team], t, tw, c);
         // RGSW parts: 0 top_a, 1 top_b, 2 bot_a, 3 bot_b (a-digits use top, b-digits bottom)
-        const uint32_t gi = start + (uint32_t)st;
+        const uint32_t gi = (uint32_t)gs;
 #pragma unroll
         for (int e = 0; e < 2; e++) {
             const uint32_t i = tid + 1024 * e;
@@ -153,6 +168,15 @@ __global__ __launch_bounds__(1024) void k_horner_pair(const uint32_t* __restrict
             acc_b[e] = add_mod(red64(sb, c), pb[(size_t)gi * 2 * N + (size_t)c * N + i], q);
         }
         __syncthreads();
+    }
+    if (lo > start) {  // the chain continues in a later launch
+#pragma unroll
+        for (int e = 0; e < 2; e++) {
+            const uint32_t i = tid + 1024 * e;
+            hs[(size_t)c * N + i] = acc_a[e];
+            hs[2 * (size_t)N + (size_t)c * N + i] = acc_b[e];
+        }
+        return;
     }
     uint64_t* o = out + ((size_t)qi * T + ct) * 4 * N;
 #pragma unroll
@@ -169,16 +193,16 @@ size_t horner_pair_xbuf_words(size_t nchains) { return nchains * 2 * 2 * 2 * N; 
 
 hipError_t launch_horner_pair(const uint32_t* pa_base, size_t pa_stride, size_t G, size_t Da, size_t T,
                               const uint32_t* packed_b, size_t packed_stride, const uint32_t* rgsw,
-                              size_t rgsw_stride, uint64_t* out, size_t nq, uint32_t* xbuf, uint32_t* flags,
-                              uint32_t* status, uint32_t spin_limit, const NttConst& nc, const CrtConst& cc,
-                              hipStream_t s) {
+                              size_t rgsw_stride, uint64_t* out, uint32_t* hstate, size_t g_lo, size_t g_hi,
+                              size_t nq, uint32_t* xbuf, uint32_t* flags, uint32_t* status, uint32_t spin_limit,
+                              const NttConst& nc, const CrtConst& cc, hipStream_t s) {
     const size_t nchains = nq * T;
-    if (nchains == 0) return hipSuccess;
+    if (nchains == 0 || g_lo >= g_hi) return hipSuccess;
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(256), 0, s, flags, 2 * nchains, 0u);
     const size_t blocks = 16 * ((nchains + 7) / 8);
     hipLaunchKernelGGL(k_horner_pair, dim3((unsigned)blocks), dim3(1024), 0, s, pa_base, pa_stride, (uint32_t)G,
-                       (uint32_t)Da, (uint32_t)T, packed_b, packed_stride, rgsw, rgsw_stride, out, xbuf, flags,
-                       status, (uint32_t)nchains, spin_limit, nc, cc);
+                       (uint32_t)Da, (uint32_t)T, packed_b, packed_stride, rgsw, rgsw_stride, out, hstate,
+                       (uint32_t)g_lo, (uint32_t)g_hi, xbuf, flags, status, (uint32_t)nchains, spin_limit, nc, cc);
     return hipGetLastError();
 }
 

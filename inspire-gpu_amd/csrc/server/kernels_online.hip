@@ -59,13 +59,15 @@ __global__ __launch_bounds__(256) void k_ntt_rns(uint32_t* __restrict__ polys, N
 // ------------------------------------------------------------ slice reduction
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
-// interm[q][limb][col] = sum over row slices of part mod q_limb.
+// interm[q][limb][col] = sum over row slices of part mod q_limb, for columns
+// col0 + [0, gridDim.x*1024); blockIdx.y = limb, blockIdx.z = query.
 __global__ __launch_bounds__(256) void k_reduce_slices(const uint32_t* __restrict__ part, size_t part_stride,
-                                                       uint32_t S, uint32_t C, uint32_t* __restrict__ interm) {
-    const uint32_t x = (blockIdx.x * 256 + threadIdx.x) * 4;  // over [2][C]
-    const uint32_t q = blockIdx.y;
-    const int limb = x >= C;
-    const u32x4* pp = reinterpret_cast<const u32x4*>(part + q * part_stride + x);
+                                                       uint32_t S, uint32_t C, uint32_t col0,
+                                                       uint32_t* __restrict__ interm) {
+    const uint32_t col = col0 + (blockIdx.x * 256 + threadIdx.x) * 4;
+    const int limb = (int)blockIdx.y;
+    const uint32_t q = blockIdx.z;
+    const u32x4* pp = reinterpret_cast<const u32x4*>(part + q * part_stride + (size_t)limb * C + col);
     uint64_t a[4] = {0, 0, 0, 0};
     for (uint32_t s = 0; s < S; s++) {
         const u32x4 v = __builtin_nontemporal_load(pp + (size_t)s * (2 * C / 4));
@@ -73,7 +75,7 @@ __global__ __launch_bounds__(256) void k_reduce_slices(const uint32_t* __restric
     }
     u32x4 o;
     o[0] = red64(a[0], limb); o[1] = red64(a[1], limb); o[2] = red64(a[2], limb); o[3] = red64(a[3], limb);
-    *reinterpret_cast<u32x4*>(interm + (size_t)q * 2 * C + x) = o;
+    *reinterpret_cast<u32x4*>(interm + (size_t)q * 2 * C + (size_t)limb * C + col) = o;
 }
 
 // ------------------------------------------------------------ pack finalize
@@ -86,9 +88,9 @@ __global__ __launch_bounds__(256) void k_pack_finalize(const uint32_t* __restric
                                                        const uint32_t* __restrict__ precomp,
                                                        const uint32_t* __restrict__ keys, size_t kstride,
                                                        uint32_t* __restrict__ packed_b, size_t pstride,
-                                                       NttConst nc) {
+                                                       uint32_t g0, NttConst nc) {
     __shared__ uint32_t buf[POLY_LDS];
-    const uint32_t g = blockIdx.x, c = blockIdx.y, qi = blockIdx.z, t = threadIdx.x;
+    const uint32_t g = g0 + blockIdx.x, c = blockIdx.y, qi = blockIdx.z, t = threadIdx.x;
     const uint32_t q = qof((int)c);
     const uint32_t* pp = interm + (size_t)qi * 2 * C + (size_t)c * C + (size_t)g * N;
 #pragma unroll
@@ -114,32 +116,46 @@ __global__ __launch_bounds__(256) void k_pack_finalize(const uint32_t* __restric
 __global__ __launch_bounds__(1024) void k_horner(const uint32_t* __restrict__ pa_base, size_t pa_stride, uint32_t G, uint32_t Da,
                                                  uint32_t T, const uint32_t* __restrict__ packed_b,
                                                  size_t pstride, const uint32_t* __restrict__ rgsw,
-                                                 size_t rstride, uint64_t* __restrict__ out, NttConst nc,
-                                                 CrtConst cc) {
+                                                 size_t rstride, uint64_t* __restrict__ out,
+                                                 uint32_t* __restrict__ hstate, uint32_t g_lo, uint32_t g_hi,
+                                                 NttConst nc, CrtConst cc) {
     __shared__ uint32_t buf[4][POLY_LDS];
     __shared__ uint32_t twl[8 * N];
     const uint32_t ct = blockIdx.x, qi = blockIdx.y, tid = threadIdx.x;
     const uint32_t team = tid >> 8, t = tid & 255;
+    const uint32_t start = ct * Da;
+    const uint32_t L = (G - start) < Da ? (G - start) : Da;
+    const uint32_t top = start + L - 1;
+    // this launch covers the chain's groups in [lo, hi), highest first
+    const uint32_t lo = start > g_lo ? start : g_lo, hi = start + L < g_hi ? start + L : g_hi;
+    if (lo >= hi) return;
     stage_twiddles(twl, nc, tid, 1024);
     const int tlimb = (int)(team & 1);
     const Tw tw = tw_lds(twl, tlimb);
     const uint32_t inv_n = nc.inv_n[tlimb], inv_n_p = nc.inv_n_p[tlimb];
-    const uint32_t start = ct * Da;
-    const uint32_t L = (G - start) < Da ? (G - start) : Da;
     const uint32_t* pb = packed_b + qi * pstride;
     const uint32_t* rg = rgsw + qi * rstride;
+    uint32_t* hs = hstate + ((size_t)qi * T + ct) * 4 * N;  // chain state between launches
     uint32_t acc[4][2];
-    {
-        const uint32_t gl = start + L - 1;
-        const uint32_t* pa = pa_base + (size_t)gl * pa_stride;
+    int first;
+    if (top < g_hi) {  // the chain starts here: acc = packed[top]
+        const uint32_t* pa = pa_base + (size_t)top * pa_stride;
 #pragma unroll
         for (int e = 0; e < 2; e++) {
             const uint32_t i = tid + 1024 * e;
             acc[0][e] = pa[i]; acc[1][e] = pa[N + i];
-            acc[2][e] = pb[(size_t)gl * 2 * N + i]; acc[3][e] = pb[(size_t)gl * 2 * N + N + i];
+            acc[2][e] = pb[(size_t)top * 2 * N + i]; acc[3][e] = pb[(size_t)top * 2 * N + N + i];
         }
+        first = (int)top - 1;
+    } else {
+#pragma unroll
+        for (int p = 0; p < 4; p++)
+#pragma unroll
+            for (int e = 0; e < 2; e++) acc[p][e] = hs[(size_t)p * N + tid + 1024 * e];
+        first = (int)hi - 1;
     }
-    for (int st = (int)L - 2; st >= 0; st--) {
+    for (int gs = first; gs >= (int)lo; gs--) {
+        const int st = gs - (int)start;
 #pragma unroll
         for (int p = 0; p < 4; p++)
 #pragma unroll
@@ -199,6 +215,13 @@ __global__ __launch_bounds__(1024) void k_horner(const uint32_t* __restrict__ pa
             }
         __syncthreads();
     }
+    if (lo > start) {  // the chain continues in a later launch
+#pragma unroll
+        for (int p = 0; p < 4; p++)
+#pragma unroll
+            for (int e = 0; e < 2; e++) hs[(size_t)p * N + tid + 1024 * e] = acc[p][e];
+        return;
+    }
     uint64_t* o = out + ((size_t)qi * T + ct) * 4 * N;
 #pragma unroll
     for (int p = 0; p < 4; p++)
@@ -224,26 +247,32 @@ hipError_t launch_ntt_rns(uint32_t* polys, size_t npoly, const NttConst& nc, boo
 }
 
 hipError_t launch_reduce_slices(const uint32_t* part, size_t part_stride, uint32_t S, size_t C, uint32_t* interm,
-                                size_t nq, hipStream_t s) {
-    hipLaunchKernelGGL(k_reduce_slices, dim3((unsigned)(2 * C / 1024), (unsigned)nq), dim3(256), 0, s, part,
-                       part_stride, S, (uint32_t)C, interm);
+                                size_t nq, size_t col0, size_t ncols, hipStream_t s) {
+    if (ncols % 1024 || col0 % 4 || col0 + ncols > C) return hipErrorInvalidValue;
+    if (nq == 0 || ncols == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_reduce_slices, dim3((unsigned)(ncols / 1024), 2, (unsigned)nq), dim3(256), 0, s, part,
+                       part_stride, S, (uint32_t)C, (uint32_t)col0, interm);
     return hipGetLastError();
 }
 
 hipError_t launch_pack_finalize(const uint32_t* interm, size_t C, const uint32_t* part2, size_t part2_stride,
-                                const uint32_t* precomp, size_t G, const uint32_t* keys,
+                                const uint32_t* precomp, size_t g0, size_t ng, const uint32_t* keys,
                                 size_t keys_stride, uint32_t* packed_b, size_t packed_stride, size_t nq,
                                 const NttConst& nc, hipStream_t s) {
-    hipLaunchKernelGGL(k_pack_finalize, dim3((unsigned)G, 2, (unsigned)nq), dim3(256), 0, s, interm, (uint32_t)C,
-                       part2, part2_stride, precomp, keys, keys_stride, packed_b, packed_stride, nc);
+    if (ng == 0 || nq == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_pack_finalize, dim3((unsigned)ng, 2, (unsigned)nq), dim3(256), 0, s, interm, (uint32_t)C,
+                       part2, part2_stride, precomp, keys, keys_stride, packed_b, packed_stride, (uint32_t)g0, nc);
     return hipGetLastError();
 }
 
 hipError_t launch_horner(const uint32_t* pa_base, size_t pa_stride, size_t G, size_t Da, size_t T,
                          const uint32_t* packed_b, size_t packed_stride, const uint32_t* rgsw, size_t rgsw_stride,
-                         uint64_t* out, size_t nq, const NttConst& nc, const CrtConst& cc, hipStream_t s) {
+                         uint64_t* out, uint32_t* hstate, size_t g_lo, size_t g_hi, size_t nq, const NttConst& nc,
+                         const CrtConst& cc, hipStream_t s) {
+    if (nq == 0 || g_lo >= g_hi) return hipSuccess;
     hipLaunchKernelGGL(k_horner, dim3((unsigned)T, (unsigned)nq), dim3(1024), 0, s, pa_base, pa_stride, (uint32_t)G,
-                       (uint32_t)Da, (uint32_t)T, packed_b, packed_stride, rgsw, rgsw_stride, out, nc, cc);
+                       (uint32_t)Da, (uint32_t)T, packed_b, packed_stride, rgsw, rgsw_stride, out, hstate,
+                       (uint32_t)g_lo, (uint32_t)g_hi, nc, cc);
     return hipGetLastError();
 }
 

@@ -248,24 +248,33 @@ const DbCand kDbCands[] = {{false, 4, 2048}, {false, 8, 2048}, {false, 2, 2048},
                            {false, 8, 1024}, {true, 0, 512},   {true, 0, 1024},  {true, 0, 2048}};
 const size_t kMfmaTargets[] = {256, 512, 1024, 2048};
 
-DbPlan cand_plan(const DbCand& c, size_t R, size_t C) {
-    return c.mfma ? dbplan_mfma(R, C, 1, c.target) : dbplan_f64(R, C, c.unroll, c.target);
+DbPlan cand_plan(const DbCand& c, size_t R, size_t ncols) {
+    return c.mfma ? dbplan_mfma(R, ncols, 1, c.target) : dbplan_f64(R, ncols, c.unroll, c.target);
 }
 }  // namespace
 
 void GpuServerCtx::alloc_slots() {
     const size_t R = pp.db_rows, C = pp.db_cols, B = cfg.max_batch;
     plan1 = dbplan_f64(R, C, 4, 2048);
-    size_t max_slices = 1;
-    for (const DbCand& c : kDbCands) {
-        const DbPlan x = cand_plan(c, R, C);
-        if (x.slices > max_slices) max_slices = x.slices;
+    plan1_target = 2048;
+    // group chunks of the pipelined answer: about 8, of at least 8 groups
+    gchunk = G / 8 >= 8 ? G / 8 : (G < 8 ? G : 8);
+    if (const char* gc = std::getenv("INSPIRE_GROUP_CHUNK")) {
+        const long v = std::atol(gc);
+        if (v > 0 && (size_t)v <= G && G % (size_t)v == 0) gchunk = (size_t)v;
     }
-    for (size_t t : kMfmaTargets)
-        for (size_t nq = 1; nq <= kMfmaMaxQueries; nq++) {
-            const DbPlan x = dbplan_mfma(R, C, nq, t);
+    size_t max_slices = 1;
+    for (size_t ncols : {C, gchunk * N}) {
+        for (const DbCand& c : kDbCands) {
+            const DbPlan x = cand_plan(c, R, ncols);
             if (x.slices > max_slices) max_slices = x.slices;
         }
+        for (size_t t : kMfmaTargets)
+            for (size_t nq = 1; nq <= kMfmaMaxQueries; nq++) {
+                const DbPlan x = dbplan_mfma(R, ncols, nq, t);
+                if (x.slices > max_slices) max_slices = x.slices;
+            }
+    }
     lay.R = R; lay.C = C; lay.G = G;
     qlen = lay.query_u64s();
     part_stride = max_slices * 2 * C;
@@ -289,6 +298,12 @@ void GpuServerCtx::alloc_slots() {
         d_kdig[l] = dalloc<int8_t>(kKdigBytes);
         d_KS[l] = dalloc<uint32_t>(64);
     }
+    d_hstate = dalloc<uint32_t>(B * T * 4 * N);
+    for (int l = 0; l < kLanes; l++) {
+        HIPX(hipStreamCreate(&hstream[l]));
+        for (auto& e : chunk_ev[l]) HIPX(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        HIPX(hipEventCreateWithFlags(&horner_done[l], hipEventDisableTiming));
+    }
     d_hxbuf = dalloc<uint32_t>(horner_pair_xbuf_words(B * T));
     d_hflags = dalloc<uint32_t>(2 * B * T);
     d_status = dalloc<uint32_t>(1);
@@ -301,7 +316,8 @@ void GpuServerCtx::alloc_slots() {
     resident_bytes = R * C * 2 + G * PC_GROUP * 4 + 8 * N * 4 +
                      B * (qlen * 8 + resp_u64s * 8 + 2 * R * 8 +
                           (keys_stride + rgsw_stride + part_stride + 2 * C + part2_stride + packed_stride) * 4) +
-                     kLanes * (R * 256 + kKdigBytes + 128 * 4) + (horner_pair_xbuf_words(B * T) + 2 * B * T) * 4;
+                     kLanes * (R * 256 + kKdigBytes + 128 * 4) +
+                     (horner_pair_xbuf_words(B * T) + 2 * B * T + B * T * 4 * N) * 4;
     autotune_dbpass();
 }
 
@@ -316,10 +332,13 @@ void GpuServerCtx::autotune_dbpass() {
     }
     if (const char* force = std::getenv("INSPIRE_DBPASS")) {
         int u = 4, t = 2048;
-        if (std::sscanf(force, "f64,%d,%d", &u, &t) == 2 && (u == 2 || u == 4 || u == 8) && t > 0)
+        if (std::sscanf(force, "f64,%d,%d", &u, &t) == 2 && (u == 2 || u == 4 || u == 8) && t > 0) {
             plan1 = dbplan_f64(R, C, u, (size_t)t);
-        else if (std::sscanf(force, "mfma,%d", &t) == 1 && t > 0)
+            plan1_target = (size_t)t;
+        } else if (std::sscanf(force, "mfma,%d", &t) == 1 && t > 0) {
             plan1 = dbplan_mfma(R, C, 1, (size_t)t);
+            plan1_target = (size_t)t;
+        }
         return;
     }
     const char* at = std::getenv("INSPIRE_AUTOTUNE");
@@ -328,11 +347,13 @@ void GpuServerCtx::autotune_dbpass() {
     HIPX(hipEventCreate(&e0));
     HIPX(hipEventCreate(&e1));
     auto time_plan = [&](const DbPlan& x, size_t nq) {
+        if (x.kind == DbPlan::MFMA) HIPX(launch_mfma_digits(d_bq, nq, R, x, d_dig[0], d_S[0], stream));
         auto run = [&]() {
             if (x.kind == DbPlan::F64)
-                HIPX(launch_dbpass_f64(db_planes(), d_bq, d_part, R, C, x, stream));
+                HIPX(launch_dbpass_f64(db_planes(), d_bq, d_part, R, C, 0, C, x, stream));
             else
-                HIPX(launch_dbpass_mfma(db_planes(), d_bq, nq, d_dig[0], d_S[0], d_part, part_stride, R, C, x, stream));
+                HIPX(launch_dbpass_mfma(db_planes(), d_dig[0], d_S[0], nq, d_part, part_stride, R, C, 0, C, x,
+                                        stream));
         };
         run();  // warm
         HIPX(hipEventRecord(e0, stream));
@@ -347,7 +368,7 @@ void GpuServerCtx::autotune_dbpass() {
     for (const DbCand& c : kDbCands) {
         const DbPlan x = cand_plan(c, R, C);
         const float ms = time_plan(x, 1);
-        if (ms < best) { best = ms; plan1 = x; }
+        if (ms < best) { best = ms; plan1 = x; plan1_target = c.target; }
     }
     // batched pass: the workgroup target at the largest batch this context serves
     const size_t nq = cfg.max_batch < kMfmaMaxQueries ? cfg.max_batch : kMfmaMaxQueries;
@@ -368,54 +389,128 @@ void GpuServerCtx::stage_prep(const uint64_t* d_in, size_t count, hipStream_t s,
     HIPX(launch_ntt_rns(d_keys + slot0 * keys_stride, count * 2 * D_EFF, nc, false, s));
 }
 
+DbPlan GpuServerCtx::plan_for(size_t nq, size_t ncols) const {
+    const size_t R = pp.db_rows;
+    if (nq == 1)
+        return plan1.kind == DbPlan::F64 ? dbplan_f64(R, ncols, (int)plan1.param, plan1_target)
+                                         : dbplan_mfma(R, ncols, 1, plan1_target);
+    return dbplan_mfma(R, ncols, nq, mfma_target);
+}
+
+void GpuServerCtx::dbpass_prep(size_t q0, size_t nq, const DbPlan& p, hipStream_t s, int lane) {
+    if (p.kind == DbPlan::MFMA)
+        HIPX(launch_mfma_digits(d_bq + q0 * 2 * pp.db_rows, nq, pp.db_rows, p, d_dig[lane], d_S[lane], s));
+}
+
+void GpuServerCtx::dbpass_range(size_t q0, size_t nq, size_t g_lo, size_t g_hi, const DbPlan& p, hipStream_t s,
+                                int lane) {
+    const size_t R = pp.db_rows, C = pp.db_cols, col0 = g_lo * N, ncols = (g_hi - g_lo) * N;
+    // one slice: straight into interm; else row-slice partials, reduced below
+    uint32_t* out = p.slices == 1 ? d_interm + q0 * 2 * C : d_part + q0 * part_stride;
+    const size_t qstride = p.slices == 1 ? 2 * C : part_stride;
+    if (p.kind == DbPlan::F64) {
+        for (size_t q = 0; q < nq; q++)
+            HIPX(launch_dbpass_f64(db_planes(), d_bq + (q0 + q) * 2 * R, out + q * qstride, R, C, col0, ncols, p, s));
+    } else {
+        HIPX(launch_dbpass_mfma(db_planes(), d_dig[lane], d_S[lane], nq, out, qstride, R, C, col0, ncols, p, s));
+    }
+    if (p.slices > 1)
+        HIPX(launch_reduce_slices(d_part + q0 * part_stride, part_stride, p.slices, C, d_interm + q0 * 2 * C, nq, col0,
+                                  ncols, s));
+}
+
 void GpuServerCtx::stage_dbpass(size_t count, hipStream_t s, size_t slot0, int lane, int force) {
     const size_t R = pp.db_rows, C = pp.db_cols;
     const size_t step = force == 0 ? 1 : kMfmaMaxQueries;
     for (size_t q0 = slot0; q0 < slot0 + count; q0 += step) {
         const size_t nq = slot0 + count - q0 < step ? slot0 + count - q0 : step;
-        DbPlan p = nq == 1 ? plan1 : dbplan_mfma(R, C, nq, mfma_target);
+        DbPlan p = plan_for(nq, C);
         if (force == 0 && p.kind != DbPlan::F64) p = dbplan_f64(R, C, 4, 2048);
         if (force == 1 && p.kind == DbPlan::F64) p = dbplan_mfma(R, C, nq, mfma_target);
-        // one slice: straight into interm; else row-slice partials, reduced below
-        uint32_t* out = p.slices == 1 ? d_interm + q0 * 2 * C : d_part + q0 * part_stride;
-        const size_t qstride = p.slices == 1 ? 2 * C : part_stride;
-        if (p.kind == DbPlan::F64)
-            HIPX(launch_dbpass_f64(db_planes(), d_bq + q0 * 2 * R, out, R, C, p, s));
-        else
-            HIPX(launch_dbpass_mfma(db_planes(), d_bq + q0 * 2 * R, nq, d_dig[lane], d_S[lane], out, qstride, R, C, p,
-                                    s));
-        if (p.slices > 1)
-            HIPX(launch_reduce_slices(d_part + q0 * part_stride, part_stride, p.slices, C, d_interm + q0 * 2 * C, nq,
-                                      s));
+        dbpass_prep(q0, nq, p, s, lane);
+        dbpass_range(q0, nq, 0, G, p, s, lane);
     }
 }
 
 void GpuServerCtx::stage_collapse(size_t count, hipStream_t s, size_t slot0, int lane) {
     for (size_t q0 = slot0; q0 < slot0 + count; q0 += kCollapseMaxQueries) {
         const size_t nq = slot0 + count - q0 < kCollapseMaxQueries ? slot0 + count - q0 : kCollapseMaxQueries;
-        HIPX(launch_collapse(d_precomp, G, d_keys + q0 * keys_stride, keys_stride, nq, d_kdig[lane], d_KS[lane],
-                             d_part2 + q0 * part2_stride, part2_stride, s));
+        HIPX(launch_collapse_keys(d_keys + q0 * keys_stride, keys_stride, nq, d_kdig[lane], d_KS[lane], s));
+        HIPX(launch_collapse(d_precomp, G, d_kdig[lane], d_KS[lane], nq, d_part2 + q0 * part2_stride, part2_stride,
+                             s));
     }
 }
 
 void GpuServerCtx::stage_finalize(size_t count, hipStream_t s, size_t slot0) {
     HIPX(launch_pack_finalize(d_interm + slot0 * 2 * pp.db_cols, pp.db_cols, d_part2 + slot0 * part2_stride,
-                              part2_stride, d_precomp, G, d_keys + slot0 * keys_stride, keys_stride,
+                              part2_stride, d_precomp, 0, G, d_keys + slot0 * keys_stride, keys_stride,
                               d_packed_b + slot0 * packed_stride, packed_stride, count, nc, s));
 }
 
-void GpuServerCtx::stage_horner(size_t count, uint64_t* d_out, hipStream_t s, size_t slot0) {
+bool GpuServerCtx::use_pair(size_t chains) const {
     // auto: the pair kernel shortens a lone chain (its per-step exchange costs
     // more than it saves once many chains share the L2s)
-    const bool pair = horner_mode == 1 || (horner_mode < 0 && count * T <= kPairMaxChains);
-    if (pair)
+    return horner_mode == 1 || (horner_mode < 0 && chains <= kPairMaxChains);
+}
+
+void GpuServerCtx::horner_range(size_t count, uint64_t* d_out, size_t g_lo, size_t g_hi, hipStream_t s,
+                                size_t slot0) {
+    if (use_pair(count * T))
         HIPX(launch_horner_pair(d_precomp + PC_A, PC_GROUP, G, Da, T, d_packed_b + slot0 * packed_stride,
-                                packed_stride, d_rgsw + slot0 * rgsw_stride, rgsw_stride, d_out, count,
+                                packed_stride, d_rgsw + slot0 * rgsw_stride, rgsw_stride, d_out,
+                                d_hstate + slot0 * T * 4 * N, g_lo, g_hi, count,
                                 d_hxbuf + horner_pair_xbuf_words(slot0 * T), d_hflags + 2 * slot0 * T, d_status,
                                 kHornerSpinLimit, nc, cc, s));
     else
         HIPX(launch_horner(d_precomp + PC_A, PC_GROUP, G, Da, T, d_packed_b + slot0 * packed_stride, packed_stride,
-                           d_rgsw + slot0 * rgsw_stride, rgsw_stride, d_out, count, nc, cc, s));
+                           d_rgsw + slot0 * rgsw_stride, rgsw_stride, d_out, d_hstate + slot0 * T * 4 * N, g_lo, g_hi,
+                           count, nc, cc, s));
+}
+
+void GpuServerCtx::stage_horner(size_t count, uint64_t* d_out, hipStream_t s, size_t slot0) {
+    horner_range(count, d_out, 0, G, s, slot0);
+}
+
+void GpuServerCtx::enqueue(const uint64_t* d_in, size_t count, uint64_t* d_out, hipStream_t s, size_t slot0,
+                           int lane) {
+    if (lane < 0 || lane >= kLanes) throw std::invalid_argument("enqueue: scratch lane out of range");
+    stage_prep(d_in, count, s, slot0);
+    const size_t C = pp.db_cols;
+    const size_t nchunks = G / gchunk;
+    // Pipelined over group chunks, highest first (the order every Horner
+    // chain consumes groups): the database pass, collapse and pack of chunk k
+    // run on s while the Horner chains of chunk k+1 (the groups above) run
+    // on the lane's second stream; s finally waits for the chains.
+    for (size_t q0 = slot0; q0 < slot0 + count; q0 += kMfmaMaxQueries) {
+        const size_t nq = slot0 + count - q0 < kMfmaMaxQueries ? slot0 + count - q0 : kMfmaMaxQueries;
+        const DbPlan p = plan_for(nq, gchunk * N);
+        dbpass_prep(q0, nq, p, s, lane);
+        HIPX(launch_collapse_keys(d_keys + q0 * keys_stride, keys_stride, nq, d_kdig[lane], d_KS[lane], s));
+        if (nchunks > 1) {
+            HIPX(hipEventRecord(horner_done[lane], s));  // the side stream starts after this batch's prologue
+            HIPX(hipStreamWaitEvent(hstream[lane], horner_done[lane], 0));
+        }
+        for (size_t k = nchunks; k-- > 0;) {
+            const size_t g_lo = k * gchunk, g_hi = g_lo + gchunk;
+            dbpass_range(q0, nq, g_lo, g_hi, p, s, lane);
+            HIPX(launch_collapse(d_precomp + g_lo * PC_GROUP, gchunk, d_kdig[lane], d_KS[lane], nq,
+                                 d_part2 + q0 * part2_stride + g_lo * 2 * N, part2_stride, s));
+            HIPX(launch_pack_finalize(d_interm + q0 * 2 * C, C, d_part2 + q0 * part2_stride, part2_stride, d_precomp,
+                                      g_lo, gchunk, d_keys + q0 * keys_stride, keys_stride,
+                                      d_packed_b + q0 * packed_stride, packed_stride, nq, nc, s));
+            if (nchunks == 1) {
+                horner_range(nq, d_out + (q0 - slot0) * resp_u64s, 0, G, s, q0);
+            } else {
+                HIPX(hipEventRecord(chunk_ev[lane][k % kChunkEvents], s));
+                HIPX(hipStreamWaitEvent(hstream[lane], chunk_ev[lane][k % kChunkEvents], 0));
+                horner_range(nq, d_out + (q0 - slot0) * resp_u64s, g_lo, g_hi, hstream[lane], q0);
+            }
+        }
+        if (nchunks > 1) {
+            HIPX(hipEventRecord(horner_done[lane], hstream[lane]));
+            HIPX(hipStreamWaitEvent(s, horner_done[lane], 0));
+        }
+    }
 }
 
 void GpuServerCtx::check_status(hipStream_t s) {
@@ -427,16 +522,6 @@ void GpuServerCtx::check_status(hipStream_t s) {
         HIPX(hipStreamSynchronize(s));
         throw std::runtime_error("Horner pair exchange exceeded its wait bound; responses of this call are invalid");
     }
-}
-
-void GpuServerCtx::enqueue(const uint64_t* d_in, size_t count, uint64_t* d_out, hipStream_t s, size_t slot0,
-                           int lane) {
-    if (lane < 0 || lane >= kLanes) throw std::invalid_argument("enqueue: scratch lane out of range");
-    stage_prep(d_in, count, s, slot0);
-    stage_dbpass(count, s, slot0, lane);
-    stage_collapse(count, s, slot0, lane);
-    stage_finalize(count, s, slot0);
-    stage_horner(count, d_out, s, slot0);
 }
 
 void GpuServerCtx::answer_host(size_t count, uint64_t* out, hipStream_t s, size_t slot0) {
@@ -520,7 +605,14 @@ void gpu_free_server(GpuServerCtx* ctx) {
     dfree(ctx->d_tw); dfree(ctx->d_db); dfree(ctx->d_precomp);
     dfree(ctx->d_q); dfree(ctx->d_resp); dfree(ctx->d_bq); dfree(ctx->d_keys); dfree(ctx->d_rgsw);
     dfree(ctx->d_part); dfree(ctx->d_interm); dfree(ctx->d_part2); dfree(ctx->d_packed_b);
-    dfree(ctx->d_hxbuf); dfree(ctx->d_hflags); dfree(ctx->d_status);
+    dfree(ctx->d_hxbuf); dfree(ctx->d_hflags); dfree(ctx->d_status); dfree(ctx->d_hstate);
+    for (int l = 0; l < GpuServerCtx::kLanes; l++) {
+        if (ctx->hstream[l]) (void)hipStreamSynchronize(ctx->hstream[l]);
+        for (auto& e : ctx->chunk_ev[l])
+            if (e) (void)hipEventDestroy(e);
+        if (ctx->horner_done[l]) (void)hipEventDestroy(ctx->horner_done[l]);
+        if (ctx->hstream[l]) (void)hipStreamDestroy(ctx->hstream[l]);
+    }
     for (int l = 0; l < GpuServerCtx::kLanes; l++) {
         dfree(ctx->d_dig[l]); dfree(ctx->d_S[l]); dfree(ctx->d_kdig[l]); dfree(ctx->d_KS[l]);
     }

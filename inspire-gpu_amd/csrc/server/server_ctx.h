@@ -33,8 +33,10 @@ struct GpuServerCtx {
 
     // per-query slot pools, max_batch each
     ipx::SlotLayout lay{};
-    ipx::DbPlan plan1{};            // database pass of a single query (autotuned)
+    ipx::DbPlan plan1{};            // database pass of a single query over all columns (autotuned)
+    size_t plan1_target = 2048;     // its workgroup target
     size_t mfma_target = 1024;      // workgroups the batched pass aims for
+    size_t gchunk = 0;              // groups per chunk of the pipelined answer
     size_t qlen = 0, resp_u64s = 0;
     size_t part_stride = 0, part2_stride = 0, keys_stride = 0, rgsw_stride = 0, packed_stride = 0;
     uint64_t* d_q = nullptr;        // flat queries (device staging)
@@ -53,6 +55,12 @@ struct GpuServerCtx {
     static constexpr size_t kKdigBytes = 2 * (2 * ipx::N / 16) * 128 * 16;
     int8_t* d_kdig[kLanes] = {};
     uint32_t* d_KS[kLanes] = {};
+    // per lane: the stream the Horner chains run on beside the passes, and
+    // the events that hand each group chunk over to it
+    static constexpr int kChunkEvents = 16;
+    hipStream_t hstream[kLanes] = {};
+    hipEvent_t chunk_ev[kLanes][kChunkEvents] = {};
+    hipEvent_t horner_done[kLanes] = {};
     uint32_t* d_packed_b = nullptr;
     // Horner: 0 = one workgroup per chain, 1 = a workgroup pair (one per
     // limb), -1 = pair when a call has at most kPairMaxChains chains
@@ -61,6 +69,7 @@ struct GpuServerCtx {
     uint32_t* d_hxbuf = nullptr;    // pair exchange buffers [B*T chains]
     uint32_t* d_hflags = nullptr;   // pair step flags [B*T][2]
     uint32_t* d_status = nullptr;   // sticky device fault word (pair wait bound exceeded)
+    uint32_t* d_hstate = nullptr;   // Horner chain state between chunk launches [B*T][4][N]
     uint64_t* h_q = nullptr;        // pinned host staging
     uint64_t* h_resp = nullptr;
 
@@ -72,8 +81,17 @@ struct GpuServerCtx {
     void autotune_dbpass();
     // Pipeline stages over slots [slot0, slot0+count) on stream s.
     void stage_prep(const uint64_t* d_in, size_t count, hipStream_t s, size_t slot0);
+    // The tuned database-pass plan for nq queries over ncols columns.
+    ipx::DbPlan plan_for(size_t nq, size_t ncols) const;
+    // Digits of queries [q0, q0+nq) for an MFMA plan (no-op for fp64).
+    void dbpass_prep(size_t q0, size_t nq, const ipx::DbPlan& p, hipStream_t s, int lane);
+    // Database pass of queries [q0, q0+nq) over the columns of groups [g_lo, g_hi) into interm.
+    void dbpass_range(size_t q0, size_t nq, size_t g_lo, size_t g_hi, const ipx::DbPlan& p, hipStream_t s, int lane);
     // force: -1 the tuned plans, 0 the fp64 pass per query, 1 the MFMA pass
     void stage_dbpass(size_t count, hipStream_t s, size_t slot0, int lane = 0, int force = -1);
+    bool use_pair(size_t chains) const;
+    // Horner chains of `count` queries over groups [g_lo, g_hi).
+    void horner_range(size_t count, uint64_t* d_out, size_t g_lo, size_t g_hi, hipStream_t s, size_t slot0);
     void stage_collapse(size_t count, hipStream_t s, size_t slot0, int lane = 0);
     void stage_finalize(size_t count, hipStream_t s, size_t slot0);
     void stage_horner(size_t count, uint64_t* d_out, hipStream_t s, size_t slot0);

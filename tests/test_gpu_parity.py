@@ -220,6 +220,27 @@ def test_pair_horner_answers_equal_single(medium8):
     assert np.array_equal(one[0], base[0]) and np.array_equal(one_single[0], base[0])
 
 
+@pytest.mark.parametrize("groups", [1, 2, 4])
+def test_pipelined_group_chunks_equal_unchunked(medium8, groups):
+    """The answer pipelined over group chunks (Horner chains on a second
+    stream, resuming between launches) equals the one-chunk answer, for both
+    Horner kernels."""
+    geo, db, srv = medium8
+    qs = [ol.make_query(geo, SEED, idx, 800 + k)[0] for k, idx in enumerate([3, 99999, 524287])]
+    base = srv.answer_batch(qs)
+    one = srv.answer_batch(qs[:1])
+    assert srv.lib.ipir_dbg_set_group_chunk(srv.h, groups) == 0
+    try:
+        got = srv.answer_batch(qs)
+        got1 = srv.answer_batch(qs[:1])
+        assert srv.lib.ipir_dbg_set_horner(srv.h, 1) == 0
+        pair = srv.answer_batch(qs)
+    finally:
+        assert srv.lib.ipir_dbg_set_horner(srv.h, -1) == 0
+        assert srv.lib.ipir_dbg_set_group_chunk(srv.h, geo["G"]) == 0
+    assert np.array_equal(got, base) and np.array_equal(got1[0], one[0]) and np.array_equal(pair, base)
+
+
 def test_batch_validation_codes(medium8):
     geo, db, srv = medium8
     lib = srv.lib
