@@ -273,7 +273,7 @@ def main():
     tfile = os.path.join(ROOT, "profiles", "dbpass_traffic.json")
     if os.path.exists(tfile):
         t = json.load(open(tfile))
-        if t.get("workload") == label and t.get("kernel") == kname.split(" ")[0]:
+        if t.get("workload") == label and t.get("kernel") == kname.split(" slices=")[0]:
             traffic = t["traffic_bytes"]
     # NTT throughput at N = 2048, both limbs per polynomial, 1024 polynomials per launch
     ntt = {}
