@@ -59,6 +59,13 @@ int ipir_dbg_set_group_chunk(ipir_server* srv, size_t groups);
  * or -1 (the default) to use the pair only for calls with at most two
  * chains. */
 int ipir_dbg_set_horner(ipir_server* srv, int mode);
+/* Set the register-ring depth (3..6) of the 17..32-query database pass a
+ * server uses; 1 for other values. */
+int ipir_dbg_set_mfma_depth(ipir_server* srv, uint32_t depth);
+/* Set the pipeline shape of one-query and many-query answers: 0 serial, 2
+ * Horner chains on a second stream per group chunk, 3 the collapse on a
+ * third stream as well; 1 for other values. */
+int ipir_dbg_set_pipeline(ipir_server* srv, int single, int batched);
 /* Per-stage device time (ms) of the last ipir_dbg_profile_batch run:
  * [unmarshal+key NTT, DB pass, collapse pass, pack finalize, Horner]. */
 int ipir_dbg_profile_batch(ipir_server* srv, const uint64_t* d_queries, size_t count, uint64_t* d_out,

@@ -231,7 +231,7 @@ int ipir_dbg_time_dbpass(ipir_server* srv, size_t nq, int iters, double* ms_per_
 int ipir_dbg_dbpass_kernel(const ipir_server* srv, size_t nq, char* out, size_t cap) {
     if (!srv || !out || cap == 0 || nq == 0 || nq > ipx::kMfmaMaxQueries) return 1;
     const GpuServerCtx* c = srv->ctx;
-    const ipx::DbPlan p = nq == 1 ? c->plan1 : ipx::dbplan_mfma(c->pp.db_rows, c->pp.db_cols, nq, c->mfma_target);
+    const ipx::DbPlan p = c->plan_for(nq, c->pp.db_cols);
     const std::string s = ipx::dbplan_kernel(p) + " slices=" + std::to_string(p.slices);
     std::snprintf(out, cap, "%s", s.c_str());
     return 0;
@@ -341,12 +341,32 @@ int ipir_dbg_set_group_chunk(ipir_server* srv, size_t groups) {
     GpuServerCtx* c = srv->ctx;
     std::lock_guard<std::mutex> lk(c->mu);
     // the partial-sum buffer was sized for the setup-time chunk width
-    const size_t R = c->pp.db_rows, ncols = groups * N;
+    const size_t ncols = groups * N;
     for (size_t nq = 1; nq <= ipx::kMfmaMaxQueries; nq++) {
-        const ipx::DbPlan x = nq == 1 ? c->plan_for(1, ncols) : ipx::dbplan_mfma(R, ncols, nq, c->mfma_target);
-        if (x.slices > 1 && x.slices * 2 * c->pp.db_cols > c->part_stride) return 1;
+        const ipx::DbPlan x = c->chunk_plan(nq, ncols);
+        if (x.slices > 1 && x.slices * 2 * ncols > c->part_stride) return 1;
     }
     c->gchunk = groups;
+    return 0;
+}
+
+int ipir_dbg_set_mfma_depth(ipir_server* srv, uint32_t depth) {
+    if (!srv) return 1;
+    for (uint32_t d : ipx::kMfmaDepths)
+        if (d == depth) {
+            std::lock_guard<std::mutex> lk(srv->ctx->mu);
+            srv->ctx->mfma_depth = depth;
+            return 0;
+        }
+    return 1;
+}
+
+int ipir_dbg_set_pipeline(ipir_server* srv, int single, int batched) {
+    auto ok = [](int v) { return v == 0 || v == 2 || v == 3; };
+    if (!srv || !ok(single) || !ok(batched)) return 1;
+    std::lock_guard<std::mutex> lk(srv->ctx->mu);
+    srv->ctx->pipe_single = single;
+    srv->ctx->pipe_batched = batched;
     return 0;
 }
 

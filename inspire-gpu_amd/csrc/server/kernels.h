@@ -35,9 +35,12 @@ hipError_t launch_ntt_rns(uint32_t* polys, size_t npoly, const NttConst& nc, boo
 // slice's rows of db[r][c]*b_{q,limb}[r] mod q_limb.
 struct DbPlan {
     enum Kind { F64, MFMA } kind;
-    uint32_t param;   // F64: 16-row blocks per iteration (2, 4, 8); MFMA: 16-row digit tiles (1, 2, 4, 8, 16)
-    uint32_t slices;  // row slices (power of two)
+    uint32_t param;      // F64: 16-row blocks per iteration (2, 4, 8); MFMA: 16-row digit tiles (1, 2, 4, 8, 16)
+    uint32_t slices;     // row slices (power of two)
+    uint32_t depth = 3;  // 16 digit tiles: register-ring slots of DB operands (kMfmaDepths)
 };
+// Ring depths the 16-tile (17..32 query) MFMA pass is built with.
+constexpr uint32_t kMfmaDepths[] = {3, 4, 5, 6};
 // Row-major staging rows (16*nkb of them) -> blocks [kb0, kb0 + nkb) of the planes.
 hipError_t launch_to_planes(const uint16_t* src, uint8_t* dst, size_t C, size_t kb0, size_t nkb, hipStream_t s);
 // Plans for a pass over ncols columns (the slice count fills the device).
@@ -45,16 +48,20 @@ DbPlan dbplan_f64(size_t R, size_t ncols, int unroll, size_t target_blocks);
 // Queries one batched (int8 MFMA) database pass serves.
 constexpr size_t kMfmaMaxQueries = 32;
 // int8-MFMA plan for nq (1..kMfmaMaxQueries) queries.
-DbPlan dbplan_mfma(size_t R, size_t ncols, size_t nq, size_t target_blocks);
+DbPlan dbplan_mfma(size_t R, size_t ncols, size_t nq, size_t target_blocks, uint32_t depth = 4);
 // Digit rows the MFMA plan stages per 16-row block (int8 scratch: R*Mp bytes, Mp <= 256).
 size_t dbplan_mp(const DbPlan& p);
 // Column granularity of a plan (one block's columns): col0 and ncols are multiples of it.
 size_t dbplan_cols(const DbPlan& p);
 // Name of the kernel a plan launches (as profilers report it, without arguments).
 std::string dbplan_kernel(const DbPlan& p);
+// Output of the database passes: out points at the first column's residue
+// (limb 0, row slice 0); limb 1 sits ostride words after limb 0 and row
+// slice k at k*2*ostride (ostride = C into interm, ncols for partials kept
+// relative to col0).
 // One query, fp64 FMA on the vector ALUs, over columns [col0, col0 + ncols).
-hipError_t launch_dbpass_f64(const uint8_t* db, const double* bq, uint32_t* out, size_t R, size_t C, size_t col0,
-                             size_t ncols, const DbPlan& p, hipStream_t s);
+hipError_t launch_dbpass_f64(const uint8_t* db, const double* bq, uint32_t* out, size_t ostride, size_t R, size_t C,
+                             size_t col0, size_t ncols, const DbPlan& p, hipStream_t s);
 // Query digits and offset sums of nq queries (bq [q][limb][R] doubles) for
 // an MFMA plan: dig (R*Mp bytes) and S (2*nq words).
 hipError_t launch_mfma_digits(const double* bq, size_t nq, size_t R, const DbPlan& p, int8_t* dig, uint32_t* S,
@@ -62,10 +69,10 @@ hipError_t launch_mfma_digits(const double* bq, size_t nq, size_t R, const DbPla
 // The int8-MFMA pass of nq queries over columns [col0, col0 + ncols), from
 // launch_mfma_digits' output for a plan with the same digit tile count.
 hipError_t launch_dbpass_mfma(const uint8_t* db, const int8_t* dig, const uint32_t* S, size_t nq, uint32_t* out,
-                              size_t out_qstride, size_t R, size_t C, size_t col0, size_t ncols, const DbPlan& p,
-                              hipStream_t s);
-// interm[q][limb][col] = sum over the S row slices of part (per query stride
-// part_stride, slice stride 2C), for columns [col0, col0 + ncols).
+                              size_t out_qstride, size_t ostride, size_t R, size_t C, size_t col0, size_t ncols,
+                              const DbPlan& p, hipStream_t s);
+// interm[q][limb][col0 + j] = sum over the S row slices of part (per query
+// stride part_stride; [slice][limb][j], j < ncols), for j < ncols.
 hipError_t launch_reduce_slices(const uint32_t* part, size_t part_stride, uint32_t S, size_t C, uint32_t* interm,
                                 size_t nq, size_t col0, size_t ncols, hipStream_t s);
 

@@ -56,10 +56,10 @@ __global__ __launch_bounds__(256) void k_to_planes(const uint16_t* __restrict__ 
 // Products db*b < 2^43 accumulate exactly in fp64 FMA for 1024 rows
 // (< 2^53), in four interleaved accumulators per limb; every 1024 rows the
 // sums are reduced mod q and folded into a canonical u32.
-// out[limb][C] for this slice at out + slice*2C.
+// out[limb][OC] for this slice at out + slice*2*OC.
 template <int U>
 __global__ __launch_bounds__(256) void k_dbpass_f64(const uint8_t* __restrict__ db, const double* __restrict__ bq,
-                                                    uint32_t* __restrict__ out, uint32_t R, uint32_t C,
+                                                    uint32_t* __restrict__ out, uint32_t R, uint32_t C, uint32_t OC,
                                                     uint32_t kbps) {
     const uint32_t col = blockIdx.x * 256 + threadIdx.x;
     const uint32_t kb0 = blockIdx.y * kbps;
@@ -102,9 +102,9 @@ __global__ __launch_bounds__(256) void k_dbpass_f64(const uint8_t* __restrict__ 
             for (int e = 0; e < 4; e++) a0[e] = a1[e] = 0.0;
         }
     }
-    uint32_t* o = out + (size_t)blockIdx.y * 2 * C;
+    uint32_t* o = out + (size_t)blockIdx.y * 2 * OC;
     o[col] = m0;
-    o[C + col] = m1;
+    o[OC + col] = m1;
 }
 
 // ------------------------------------------------------------ batched pass (int8 MFMA)
@@ -177,7 +177,7 @@ __global__ __launch_bounds__(256) void k_bsum(const double* __restrict__ bq, uin
 // Recombine one lane's accumulators of pair pr and store the canonical residue.
 __device__ __forceinline__ void store_pair(const i32x4& al, const i32x4& ah, uint32_t pr, uint32_t nq, bool corr,
                                            const uint32_t* __restrict__ S, uint32_t* __restrict__ out,
-                                           size_t out_qstride, uint32_t slice, uint32_t C, uint32_t col) {
+                                           size_t out_qstride, uint32_t slice, uint32_t OC, uint32_t col) {
     const uint32_t q = pr >> 1;
     const int c = (int)(pr & 1);
     if (q >= nq) return;
@@ -188,7 +188,8 @@ __device__ __forceinline__ void store_pair(const i32x4& al, const i32x4& ah, uin
     // of q_c above that to make it non-negative
     const uint64_t qc = c ? Q1 : Q0;
     const uint64_t off = qc * ((1ull << 59) / qc);
-    out[(size_t)q * out_qstride + (size_t)slice * 2 * C + (size_t)c * C + col] = red64((uint64_t)(s + (int64_t)off), c);
+    out[(size_t)q * out_qstride + (size_t)slice * 2 * OC + (size_t)c * OC + col] =
+        red64((uint64_t)(s + (int64_t)off), c);
 }
 
 // Up to 2*MT queries (Mp = 16*MT digit rows): the 4 waves of a block own
@@ -200,7 +201,7 @@ __global__ __launch_bounds__(256, 2) void k_dbpass_mfma(const uint8_t* __restric
                                                         const int8_t* __restrict__ dig,
                                                         const uint32_t* __restrict__ S,
                                                         uint32_t* __restrict__ out, size_t out_qstride,
-                                                        uint32_t C, uint32_t nq, uint32_t kbps) {
+                                                        uint32_t C, uint32_t OC, uint32_t nq, uint32_t kbps) {
     constexpr uint32_t Mp = 16 * MT;
     const uint32_t w = threadIdx.x >> 6, l = threadIdx.x & 63;
     const uint32_t slice = blockIdx.y;
@@ -252,20 +253,21 @@ __global__ __launch_bounds__(256, 2) void k_dbpass_mfma(const uint8_t* __restric
 #pragma unroll
         for (int n = 0; n < NT; n++)
             store_pair(acc[a][n][0], acc[a][n][1], (uint32_t)a * 4 + (l >> 4), nq, slice == 0, S, out, out_qstride,
-                       slice, C, col0 + n * 16 + (l & 15));
+                       slice, OC, col0 + n * 16 + (l & 15));
 }
 
 // Many queries (MT = 8 or 16 digit tiles, up to 16 or 32 queries): as above,
 // but the block's 4 waves share the step's digit tile (Mp x 64 bytes)
-// through LDS instead of each reading it from L2: the tile for step ks+1
-// is loaded into registers during step ks and written to the other half of
-// a double buffer after it, one barrier per step.
-template <int MT, int NT>
-__global__ __launch_bounds__(256, 2) void k_dbpass_mfma_sa(const uint8_t* __restrict__ db,
+// through an LDS double buffer (one barrier per step), and each wave keeps
+// its DB operands DEPTH steps ahead in a register ring: with few columns
+// per wave, one step of prefetch leaves too few bytes in flight per CU to
+// cover the HBM latency.
+template <int MT, int NT, int OCC, int DEPTH>
+__global__ __launch_bounds__(256, OCC) void k_dbpass_mfma_sa(const uint8_t* __restrict__ db,
                                                            const int8_t* __restrict__ dig,
                                                            const uint32_t* __restrict__ S,
                                                            uint32_t* __restrict__ out, size_t out_qstride,
-                                                           uint32_t C, uint32_t nq, uint32_t kbps) {
+                                                           uint32_t C, uint32_t OC, uint32_t nq, uint32_t kbps) {
     constexpr uint32_t Mp = 16 * MT;
     constexpr uint32_t AU = Mp * 4 / 256;  // 16-byte units of the step's digit tile per thread
     __shared__ i32x4 atile[2][4 * Mp];      // [buffer][block of the step][row]
@@ -284,62 +286,67 @@ __global__ __launch_bounds__(256, 2) void k_dbpass_mfma_sa(const uint8_t* __rest
     for (int a = 0; a < MT; a++)
 #pragma unroll
         for (int n = 0; n < NT; n++) acc[a][n][0] = acc[a][n][1] = i32x4{0, 0, 0, 0};
-    i32x4 v[2][NT][2];
+    i32x4 v[DEPTH][NT][2];
     i32x4 apre[AU];
-    auto load_b = [&](int bi, uint32_t ks) {
+    auto load_b = [&](int slot, uint32_t ks) {
 #pragma unroll
         for (int n = 0; n < NT; n++) {
-            v[bi][n][0] = __builtin_nontemporal_load(bsrc + ks * bstep + n * 16);
-            v[bi][n][1] = __builtin_nontemporal_load(bsrc + ks * bstep + n * 16 + C);
+            v[slot][n][0] = __builtin_nontemporal_load(bsrc + ks * bstep + n * 16);
+            v[slot][n][1] = __builtin_nontemporal_load(bsrc + ks * bstep + n * 16 + C);
         }
     };
     auto load_a = [&](uint32_t ks) {
 #pragma unroll
         for (int u = 0; u < (int)AU; u++) apre[u] = atile_src[(size_t)ks * 4 * Mp + tid + 256 * u];
     };
-    auto store_a = [&](int bi) {
+    auto store_a = [&](int buf) {
 #pragma unroll
-        for (int u = 0; u < (int)AU; u++) atile[bi][tid + 256 * u] = apre[u];
+        for (int u = 0; u < (int)AU; u++) atile[buf][tid + 256 * u] = apre[u];
     };
-    auto compute = [&](int bi, int abuf) {
+    auto compute = [&](int slot, int buf) {
 #pragma unroll
         for (int n = 0; n < NT; n++) {
-            const i32x4 Bl = v[bi][n][0] ^ flip, Bh = v[bi][n][1] ^ flip;
+            const i32x4 Bl = v[slot][n][0] ^ flip, Bh = v[slot][n][1] ^ flip;
 #pragma unroll
             for (int a = 0; a < MT; a++) {
-                const i32x4 A = atile[abuf][(l >> 4) * Mp + a * 16 + (l & 15)];
+                const i32x4 A = atile[buf][(l >> 4) * Mp + a * 16 + (l & 15)];
                 acc[a][n][0] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A, Bl, acc[a][n][0], 0, 0, 0);
                 acc[a][n][1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A, Bh, acc[a][n][1], 0, 0, 0);
             }
         }
     };
+    // Step s: compute from ring slot s%DEPTH and digit buffer s&1, then
+    // publish the digits of step s+1 in buffer (s+1)&1 (every wave finished
+    // step s-1, which read that buffer, before the previous barrier). The
+    // step count is uniform over the block, so every barrier is reached by
+    // all waves.
     load_a(0);
     store_a(0);
-    load_b(0, 0);
     if (nks > 1) load_a(1);
+#pragma unroll
+    for (int j = 0; j < DEPTH - 1; j++)
+        if ((uint32_t)j < nks) load_b(j, (uint32_t)j);
     __syncthreads();
-    for (uint32_t ks = 0; ks < nks; ks += 2) {  // nks is even
-        load_b(1, ks + 1);
-        compute(0, 0);
-        store_a(1);  // digits of step ks+1
-        __syncthreads();
-        if (ks + 2 < nks) {
-            load_b(0, ks + 2);
-            load_a(ks + 2);
+    for (uint32_t ks = 0; ks < nks; ks += DEPTH) {
+#pragma unroll
+        for (int j = 0; j < DEPTH; j++) {
+            const uint32_t step = ks + (uint32_t)j;
+            if (step >= nks) break;
+            if (step + DEPTH - 1 < nks) load_b((j + DEPTH - 1) % DEPTH, step + DEPTH - 1);
+            compute(j, (int)(step & 1));
+            if (step + 1 < nks) {
+                store_a((int)((step + 1) & 1));
+                if (step + 2 < nks) load_a(step + 2);
+            }
+            __syncthreads();
         }
-        compute(1, 1);
-        if (ks + 2 < nks) {
-            store_a(0);  // digits of step ks+2
-            if (ks + 3 < nks) load_a(ks + 3);
-        }
-        __syncthreads();
     }
 #pragma unroll
     for (int a = 0; a < MT; a++)
 #pragma unroll
         for (int n = 0; n < NT; n++)
             store_pair(acc[a][n][0], acc[a][n][1], (uint32_t)a * 4 + (l >> 4), nq, slice == 0, S, out, out_qstride,
-                       slice, C, col0 + n * 16 + (l & 15));
+                       slice, OC, col0 + n * 16 + (l & 15));
 }
 
 // Largest power-of-two slice count s with blocks*s <= target that keeps
@@ -371,10 +378,11 @@ DbPlan dbplan_f64(size_t R, size_t ncols, int unroll, size_t target_blocks) {
     return p;
 }
 
-DbPlan dbplan_mfma(size_t R, size_t ncols, size_t nq, size_t target_blocks) {
+DbPlan dbplan_mfma(size_t R, size_t ncols, size_t nq, size_t target_blocks, uint32_t depth) {
     DbPlan p;
     p.kind = DbPlan::MFMA;
     p.param = nq <= 2 ? 1 : nq <= 4 ? 2 : nq <= 8 ? 4 : nq <= 16 ? 8 : 16;  // 2 queries per 16-row digit tile
+    p.depth = p.param == 16 ? depth : 3;
     const size_t cols = 64 * mfma_nt(p.param);
     // slices: whole number of 128-row (8-block) chunks so the step count is even
     p.slices = pick_slices(R / 16, ncols / cols, target_blocks, 8);
@@ -390,23 +398,26 @@ size_t dbplan_cols(const DbPlan& p) { return p.kind == DbPlan::MFMA ? 64 * mfma_
 
 std::string dbplan_kernel(const DbPlan& p) {
     if (p.kind == DbPlan::F64) return "k_dbpass_f64<" + std::to_string(p.param) + ">";
-    const uint32_t nt = mfma_nt(p.param);
-    return std::string(p.param >= 8 ? "k_dbpass_mfma_sa<" : "k_dbpass_mfma<") + std::to_string(p.param) + ", " +
-           std::to_string(nt) + ">";
+    const std::string mt_nt = std::to_string(p.param) + ", " + std::to_string(mfma_nt(p.param));
+    if (p.param < 8) return "k_dbpass_mfma<" + mt_nt + ">";
+    return "k_dbpass_mfma_sa<" + mt_nt + ", 2, " + std::to_string(p.depth) + ">";
 }
 
-hipError_t launch_dbpass_f64(const uint8_t* db, const double* bq, uint32_t* out, size_t R, size_t C, size_t col0,
-                             size_t ncols, const DbPlan& p, hipStream_t s) {
-    if (p.kind != DbPlan::F64 || ncols % 256 || col0 % 256 || col0 + ncols > C) return hipErrorInvalidValue;
+hipError_t launch_dbpass_f64(const uint8_t* db, const double* bq, uint32_t* out, size_t ostride, size_t R, size_t C,
+                             size_t col0, size_t ncols, const DbPlan& p, hipStream_t s) {
+    if (p.kind != DbPlan::F64 || ncols % 256 || col0 % 256 || col0 + ncols > C || ostride < ncols)
+        return hipErrorInvalidValue;
     const uint32_t kbps = (uint32_t)(R / 16 / p.slices);
     const dim3 grid((unsigned)(ncols / 256), p.slices);
     const uint8_t* d = db + col0 * 16;
-    uint32_t* o = out + col0;
+    const uint32_t Cc = (uint32_t)C, OC = (uint32_t)ostride;
+#define F64(U) hipLaunchKernelGGL(k_dbpass_f64<U>, grid, dim3(256), 0, s, d, bq, out, (uint32_t)R, Cc, OC, kbps)
     switch (p.param) {
-        case 2: hipLaunchKernelGGL(k_dbpass_f64<2>, grid, dim3(256), 0, s, d, bq, o, (uint32_t)R, (uint32_t)C, kbps); break;
-        case 8: hipLaunchKernelGGL(k_dbpass_f64<8>, grid, dim3(256), 0, s, d, bq, o, (uint32_t)R, (uint32_t)C, kbps); break;
-        default: hipLaunchKernelGGL(k_dbpass_f64<4>, grid, dim3(256), 0, s, d, bq, o, (uint32_t)R, (uint32_t)C, kbps); break;
+        case 2: F64(2); break;
+        case 8: F64(8); break;
+        default: F64(4); break;
     }
+#undef F64
     return hipGetLastError();
 }
 
@@ -423,24 +434,33 @@ hipError_t launch_mfma_digits(const double* bq, size_t nq, size_t R, const DbPla
 }
 
 hipError_t launch_dbpass_mfma(const uint8_t* db, const int8_t* dig, const uint32_t* S, size_t nq, uint32_t* out,
-                              size_t out_qstride, size_t R, size_t C, size_t col0, size_t ncols, const DbPlan& p,
-                              hipStream_t s) {
+                              size_t out_qstride, size_t ostride, size_t R, size_t C, size_t col0, size_t ncols,
+                              const DbPlan& p, hipStream_t s) {
     if (p.kind != DbPlan::MFMA || nq == 0 || nq > kMfmaMaxQueries) return hipErrorInvalidValue;
     const uint32_t Mp = (uint32_t)dbplan_mp(p);
     const uint32_t kbps = (uint32_t)(R / 16) / p.slices;
     const size_t cols = dbplan_cols(p);
-    if (2 * nq > Mp / 4 || kbps % 8 || ncols % cols || col0 % cols || col0 + ncols > C) return hipErrorInvalidValue;
-    const uint32_t Cc = (uint32_t)C, Q = (uint32_t)nq;
+    if (2 * nq > Mp / 4 || kbps % 8 || ncols % cols || col0 % cols || col0 + ncols > C || ostride < ncols)
+        return hipErrorInvalidValue;
+    const uint32_t Cc = (uint32_t)C, OC = (uint32_t)ostride, Q = (uint32_t)nq;
     const dim3 grid((unsigned)(ncols / cols), p.slices);
     const uint8_t* d = db + col0 * 16;
-    uint32_t* o = out + col0;
-#define MFMA(K, MT, NT) hipLaunchKernelGGL((K<MT, NT>), grid, dim3(256), 0, s, d, dig, S, o, out_qstride, Cc, Q, kbps)
+#define MFMA(K, ...) \
+    hipLaunchKernelGGL((K<__VA_ARGS__>), grid, dim3(256), 0, s, d, dig, S, out, out_qstride, Cc, OC, Q, kbps)
     switch (p.param) {
         case 1: MFMA(k_dbpass_mfma, 1, 4); break;
         case 2: MFMA(k_dbpass_mfma, 2, 4); break;
         case 4: MFMA(k_dbpass_mfma, 4, 4); break;
-        case 8: MFMA(k_dbpass_mfma_sa, 8, 2); break;
-        case 16: MFMA(k_dbpass_mfma_sa, 16, 1); break;
+        case 8: MFMA(k_dbpass_mfma_sa, 8, 2, 2, 3); break;
+        case 16:
+            switch (p.depth) {
+                case 3: MFMA(k_dbpass_mfma_sa, 16, 1, 2, 3); break;
+                case 4: MFMA(k_dbpass_mfma_sa, 16, 1, 2, 4); break;
+                case 5: MFMA(k_dbpass_mfma_sa, 16, 1, 2, 5); break;
+                case 6: MFMA(k_dbpass_mfma_sa, 16, 1, 2, 6); break;
+                default: return hipErrorInvalidValue;
+            }
+            break;
         default: return hipErrorInvalidValue;
     }
 #undef MFMA

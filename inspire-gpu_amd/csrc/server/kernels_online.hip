@@ -62,15 +62,15 @@ typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 // interm[q][limb][col] = sum over row slices of part mod q_limb, for columns
 // col0 + [0, gridDim.x*1024); blockIdx.y = limb, blockIdx.z = query.
 __global__ __launch_bounds__(256) void k_reduce_slices(const uint32_t* __restrict__ part, size_t part_stride,
-                                                       uint32_t S, uint32_t C, uint32_t col0,
+                                                       uint32_t S, uint32_t C, uint32_t col0, uint32_t ncols,
                                                        uint32_t* __restrict__ interm) {
-    const uint32_t col = col0 + (blockIdx.x * 256 + threadIdx.x) * 4;
+    const uint32_t j = (blockIdx.x * 256 + threadIdx.x) * 4, col = col0 + j;
     const int limb = (int)blockIdx.y;
     const uint32_t q = blockIdx.z;
-    const u32x4* pp = reinterpret_cast<const u32x4*>(part + q * part_stride + (size_t)limb * C + col);
+    const u32x4* pp = reinterpret_cast<const u32x4*>(part + q * part_stride + (size_t)limb * ncols + j);
     uint64_t a[4] = {0, 0, 0, 0};
     for (uint32_t s = 0; s < S; s++) {
-        const u32x4 v = __builtin_nontemporal_load(pp + (size_t)s * (2 * C / 4));
+        const u32x4 v = __builtin_nontemporal_load(pp + (size_t)s * (2 * ncols / 4));
         a[0] += v[0]; a[1] += v[1]; a[2] += v[2]; a[3] += v[3];
     }
     u32x4 o;
@@ -251,7 +251,7 @@ hipError_t launch_reduce_slices(const uint32_t* part, size_t part_stride, uint32
     if (ncols % 1024 || col0 % 4 || col0 + ncols > C) return hipErrorInvalidValue;
     if (nq == 0 || ncols == 0) return hipSuccess;
     hipLaunchKernelGGL(k_reduce_slices, dim3((unsigned)(ncols / 1024), 2, (unsigned)nq), dim3(256), 0, s, part,
-                       part_stride, S, (uint32_t)C, (uint32_t)col0, interm);
+                       part_stride, S, (uint32_t)C, (uint32_t)col0, (uint32_t)ncols, interm);
     return hipGetLastError();
 }
 

@@ -251,6 +251,17 @@ const size_t kMfmaTargets[] = {256, 512, 1024, 2048};
 DbPlan cand_plan(const DbCand& c, size_t R, size_t ncols) {
     return c.mfma ? dbplan_mfma(R, ncols, 1, c.target) : dbplan_f64(R, ncols, c.unroll, c.target);
 }
+
+// The plan of `full` (over all C columns) narrowed to ncols columns: the
+// same kernel, with row slices multiplied so the grid keeps the full
+// plan's workgroup count (a narrow grid leaves most CUs idle).
+DbPlan narrow_plan(const DbPlan& full, size_t target, size_t R, size_t C, size_t ncols, size_t nq) {
+    if (ncols == C) return full;
+    const size_t wgs = C / dbplan_cols(full) * full.slices;
+    const size_t t = wgs > target ? wgs : target;
+    return full.kind == DbPlan::F64 ? dbplan_f64(R, ncols, (int)full.param, t)
+                                     : dbplan_mfma(R, ncols, nq, t, full.depth);
+}
 }  // namespace
 
 void GpuServerCtx::alloc_slots() {
@@ -263,21 +274,27 @@ void GpuServerCtx::alloc_slots() {
         const long v = std::atol(gc);
         if (v > 0 && (size_t)v <= G && G % (size_t)v == 0) gchunk = (size_t)v;
     }
-    size_t max_slices = 1;
-    for (size_t ncols : {C, gchunk * N}) {
-        for (const DbCand& c : kDbCands) {
-            const DbPlan x = cand_plan(c, R, ncols);
-            if (x.slices > max_slices) max_slices = x.slices;
-        }
-        for (size_t t : kMfmaTargets)
-            for (size_t nq = 1; nq <= kMfmaMaxQueries; nq++) {
-                const DbPlan x = dbplan_mfma(R, ncols, nq, t);
-                if (x.slices > max_slices) max_slices = x.slices;
-            }
+    // row-slice partials of one query: [slices][2][ncols] words for the
+    // widest plan any candidate (full width or narrowed to a chunk) needs
+    size_t part_words = 2 * C;
+    auto need = [&](const DbPlan& x, size_t ncols) {
+        if (x.slices > 1 && x.slices * 2 * ncols > part_words) part_words = x.slices * 2 * ncols;
+    };
+    const size_t ccols = gchunk * N;
+    for (const DbCand& c : kDbCands) {
+        const DbPlan full = cand_plan(c, R, C);
+        need(full, C);
+        need(narrow_plan(full, c.target, R, C, ccols, 1), ccols);
     }
+    for (size_t t : kMfmaTargets)
+        for (size_t nq = 1; nq <= kMfmaMaxQueries; nq++) {
+            const DbPlan full = dbplan_mfma(R, C, nq, t);
+            need(full, C);
+            need(narrow_plan(full, t, R, C, ccols, nq), ccols);
+        }
     lay.R = R; lay.C = C; lay.G = G;
     qlen = lay.query_u64s();
-    part_stride = max_slices * 2 * C;
+    part_stride = part_words;
     part2_stride = G * 2 * N;
     keys_stride = lay.keys_words();
     rgsw_stride = lay.rgsw_words();
@@ -301,7 +318,10 @@ void GpuServerCtx::alloc_slots() {
     d_hstate = dalloc<uint32_t>(B * T * 4 * N);
     for (int l = 0; l < kLanes; l++) {
         HIPX(hipStreamCreate(&hstream[l]));
+        HIPX(hipStreamCreate(&cstream[l]));
         for (auto& e : chunk_ev[l]) HIPX(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        for (auto& e : coll_ev[l]) HIPX(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        HIPX(hipEventCreateWithFlags(&prep_ev[l], hipEventDisableTiming));
         HIPX(hipEventCreateWithFlags(&horner_done[l], hipEventDisableTiming));
     }
     d_hxbuf = dalloc<uint32_t>(horner_pair_xbuf_words(B * T));
@@ -310,6 +330,16 @@ void GpuServerCtx::alloc_slots() {
     HIPX(hipMemset(d_status, 0, 4));
     if (const char* h = std::getenv("INSPIRE_HORNER"))
         horner_mode = std::strcmp(h, "pair") == 0 ? 1 : std::strcmp(h, "single") == 0 ? 0 : -1;
+    // INSPIRE_PIPELINE=<single>,<batched>: the pipeline shape (0 serial, 2
+    // two streams, 3 three streams) of one-query and many-query calls
+    if (const char* pl = std::getenv("INSPIRE_PIPELINE")) {
+        int a1 = -1, a2 = -1;
+        if (std::sscanf(pl, "%d,%d", &a1, &a2) == 2 && (a1 == 0 || a1 == 2 || a1 == 3) &&
+            (a2 == 0 || a2 == 2 || a2 == 3)) {
+            pipe_single = a1;
+            pipe_batched = a2;
+        }
+    }
     HIPX(hipMemset(d_bq, 0, B * 2 * R * sizeof(double)));
     HIPX(hipHostMalloc((void**)&h_q, B * qlen * 8, hipHostMallocDefault));
     HIPX(hipHostMalloc((void**)&h_resp, B * resp_u64s * 8, hipHostMallocDefault));
@@ -329,6 +359,12 @@ void GpuServerCtx::autotune_dbpass() {
     if (const char* t = std::getenv("INSPIRE_MFMA_TARGET")) {
         const long v = std::atol(t);
         if (v > 0) mfma_target = (size_t)v;
+    }
+    bool depth_forced = false;
+    if (const char* t = std::getenv("INSPIRE_MFMA_DEPTH")) {
+        const long v = std::atol(t);
+        for (uint32_t d : kMfmaDepths)
+            if (v == (long)d) { mfma_depth = d; depth_forced = true; }
     }
     if (const char* force = std::getenv("INSPIRE_DBPASS")) {
         int u = 4, t = 2048;
@@ -350,9 +386,9 @@ void GpuServerCtx::autotune_dbpass() {
         if (x.kind == DbPlan::MFMA) HIPX(launch_mfma_digits(d_bq, nq, R, x, d_dig[0], d_S[0], stream));
         auto run = [&]() {
             if (x.kind == DbPlan::F64)
-                HIPX(launch_dbpass_f64(db_planes(), d_bq, d_part, R, C, 0, C, x, stream));
+                HIPX(launch_dbpass_f64(db_planes(), d_bq, d_part, C, R, C, 0, C, x, stream));
             else
-                HIPX(launch_dbpass_mfma(db_planes(), d_dig[0], d_S[0], nq, d_part, part_stride, R, C, 0, C, x,
+                HIPX(launch_dbpass_mfma(db_planes(), d_dig[0], d_S[0], nq, d_part, part_stride, C, R, C, 0, C, x,
                                         stream));
         };
         run();  // warm
@@ -374,10 +410,13 @@ void GpuServerCtx::autotune_dbpass() {
     const size_t nq = cfg.max_batch < kMfmaMaxQueries ? cfg.max_batch : kMfmaMaxQueries;
     if (nq > 1) {
         best = 1e30f;
-        for (size_t t : kMfmaTargets) {
-            const float ms = time_plan(dbplan_mfma(R, C, nq, t), nq);
-            if (ms < best) { best = ms; mfma_target = t; }
-        }
+        const bool ring = dbplan_mfma(R, C, nq, mfma_target).param == 16 && !depth_forced;
+        for (size_t t : kMfmaTargets)
+            for (uint32_t d : kMfmaDepths) {
+                if (!ring && d != mfma_depth) continue;
+                const float ms = time_plan(dbplan_mfma(R, C, nq, t, d), nq);
+                if (ms < best) { best = ms; mfma_target = t; mfma_depth = d; }
+            }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -394,7 +433,12 @@ DbPlan GpuServerCtx::plan_for(size_t nq, size_t ncols) const {
     if (nq == 1)
         return plan1.kind == DbPlan::F64 ? dbplan_f64(R, ncols, (int)plan1.param, plan1_target)
                                          : dbplan_mfma(R, ncols, 1, plan1_target);
-    return dbplan_mfma(R, ncols, nq, mfma_target);
+    return dbplan_mfma(R, ncols, nq, mfma_target, mfma_depth);
+}
+
+DbPlan GpuServerCtx::chunk_plan(size_t nq, size_t ncols) const {
+    return narrow_plan(plan_for(nq, pp.db_cols), nq == 1 ? plan1_target : mfma_target, pp.db_rows, pp.db_cols, ncols,
+                       nq);
 }
 
 void GpuServerCtx::dbpass_prep(size_t q0, size_t nq, const DbPlan& p, hipStream_t s, int lane) {
@@ -405,14 +449,20 @@ void GpuServerCtx::dbpass_prep(size_t q0, size_t nq, const DbPlan& p, hipStream_
 void GpuServerCtx::dbpass_range(size_t q0, size_t nq, size_t g_lo, size_t g_hi, const DbPlan& p, hipStream_t s,
                                 int lane) {
     const size_t R = pp.db_rows, C = pp.db_cols, col0 = g_lo * N, ncols = (g_hi - g_lo) * N;
-    // one slice: straight into interm; else row-slice partials, reduced below
-    uint32_t* out = p.slices == 1 ? d_interm + q0 * 2 * C : d_part + q0 * part_stride;
-    const size_t qstride = p.slices == 1 ? 2 * C : part_stride;
+    // one slice: straight into interm; else row-slice partials [slice][2][ncols]
+    // relative to col0, reduced below
+    const bool direct = p.slices == 1;
+    if (!direct && p.slices * 2 * ncols > part_stride)
+        throw std::logic_error("database pass: row-slice partials exceed their buffer");
+    uint32_t* out = direct ? d_interm + q0 * 2 * C + col0 : d_part + q0 * part_stride;
+    const size_t qstride = direct ? 2 * C : part_stride, ostride = direct ? C : ncols;
     if (p.kind == DbPlan::F64) {
         for (size_t q = 0; q < nq; q++)
-            HIPX(launch_dbpass_f64(db_planes(), d_bq + (q0 + q) * 2 * R, out + q * qstride, R, C, col0, ncols, p, s));
+            HIPX(launch_dbpass_f64(db_planes(), d_bq + (q0 + q) * 2 * R, out + q * qstride, ostride, R, C, col0, ncols,
+                                   p, s));
     } else {
-        HIPX(launch_dbpass_mfma(db_planes(), d_dig[lane], d_S[lane], nq, out, qstride, R, C, col0, ncols, p, s));
+        HIPX(launch_dbpass_mfma(db_planes(), d_dig[lane], d_S[lane], nq, out, qstride, ostride, R, C, col0, ncols, p,
+                                s));
     }
     if (p.slices > 1)
         HIPX(launch_reduce_slices(d_part + q0 * part_stride, part_stride, p.slices, C, d_interm + q0 * 2 * C, nq, col0,
@@ -426,7 +476,7 @@ void GpuServerCtx::stage_dbpass(size_t count, hipStream_t s, size_t slot0, int l
         const size_t nq = slot0 + count - q0 < step ? slot0 + count - q0 : step;
         DbPlan p = plan_for(nq, C);
         if (force == 0 && p.kind != DbPlan::F64) p = dbplan_f64(R, C, 4, 2048);
-        if (force == 1 && p.kind == DbPlan::F64) p = dbplan_mfma(R, C, nq, mfma_target);
+        if (force == 1 && p.kind == DbPlan::F64) p = dbplan_mfma(R, C, nq, mfma_target, mfma_depth);
         dbpass_prep(q0, nq, p, s, lane);
         dbpass_range(q0, nq, 0, G, p, s, lane);
     }
@@ -477,39 +527,65 @@ void GpuServerCtx::enqueue(const uint64_t* d_in, size_t count, uint64_t* d_out, 
     stage_prep(d_in, count, s, slot0);
     const size_t C = pp.db_cols;
     const size_t nchunks = G / gchunk;
+    const hipStream_t hs = hstream[lane];
     // Pipelined over group chunks, highest first (the order every Horner
-    // chain consumes groups): the database pass, collapse and pack of chunk k
-    // run on s while the Horner chains of chunk k+1 (the groups above) run
-    // on the lane's second stream; s finally waits for the chains.
+    // chain consumes groups). The database pass (on s) and the collapse (on
+    // s, or on the lane's collapse stream) need nothing from each other; the
+    // pack and the Horner chains of a chunk (on the lane's Horner stream)
+    // wait for both halves of that chunk. s finally waits for the chains,
+    // and through their waits for every collapse.
+    bool joined = false;
     for (size_t q0 = slot0; q0 < slot0 + count; q0 += kMfmaMaxQueries) {
         const size_t nq = slot0 + count - q0 < kMfmaMaxQueries ? slot0 + count - q0 : kMfmaMaxQueries;
-        const DbPlan p = plan_for(nq, gchunk * N);
-        dbpass_prep(q0, nq, p, s, lane);
-        HIPX(launch_collapse_keys(d_keys + q0 * keys_stride, keys_stride, nq, d_kdig[lane], d_KS[lane], s));
-        if (nchunks > 1) {
-            HIPX(hipEventRecord(horner_done[lane], s));  // the side stream starts after this batch's prologue
-            HIPX(hipStreamWaitEvent(hstream[lane], horner_done[lane], 0));
+        const int shape = nchunks == 1 ? 0 : nq == 1 ? pipe_single : pipe_batched;
+        uint32_t* p2 = d_part2 + q0 * part2_stride;
+        const uint32_t* keys = d_keys + q0 * keys_stride;
+        uint32_t* packed = d_packed_b + q0 * packed_stride;
+        uint64_t* out = d_out + (q0 - slot0) * resp_u64s;
+        if (shape == 0) {
+            const DbPlan p = plan_for(nq, C);
+            dbpass_prep(q0, nq, p, s, lane);
+            dbpass_range(q0, nq, 0, G, p, s, lane);
+            HIPX(launch_collapse_keys(keys, keys_stride, nq, d_kdig[lane], d_KS[lane], s));
+            HIPX(launch_collapse(d_precomp, G, d_kdig[lane], d_KS[lane], nq, p2, part2_stride, s));
+            HIPX(launch_pack_finalize(d_interm + q0 * 2 * C, C, p2, part2_stride, d_precomp, 0, G, keys, keys_stride,
+                                      packed, packed_stride, nq, nc, s));
+            horner_range(nq, out, 0, G, s, q0);
+            continue;
         }
+        // shape 2 keeps the collapse on s (two HBM-bound passes side by side
+        // share the bandwidth worse than back to back); shape 3 runs it beside
+        const hipStream_t cs = shape == 3 ? cstream[lane] : s;
+        joined = true;
+        const DbPlan p = chunk_plan(nq, gchunk * N);
+        if (cs != s) {
+            HIPX(hipEventRecord(prep_ev[lane], s));  // the keys are in place
+            HIPX(hipStreamWaitEvent(cs, prep_ev[lane], 0));
+        }
+        HIPX(launch_collapse_keys(keys, keys_stride, nq, d_kdig[lane], d_KS[lane], cs));
+        dbpass_prep(q0, nq, p, s, lane);
         for (size_t k = nchunks; k-- > 0;) {
             const size_t g_lo = k * gchunk, g_hi = g_lo + gchunk;
+            hipEvent_t dbe = chunk_ev[lane][k % kChunkEvents], ce = coll_ev[lane][k % kChunkEvents];
             dbpass_range(q0, nq, g_lo, g_hi, p, s, lane);
+            HIPX(hipEventRecord(dbe, s));
             HIPX(launch_collapse(d_precomp + g_lo * PC_GROUP, gchunk, d_kdig[lane], d_KS[lane], nq,
-                                 d_part2 + q0 * part2_stride + g_lo * 2 * N, part2_stride, s));
-            HIPX(launch_pack_finalize(d_interm + q0 * 2 * C, C, d_part2 + q0 * part2_stride, part2_stride, d_precomp,
-                                      g_lo, gchunk, d_keys + q0 * keys_stride, keys_stride,
-                                      d_packed_b + q0 * packed_stride, packed_stride, nq, nc, s));
-            if (nchunks == 1) {
-                horner_range(nq, d_out + (q0 - slot0) * resp_u64s, 0, G, s, q0);
-            } else {
-                HIPX(hipEventRecord(chunk_ev[lane][k % kChunkEvents], s));
-                HIPX(hipStreamWaitEvent(hstream[lane], chunk_ev[lane][k % kChunkEvents], 0));
-                horner_range(nq, d_out + (q0 - slot0) * resp_u64s, g_lo, g_hi, hstream[lane], q0);
-            }
+                                 p2 + g_lo * 2 * N, part2_stride, cs));
+            HIPX(hipEventRecord(ce, cs));
+            HIPX(hipStreamWaitEvent(hs, dbe, 0));
+            HIPX(hipStreamWaitEvent(hs, ce, 0));
+            HIPX(launch_pack_finalize(d_interm + q0 * 2 * C, C, p2, part2_stride, d_precomp, g_lo, gchunk, keys,
+                                      keys_stride, packed, packed_stride, nq, nc, hs));
+            horner_range(nq, out, g_lo, g_hi, hs, q0);
         }
-        if (nchunks > 1) {
-            HIPX(hipEventRecord(horner_done[lane], hstream[lane]));
-            HIPX(hipStreamWaitEvent(s, horner_done[lane], 0));
+        if (cs != s) {  // the next batch's key digits overwrite the lane's scratch
+            HIPX(hipEventRecord(prep_ev[lane], cs));
+            HIPX(hipStreamWaitEvent(s, prep_ev[lane], 0));
         }
+    }
+    if (joined) {
+        HIPX(hipEventRecord(horner_done[lane], hs));
+        HIPX(hipStreamWaitEvent(s, horner_done[lane], 0));
     }
 }
 
@@ -608,10 +684,15 @@ void gpu_free_server(GpuServerCtx* ctx) {
     dfree(ctx->d_hxbuf); dfree(ctx->d_hflags); dfree(ctx->d_status); dfree(ctx->d_hstate);
     for (int l = 0; l < GpuServerCtx::kLanes; l++) {
         if (ctx->hstream[l]) (void)hipStreamSynchronize(ctx->hstream[l]);
+        if (ctx->cstream[l]) (void)hipStreamSynchronize(ctx->cstream[l]);
         for (auto& e : ctx->chunk_ev[l])
             if (e) (void)hipEventDestroy(e);
+        for (auto& e : ctx->coll_ev[l])
+            if (e) (void)hipEventDestroy(e);
+        if (ctx->prep_ev[l]) (void)hipEventDestroy(ctx->prep_ev[l]);
         if (ctx->horner_done[l]) (void)hipEventDestroy(ctx->horner_done[l]);
         if (ctx->hstream[l]) (void)hipStreamDestroy(ctx->hstream[l]);
+        if (ctx->cstream[l]) (void)hipStreamDestroy(ctx->cstream[l]);
     }
     for (int l = 0; l < GpuServerCtx::kLanes; l++) {
         dfree(ctx->d_dig[l]); dfree(ctx->d_S[l]); dfree(ctx->d_kdig[l]); dfree(ctx->d_KS[l]);

@@ -36,7 +36,11 @@ struct GpuServerCtx {
     ipx::DbPlan plan1{};            // database pass of a single query over all columns (autotuned)
     size_t plan1_target = 2048;     // its workgroup target
     size_t mfma_target = 1024;      // workgroups the batched pass aims for
+    uint32_t mfma_depth = 4;        // register-ring depth of the 17..32-query pass
     size_t gchunk = 0;              // groups per chunk of the pipelined answer
+    // pipeline shape of one-query and many-query calls: 0 serial on the
+    // caller's stream, 2 Horner chains beside the passes, 3 collapse beside too
+    int pipe_single = 0, pipe_batched = 2;
     size_t qlen = 0, resp_u64s = 0;
     size_t part_stride = 0, part2_stride = 0, keys_stride = 0, rgsw_stride = 0, packed_stride = 0;
     uint64_t* d_q = nullptr;        // flat queries (device staging)
@@ -55,11 +59,14 @@ struct GpuServerCtx {
     static constexpr size_t kKdigBytes = 2 * (2 * ipx::N / 16) * 128 * 16;
     int8_t* d_kdig[kLanes] = {};
     uint32_t* d_KS[kLanes] = {};
-    // per lane: the stream the Horner chains run on beside the passes, and
-    // the events that hand each group chunk over to it
+    // per lane: the streams the collapse and the Horner chains run on beside
+    // the database pass, and the events that hand each group chunk over
     static constexpr int kChunkEvents = 16;
     hipStream_t hstream[kLanes] = {};
-    hipEvent_t chunk_ev[kLanes][kChunkEvents] = {};
+    hipStream_t cstream[kLanes] = {};
+    hipEvent_t chunk_ev[kLanes][kChunkEvents] = {};  // database pass of a chunk done
+    hipEvent_t coll_ev[kLanes][kChunkEvents] = {};   // collapse of a chunk done
+    hipEvent_t prep_ev[kLanes] = {};                 // keys of a batch in place
     hipEvent_t horner_done[kLanes] = {};
     uint32_t* d_packed_b = nullptr;
     // Horner: 0 = one workgroup per chain, 1 = a workgroup pair (one per
@@ -83,6 +90,8 @@ struct GpuServerCtx {
     void stage_prep(const uint64_t* d_in, size_t count, hipStream_t s, size_t slot0);
     // The tuned database-pass plan for nq queries over ncols columns.
     ipx::DbPlan plan_for(size_t nq, size_t ncols) const;
+    // The tuned full-width plan narrowed to ncols columns at its workgroup count.
+    ipx::DbPlan chunk_plan(size_t nq, size_t ncols) const;
     // Digits of queries [q0, q0+nq) for an MFMA plan (no-op for fp64).
     void dbpass_prep(size_t q0, size_t nq, const ipx::DbPlan& p, hipStream_t s, int lane);
     // Database pass of queries [q0, q0+nq) over the columns of groups [g_lo, g_hi) into interm.

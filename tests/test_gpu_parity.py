@@ -181,6 +181,42 @@ def test_db_pass_paths_match_oracle(medium8, path):
     assert srv.lib.ipir_dbg_db_pass(srv.h, ia.ptr(bad), 1, path, ia.ptr(np.zeros((1, 2, Cc), np.uint32))) == 1
 
 
+@pytest.mark.parametrize("depth", [3, 4, 5, 6])
+def test_many_query_ring_depths_match_oracle(medium8, depth):
+    """Every register-ring depth of the 17..32-query MFMA pass against the
+    oracle's mat-vec (17 and 24 queries)."""
+    geo, db, srv = medium8
+    R, Cc = geo["R"], geo["C"]
+    enc = np.zeros((R, Cc), np.uint16)
+    assert srv.lib.ipir_dbg_db_rows(srv.h, 0, R, ia.ptr(enc)) == 0
+    flat = enc.reshape(-1)
+    rng = np.random.default_rng(400 + depth)
+    name = C.create_string_buffer(128)
+    assert srv.lib.ipir_dbg_dbpass_kernel(srv.h, MB8, name, 128) == 0
+    tuned = int(name.value.decode().split(">")[0].split(",")[-1])  # k_dbpass_mfma_sa<16, 1, 2, depth>
+    assert srv.lib.ipir_dbg_set_mfma_depth(srv.h, 2) == 1
+    assert srv.lib.ipir_dbg_set_mfma_depth(srv.h, depth) == 0
+    try:
+        for nq in (17, MB8):
+            _ring_case(srv, flat, R, Cc, rng, nq, depth)
+    finally:
+        assert srv.lib.ipir_dbg_set_mfma_depth(srv.h, tuned) == 0
+
+
+def _ring_case(srv, flat, R, Cc, rng, nq, depth):
+    b = np.zeros((nq, 2, R), np.uint64)
+    b[:, 0] = rng.integers(0, Q0, (nq, R), dtype=np.uint64)
+    b[:, 1] = rng.integers(0, Q1, (nq, R), dtype=np.uint64)
+    b[-1, :, :R // 8] = np.array([Q0 - 1, Q1 - 1], np.uint64)[:, None]
+    got = np.zeros((nq, 2, Cc), np.uint32)
+    assert srv.lib.ipir_dbg_db_pass(srv.h, ia.ptr(b), nq, 1, ia.ptr(got)) == 0
+    for k in range(nq):
+        want = np.zeros(2 * Cc, np.uint64)
+        ol.oracle().orc_matvec(flat, R, Cc, np.ascontiguousarray(b[k, 0]), np.ascontiguousarray(b[k, 1]),
+                               0, Cc, want)
+        assert np.array_equal(got[k].reshape(-1).astype(np.uint64), want), f"depth {depth} nq {nq} query {k}"
+
+
 def test_batch_equals_singles_and_extracts(medium8):
     geo, db, srv = medium8
     idxs = [0, 42, 1000, 100000, 524287] * 4  # 20: a 17..32-query DB pass, two collapse passes
@@ -220,16 +256,19 @@ def test_pair_horner_answers_equal_single(medium8):
     assert np.array_equal(one[0], base[0]) and np.array_equal(one_single[0], base[0])
 
 
-@pytest.mark.parametrize("groups", [1, 2, 4])
-def test_pipelined_group_chunks_equal_unchunked(medium8, groups):
+@pytest.mark.parametrize("groups,shape", [(1, 2), (2, 2), (4, 2), (1, 3), (2, 3)])
+def test_pipelined_group_chunks_equal_unchunked(medium8, groups, shape):
     """The answer pipelined over group chunks (Horner chains on a second
-    stream, resuming between launches) equals the one-chunk answer, for both
-    Horner kernels."""
+    stream, resuming between launches; shape 3 also the collapse on a third)
+    equals the serial one-chunk answer, for both Horner kernels, one query
+    and three."""
     geo, db, srv = medium8
     qs = [ol.make_query(geo, SEED, idx, 800 + k)[0] for k, idx in enumerate([3, 99999, 524287])]
     base = srv.answer_batch(qs)
     one = srv.answer_batch(qs[:1])
+    assert srv.lib.ipir_dbg_set_pipeline(srv.h, 1, shape) == 1
     assert srv.lib.ipir_dbg_set_group_chunk(srv.h, groups) == 0
+    assert srv.lib.ipir_dbg_set_pipeline(srv.h, shape, shape) == 0
     try:
         got = srv.answer_batch(qs)
         got1 = srv.answer_batch(qs[:1])
@@ -237,6 +276,7 @@ def test_pipelined_group_chunks_equal_unchunked(medium8, groups):
         pair = srv.answer_batch(qs)
     finally:
         assert srv.lib.ipir_dbg_set_horner(srv.h, -1) == 0
+        assert srv.lib.ipir_dbg_set_pipeline(srv.h, 0, 2) == 0
         assert srv.lib.ipir_dbg_set_group_chunk(srv.h, geo["G"]) == 0
     assert np.array_equal(got, base) and np.array_equal(got1[0], one[0]) and np.array_equal(pair, base)
 
