@@ -59,8 +59,8 @@ int ipir_dbg_set_group_chunk(ipir_server* srv, size_t groups);
  * or -1 (the default) to use the pair only for calls with at most two
  * chains. */
 int ipir_dbg_set_horner(ipir_server* srv, int mode);
-/* Set the register-ring depth (3..6) of the 17..32-query database pass a
- * server uses; 1 for other values. */
+/* Set the register-ring depth (3..5; 9..16-query plans cap it at 4) of the
+ * shared-digit database passes a server uses; 1 for other values. */
 int ipir_dbg_set_mfma_depth(ipir_server* srv, uint32_t depth);
 /* Set the pipeline shape of one-query and many-query answers: 0 serial, 2
  * Horner chains on a second stream per group chunk, 3 the collapse on a

@@ -37,10 +37,11 @@ struct DbPlan {
     enum Kind { F64, MFMA } kind;
     uint32_t param;      // F64: 16-row blocks per iteration (2, 4, 8); MFMA: 16-row digit tiles (1, 2, 4, 8, 16)
     uint32_t slices;     // row slices (power of two)
-    uint32_t depth = 3;  // 16 digit tiles: register-ring slots of DB operands (kMfmaDepths)
+    uint32_t depth = 3;  // 8 or 16 digit tiles: register-ring slots of DB operands (kMfmaDepths)
 };
-// Ring depths the 16-tile (17..32 query) MFMA pass is built with.
-constexpr uint32_t kMfmaDepths[] = {3, 4, 5, 6};
+// Ring depths of the shared-digit (9..32 query) MFMA passes; 8-tile plans
+// cap the depth at 4.
+constexpr uint32_t kMfmaDepths[] = {3, 4, 5};
 // Row-major staging rows (16*nkb of them) -> blocks [kb0, kb0 + nkb) of the planes.
 hipError_t launch_to_planes(const uint16_t* src, uint8_t* dst, size_t C, size_t kb0, size_t nkb, hipStream_t s);
 // Plans for a pass over ncols columns (the slice count fills the device).

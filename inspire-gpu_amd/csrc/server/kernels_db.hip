@@ -258,16 +258,24 @@ __global__ __launch_bounds__(256, 2) void k_dbpass_mfma(const uint8_t* __restric
 
 // Many queries (MT = 8 or 16 digit tiles, up to 16 or 32 queries): as above,
 // but the block's 4 waves share the step's digit tile (Mp x 64 bytes)
-// through an LDS double buffer (one barrier per step), and each wave keeps
-// its DB operands DEPTH steps ahead in a register ring: with few columns
-// per wave, one step of prefetch leaves too few bytes in flight per CU to
-// cover the HBM latency.
+// through an LDS double buffer (one barrier per step). Every global load is
+// issued L = DEPTH-1 steps before its step: the DB operands into a register
+// ring of DEPTH slots, the thread's share of the digit tile into a ring of L
+// slots (stored to LDS one step ahead). Vector-memory loads complete in issue
+// order, so a load issued closer to its use (a digit prefetch one step ahead)
+// would make the wait for it also wait for every DB operand issued before it;
+// with one lead for all loads each wait covers exactly one step's loads. Past
+// the slice end the loads repeat the last step (in bounds, never used), so
+// every step issues the same instructions.
 template <int MT, int NT, int OCC, int DEPTH>
 __global__ __launch_bounds__(256, OCC) void k_dbpass_mfma_sa(const uint8_t* __restrict__ db,
                                                            const int8_t* __restrict__ dig,
                                                            const uint32_t* __restrict__ S,
                                                            uint32_t* __restrict__ out, size_t out_qstride,
                                                            uint32_t C, uint32_t OC, uint32_t nq, uint32_t kbps) {
+    static_assert(DEPTH >= 3, "the digit ring needs two slots");
+    constexpr int L = DEPTH - 1;            // lead of every load, in steps
+    constexpr int P = DEPTH * L;            // period of both rings' slot pattern
     constexpr uint32_t Mp = 16 * MT;
     constexpr uint32_t AU = Mp * 4 / 256;  // 16-byte units of the step's digit tile per thread
     __shared__ i32x4 atile[2][4 * Mp];      // [buffer][block of the step][row]
@@ -279,7 +287,7 @@ __global__ __launch_bounds__(256, OCC) void k_dbpass_mfma_sa(const uint8_t* __re
     const i32x4* bsrc = reinterpret_cast<const i32x4*>(db) + (size_t)(kb0 + (l >> 4)) * 2 * C + col0 + (l & 15);
     const size_t bstep = (size_t)8 * C;
     // the step's digit tile is 4*Mp contiguous 16-byte units starting at block 4*ks
-    const i32x4* atile_src = reinterpret_cast<const i32x4*>(dig) + (size_t)kb0 * Mp;
+    const i32x4* atile_src = reinterpret_cast<const i32x4*>(dig) + (size_t)kb0 * Mp + tid;
     const i32x4 flip = {(int32_t)0x80808080, (int32_t)0x80808080, (int32_t)0x80808080, (int32_t)0x80808080};
     i32x4 acc[MT][NT][2];
 #pragma unroll
@@ -287,26 +295,25 @@ __global__ __launch_bounds__(256, OCC) void k_dbpass_mfma_sa(const uint8_t* __re
 #pragma unroll
         for (int n = 0; n < NT; n++) acc[a][n][0] = acc[a][n][1] = i32x4{0, 0, 0, 0};
     i32x4 v[DEPTH][NT][2];
-    i32x4 apre[AU];
-    auto load_b = [&](int slot, uint32_t ks) {
+    i32x4 ar[L][AU];
+    auto issue = [&](int bs, int as, uint32_t step) {
+        const uint32_t k = step < nks ? step : nks - 1;
+#pragma unroll
+        for (int u = 0; u < (int)AU; u++) ar[as][u] = atile_src[(size_t)k * 4 * Mp + 256 * u];
 #pragma unroll
         for (int n = 0; n < NT; n++) {
-            v[slot][n][0] = __builtin_nontemporal_load(bsrc + ks * bstep + n * 16);
-            v[slot][n][1] = __builtin_nontemporal_load(bsrc + ks * bstep + n * 16 + C);
+            v[bs][n][0] = __builtin_nontemporal_load(bsrc + k * bstep + n * 16);
+            v[bs][n][1] = __builtin_nontemporal_load(bsrc + k * bstep + n * 16 + C);
         }
     };
-    auto load_a = [&](uint32_t ks) {
+    auto store_a = [&](int buf, int as) {
 #pragma unroll
-        for (int u = 0; u < (int)AU; u++) apre[u] = atile_src[(size_t)ks * 4 * Mp + tid + 256 * u];
+        for (int u = 0; u < (int)AU; u++) atile[buf][tid + 256 * u] = ar[as][u];
     };
-    auto store_a = [&](int buf) {
-#pragma unroll
-        for (int u = 0; u < (int)AU; u++) atile[buf][tid + 256 * u] = apre[u];
-    };
-    auto compute = [&](int slot, int buf) {
+    auto compute = [&](int bs, int buf) {
 #pragma unroll
         for (int n = 0; n < NT; n++) {
-            const i32x4 Bl = v[slot][n][0] ^ flip, Bh = v[slot][n][1] ^ flip;
+            const i32x4 Bl = v[bs][n][0] ^ flip, Bh = v[bs][n][1] ^ flip;
 #pragma unroll
             for (int a = 0; a < MT; a++) {
                 const i32x4 A = atile[buf][(l >> 4) * Mp + a * 16 + (l & 15)];
@@ -315,29 +322,24 @@ __global__ __launch_bounds__(256, OCC) void k_dbpass_mfma_sa(const uint8_t* __re
             }
         }
     };
-    // Step s: compute from ring slot s%DEPTH and digit buffer s&1, then
-    // publish the digits of step s+1 in buffer (s+1)&1 (every wave finished
-    // step s-1, which read that buffer, before the previous barrier). The
-    // step count is uniform over the block, so every barrier is reached by
-    // all waves.
-    load_a(0);
-    store_a(0);
-    if (nks > 1) load_a(1);
+    // Step s: issue the loads of step s+L (B slot (s+L)%DEPTH held step s-1's
+    // operands, digit slot (s+L)%L held step s's digits, stored last step),
+    // compute from B slot s%DEPTH and LDS buffer s&1, then store step s+1's
+    // digits in buffer (s+1)&1 (every wave finished step s-1, its last
+    // reader, before the previous barrier). The step count is uniform over
+    // the block, so every barrier is reached by all waves.
 #pragma unroll
-    for (int j = 0; j < DEPTH - 1; j++)
-        if ((uint32_t)j < nks) load_b(j, (uint32_t)j);
+    for (int j = 0; j < L; j++) issue(j, j, (uint32_t)j);
+    store_a(0, 0);
     __syncthreads();
-    for (uint32_t ks = 0; ks < nks; ks += DEPTH) {
+    for (uint32_t ks = 0; ks < nks; ks += P) {
 #pragma unroll
-        for (int j = 0; j < DEPTH; j++) {
+        for (int j = 0; j < P; j++) {
             const uint32_t step = ks + (uint32_t)j;
             if (step >= nks) break;
-            if (step + DEPTH - 1 < nks) load_b((j + DEPTH - 1) % DEPTH, step + DEPTH - 1);
-            compute(j, (int)(step & 1));
-            if (step + 1 < nks) {
-                store_a((int)((step + 1) & 1));
-                if (step + 2 < nks) load_a(step + 2);
-            }
+            issue((j + L) % DEPTH, (j + L) % L, step + L);
+            compute(j % DEPTH, (int)(step & 1));
+            store_a((int)((step + 1) & 1), (j + 1) % L);
             __syncthreads();
         }
     }
@@ -345,8 +347,8 @@ __global__ __launch_bounds__(256, OCC) void k_dbpass_mfma_sa(const uint8_t* __re
     for (int a = 0; a < MT; a++)
 #pragma unroll
         for (int n = 0; n < NT; n++)
-            store_pair(acc[a][n][0], acc[a][n][1], (uint32_t)a * 4 + (l >> 4), nq, slice == 0, S, out, out_qstride,
-                       slice, OC, col0 + n * 16 + (l & 15));
+            store_pair(acc[a][n][0], acc[a][n][1], (uint32_t)a * 4 + (l >> 4), nq, slice == 0, S, out,
+                       out_qstride, slice, OC, col0 + n * 16 + (l & 15));
 }
 
 // Largest power-of-two slice count s with blocks*s <= target that keeps
@@ -382,8 +384,9 @@ DbPlan dbplan_mfma(size_t R, size_t ncols, size_t nq, size_t target_blocks, uint
     DbPlan p;
     p.kind = DbPlan::MFMA;
     p.param = nq <= 2 ? 1 : nq <= 4 ? 2 : nq <= 8 ? 4 : nq <= 16 ? 8 : 16;  // 2 queries per 16-row digit tile
-    p.depth = p.param == 16 ? depth : 3;
-    const size_t cols = 64 * mfma_nt(p.param);
+    // ring depths the shared-digit kernels are built with: 3..4 (8 tiles), 3..5 (16)
+    p.depth = p.param == 16 ? depth : p.param == 8 ? (depth < 4 ? depth : 4) : 3;
+    const size_t cols = dbplan_cols(p);
     // slices: whole number of 128-row (8-block) chunks so the step count is even
     p.slices = pick_slices(R / 16, ncols / cols, target_blocks, 8);
     // at most 2^16 rows per slice keeps |H_k|, |L_k| <= 2^30 (i32-exact) and
@@ -451,13 +454,18 @@ hipError_t launch_dbpass_mfma(const uint8_t* db, const int8_t* dig, const uint32
         case 1: MFMA(k_dbpass_mfma, 1, 4); break;
         case 2: MFMA(k_dbpass_mfma, 2, 4); break;
         case 4: MFMA(k_dbpass_mfma, 4, 4); break;
-        case 8: MFMA(k_dbpass_mfma_sa, 8, 2, 2, 3); break;
+        case 8:
+            switch (p.depth) {
+                case 3: MFMA(k_dbpass_mfma_sa, 8, 2, 2, 3); break;
+                case 4: MFMA(k_dbpass_mfma_sa, 8, 2, 2, 4); break;
+                default: return hipErrorInvalidValue;
+            }
+            break;
         case 16:
             switch (p.depth) {
                 case 3: MFMA(k_dbpass_mfma_sa, 16, 1, 2, 3); break;
                 case 4: MFMA(k_dbpass_mfma_sa, 16, 1, 2, 4); break;
                 case 5: MFMA(k_dbpass_mfma_sa, 16, 1, 2, 5); break;
-                case 6: MFMA(k_dbpass_mfma_sa, 16, 1, 2, 6); break;
                 default: return hipErrorInvalidValue;
             }
             break;

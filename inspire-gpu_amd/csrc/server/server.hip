@@ -360,6 +360,7 @@ void GpuServerCtx::autotune_dbpass() {
         const long v = std::atol(t);
         if (v > 0) mfma_target = (size_t)v;
     }
+    // INSPIRE_MFMA_DEPTH forces the shared-digit passes' ring depth
     bool depth_forced = false;
     if (const char* t = std::getenv("INSPIRE_MFMA_DEPTH")) {
         const long v = std::atol(t);
@@ -410,11 +411,13 @@ void GpuServerCtx::autotune_dbpass() {
     const size_t nq = cfg.max_batch < kMfmaMaxQueries ? cfg.max_batch : kMfmaMaxQueries;
     if (nq > 1) {
         best = 1e30f;
-        const bool ring = dbplan_mfma(R, C, nq, mfma_target).param == 16 && !depth_forced;
+        const bool ring = dbplan_mfma(R, C, nq, mfma_target).param >= 8 && !depth_forced;
         for (size_t t : kMfmaTargets)
             for (uint32_t d : kMfmaDepths) {
                 if (!ring && d != mfma_depth) continue;
-                const float ms = time_plan(dbplan_mfma(R, C, nq, t, d), nq);
+                const DbPlan x = dbplan_mfma(R, C, nq, t, d);
+                if (x.depth != d) continue;  // a depth this kernel is not built with
+                const float ms = time_plan(x, nq);
                 if (ms < best) { best = ms; mfma_target = t; mfma_depth = d; }
             }
     }

@@ -36,7 +36,7 @@ struct GpuServerCtx {
     ipx::DbPlan plan1{};            // database pass of a single query over all columns (autotuned)
     size_t plan1_target = 2048;     // its workgroup target
     size_t mfma_target = 1024;      // workgroups the batched pass aims for
-    uint32_t mfma_depth = 4;        // register-ring depth of the 17..32-query pass
+    uint32_t mfma_depth = 4;        // register-ring depth of the shared-digit passes
     size_t gchunk = 0;              // groups per chunk of the pipelined answer
     // pipeline shape of one-query and many-query calls: 0 serial on the
     // caller's stream, 2 Horner chains beside the passes, 3 collapse beside too

@@ -181,10 +181,11 @@ def test_db_pass_paths_match_oracle(medium8, path):
     assert srv.lib.ipir_dbg_db_pass(srv.h, ia.ptr(bad), 1, path, ia.ptr(np.zeros((1, 2, Cc), np.uint32))) == 1
 
 
-@pytest.mark.parametrize("depth", [3, 4, 5, 6])
+@pytest.mark.parametrize("depth", [3, 4, 5])
 def test_many_query_ring_depths_match_oracle(medium8, depth):
-    """Every register-ring depth of the 17..32-query MFMA pass against the
-    oracle's mat-vec (17 and 24 queries)."""
+    """Every register-ring depth of the shared-digit MFMA passes against the
+    oracle's mat-vec (12 queries: 8 digit tiles, depth capped at 4; 17 and
+    24 queries: 16 tiles)."""
     geo, db, srv = medium8
     R, Cc = geo["R"], geo["C"]
     enc = np.zeros((R, Cc), np.uint16)
@@ -195,9 +196,10 @@ def test_many_query_ring_depths_match_oracle(medium8, depth):
     assert srv.lib.ipir_dbg_dbpass_kernel(srv.h, MB8, name, 128) == 0
     tuned = int(name.value.decode().split(">")[0].split(",")[-1])  # k_dbpass_mfma_sa<16, 1, 2, depth>
     assert srv.lib.ipir_dbg_set_mfma_depth(srv.h, 2) == 1
+    assert srv.lib.ipir_dbg_set_mfma_depth(srv.h, 6) == 1
     assert srv.lib.ipir_dbg_set_mfma_depth(srv.h, depth) == 0
     try:
-        for nq in (17, MB8):
+        for nq in (12, 17, MB8):
             _ring_case(srv, flat, R, Cc, rng, nq, depth)
     finally:
         assert srv.lib.ipir_dbg_set_mfma_depth(srv.h, tuned) == 0
