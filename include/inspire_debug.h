@@ -60,8 +60,10 @@ int ipir_dbg_set_group_chunk(ipir_server* srv, size_t groups);
  * chains. */
 int ipir_dbg_set_horner(ipir_server* srv, int mode);
 /* Set the register-ring depth (3..5; 9..16-query plans cap it at 4) of the
- * shared-digit database passes a server uses; 1 for other values. */
-int ipir_dbg_set_mfma_depth(ipir_server* srv, uint32_t depth);
+ * batched database passes a server uses, and for 17..32 queries the form
+ * (dbtile = 1: the waves share the DB tile, depth 3..4; 0: the digit
+ * tile); 1 for other values. */
+int ipir_dbg_set_mfma_depth(ipir_server* srv, uint32_t depth, int dbtile);
 /* Set the pipeline shape of one-query and many-query answers: 0 serial, 2
  * Horner chains on a second stream per group chunk, 3 the collapse on a
  * third stream as well; 1 for other values. */

@@ -350,12 +350,13 @@ int ipir_dbg_set_group_chunk(ipir_server* srv, size_t groups) {
     return 0;
 }
 
-int ipir_dbg_set_mfma_depth(ipir_server* srv, uint32_t depth) {
-    if (!srv) return 1;
+int ipir_dbg_set_mfma_depth(ipir_server* srv, uint32_t depth, int dbtile) {
+    if (!srv || dbtile < 0 || dbtile > 1 || (dbtile && depth > 4)) return 1;
     for (uint32_t d : ipx::kMfmaDepths)
         if (d == depth) {
             std::lock_guard<std::mutex> lk(srv->ctx->mu);
             srv->ctx->mfma_depth = depth;
+            srv->ctx->mfma_dbtile = dbtile != 0;
             return 0;
         }
     return 1;

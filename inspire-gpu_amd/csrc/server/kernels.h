@@ -38,9 +38,10 @@ struct DbPlan {
     uint32_t param;      // F64: 16-row blocks per iteration (2, 4, 8); MFMA: 16-row digit tiles (1, 2, 4, 8, 16)
     uint32_t slices;     // row slices (power of two)
     uint32_t depth = 3;  // 8 or 16 digit tiles: register-ring slots of DB operands (kMfmaDepths)
+    bool dbtile = false; // 16 digit tiles: the waves share the DB tile (k_dbpass_mfma_bs), not the digit tile
 };
-// Ring depths of the shared-digit (9..32 query) MFMA passes; 8-tile plans
-// cap the depth at 4.
+// Ring depths of the 9..32-query MFMA passes; 8-tile and shared-DB-tile
+// plans cap the depth at 4.
 constexpr uint32_t kMfmaDepths[] = {3, 4, 5};
 // Row-major staging rows (16*nkb of them) -> blocks [kb0, kb0 + nkb) of the planes.
 hipError_t launch_to_planes(const uint16_t* src, uint8_t* dst, size_t C, size_t kb0, size_t nkb, hipStream_t s);
@@ -49,7 +50,8 @@ DbPlan dbplan_f64(size_t R, size_t ncols, int unroll, size_t target_blocks);
 // Queries one batched (int8 MFMA) database pass serves.
 constexpr size_t kMfmaMaxQueries = 32;
 // int8-MFMA plan for nq (1..kMfmaMaxQueries) queries.
-DbPlan dbplan_mfma(size_t R, size_t ncols, size_t nq, size_t target_blocks, uint32_t depth = 4);
+DbPlan dbplan_mfma(size_t R, size_t ncols, size_t nq, size_t target_blocks, uint32_t depth = 4,
+                   bool dbtile = false);
 // Digit rows the MFMA plan stages per 16-row block (int8 scratch: R*Mp bytes, Mp <= 256).
 size_t dbplan_mp(const DbPlan& p);
 // Column granularity of a plan (one block's columns): col0 and ncols are multiples of it.

@@ -351,6 +351,105 @@ __global__ __launch_bounds__(256, OCC) void k_dbpass_mfma_sa(const uint8_t* __re
                        out_qstride, slice, OC, col0 + n * 16 + (l & 15));
 }
 
+// 17..32 queries with the DB tile shared instead: each of the block's 4
+// waves owns MT/4 digit tiles (8 queries) for all of the block's NT*16
+// columns, reads its digit operands straight from memory (they are
+// L2-resident: every block reads the same R*Mp bytes), and takes its DB
+// operands from a per-step tile the 4 waves load once into an LDS double
+// buffer (flipped to signed bytes on the way in). Per MFMA this moves half
+// the LDS bytes of the shared-digit form, whose every wave reads the whole
+// Mp-row digit tile for only 16 columns. Loads follow the same one-lead
+// rule: the digits of step s+L go to a register ring of DEPTH slots, the
+// thread's share of the DB tile of step s+L to a ring of L slots (stored to
+// LDS one step ahead).
+template <int MT, int NT, int DEPTH>
+__global__ __launch_bounds__(256, 2) void k_dbpass_mfma_bs(const uint8_t* __restrict__ db,
+                                                           const int8_t* __restrict__ dig,
+                                                           const uint32_t* __restrict__ S,
+                                                           uint32_t* __restrict__ out, size_t out_qstride,
+                                                           uint32_t C, uint32_t OC, uint32_t nq, uint32_t kbps) {
+    static_assert(DEPTH >= 3 && MT % 4 == 0, "shape");
+    constexpr int L = DEPTH - 1;
+    constexpr int P = DEPTH * L;
+    constexpr int MW = MT / 4;                  // digit tiles per wave
+    constexpr uint32_t Mp = 16 * MT;
+    constexpr uint32_t TC = 16 * NT;            // columns per block
+    constexpr uint32_t BU = 4 * 2 * TC / 256;   // 16-byte units of the step's DB tile per thread
+    __shared__ i32x4 btile[2][4][2][TC];        // [buffer][block of the step][plane][column]
+    const uint32_t tid = threadIdx.x, w = tid >> 6, l = tid & 63;
+    const uint32_t slice = blockIdx.y;
+    const uint32_t bcol0 = blockIdx.x * TC;
+    const uint32_t kb0 = slice * kbps;
+    const uint32_t nks = kbps / 4;
+    // this thread's DB units: unit u = tid + 256*i -> (block u/(2*TC), plane (u/TC)%2, column u%TC)
+    const i32x4* bsrc = reinterpret_cast<const i32x4*>(db) + (size_t)kb0 * 2 * C + bcol0;
+    // digit operand of lane l for tile a: row 16*(MW*w + a) + l%16 of block 4*ks + l/16
+    const i32x4* asrc = reinterpret_cast<const i32x4*>(dig) + ((size_t)kb0 + (l >> 4)) * Mp + 16 * MW * w + (l & 15);
+    const i32x4 flip = {(int32_t)0x80808080, (int32_t)0x80808080, (int32_t)0x80808080, (int32_t)0x80808080};
+    i32x4 acc[MW][NT][2];
+#pragma unroll
+    for (int a = 0; a < MW; a++)
+#pragma unroll
+        for (int n = 0; n < NT; n++) acc[a][n][0] = acc[a][n][1] = i32x4{0, 0, 0, 0};
+    i32x4 av[DEPTH][MW];
+    i32x4 br[L][BU];
+    auto issue = [&](int as, int bs, uint32_t step) {
+        const uint32_t k = step < nks ? step : nks - 1;
+#pragma unroll
+        for (int a = 0; a < MW; a++) av[as][a] = asrc[(size_t)k * 4 * Mp + 16 * a];
+#pragma unroll
+        for (int i = 0; i < (int)BU; i++) {
+            const uint32_t u = tid + 256 * i, kb = u / (2 * TC), pl = (u / TC) % 2, c = u % TC;
+            br[bs][i] = __builtin_nontemporal_load(bsrc + ((size_t)(4 * k + kb) * 2 + pl) * C + c);
+        }
+    };
+    auto store_b = [&](int buf, int bs) {
+#pragma unroll
+        for (int i = 0; i < (int)BU; i++) {
+            const uint32_t u = tid + 256 * i, kb = u / (2 * TC), pl = (u / TC) % 2, c = u % TC;
+            btile[buf][kb][pl][c] = br[bs][i] ^ flip;
+        }
+    };
+    auto compute = [&](int as, int buf) {
+#pragma unroll
+        for (int n = 0; n < NT; n++)
+#pragma unroll
+            for (int pl = 0; pl < 2; pl++) {
+                const i32x4 B = btile[buf][l >> 4][pl][n * 16 + (l & 15)];
+#pragma unroll
+                for (int a = 0; a < MW; a++)
+                    acc[a][n][pl] = __builtin_amdgcn_mfma_i32_16x16x64_i8(av[as][a], B, acc[a][n][pl], 0, 0, 0);
+            }
+    };
+    // Step s: issue step s+L's loads (digit slot (s+L)%DEPTH held step s-1's
+    // operands; DB slot (s+L)%L held step s's share, stored last step),
+    // compute from digit slot s%DEPTH and LDS buffer s&1, then store step
+    // s+1's share in buffer (s+1)&1 (its last reader, step s-1, finished
+    // before the previous barrier). The step count is uniform over the
+    // block, so every barrier is reached by all waves.
+#pragma unroll
+    for (int j = 0; j < L; j++) issue(j, j, (uint32_t)j);
+    store_b(0, 0);
+    __syncthreads();
+    for (uint32_t ks = 0; ks < nks; ks += P) {
+#pragma unroll
+        for (int j = 0; j < P; j++) {
+            const uint32_t step = ks + (uint32_t)j;
+            if (step >= nks) break;
+            issue((j + L) % DEPTH, (j + L) % L, step + L);
+            compute(j % DEPTH, (int)(step & 1));
+            store_b((int)((step + 1) & 1), (j + 1) % L);
+            __syncthreads();
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < MW; a++)
+#pragma unroll
+        for (int n = 0; n < NT; n++)
+            store_pair(acc[a][n][0], acc[a][n][1], (uint32_t)(MW * w + a) * 4 + (l >> 4), nq, slice == 0, S, out,
+                       out_qstride, slice, OC, bcol0 + n * 16 + (l & 15));
+}
+
 // Largest power-of-two slice count s with blocks*s <= target that keeps
 // every slice a whole number of `unit`-block chunks.
 uint32_t pick_slices(size_t nkb, size_t blocks, size_t target, size_t unit) {
@@ -380,12 +479,13 @@ DbPlan dbplan_f64(size_t R, size_t ncols, int unroll, size_t target_blocks) {
     return p;
 }
 
-DbPlan dbplan_mfma(size_t R, size_t ncols, size_t nq, size_t target_blocks, uint32_t depth) {
+DbPlan dbplan_mfma(size_t R, size_t ncols, size_t nq, size_t target_blocks, uint32_t depth, bool dbtile) {
     DbPlan p;
     p.kind = DbPlan::MFMA;
     p.param = nq <= 2 ? 1 : nq <= 4 ? 2 : nq <= 8 ? 4 : nq <= 16 ? 8 : 16;  // 2 queries per 16-row digit tile
-    // ring depths the shared-digit kernels are built with: 3..4 (8 tiles), 3..5 (16)
-    p.depth = p.param == 16 ? depth : p.param == 8 ? (depth < 4 ? depth : 4) : 3;
+    p.dbtile = dbtile && p.param == 16;
+    // ring depths the kernels are built with: 3..5 (16 tiles, shared digits), else 3..4
+    p.depth = p.param == 16 && !p.dbtile ? depth : p.param >= 8 ? (depth < 4 ? depth : 4) : 3;
     const size_t cols = dbplan_cols(p);
     // slices: whole number of 128-row (8-block) chunks so the step count is even
     p.slices = pick_slices(R / 16, ncols / cols, target_blocks, 8);
@@ -403,6 +503,7 @@ std::string dbplan_kernel(const DbPlan& p) {
     if (p.kind == DbPlan::F64) return "k_dbpass_f64<" + std::to_string(p.param) + ">";
     const std::string mt_nt = std::to_string(p.param) + ", " + std::to_string(mfma_nt(p.param));
     if (p.param < 8) return "k_dbpass_mfma<" + mt_nt + ">";
+    if (p.dbtile) return "k_dbpass_mfma_bs<16, 4, " + std::to_string(p.depth) + ">";
     return "k_dbpass_mfma_sa<" + mt_nt + ", 2, " + std::to_string(p.depth) + ">";
 }
 
@@ -462,6 +563,14 @@ hipError_t launch_dbpass_mfma(const uint8_t* db, const int8_t* dig, const uint32
             }
             break;
         case 16:
+            if (p.dbtile) {
+                switch (p.depth) {
+                    case 3: MFMA(k_dbpass_mfma_bs, 16, 4, 3); break;
+                    case 4: MFMA(k_dbpass_mfma_bs, 16, 4, 4); break;
+                    default: return hipErrorInvalidValue;
+                }
+                break;
+            }
             switch (p.depth) {
                 case 3: MFMA(k_dbpass_mfma_sa, 16, 1, 2, 3); break;
                 case 4: MFMA(k_dbpass_mfma_sa, 16, 1, 2, 4); break;

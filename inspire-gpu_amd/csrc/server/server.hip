@@ -260,7 +260,7 @@ DbPlan narrow_plan(const DbPlan& full, size_t target, size_t R, size_t C, size_t
     const size_t wgs = C / dbplan_cols(full) * full.slices;
     const size_t t = wgs > target ? wgs : target;
     return full.kind == DbPlan::F64 ? dbplan_f64(R, ncols, (int)full.param, t)
-                                     : dbplan_mfma(R, ncols, nq, t, full.depth);
+                                     : dbplan_mfma(R, ncols, nq, t, full.depth, full.dbtile);
 }
 }  // namespace
 
@@ -360,12 +360,14 @@ void GpuServerCtx::autotune_dbpass() {
         const long v = std::atol(t);
         if (v > 0) mfma_target = (size_t)v;
     }
-    // INSPIRE_MFMA_DEPTH forces the shared-digit passes' ring depth
+    // INSPIRE_MFMA_DEPTH=<depth>[,dbtile] forces the batched passes' ring
+    // depth and (",dbtile") the shared-DB-tile form for 17..32 queries
     bool depth_forced = false;
     if (const char* t = std::getenv("INSPIRE_MFMA_DEPTH")) {
         const long v = std::atol(t);
         for (uint32_t d : kMfmaDepths)
             if (v == (long)d) { mfma_depth = d; depth_forced = true; }
+        mfma_dbtile = depth_forced && std::strstr(t, "dbtile") != nullptr;
     }
     if (const char* force = std::getenv("INSPIRE_DBPASS")) {
         int u = 4, t = 2048;
@@ -411,14 +413,17 @@ void GpuServerCtx::autotune_dbpass() {
     const size_t nq = cfg.max_batch < kMfmaMaxQueries ? cfg.max_batch : kMfmaMaxQueries;
     if (nq > 1) {
         best = 1e30f;
-        const bool ring = dbplan_mfma(R, C, nq, mfma_target).param >= 8 && !depth_forced;
+        const uint32_t param = dbplan_mfma(R, C, nq, mfma_target).param;
         for (size_t t : kMfmaTargets)
-            for (uint32_t d : kMfmaDepths) {
-                if (!ring && d != mfma_depth) continue;
-                const DbPlan x = dbplan_mfma(R, C, nq, t, d);
-                if (x.depth != d) continue;  // a depth this kernel is not built with
-                const float ms = time_plan(x, nq);
-                if (ms < best) { best = ms; mfma_target = t; mfma_depth = d; }
+            for (bool dbt : {false, true}) {
+                if (depth_forced ? dbt != mfma_dbtile : (dbt && param != 16)) continue;
+                for (uint32_t d : kMfmaDepths) {
+                    if ((depth_forced || param < 8) && d != mfma_depth) continue;
+                    const DbPlan x = dbplan_mfma(R, C, nq, t, d, dbt);
+                    if (x.depth != d && param >= 8) continue;  // a depth this kernel is not built with
+                    const float ms = time_plan(x, nq);
+                    if (ms < best) { best = ms; mfma_target = t; mfma_depth = d; mfma_dbtile = dbt; }
+                }
             }
     }
     (void)hipEventDestroy(e0);
@@ -436,7 +441,7 @@ DbPlan GpuServerCtx::plan_for(size_t nq, size_t ncols) const {
     if (nq == 1)
         return plan1.kind == DbPlan::F64 ? dbplan_f64(R, ncols, (int)plan1.param, plan1_target)
                                          : dbplan_mfma(R, ncols, 1, plan1_target);
-    return dbplan_mfma(R, ncols, nq, mfma_target, mfma_depth);
+    return dbplan_mfma(R, ncols, nq, mfma_target, mfma_depth, mfma_dbtile);
 }
 
 DbPlan GpuServerCtx::chunk_plan(size_t nq, size_t ncols) const {
@@ -479,7 +484,7 @@ void GpuServerCtx::stage_dbpass(size_t count, hipStream_t s, size_t slot0, int l
         const size_t nq = slot0 + count - q0 < step ? slot0 + count - q0 : step;
         DbPlan p = plan_for(nq, C);
         if (force == 0 && p.kind != DbPlan::F64) p = dbplan_f64(R, C, 4, 2048);
-        if (force == 1 && p.kind == DbPlan::F64) p = dbplan_mfma(R, C, nq, mfma_target, mfma_depth);
+        if (force == 1 && p.kind == DbPlan::F64) p = dbplan_mfma(R, C, nq, mfma_target, mfma_depth, mfma_dbtile);
         dbpass_prep(q0, nq, p, s, lane);
         dbpass_range(q0, nq, 0, G, p, s, lane);
     }

@@ -36,7 +36,8 @@ struct GpuServerCtx {
     ipx::DbPlan plan1{};            // database pass of a single query over all columns (autotuned)
     size_t plan1_target = 2048;     // its workgroup target
     size_t mfma_target = 1024;      // workgroups the batched pass aims for
-    uint32_t mfma_depth = 4;        // register-ring depth of the shared-digit passes
+    uint32_t mfma_depth = 4;        // register-ring depth of the 9..32-query passes
+    bool mfma_dbtile = false;       // 17..32 queries: waves share the DB tile instead of the digit tile
     size_t gchunk = 0;              // groups per chunk of the pipelined answer
     // pipeline shape of one-query and many-query calls: 0 serial on the
     // caller's stream, 2 Horner chains beside the passes, 3 collapse beside too
