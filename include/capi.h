@@ -111,6 +111,20 @@ ipir_client_query* ipir_query_build_seeded(const ipir_params* params, uint64_t i
 int ipir_answer_batch_device(ipir_server* srv, const uint64_t* d_queries, size_t count,
                              uint64_t* d_out_resp, void* hip_stream);
 
+/* Answer `count` queries given in the packed wire form (ipir_query_pack:
+ * count * ipir_query_packed_bytes bytes back to back) and return the
+ * responses compressed (ipir_response_compress: count * num_cts * 12288
+ * bytes, query-major), with the unpacking and the compression run on the
+ * device: the host-device copies carry the packed sizes (the reference
+ * client's codecs, src/capi_client.cpp:246-385, run on the host only).
+ * Blocking; return codes as ipir_answer_batch. */
+int ipir_answer_batch_packed(ipir_server* srv, const uint8_t* packed_queries, size_t count,
+                             uint8_t* out_compressed);
+/* The same over DEVICE buffers, asynchronous on the given HIP stream (NULL
+ * = the server's own stream), like ipir_answer_batch_device. */
+int ipir_answer_batch_packed_device(ipir_server* srv, const uint8_t* d_packed_queries, size_t count,
+                                    uint8_t* d_out_compressed, void* hip_stream);
+
 /* Continuous-batching serving loop over one server. Requests are admitted
  * into free slots as they arrive, batches of different requests overlap on
  * the device, and each request completes independently.

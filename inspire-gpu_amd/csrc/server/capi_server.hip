@@ -108,6 +108,34 @@ int ipir_answer_batch_device(ipir_server* srv, const uint64_t* d_queries, size_t
     }
 }
 
+int ipir_answer_batch_packed(ipir_server* srv, const uint8_t* packed_queries, size_t count,
+                             uint8_t* out_compressed) {
+    if (!srv || !packed_queries || !out_compressed) return 1;
+    if (count == 0) return 0;
+    if (count > srv->ctx->cfg.max_batch) return 2;
+    try {
+        std::lock_guard<std::mutex> lk(srv->ctx->mu);
+        srv->ctx->answer_packed_host(packed_queries, count, out_compressed);
+        return 0;
+    } catch (...) {
+        return 4;
+    }
+}
+
+int ipir_answer_batch_packed_device(ipir_server* srv, const uint8_t* d_packed_queries, size_t count,
+                                    uint8_t* d_out_compressed, void* hip_stream) {
+    if (!srv || !d_packed_queries || !d_out_compressed) return 1;
+    if (count == 0) return 0;
+    if (count > srv->ctx->cfg.max_batch) return 2;
+    try {
+        hipStream_t s = hip_stream ? static_cast<hipStream_t>(hip_stream) : srv->ctx->stream;
+        srv->ctx->enqueue_packed(d_packed_queries, count, d_out_compressed, s);
+        return 0;
+    } catch (...) {
+        return 4;
+    }
+}
+
 void ipir_server_destroy(ipir_server* srv) {
     if (!srv) return;
     gpu_free_server(srv->ctx);

@@ -29,6 +29,18 @@ hipError_t launch_unmarshal(const uint64_t* d_q, size_t count, size_t R, double*
 // In-place forward NTT of `npoly` RNS polynomials laid out [npoly][2][N].
 hipError_t launch_ntt_rns(uint32_t* polys, size_t npoly, const NttConst& nc, bool inverse, hipStream_t s);
 
+// Wire codecs on the device (the client library's byte formats).
+// Bytes of one query in the 53-bit CRT-packed form.
+size_t query_packed_bytes(size_t R);
+// count packed queries (query_packed_bytes(R) bytes each, back to back) ->
+// flat u64 queries (qlen u64s apart).
+hipError_t launch_query_unpack(const uint8_t* packed, size_t count, size_t R, uint64_t* flat, size_t qlen,
+                               hipStream_t s);
+// nct response cts (flat u64, NTT form, [ct][a|b][limb][N]) -> 12288
+// compressed bytes each; scratch holds nct*4*N words.
+hipError_t launch_compress(const uint64_t* resp, size_t nct, uint32_t* scratch, uint8_t* out, const NttConst& nc,
+                           const CrtConst& cc, hipStream_t s);
+
 // Database pass over the resident DB, byte planes in 16-row blocks: the
 // low (p = 0) and high (p = 1) byte of (r, c) at ((r/16)*2 + p)*C*16 +
 // c*16 + r%16. out + q*out_qstride + slice*2C holds [limb][C] = sum over the

@@ -340,6 +340,11 @@ void GpuServerCtx::alloc_slots() {
             pipe_batched = a2;
         }
     }
+    // packed-wire path: packed queries, coefficient-form scratch, compressed responses
+    qpack_bytes = query_packed_bytes(R);
+    d_qpack = dalloc<uint8_t>(B * qpack_bytes);
+    d_cmp = dalloc<uint32_t>(B * T * 4 * N);
+    d_rcmp = dalloc<uint8_t>(B * T * kCompressedCtBytes);
     HIPX(hipMemset(d_bq, 0, B * 2 * R * sizeof(double)));
     HIPX(hipHostMalloc((void**)&h_q, B * qlen * 8, hipHostMallocDefault));
     HIPX(hipHostMalloc((void**)&h_resp, B * resp_u64s * 8, hipHostMallocDefault));
@@ -347,7 +352,8 @@ void GpuServerCtx::alloc_slots() {
                      B * (qlen * 8 + resp_u64s * 8 + 2 * R * 8 +
                           (keys_stride + rgsw_stride + part_stride + 2 * C + part2_stride + packed_stride) * 4) +
                      kLanes * (R * 256 + kKdigBytes + 128 * 4) +
-                     (horner_pair_xbuf_words(B * T) + 2 * B * T + B * T * 4 * N) * 4;
+                     (horner_pair_xbuf_words(B * T) + 2 * B * T + B * T * 4 * N) * 4 +
+                     B * (qpack_bytes + T * 4 * N * 4 + T * kCompressedCtBytes);
     autotune_dbpass();
 }
 
@@ -597,6 +603,28 @@ void GpuServerCtx::enqueue(const uint64_t* d_in, size_t count, uint64_t* d_out, 
     }
 }
 
+void GpuServerCtx::enqueue_packed(const uint8_t* d_in, size_t count, uint8_t* d_out, hipStream_t s, size_t slot0,
+                                  int lane) {
+    uint64_t* dq = d_q + slot0 * qlen;
+    uint64_t* dr = d_resp + slot0 * resp_u64s;
+    HIPX(launch_query_unpack(d_in, count, pp.db_rows, dq, qlen, s));
+    enqueue(dq, count, dr, s, slot0, lane);
+    HIPX(launch_compress(dr, count * T, d_cmp + slot0 * T * 4 * N, d_out, nc, cc, s));
+}
+
+void GpuServerCtx::answer_packed_host(const uint8_t* in, size_t count, uint8_t* out) {
+    hipStream_t s = stream;
+    uint8_t* hq = reinterpret_cast<uint8_t*>(h_q);     // packed queries fit the flat staging
+    uint8_t* hr = reinterpret_cast<uint8_t*>(h_resp);  // so do compressed responses
+    std::memcpy(hq, in, count * qpack_bytes);
+    HIPX(hipMemcpyAsync(d_qpack, hq, count * qpack_bytes, hipMemcpyHostToDevice, s));
+    enqueue_packed(d_qpack, count, d_rcmp, s);
+    HIPX(hipMemcpyAsync(hr, d_rcmp, count * T * kCompressedCtBytes, hipMemcpyDeviceToHost, s));
+    HIPX(hipStreamSynchronize(s));
+    if (horner_mode != 0) check_status(s);
+    std::memcpy(out, hr, count * T * kCompressedCtBytes);
+}
+
 void GpuServerCtx::check_status(hipStream_t s) {
     uint32_t st = 0;
     HIPX(hipMemcpyAsync(&st, d_status, 4, hipMemcpyDeviceToHost, s));
@@ -690,6 +718,7 @@ void gpu_free_server(GpuServerCtx* ctx) {
     dfree(ctx->d_q); dfree(ctx->d_resp); dfree(ctx->d_bq); dfree(ctx->d_keys); dfree(ctx->d_rgsw);
     dfree(ctx->d_part); dfree(ctx->d_interm); dfree(ctx->d_part2); dfree(ctx->d_packed_b);
     dfree(ctx->d_hxbuf); dfree(ctx->d_hflags); dfree(ctx->d_status); dfree(ctx->d_hstate);
+    dfree(ctx->d_qpack); dfree(ctx->d_cmp); dfree(ctx->d_rcmp);
     for (int l = 0; l < GpuServerCtx::kLanes; l++) {
         if (ctx->hstream[l]) (void)hipStreamSynchronize(ctx->hstream[l]);
         if (ctx->cstream[l]) (void)hipStreamSynchronize(ctx->cstream[l]);

@@ -81,6 +81,13 @@ struct GpuServerCtx {
     uint64_t* h_q = nullptr;        // pinned host staging
     uint64_t* h_resp = nullptr;
 
+    // packed-wire path (53-bit CRT-packed queries in, compressed responses out)
+    static constexpr size_t kCompressedCtBytes = ipx::N * (28 + 20) / 8;  // 12288
+    size_t qpack_bytes = 0;
+    uint8_t* d_qpack = nullptr;     // packed query staging [B][qpack_bytes]
+    uint32_t* d_cmp = nullptr;      // coefficient-form responses [B*T][4][N]
+    uint8_t* d_rcmp = nullptr;      // compressed response staging [B*T][12288]
+
     size_t resident_bytes = 0;
     std::mutex mu;  // serialises host-staged answers on this context
 
@@ -118,6 +125,12 @@ struct GpuServerCtx {
     // Host-staged round trip through the pinned staging of slots
     // [slot0, slot0+count): h_q -> device -> h_resp, then copy to out.
     void answer_host(size_t count, uint64_t* out, hipStream_t s = nullptr, size_t slot0 = 0);
+    // enqueue() behind the device wire codecs: count packed queries at d_in
+    // (device, qpack_bytes each) -> count*T compressed cts at d_out (device).
+    void enqueue_packed(const uint8_t* d_in, size_t count, uint8_t* d_out, hipStream_t s, size_t slot0 = 0,
+                        int lane = 0);
+    // Host round trip of the packed forms through the pinned staging.
+    void answer_packed_host(const uint8_t* in, size_t count, uint8_t* out);
 };
 
 }  // namespace inspire

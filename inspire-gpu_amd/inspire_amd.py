@@ -71,6 +71,8 @@ SERVER_FUNCS = [
     ("ipir_answer_batch", C.c_int, [VP, P(ipir_query_view), SZ, VP]),
     ("ipir_server_destroy", None, [VP]),
     ("ipir_answer_batch_device", C.c_int, [VP, VP, SZ, VP, VP]),
+    ("ipir_answer_batch_packed", C.c_int, [VP, VP, SZ, VP]),
+    ("ipir_answer_batch_packed_device", C.c_int, [VP, VP, SZ, VP, VP]),
     ("ipir_loop_start", VP, [VP, SZ]),
     ("ipir_loop_submit", U64, [VP, P(ipir_query_view), VP]),
     ("ipir_loop_wait", C.c_int, [VP, U64, P(C.c_double)]),
@@ -198,6 +200,25 @@ class Client:
             raise ValueError("ipir_query_build failed")
         return h, q
 
+    def pack_query(self, q):
+        out = np.zeros(self.lib.ipir_query_packed_bytes(self.p.ref()), np.uint8)
+        if self.lib.ipir_query_pack(self.p.ref(), ptr(np.ascontiguousarray(q, np.uint64)), ptr(out)):
+            raise ValueError("ipir_query_pack failed")
+        return out
+
+    def compress(self, resp):
+        out = np.zeros(self.lib.ipir_response_compressed_bytes(self.p.ref()), np.uint8)
+        if self.lib.ipir_response_compress(self.p.ref(), ptr(np.ascontiguousarray(resp, np.uint64)), ptr(out)):
+            raise ValueError("ipir_response_compress failed")
+        return out
+
+    def extract_compressed(self, h, buf):
+        out = np.zeros(self.entry_slots(), np.uint16)
+        rc = self.lib.ipir_extract_compressed(self.p.ref(), h, ptr(np.ascontiguousarray(buf, np.uint8)), ptr(out))
+        if rc:
+            raise ValueError(f"ipir_extract_compressed rc={rc}")
+        return out
+
     def extract(self, h, resp):
         out = np.zeros(self.entry_slots(), np.uint16)
         rc = self.lib.ipir_extract(self.p.ref(), h, ptr(np.ascontiguousarray(resp, np.uint64)), ptr(out))
@@ -269,6 +290,17 @@ class Server:
         rc = self.lib.ipir_answer_batch(self.h, self.views(queries), len(queries), ptr(out))
         if rc:
             raise RuntimeError(f"ipir_answer_batch rc={rc}")
+        return out
+
+    def answer_packed(self, packed):
+        """Packed wire queries (ipir_query_pack bytes, one row per query) ->
+        compressed responses (num_cts * 12288 bytes per query)."""
+        packed = np.ascontiguousarray(packed, np.uint8)
+        T = self.params.num_cts
+        out = np.zeros((packed.shape[0], T * 12288), np.uint8)
+        rc = self.lib.ipir_answer_batch_packed(self.h, ptr(packed), packed.shape[0], ptr(out))
+        if rc:
+            raise RuntimeError(f"ipir_answer_batch_packed rc={rc}")
         return out
 
     def close(self):

@@ -285,6 +285,43 @@ def test_pipelined_group_chunks_equal_unchunked(medium8, groups, shape):
     assert np.array_equal(got, base) and np.array_equal(got1[0], one[0]) and np.array_equal(pair, base)
 
 
+def test_packed_wire_answers_match_host_codecs(medium8):
+    """Queries in the 53-bit packed wire form, unpacked on the device, and
+    responses compressed on the device give the bytes of the client
+    library's host codecs (pack, flat answer, compress), extract to their
+    entries, and the device-buffer entry point agrees."""
+    geo, db, srv = medium8
+    cl = ia.Client(srv.params, srv.lib)
+    idxs = [0, 77777, 524287, 4242]
+    hs, qs = [], []
+    for k, idx in enumerate(idxs):
+        h, q = cl.build(idx, 900 + k)
+        hs.append(h)
+        qs.append(q)
+    packed = np.stack([cl.pack_query(q) for q in qs])
+    assert packed.shape[1] == srv.lib.ipir_query_packed_bytes(srv.params.ref())
+    got = srv.answer_packed(packed)
+    flat = srv.answer_batch(qs)
+    for k, idx in enumerate(idxs):
+        assert np.array_equal(got[k], cl.compress(flat[k])), f"query {k}"
+        assert np.array_equal(cl.extract_compressed(hs[k], got[k]), ol.expected_slots(geo, db, idx))
+    assert np.array_equal(srv.answer_packed(packed[2:3])[0], got[2])  # alone equals batched
+    dq = ia.DeviceBuffer(packed.nbytes, srv.lib)
+    dq.upload(packed)
+    dout = ia.DeviceBuffer(got.nbytes, srv.lib)
+    assert srv.lib.ipir_answer_batch_packed_device(srv.h, C.c_void_p(dq.p), len(idxs), C.c_void_p(dout.p),
+                                                   None) == 0
+    srv.lib.ipir_dev_sync()
+    assert np.array_equal(dout.download(np.zeros_like(got)), got)
+    out = np.zeros((MB8 + 1, 12288), np.uint8)
+    big = np.zeros((MB8 + 1, packed.shape[1]), np.uint8)
+    assert srv.lib.ipir_answer_batch_packed(srv.h, ia.ptr(big), MB8 + 1, ia.ptr(out)) == 2
+    assert srv.lib.ipir_answer_batch_packed(srv.h, None, 1, ia.ptr(out)) == 1
+    assert srv.lib.ipir_answer_batch_packed(srv.h, ia.ptr(big), 0, ia.ptr(out)) == 0
+    for h in hs:
+        cl.destroy(h)
+
+
 def test_batch_validation_codes(medium8):
     geo, db, srv = medium8
     lib = srv.lib
@@ -390,5 +427,12 @@ def test_two_ciphertext_responses_extract():
         assert srv.lib.ipir_dbg_set_horner(srv.h, 0) == 0
         one = srv.answer_batch(qs)
         assert np.array_equal(pair, batch) and np.array_equal(one, batch)
+        # the packed wire path: both cts compressed on the device, query-major
+        cl = ia.Client(srv.params, srv.lib)
+        packed = np.stack([cl.pack_query(q) for q in qs])
+        comp = srv.answer_packed(packed)
+        assert comp.shape == (3, 2 * 12288)
+        for k in range(3):
+            assert np.array_equal(comp[k], cl.compress(batch[k])), f"compressed response {k}"
     finally:
         srv.close()
