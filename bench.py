@@ -30,6 +30,8 @@ CONFIGS = {
     "c1": (2**20, 60, 32768, "plain small, 2^20 x 60 B (0.13 GB DB)"),
     "c2": (2**23, 120, 32768, "1 GB DB, 2^23 x 120 B"),
     "c3": (2**27, 120, 32768, "16 GB DB, 2^27 x 120 B"),
+    "c4": (2**29, 120, 32768, "64 GB DB, 2^29 x 120 B"),
+    "c5": (2**25, 3840, 32768, "128 GB DB, 2^25 x 3840 B (wide records)"),
 }
 HBM_PEAK_GBS = 8000.0
 
@@ -50,23 +52,23 @@ def gen_db(R, Cc, seed=0):
     return db
 
 
-def cpu_baseline(geo, n_groups_sample=8, col_frac=4, horner_steps=16):
+def cpu_baseline(geo, n_groups_sample=8, sample_cols=65536, horner_steps=16):
     """Time the oracle (scalar, one core) on a bounded sample of one query and
-    scale to a whole query: mat-vec over 1/col_frac of the columns, the
+    scale to a whole query: mat-vec over sample_cols columns (all rows), the
     collapse of n_groups_sample groups, horner_steps external products."""
     import oracle_lib as ol
     o = ol.oracle()
     N = ol.N
     R, Cc, G, Da = geo["R"], geo["C"], geo["G"], geo["Da"]
     rng = np.random.default_rng(5)
-    cw = Cc // col_frac
+    cw = min(Cc, sample_cols)
     dbs = rng.integers(0, 65535, size=(R, cw), dtype=np.uint16)
     b0 = rng.integers(0, ol.Q0, R, dtype=np.uint64)
     b1 = rng.integers(0, ol.Q1, R, dtype=np.uint64)
     out = np.zeros(2 * cw, np.uint64)
     t = time.perf_counter()
     o.orc_matvec(dbs.reshape(-1), R, cw, b0, b1, 0, cw, out)
-    t_mv = (time.perf_counter() - t) * col_frac
+    t_mv = (time.perf_counter() - t) * Cc / cw
     Dp = ol.rand_poly(rng, (ol.NSTEPS, 2))  # [step][digit][limb] shapes; values are irrelevant to timing
     Dp = np.ascontiguousarray(np.transpose(Dp, (2, 0, 1, 3)))
     Df = ol.rand_poly(rng, (2,))
@@ -87,7 +89,7 @@ def cpu_baseline(geo, n_groups_sample=8, col_frac=4, horner_steps=16):
     total = t_mv + t_pack + t_h
     return {
         "value": 1.0 / total, "unit": "queries/s", "cores": 1, "kind": "port",
-        "sample": (f"oracle (scalar C++, 1 thread) on one query: mat-vec over 1/{col_frac} of the columns, "
+        "sample": (f"oracle (scalar C++, 1 thread) on one query: mat-vec over {cw} of {Cc} columns, "
                    f"collapse of {n_groups_sample} of {G} groups, {horner_steps} of {Da - 1} Horner steps; "
                    f"scaled to the whole query (mat-vec {t_mv:.1f} s, collapse {t_pack:.1f} s, Horner {t_h:.2f} s)"),
     }
@@ -167,6 +169,8 @@ def main():
     ap.add_argument("--config", default="c3", choices=sorted(CONFIGS))
     ap.add_argument("--batch", type=int, default=32)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--verify", type=int, default=0,
+                    help="also answer and extract this many further random queries (batches of --batch)")
     ap.add_argument("--serving", action="store_true",
                     help="also run Poisson arrivals through the serving loop at 0.25/0.5/0.9x capacity")
     args = ap.parse_args()
@@ -222,6 +226,25 @@ def main():
     ref1 = srv.answer_batch([qs[0]])[0]
     ok = ok and np.array_equal(ref1, resp[0])
     log(f"[bench] extract check ({B} queries) and batch==single: {'ok' if ok else 'FAILED'}")
+    verified = None
+    if args.verify > 0:
+        vr = np.random.default_rng(11).integers(0, n, args.verify)
+        bad = 0
+        t0 = time.time()
+        for c0 in range(0, args.verify, B):
+            chunk = vr[c0:c0 + B]
+            built = [cl.build(int(i), 3 * 1000003 + c0 + k) for k, i in enumerate(chunk)]
+            dq.upload(np.ascontiguousarray(np.stack([q for _, q in built])))
+            step(len(chunk))
+            lib.ipir_dev_sync()
+            r = dout.download(np.zeros((len(chunk), resp_u64s), np.uint64))
+            for k, (h, _) in enumerate(built):
+                bad += not np.array_equal(cl.extract(h, r[k]), entry(int(chunk[k])))
+                cl.destroy(h)
+        verified = {"queries": int(args.verify), "mismatches": int(bad)}
+        ok = ok and bad == 0
+        log(f"[bench] verified {args.verify} further random queries: {bad} mismatches ({time.time() - t0:.1f} s)")
+        dq.upload(flat)
     del db
 
     def timed(count, steps, warmup):
@@ -284,6 +307,29 @@ def main():
                                          "us_per_launch_1024x2": round(ms.value * 1e3, 2)}
     log(f"[bench] NTT throughput (2048-point, per limb): {ntt}")
 
+    # host-staged wire paths (PCIe-inclusive; the headline value keeps the
+    # queries resident): flat u64 residues vs the packed wire form with the
+    # codecs on the device
+    def host_rate(fn, steps=5):
+        fn()
+        t = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        return B * steps / (time.perf_counter() - t)
+    views = srv.views(qs)
+    flat_out = np.zeros((B, resp_u64s), np.uint64)
+    packed = np.stack([cl.pack_query(q) for q in qs])
+    comp = np.zeros((B, T * 12288), np.uint8)
+    qps_flat = host_rate(lambda: lib.ipir_answer_batch(srv.h, views, B, ia.ptr(flat_out)))
+    qps_packed = host_rate(lambda: lib.ipir_answer_batch_packed(srv.h, ia.ptr(packed), B, ia.ptr(comp)))
+    wire_ok = all(np.array_equal(comp[k], cl.compress(flat_out[k])) for k in range(B))
+    wire = {"flat_qps": round(qps_flat, 1), "packed_qps": round(qps_packed, 1),
+            "h2d_bytes_per_query": {"flat": int(qlen * 8), "packed": int(packed.shape[1])},
+            "d2h_bytes_per_query": {"flat": int(resp_u64s * 8), "packed": int(T * 12288)},
+            "packed_equals_host_codecs": bool(wire_ok)}
+    log(f"[bench] host-staged B={B}: flat {qps_flat:.1f} q/s, packed wire {qps_packed:.1f} q/s "
+        f"(codecs on device; bytes equal host codecs: {wire_ok})")
+
     serve = None
     if args.serving:
         serve = serving(lib, srv, qs, qps32, resp_u64s)
@@ -294,7 +340,8 @@ def main():
         cpu = cpu_baseline(geo)
 
     out = {
-        "metric": "queries/s at 16 GB DB, batch 32 vs batch 1 (bit-exact); p50/p99 latency",
+        "metric": ("queries/s at 16 GB DB, batch 32 vs batch 1 (bit-exact); p50/p99 latency" if args.config == "c3"
+                   else f"queries/s at {label}, batch {B} vs batch 1 (bit-exact); p50/p99 latency"),
         "value": round(qps32, 2),
         "unit": "queries/s",
         "n_gpus": 1,
@@ -319,6 +366,7 @@ def main():
         "stages_ms": {"batch1": stages1, f"batch{B}": stages32},
         "preprocess_s": round(t_pre, 2),
         "bit_exact_check": bool(ok),
+        "verified_queries": verified,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": f"{kname} (database pass, batch 1; {dbpass_ms:.3f} ms per launch)"},
@@ -330,6 +378,7 @@ def main():
         "horner_step_us": {"batch1": round(st1[4] * 1e3 / max(1, geo["Da"] - 1), 2),
                            f"batch{B}": round(st32[4] * 1e3 / max(1, geo["Da"] - 1), 2)},
         "ntt_throughput": ntt,
+        "host_wire": wire,
         "cpu_baseline": cpu,
     }
     if serve is not None:
