@@ -254,10 +254,13 @@ DbPlan cand_plan(const DbCand& c, size_t R, size_t ncols) {
 
 // The plan of `full` (over all C columns) narrowed to ncols columns: the
 // same kernel, with row slices multiplied so the grid keeps the full
-// plan's workgroup count (a narrow grid leaves most CUs idle).
+// plan's workgroup count up to kChunkWgs (a narrow grid leaves most CUs
+// idle; past that, more slices only add partial sums to write and reduce).
+constexpr size_t kChunkWgs = 2048;
 DbPlan narrow_plan(const DbPlan& full, size_t target, size_t R, size_t C, size_t ncols, size_t nq) {
     if (ncols == C) return full;
-    const size_t wgs = C / dbplan_cols(full) * full.slices;
+    size_t wgs = C / dbplan_cols(full) * full.slices;
+    if (wgs > kChunkWgs) wgs = kChunkWgs;
     const size_t t = wgs > target ? wgs : target;
     return full.kind == DbPlan::F64 ? dbplan_f64(R, ncols, (int)full.param, t)
                                      : dbplan_mfma(R, ncols, nq, t, full.depth, full.dbtile);
