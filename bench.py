@@ -173,7 +173,18 @@ def main():
                     help="also answer and extract this many further random queries (batches of --batch)")
     ap.add_argument("--serving", action="store_true",
                     help="also run Poisson arrivals through the serving loop at 0.25/0.5/0.9x capacity")
+    ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
+                    help="only run a plain device copy of this many GiB (3 launches) and print its rate: the "
+                         "known-size reference for the FETCH_SIZE/WRITE_SIZE counter scale")
     args = ap.parse_args()
+    if args.copy_calibration > 0:
+        import inspire_amd as ia
+        nbytes = int(args.copy_calibration * 2 ** 30) // 16 * 16
+        ms = C.c_double()
+        assert ia.load("server").ipir_dbg_copy_bench(nbytes, 3, C.byref(ms)) == 0
+        print(json.dumps({"copy_bytes": nbytes, "ms_per_copy": round(ms.value, 4),
+                          "GBs_read_plus_write": round(2 * nbytes / (ms.value * 1e-3) / 1e9, 1)}))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus > 1 is not supported "
                                    "(queries shard by replica or by group, see the design notes)"}))

@@ -48,6 +48,10 @@ int ipir_dbg_ntt(const uint32_t* in, uint32_t* out, size_t npoly, int inverse);
 /* Device time of one launch transforming npoly RNS polynomials (both limbs)
  * resident in HBM, averaged over iters launches. */
 int ipir_dbg_ntt_bench(size_t npoly, int inverse, int iters, double* ms_per_call);
+/* Time a plain device copy of `bytes` (a multiple of 16) between two fresh
+ * buffers: the known-size reference that calibrates memory-traffic
+ * counters (FETCH_SIZE, WRITE_SIZE) of the other kernels. */
+int ipir_dbg_copy_bench(size_t bytes, int iters, double* ms_per_copy);
 int ipir_dbg_horner(const uint64_t* cts, size_t L, const uint64_t* rgsw, uint64_t* out);
 /* horner with a chosen kernel: variant 0 = one workgroup per chain, 1 = a
  * workgroup pair (one per limb). */

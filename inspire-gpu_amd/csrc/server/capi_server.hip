@@ -360,6 +360,31 @@ int ipir_dbg_ntt_bench(size_t npoly, int inverse, int iters, double* ms_per_call
     return rc;
 }
 
+int ipir_dbg_copy_bench(size_t bytes, int iters, double* ms_per_copy) {
+    if (bytes == 0 || bytes % 16 || iters <= 0 || !ms_per_copy) return 1;
+    void *a = nullptr, *b = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = 0;
+    if (hipMalloc(&a, bytes) || hipMalloc(&b, bytes) || hipMemset(a, 1, bytes) || hipEventCreate(&e0) ||
+        hipEventCreate(&e1))
+        rc = 4;
+    if (!rc && ipx::launch_copy16(a, b, bytes, nullptr)) rc = 4;  // warm-up
+    if (!rc) {
+        (void)hipEventRecord(e0, nullptr);
+        for (int i = 0; i < iters && !rc; i++)
+            if (ipx::launch_copy16(a, b, bytes, nullptr)) rc = 4;
+        (void)hipEventRecord(e1, nullptr);
+        float ms = 0;
+        if (!rc && (hipEventSynchronize(e1) || hipEventElapsedTime(&ms, e0, e1))) rc = 4;
+        *ms_per_copy = ms / iters;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    dfree(a);
+    dfree(b);
+    return rc;
+}
+
 int ipir_dbg_horner(const uint64_t* cts, size_t L, const uint64_t* rgsw, uint64_t* out) {
     return ipir_dbg_horner_variant(cts, L, rgsw, out, 0);
 }

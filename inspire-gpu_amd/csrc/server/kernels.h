@@ -28,6 +28,8 @@ hipError_t launch_unmarshal(const uint64_t* d_q, size_t count, size_t R, double*
                             uint32_t* rgsw, hipStream_t s);
 // In-place forward NTT of `npoly` RNS polynomials laid out [npoly][2][N].
 hipError_t launch_ntt_rns(uint32_t* polys, size_t npoly, const NttConst& nc, bool inverse, hipStream_t s);
+// dst <- src, bytes a multiple of 16 (grid-stride, 16 bytes per lane).
+hipError_t launch_copy16(const void* src, void* dst, size_t bytes, hipStream_t s);
 
 // Wire codecs on the device (the client library's byte formats).
 // Bytes of one query in the 53-bit CRT-packed form.

@@ -86,6 +86,7 @@ SERVER_FUNCS = [
     ("ipir_dbg_answer_taps", C.c_int, [VP, VP, VP, VP, VP]),
     ("ipir_dbg_ntt", C.c_int, [VP, VP, SZ, C.c_int]),
     ("ipir_dbg_ntt_bench", C.c_int, [SZ, C.c_int, C.c_int, P(C.c_double)]),
+    ("ipir_dbg_copy_bench", C.c_int, [SZ, C.c_int, P(C.c_double)]),
     ("ipir_dbg_horner", C.c_int, [VP, SZ, VP, VP]),
     ("ipir_dbg_horner_variant", C.c_int, [VP, SZ, VP, VP, C.c_int]),
     ("ipir_dbg_set_horner", C.c_int, [VP, C.c_int]),
