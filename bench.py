@@ -52,46 +52,104 @@ def gen_db(R, Cc, seed=0):
     return db
 
 
-def cpu_baseline(geo, n_groups_sample=8, sample_cols=65536, horner_steps=16):
-    """Time the oracle (scalar, one core) on a bounded sample of one query and
-    scale to a whole query: mat-vec over sample_cols columns (all rows), the
-    collapse of n_groups_sample groups, horner_steps external products."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(geo, n_groups_sample=8, sample_cols=65536, horner_steps=16, threads=None):
+    """Time the oracle (scalar C++ restatement of the reference's CPU
+    primitives) on the host cores over a bounded sample of one query and
+    scale to a whole query: the mat-vec over sample_cols columns (all rows)
+    split across `threads` workers, the collapse+pack of groups spread over
+    the workers, the Horner chains (one per response ct, in parallel).
+    Also one-thread figures and per-primitive times."""
+    import concurrent.futures as cf
     import oracle_lib as ol
     o = ol.oracle()
     N = ol.N
-    R, Cc, G, Da = geo["R"], geo["C"], geo["G"], geo["Da"]
+    threads = threads or min(16, os.cpu_count() or 1)
+    R, Cc, G, Da, T = geo["R"], geo["C"], geo["G"], geo["Da"], geo["T"]
     rng = np.random.default_rng(5)
     cw = min(Cc, sample_cols)
     dbs = rng.integers(0, 65535, size=(R, cw), dtype=np.uint16)
+    flat = dbs.reshape(-1)
     b0 = rng.integers(0, ol.Q0, R, dtype=np.uint64)
     b1 = rng.integers(0, ol.Q1, R, dtype=np.uint64)
-    out = np.zeros(2 * cw, np.uint64)
+    pool = cf.ThreadPoolExecutor(threads)  # ctypes calls release the GIL
+
+    def mv(c0, c1):
+        out = np.zeros(2 * (c1 - c0), np.uint64)
+        o.orc_matvec(flat, R, cw, b0, b1, c0, c1, out)
+    part = max(1, cw // threads)
+    mv(0, 8)  # first call: the oracle's tables
     t = time.perf_counter()
-    o.orc_matvec(dbs.reshape(-1), R, cw, b0, b1, 0, cw, out)
-    t_mv = (time.perf_counter() - t) * Cc / cw
+    mv(0, part)
+    t_mv1 = (time.perf_counter() - t) * Cc / part
+    t = time.perf_counter()
+    list(pool.map(lambda k: mv(k * part, min(cw, (k + 1) * part)), range(threads)))
+    t_mv = (time.perf_counter() - t) * Cc / (part * threads)
     Dp = ol.rand_poly(rng, (ol.NSTEPS, 2))  # [step][digit][limb] shapes; values are irrelevant to timing
-    Dp = np.ascontiguousarray(np.transpose(Dp, (2, 0, 1, 3)))
-    Df = ol.rand_poly(rng, (2,))
-    k5 = ol.rand_poly(rng, (2,))
-    interm = ol.rand_poly(rng)
-    ob = np.zeros(2 * N, np.uint64)
+    Dp = np.ascontiguousarray(np.transpose(Dp, (2, 0, 1, 3))).reshape(-1)
+    Df = ol.rand_poly(rng, (2,)).reshape(-1)
+    k5 = ol.rand_poly(rng, (2,)).reshape(-1)
+    interm = ol.rand_poly(rng).reshape(-1)
+
+    def pack(_):
+        ob = np.zeros(2 * N, np.uint64)
+        o.orc_pack_group(interm, Dp, Dp, Df, k5, k5, ob)
+    pack(0)
     t = time.perf_counter()
-    for _ in range(n_groups_sample):
-        o.orc_pack_group(interm.reshape(-1), Dp.reshape(-1), Dp.reshape(-1), Df.reshape(-1),
-                         k5.reshape(-1), k5.reshape(-1), ob)
-    t_pack = (time.perf_counter() - t) / n_groups_sample * G
+    pack(0)
+    t_pack1 = (time.perf_counter() - t) * G
+    ng = max(n_groups_sample, threads)
+    t = time.perf_counter()
+    list(pool.map(pack, range(ng)))
+    t_pack = (time.perf_counter() - t) / ng * G
     rg = ol.rand_poly(rng, (4, 2)).reshape(-1)
     cts = ol.rand_poly(rng, (horner_steps + 1, 2)).reshape(-1)
     ho = np.zeros(4 * N, np.uint64)
     t = time.perf_counter()
     o.orc_horner(cts, horner_steps + 1, rg, ho)
-    t_h = (time.perf_counter() - t) / horner_steps * max(0, Da - 1) * geo["T"]
+    step = (time.perf_counter() - t) / horner_steps
+    t_h1 = step * max(0, Da - 1) * T
+    t_h = step * max(0, Da - 1) * -(-T // threads)
+    pool.shutdown()
+    # per-primitive (one thread), microseconds
+    prim = {}
+    poly = ol.rand_poly(rng).reshape(-1)
+
+    def us(fn, reps):
+        t = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        return round((time.perf_counter() - t) / reps * 1e6, 1)
+    a = poly[:N].copy()
+    prim["ntt_fwd_q0"] = us(lambda: o.orc_ntt_fwd(a, 0), 50)
+    prim["ntt_fwd_q1"] = us(lambda: o.orc_ntt_fwd(a, 1), 50)
+    prim["ntt_inv_q0"] = us(lambda: o.orc_ntt_inv(a, 0), 50)
+    prim["ntt_inv_q1"] = us(lambda: o.orc_ntt_inv(a, 1), 50)
+    dec = np.zeros(2 * 2 * N, np.uint64)
+    prim["decomp"] = us(lambda: o.orc_decomp(poly, dec), 50)
+    ct = ol.rand_poly(rng, (2,)).reshape(-1)
+    res = np.zeros(4 * N, np.uint64)
+    prim["ext_prod"] = us(lambda: o.orc_ext_prod(rg, ct, res), 10)
+    prim[f"horner_eval_Da{Da}"] = round(step * max(0, Da - 1) * 1e6, 1)
     total = t_mv + t_pack + t_h
+    total1 = t_mv1 + t_pack1 + t_h1
     return {
-        "value": 1.0 / total, "unit": "queries/s", "cores": 1, "kind": "port",
-        "sample": (f"oracle (scalar C++, 1 thread) on one query: mat-vec over {cw} of {Cc} columns, "
-                   f"collapse of {n_groups_sample} of {G} groups, {horner_steps} of {Da - 1} Horner steps; "
-                   f"scaled to the whole query (mat-vec {t_mv:.1f} s, collapse {t_pack:.1f} s, Horner {t_h:.2f} s)"),
+        "value": 1.0 / total, "unit": "queries/s", "cores": threads, "kind": "port",
+        "sample": (f"oracle (scalar C++ restatement of the reference CPU primitives) on {threads} host threads "
+                   f"({cpu_model()}): mat-vec over {cw} of {Cc} columns, collapse+pack of {ng} groups, "
+                   f"{horner_steps} of {Da - 1} Horner steps; scaled to one whole query (mat-vec {t_mv:.2f} s, "
+                   f"collapse {t_pack:.2f} s, Horner {t_h:.2f} s)"),
+        "single_thread_qps": 1.0 / total1,
+        "primitives_us": prim,
     }
 
 

@@ -322,6 +322,48 @@ def test_packed_wire_answers_match_host_codecs(medium8):
         cl.destroy(h)
 
 
+def test_generation_rollover_while_serving(medium8):
+    """A database generation is one server (reference src/gpu_protocol.h:
+    11-15, docs/ARCHITECTURE.md:292-297): the next generation (a new DB) is
+    preprocessed on another host thread while the current one keeps
+    answering, admitted because device_free_bytes holds another resident
+    footprint; both answer their own DB side by side, and freeing one
+    generation leaves the other answering."""
+    import threading
+    geo, db, srv = medium8
+    n, w, rows = 524288, 120, 2048
+    caps = srv.caps()
+    assert caps.device_free_bytes > 2 * caps.resident_bytes
+    db2 = np.random.default_rng(43).integers(0, P, size=(geo["R"], geo["C"]), dtype=np.uint16)
+    box = {}
+
+    def build():
+        box["srv"] = ia.Server(db2, n, w, rows, max_batch=4, crs_seed=bytes(SEED))
+    th = threading.Thread(target=build)
+    th.start()
+    served = 0
+    while th.is_alive() or served == 0:
+        q, st = ol.make_query(geo, SEED, 1000 + served, 5000 + served)
+        r = srv.answer_batch([q])[0]
+        assert np.array_equal(ol.extract(geo, st, r.reshape(geo["T"], -1)), ol.expected_slots(geo, db, 1000 + served))
+        served += 1
+    th.join()
+    new = box["srv"]
+    try:
+        for k, idx in enumerate([7, 300000]):
+            q, st = ol.make_query(geo, SEED, idx, 6000 + k)
+            r_new = new.answer_batch([q])[0]
+            assert np.array_equal(ol.extract(geo, st, r_new.reshape(geo["T"], -1)), ol.expected_slots(geo, db2, idx))
+            r_old = srv.answer_batch([q])[0]
+            assert np.array_equal(ol.extract(geo, st, r_old.reshape(geo["T"], -1)), ol.expected_slots(geo, db, idx))
+        say(f"[rollover] {served} answers served on the old generation while the new one was built")
+    finally:
+        new.close()
+    q, st = ol.make_query(geo, SEED, 424242, 6100)
+    r = srv.answer_batch([q])[0]
+    assert np.array_equal(ol.extract(geo, st, r.reshape(geo["T"], -1)), ol.expected_slots(geo, db, 424242))
+
+
 def test_batch_validation_codes(medium8):
     geo, db, srv = medium8
     lib = srv.lib
