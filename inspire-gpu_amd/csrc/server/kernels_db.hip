@@ -314,12 +314,25 @@ __global__ __launch_bounds__(256, OCC) void k_dbpass_mfma_sa(const uint8_t* __re
 #pragma unroll
         for (int n = 0; n < NT; n++) {
             const i32x4 Bl = v[bs][n][0] ^ flip, Bh = v[bs][n][1] ^ flip;
+            constexpr int AW = 4;  // digit tiles read ahead of their MFMAs
+            i32x4 Aw[AW];
+#pragma unroll
+            for (int a = 0; a < AW; a++) Aw[a] = atile[buf][(l >> 4) * Mp + a * 16 + (l & 15)];
 #pragma unroll
             for (int a = 0; a < MT; a++) {
-                const i32x4 A = atile[buf][(l >> 4) * Mp + a * 16 + (l & 15)];
+                const i32x4 A = Aw[a % AW];
+                if (a + AW < MT) Aw[a % AW] = atile[buf][(l >> 4) * Mp + (a + AW) * 16 + (l & 15)];
                 acc[a][n][0] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A, Bl, acc[a][n][0], 0, 0, 0);
                 acc[a][n][1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A, Bh, acc[a][n][1], 0, 0, 0);
             }
+            // keep the digit-tile reads AW tiles ahead of their MFMAs
+            __builtin_amdgcn_sched_group_barrier(0x100, AW, 0);
+#pragma unroll
+            for (int a = 0; a + AW < MT; a++) {
+                __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);
+                __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+            }
+            __builtin_amdgcn_sched_group_barrier(0x008, 2 * AW, 0);
         }
     };
     // Step s: issue the loads of step s+L (B slot (s+L)%DEPTH held step s-1's
