@@ -63,11 +63,12 @@ int ipir_dbg_set_group_chunk(ipir_server* srv, size_t groups);
  * or -1 (the default) to use the pair only for calls with at most two
  * chains. */
 int ipir_dbg_set_horner(ipir_server* srv, int mode);
-/* Set the register-ring depth (3..5; 9..16-query plans cap it at 4) of the
- * batched database passes a server uses, and for 17..32 queries the form
- * (dbtile = 1: the waves share the DB tile, depth 3..4; 0: the digit
- * tile); 1 for other values. */
-int ipir_dbg_set_mfma_depth(ipir_server* srv, uint32_t depth, int dbtile);
+/* Set the register-ring depth of the batched database passes a server uses
+ * and their form: 0 the waves of a 4-wave block share the digit tile (depth
+ * 3..5), 1 (17..32 queries) they share the DB tile (3..4), 2 the waves of an
+ * 8-wave block share the digit tile (3..7); 9..16-query plans cap the depth
+ * at 4. 1 for other values. */
+int ipir_dbg_set_mfma_depth(ipir_server* srv, uint32_t depth, int form);
 /* Set the pipeline shape of one-query and many-query answers: 0 serial, 2
  * Horner chains on a second stream per group chunk, 3 the collapse on a
  * third stream as well; 1 for other values. */

@@ -53,10 +53,12 @@ struct DbPlan {
     uint32_t slices;     // row slices (power of two)
     uint32_t depth = 3;  // 8 or 16 digit tiles: register-ring slots of DB operands (kMfmaDepths)
     bool dbtile = false; // 16 digit tiles: the waves share the DB tile (k_dbpass_mfma_bs), not the digit tile
+    uint32_t waves = 4;  // 8 or 16 digit tiles, shared digits: waves per block (4, or 8 = one block per CU)
 };
-// Ring depths of the 9..32-query MFMA passes; 8-tile and shared-DB-tile
-// plans cap the depth at 4.
-constexpr uint32_t kMfmaDepths[] = {3, 4, 5};
+// Ring depths of the 9..32-query MFMA passes; plans cap them at what each
+// kernel is built with (7 for 16 shared-digit tiles in 8-wave blocks, 5 in
+// 4-wave blocks, 4 otherwise).
+constexpr uint32_t kMfmaDepths[] = {3, 4, 5, 6, 7};
 // Row-major staging rows (16*nkb of them) -> blocks [kb0, kb0 + nkb) of the planes.
 hipError_t launch_to_planes(const uint16_t* src, uint8_t* dst, size_t C, size_t kb0, size_t nkb, hipStream_t s);
 // Plans for a pass over ncols columns (the slice count fills the device).
@@ -65,7 +67,7 @@ DbPlan dbplan_f64(size_t R, size_t ncols, int unroll, size_t target_blocks);
 constexpr size_t kMfmaMaxQueries = 32;
 // int8-MFMA plan for nq (1..kMfmaMaxQueries) queries.
 DbPlan dbplan_mfma(size_t R, size_t ncols, size_t nq, size_t target_blocks, uint32_t depth = 4,
-                   bool dbtile = false);
+                   bool dbtile = false, uint32_t waves = 4);
 // Digit rows the MFMA plan stages per 16-row block (int8 scratch: R*Mp bytes, Mp <= 256).
 size_t dbplan_mp(const DbPlan& p);
 // Column granularity of a plan (one block's columns): col0 and ncols are multiples of it.

@@ -257,8 +257,11 @@ __global__ __launch_bounds__(256, 2) void k_dbpass_mfma(const uint8_t* __restric
 }
 
 // Many queries (MT = 8 or 16 digit tiles, up to 16 or 32 queries): as above,
-// but the block's 4 waves share the step's digit tile (Mp x 64 bytes)
-// through an LDS double buffer (one barrier per step). Every global load is
+// but the block's WAVES waves share the step's digit tile (Mp x 64 bytes)
+// through an LDS double buffer (one barrier per step). Every block reads the
+// whole digit tile each step, twice the bytes of its 64 columns' DB operands
+// at 4 waves; 8 waves (both waves of every SIMD of a CU) halve that ratio,
+// and the digit loads are what the vector-memory path spends most on. Every global load is
 // issued L = DEPTH-1 steps before its step: the DB operands into a register
 // ring of DEPTH slots, the thread's share of the digit tile into a ring of L
 // slots (stored to LDS one step ahead). Vector-memory loads complete in issue
@@ -267,8 +270,8 @@ __global__ __launch_bounds__(256, 2) void k_dbpass_mfma(const uint8_t* __restric
 // with one lead for all loads each wait covers exactly one step's loads. Past
 // the slice end the loads repeat the last step (in bounds, never used), so
 // every step issues the same instructions.
-template <int MT, int NT, int OCC, int DEPTH>
-__global__ __launch_bounds__(256, OCC) void k_dbpass_mfma_sa(const uint8_t* __restrict__ db,
+template <int MT, int NT, int OCC, int DEPTH, int WAVES = 4>
+__global__ __launch_bounds__(64 * WAVES, OCC) void k_dbpass_mfma_sa(const uint8_t* __restrict__ db,
                                                            const int8_t* __restrict__ dig,
                                                            const uint32_t* __restrict__ S,
                                                            uint32_t* __restrict__ out, size_t out_qstride,
@@ -277,11 +280,12 @@ __global__ __launch_bounds__(256, OCC) void k_dbpass_mfma_sa(const uint8_t* __re
     constexpr int L = DEPTH - 1;            // lead of every load, in steps
     constexpr int P = DEPTH * L;            // period of both rings' slot pattern
     constexpr uint32_t Mp = 16 * MT;
-    constexpr uint32_t AU = Mp * 4 / 256;  // 16-byte units of the step's digit tile per thread
+    constexpr uint32_t TH = 64 * WAVES;    // threads per block
+    constexpr uint32_t AU = Mp * 4 / TH;   // 16-byte units of the step's digit tile per thread
     __shared__ i32x4 atile[2][4 * Mp];      // [buffer][block of the step][row]
     const uint32_t tid = threadIdx.x, w = tid >> 6, l = tid & 63;
     const uint32_t slice = blockIdx.y;
-    const uint32_t col0 = (blockIdx.x * 4 + w) * NT * 16;
+    const uint32_t col0 = (blockIdx.x * WAVES + w) * NT * 16;
     const uint32_t kb0 = slice * kbps;
     const uint32_t nks = kbps / 4;
     const i32x4* bsrc = reinterpret_cast<const i32x4*>(db) + (size_t)(kb0 + (l >> 4)) * 2 * C + col0 + (l & 15);
@@ -299,7 +303,7 @@ __global__ __launch_bounds__(256, OCC) void k_dbpass_mfma_sa(const uint8_t* __re
     auto issue = [&](int bs, int as, uint32_t step) {
         const uint32_t k = step < nks ? step : nks - 1;
 #pragma unroll
-        for (int u = 0; u < (int)AU; u++) ar[as][u] = atile_src[(size_t)k * 4 * Mp + 256 * u];
+        for (int u = 0; u < (int)AU; u++) ar[as][u] = atile_src[(size_t)k * 4 * Mp + TH * u];
 #pragma unroll
         for (int n = 0; n < NT; n++) {
             v[bs][n][0] = __builtin_nontemporal_load(bsrc + k * bstep + n * 16);
@@ -308,7 +312,7 @@ __global__ __launch_bounds__(256, OCC) void k_dbpass_mfma_sa(const uint8_t* __re
     };
     auto store_a = [&](int buf, int as) {
 #pragma unroll
-        for (int u = 0; u < (int)AU; u++) atile[buf][tid + 256 * u] = ar[as][u];
+        for (int u = 0; u < (int)AU; u++) atile[buf][tid + TH * u] = ar[as][u];
     };
     auto compute = [&](int bs, int buf) {
 #pragma unroll
@@ -492,13 +496,17 @@ DbPlan dbplan_f64(size_t R, size_t ncols, int unroll, size_t target_blocks) {
     return p;
 }
 
-DbPlan dbplan_mfma(size_t R, size_t ncols, size_t nq, size_t target_blocks, uint32_t depth, bool dbtile) {
+DbPlan dbplan_mfma(size_t R, size_t ncols, size_t nq, size_t target_blocks, uint32_t depth, bool dbtile,
+                   uint32_t waves) {
     DbPlan p;
     p.kind = DbPlan::MFMA;
     p.param = nq <= 2 ? 1 : nq <= 4 ? 2 : nq <= 8 ? 4 : nq <= 16 ? 8 : 16;  // 2 queries per 16-row digit tile
     p.dbtile = dbtile && p.param == 16;
-    // ring depths the kernels are built with: 3..5 (16 tiles, shared digits), else 3..4
-    p.depth = p.param == 16 && !p.dbtile ? depth : p.param >= 8 ? (depth < 4 ? depth : 4) : 3;
+    p.waves = p.param >= 8 && !p.dbtile && waves == 8 ? 8 : 4;
+    // ring depths the kernels are built with (what fits 256 VGPRs): 16 tiles
+    // with shared digits 3..7 (8-wave blocks) or 3..5, otherwise 3..4
+    const uint32_t maxd = p.param == 16 && !p.dbtile ? (p.waves == 8 ? 7 : 5) : 4;
+    p.depth = p.param >= 8 ? (depth < 3 ? 3 : depth > maxd ? maxd : depth) : 3;
     const size_t cols = dbplan_cols(p);
     // slices: whole number of 128-row (8-block) chunks so the step count is even
     p.slices = pick_slices(R / 16, ncols / cols, target_blocks, 8);
@@ -510,14 +518,14 @@ DbPlan dbplan_mfma(size_t R, size_t ncols, size_t nq, size_t target_blocks, uint
 
 size_t dbplan_mp(const DbPlan& p) { return p.kind == DbPlan::MFMA ? 16 * p.param : 0; }
 
-size_t dbplan_cols(const DbPlan& p) { return p.kind == DbPlan::MFMA ? 64 * mfma_nt(p.param) : 256; }
+size_t dbplan_cols(const DbPlan& p) { return p.kind == DbPlan::MFMA ? 16 * mfma_nt(p.param) * p.waves : 256; }
 
 std::string dbplan_kernel(const DbPlan& p) {
     if (p.kind == DbPlan::F64) return "k_dbpass_f64<" + std::to_string(p.param) + ">";
     const std::string mt_nt = std::to_string(p.param) + ", " + std::to_string(mfma_nt(p.param));
     if (p.param < 8) return "k_dbpass_mfma<" + mt_nt + ">";
     if (p.dbtile) return "k_dbpass_mfma_bs<16, 4, " + std::to_string(p.depth) + ">";
-    return "k_dbpass_mfma_sa<" + mt_nt + ", 2, " + std::to_string(p.depth) + ">";
+    return "k_dbpass_mfma_sa<" + mt_nt + ", 2, " + std::to_string(p.depth) + (p.waves == 8 ? ", 8>" : ">");
 }
 
 hipError_t launch_dbpass_f64(const uint8_t* db, const double* bq, uint32_t* out, size_t ostride, size_t R, size_t C,
@@ -564,14 +572,18 @@ hipError_t launch_dbpass_mfma(const uint8_t* db, const int8_t* dig, const uint32
     const uint8_t* d = db + col0 * 16;
 #define MFMA(K, ...) \
     hipLaunchKernelGGL((K<__VA_ARGS__>), grid, dim3(256), 0, s, d, dig, S, out, out_qstride, Cc, OC, Q, kbps)
+#define MFMA8(K, ...) \
+    hipLaunchKernelGGL((K<__VA_ARGS__, 8>), grid, dim3(512), 0, s, d, dig, S, out, out_qstride, Cc, OC, Q, kbps)
     switch (p.param) {
         case 1: MFMA(k_dbpass_mfma, 1, 4); break;
         case 2: MFMA(k_dbpass_mfma, 2, 4); break;
         case 4: MFMA(k_dbpass_mfma, 4, 4); break;
         case 8:
-            switch (p.depth) {
-                case 3: MFMA(k_dbpass_mfma_sa, 8, 2, 2, 3); break;
-                case 4: MFMA(k_dbpass_mfma_sa, 8, 2, 2, 4); break;
+            switch (p.depth * 10 + p.waves) {
+                case 34: MFMA(k_dbpass_mfma_sa, 8, 2, 2, 3); break;
+                case 44: MFMA(k_dbpass_mfma_sa, 8, 2, 2, 4); break;
+                case 38: MFMA8(k_dbpass_mfma_sa, 8, 2, 2, 3); break;
+                case 48: MFMA8(k_dbpass_mfma_sa, 8, 2, 2, 4); break;
                 default: return hipErrorInvalidValue;
             }
             break;
@@ -584,16 +596,22 @@ hipError_t launch_dbpass_mfma(const uint8_t* db, const int8_t* dig, const uint32
                 }
                 break;
             }
-            switch (p.depth) {
-                case 3: MFMA(k_dbpass_mfma_sa, 16, 1, 2, 3); break;
-                case 4: MFMA(k_dbpass_mfma_sa, 16, 1, 2, 4); break;
-                case 5: MFMA(k_dbpass_mfma_sa, 16, 1, 2, 5); break;
+            switch (p.depth * 10 + p.waves) {
+                case 34: MFMA(k_dbpass_mfma_sa, 16, 1, 2, 3); break;
+                case 44: MFMA(k_dbpass_mfma_sa, 16, 1, 2, 4); break;
+                case 54: MFMA(k_dbpass_mfma_sa, 16, 1, 2, 5); break;
+                case 38: MFMA8(k_dbpass_mfma_sa, 16, 1, 2, 3); break;
+                case 48: MFMA8(k_dbpass_mfma_sa, 16, 1, 2, 4); break;
+                case 58: MFMA8(k_dbpass_mfma_sa, 16, 1, 2, 5); break;
+                case 68: MFMA8(k_dbpass_mfma_sa, 16, 1, 2, 6); break;
+                case 78: MFMA8(k_dbpass_mfma_sa, 16, 1, 2, 7); break;
                 default: return hipErrorInvalidValue;
             }
             break;
         default: return hipErrorInvalidValue;
     }
 #undef MFMA
+#undef MFMA8
     return hipGetLastError();
 }
 

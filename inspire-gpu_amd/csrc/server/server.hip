@@ -263,7 +263,7 @@ DbPlan narrow_plan(const DbPlan& full, size_t target, size_t R, size_t C, size_t
     if (wgs > kChunkWgs) wgs = kChunkWgs;
     const size_t t = wgs > target ? wgs : target;
     return full.kind == DbPlan::F64 ? dbplan_f64(R, ncols, (int)full.param, t)
-                                     : dbplan_mfma(R, ncols, nq, t, full.depth, full.dbtile);
+                                     : dbplan_mfma(R, ncols, nq, t, full.depth, full.dbtile, full.waves);
 }
 }  // namespace
 
@@ -369,14 +369,16 @@ void GpuServerCtx::autotune_dbpass() {
         const long v = std::atol(t);
         if (v > 0) mfma_target = (size_t)v;
     }
-    // INSPIRE_MFMA_DEPTH=<depth>[,dbtile] forces the batched passes' ring
-    // depth and (",dbtile") the shared-DB-tile form for 17..32 queries
+    // INSPIRE_MFMA_DEPTH=<depth>[,dbtile|,w8] forces the batched passes'
+    // ring depth and the form: shared digits in 4-wave blocks (default),
+    // the shared DB tile (17..32 queries), or shared digits in 8-wave blocks
     bool depth_forced = false;
     if (const char* t = std::getenv("INSPIRE_MFMA_DEPTH")) {
         const long v = std::atol(t);
         for (uint32_t d : kMfmaDepths)
             if (v == (long)d) { mfma_depth = d; depth_forced = true; }
         mfma_dbtile = depth_forced && std::strstr(t, "dbtile") != nullptr;
+        mfma_waves = depth_forced && std::strstr(t, "w8") != nullptr ? 8 : 4;
     }
     if (const char* force = std::getenv("INSPIRE_DBPASS")) {
         int u = 4, t = 2048;
@@ -423,15 +425,24 @@ void GpuServerCtx::autotune_dbpass() {
     if (nq > 1) {
         best = 1e30f;
         const uint32_t param = dbplan_mfma(R, C, nq, mfma_target).param;
+        // forms: 0 shared digits (4-wave blocks), 1 shared DB tile, 2 shared digits (8-wave blocks)
+        const int forced_form = mfma_dbtile ? 1 : mfma_waves == 8 ? 2 : 0;
         for (size_t t : kMfmaTargets)
-            for (bool dbt : {false, true}) {
-                if (depth_forced ? dbt != mfma_dbtile : (dbt && param != 16)) continue;
+            for (int form = 0; form < 3; form++) {
+                if (depth_forced ? form != forced_form : (form == 1 && param != 16) || (form > 0 && param < 8))
+                    continue;
                 for (uint32_t d : kMfmaDepths) {
                     if ((depth_forced || param < 8) && d != mfma_depth) continue;
-                    const DbPlan x = dbplan_mfma(R, C, nq, t, d, dbt);
+                    const DbPlan x = dbplan_mfma(R, C, nq, t, d, form == 1, form == 2 ? 8 : 4);
                     if (x.depth != d && param >= 8) continue;  // a depth this kernel is not built with
                     const float ms = time_plan(x, nq);
-                    if (ms < best) { best = ms; mfma_target = t; mfma_depth = d; mfma_dbtile = dbt; }
+                    if (ms < best) {
+                        best = ms;
+                        mfma_target = t;
+                        mfma_depth = d;
+                        mfma_dbtile = form == 1;
+                        mfma_waves = form == 2 ? 8 : 4;
+                    }
                 }
             }
     }
@@ -450,7 +461,7 @@ DbPlan GpuServerCtx::plan_for(size_t nq, size_t ncols) const {
     if (nq == 1)
         return plan1.kind == DbPlan::F64 ? dbplan_f64(R, ncols, (int)plan1.param, plan1_target)
                                          : dbplan_mfma(R, ncols, 1, plan1_target);
-    return dbplan_mfma(R, ncols, nq, mfma_target, mfma_depth, mfma_dbtile);
+    return dbplan_mfma(R, ncols, nq, mfma_target, mfma_depth, mfma_dbtile, mfma_waves);
 }
 
 DbPlan GpuServerCtx::chunk_plan(size_t nq, size_t ncols) const {
@@ -493,7 +504,8 @@ void GpuServerCtx::stage_dbpass(size_t count, hipStream_t s, size_t slot0, int l
         const size_t nq = slot0 + count - q0 < step ? slot0 + count - q0 : step;
         DbPlan p = plan_for(nq, C);
         if (force == 0 && p.kind != DbPlan::F64) p = dbplan_f64(R, C, 4, 2048);
-        if (force == 1 && p.kind == DbPlan::F64) p = dbplan_mfma(R, C, nq, mfma_target, mfma_depth, mfma_dbtile);
+        if (force == 1 && p.kind == DbPlan::F64)
+            p = dbplan_mfma(R, C, nq, mfma_target, mfma_depth, mfma_dbtile, mfma_waves);
         dbpass_prep(q0, nq, p, s, lane);
         dbpass_range(q0, nq, 0, G, p, s, lane);
     }

@@ -38,6 +38,7 @@ struct GpuServerCtx {
     size_t mfma_target = 1024;      // workgroups the batched pass aims for
     uint32_t mfma_depth = 4;        // register-ring depth of the 9..32-query passes
     bool mfma_dbtile = false;       // 17..32 queries: waves share the DB tile instead of the digit tile
+    uint32_t mfma_waves = 4;        // shared-digit passes: waves per block (4 or 8)
     size_t gchunk = 0;              // groups per chunk of the pipelined answer
     // pipeline shape of one-query and many-query calls: 0 serial on the
     // caller's stream, 2 Horner chains beside the passes, 3 collapse beside too

@@ -181,11 +181,12 @@ def test_db_pass_paths_match_oracle(medium8, path):
     assert srv.lib.ipir_dbg_db_pass(srv.h, ia.ptr(bad), 1, path, ia.ptr(np.zeros((1, 2, Cc), np.uint32))) == 1
 
 
-@pytest.mark.parametrize("depth,dbtile", [(3, 0), (4, 0), (5, 0), (3, 1), (4, 1)])
-def test_many_query_ring_depths_match_oracle(medium8, depth, dbtile):
+@pytest.mark.parametrize("depth,form", [(3, 0), (4, 0), (5, 0), (3, 1), (4, 1), (3, 2), (5, 2), (6, 2), (7, 2)])
+def test_many_query_ring_depths_match_oracle(medium8, depth, form):
     """Every register-ring depth and form of the batched MFMA passes against
     the oracle's mat-vec (12 queries: 8 digit tiles, depth capped at 4; 17
-    and 24 queries: 16 tiles, the waves sharing the digit or the DB tile)."""
+    and 24 queries: 16 tiles; the waves of 4- or 8-wave blocks sharing the
+    digit tile, or the DB tile)."""
     geo, db, srv = medium8
     R, Cc = geo["R"], geo["C"]
     enc = np.zeros((R, Cc), np.uint16)
@@ -194,12 +195,15 @@ def test_many_query_ring_depths_match_oracle(medium8, depth, dbtile):
     rng = np.random.default_rng(400 + depth)
     name = C.create_string_buffer(128)
     assert srv.lib.ipir_dbg_dbpass_kernel(srv.h, MB8, name, 128) == 0
-    kname = name.value.decode()  # k_dbpass_mfma_sa<16, 1, 2, depth> or k_dbpass_mfma_bs<16, 4, depth>
-    tuned = (int(kname.split(">")[0].split(",")[-1]), int("_bs<" in kname))
+    kname = name.value.decode()  # k_dbpass_mfma_sa<16, 1, 2, depth[, 8]> or k_dbpass_mfma_bs<16, 4, depth>
+    args = [int(x) for x in kname.split("<")[1].split(">")[0].split(",")]
+    tuned = (args[2], 1) if "_bs<" in kname else (args[3], 2 if len(args) == 5 else 0)
     assert srv.lib.ipir_dbg_set_mfma_depth(srv.h, 2, 0) == 1
     assert srv.lib.ipir_dbg_set_mfma_depth(srv.h, 6, 0) == 1
     assert srv.lib.ipir_dbg_set_mfma_depth(srv.h, 5, 1) == 1
-    assert srv.lib.ipir_dbg_set_mfma_depth(srv.h, depth, dbtile) == 0
+    assert srv.lib.ipir_dbg_set_mfma_depth(srv.h, 8, 2) == 1
+    assert srv.lib.ipir_dbg_set_mfma_depth(srv.h, 3, 3) == 1
+    assert srv.lib.ipir_dbg_set_mfma_depth(srv.h, depth, form) == 0
     try:
         for nq in (12, 17, MB8):
             _ring_case(srv, flat, R, Cc, rng, nq, depth)
