@@ -123,13 +123,27 @@ __device__ __forceinline__ uint32_t perm_slot(uint32_t t, uint32_t v /* 2BR(i)+1
 constexpr uint32_t POLY_LDS = N + N / 32;
 __device__ __forceinline__ uint32_t ls(uint32_t e) { return e + (e >> 5); }
 
+// Lazy butterflies (Harvey): values stay below 4q (forward) or 2q
+// (inverse) between stages, which the 27-bit primes keep far below 2^32.
+// x in [0, 2m) -> x mod m: x - m wraps above x exactly when x < m.
+__device__ __forceinline__ uint32_t fold(uint32_t x, uint32_t m) {
+    const uint32_t y = x - m;
+    return y < x ? y : x;  // unsigned: a v_min_u32
+}
+// x*w mod q up to one extra q (result in [0, 2q)), any x < 2^32.
+__device__ __forceinline__ uint32_t mul_shoup_lazy(uint32_t x, uint32_t w, uint32_t wp, uint32_t q) {
+    return x * w - mulhi32(x, wp) * q;
+}
+
 // One register-blocked phase of the forward transform: the lane owns the 8
 // coefficients base + S*k (k < 8), which are closed under the butterflies
-// of strides 4S, 2S, S (NST = 3) or 2S, S (NST = 2).
-template <int S, int NST>
+// of strides 4S, 2S, S (NST = 3) or 2S, S (NST = 2). Inputs below 4q,
+// outputs below 4q, canonical after the LAST phase.
+template <int S, int NST, bool LAST>
 __device__ __forceinline__ void fwd_phase(uint32_t* poly, uint32_t t, const uint32_t* __restrict__ w,
                                           const uint32_t* __restrict__ wp, uint32_t q) {
     const uint32_t base = (t / S) * 8 * S + (t % S);
+    const uint32_t q2 = 2 * q;
     uint32_t x[8];
 #pragma unroll
     for (int k = 0; k < 8; k++) x[k] = poly[ls(base + S * k)];
@@ -142,21 +156,28 @@ __device__ __forceinline__ void fwd_phase(uint32_t* poly, uint32_t t, const uint
             if (k & (1 << b)) continue;
             const uint32_t j = base + S * k;
             const uint32_t g = j / (2 * st);
-            const uint32_t u = x[k];
-            const uint32_t v = mul_shoup(x[k + (1 << b)], w[m + g], wp[m + g], q);
-            x[k] = add_mod(u, v, q);
-            x[k + (1 << b)] = sub_mod(u, v, q);
+            const uint32_t u = fold(x[k], q2);                                   // [0, 2q)
+            const uint32_t v = mul_shoup_lazy(x[k + (1 << b)], w[m + g], wp[m + g], q);  // [0, 2q)
+            x[k] = u + v;                                                        // [0, 4q)
+            x[k + (1 << b)] = u + q2 - v;                                        // (0, 4q)
         }
+    }
+    if (LAST) {
+#pragma unroll
+        for (int k = 0; k < 8; k++) x[k] = fold(fold(x[k], q2), q);
     }
 #pragma unroll
     for (int k = 0; k < 8; k++) poly[ls(base + S * k)] = x[k];
 }
 
+// Inverse phase: inputs below 2q, outputs below 2q; the scaling phase
+// (by N^-1, a full Shoup product) leaves them canonical.
 template <int S, int NST>
 __device__ __forceinline__ void inv_phase(uint32_t* poly, uint32_t t, const uint32_t* __restrict__ w,
                                           const uint32_t* __restrict__ wp, uint32_t q, bool scale,
                                           uint32_t inv_n, uint32_t inv_n_p) {
     const uint32_t base = (t / S) * 8 * S + (t % S);
+    const uint32_t q2 = 2 * q;
     uint32_t x[8];
 #pragma unroll
     for (int k = 0; k < 8; k++) x[k] = poly[ls(base + S * k)];
@@ -170,8 +191,8 @@ __device__ __forceinline__ void inv_phase(uint32_t* poly, uint32_t t, const uint
             const uint32_t j = base + S * k;
             const uint32_t g = j / (2 * st);
             const uint32_t u = x[k], v = x[k + (1 << b)];
-            x[k] = add_mod(u, v, q);
-            x[k + (1 << b)] = mul_shoup(sub_mod(u, v, q), w[m + g], wp[m + g], q);
+            x[k] = fold(u + v, q2);                                                   // [0, 2q)
+            x[k + (1 << b)] = mul_shoup_lazy(u + q2 - v, w[m + g], wp[m + g], q);    // [0, 2q)
         }
     }
     if (scale) {
@@ -212,13 +233,13 @@ __device__ __forceinline__ Tw tw_lds(const uint32_t* lds_tw, int limb) {
 // (input visible); the routine ends with a barrier (output visible).
 __device__ __forceinline__ void ntt_fwd_lds(uint32_t* poly, uint32_t t, const Tw& tw, int limb, bool active = true) {
     const uint32_t q = qof(limb);
-    if (active) fwd_phase<256, 3>(poly, t, tw.w, tw.wp, q);
+    if (active) fwd_phase<256, 3, false>(poly, t, tw.w, tw.wp, q);
     __syncthreads();
-    if (active) fwd_phase<32, 3>(poly, t, tw.w, tw.wp, q);
+    if (active) fwd_phase<32, 3, false>(poly, t, tw.w, tw.wp, q);
     __syncthreads();
-    if (active) fwd_phase<4, 3>(poly, t, tw.w, tw.wp, q);
+    if (active) fwd_phase<4, 3, false>(poly, t, tw.w, tw.wp, q);
     __syncthreads();
-    if (active) fwd_phase<1, 2>(poly, t, tw.w, tw.wp, q);
+    if (active) fwd_phase<1, 2, true>(poly, t, tw.w, tw.wp, q);
     __syncthreads();
 }
 
