@@ -69,6 +69,10 @@ int ipir_dbg_set_horner(ipir_server* srv, int mode);
  * 8-wave block share the digit tile (3..7); 9..16-query plans cap the depth
  * at 4. 1 for other values. */
 int ipir_dbg_set_mfma_depth(ipir_server* srv, uint32_t depth, int form);
+/* Row slices of the pipelined answer's database-pass chunks for batches
+ * planned like an nq-query batch (0: back to the narrowing rule; a power of
+ * two up to 64, used when the partial-sum buffer holds it). */
+int ipir_dbg_set_chunk_slices(ipir_server* srv, size_t nq, uint32_t slices);
 /* Set the pipeline shape of one-query and many-query answers: 0 serial, 2
  * Horner chains on a second stream per group chunk, 3 the collapse on a
  * third stream as well; 1 for other values. */

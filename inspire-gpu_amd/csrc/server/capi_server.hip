@@ -403,6 +403,15 @@ int ipir_dbg_set_group_chunk(ipir_server* srv, size_t groups) {
     return 0;
 }
 
+int ipir_dbg_set_chunk_slices(ipir_server* srv, size_t nq, uint32_t slices) {
+    if (!srv || nq < 2 || nq > ipx::kMfmaMaxQueries || slices > 64 || (slices & (slices - 1))) return 1;
+    GpuServerCtx* c = srv->ctx;
+    std::lock_guard<std::mutex> lk(c->mu);
+    c->chunk_slices = slices;
+    c->chunk_param = slices ? c->plan_for(nq, c->pp.db_cols).param : 0;
+    return 0;
+}
+
 int ipir_dbg_set_mfma_depth(ipir_server* srv, uint32_t depth, int form) {
     if (!srv || form < 0 || form > 2) return 1;
     // a depth the 32-query kernel of this form is built with

@@ -445,6 +445,42 @@ void GpuServerCtx::autotune_dbpass() {
                     }
                 }
             }
+        // the pipelined answer's chunk plan: the same kernel over one chunk's
+        // columns, its row-slice count timed with the slice reduction (a
+        // narrow grid leaves CUs idle; short slices pay ring fill and partials)
+        if (G / gchunk > 1) {
+            const size_t ccols = gchunk * N;
+            const DbPlan full = plan_for(nq, C);
+            const size_t blocks = ccols / dbplan_cols(full);
+            HIPX(launch_mfma_digits(d_bq, nq, R, full, d_dig[0], d_S[0], stream));
+            best = 1e30f;
+            for (uint32_t sl = 1; sl <= 16; sl *= 2) {
+                const DbPlan x = dbplan_mfma(R, ccols, nq, blocks * sl, mfma_depth, mfma_dbtile, mfma_waves);
+                if (x.slices != sl || (sl > 1 && sl * 2 * ccols > part_stride)) continue;
+                auto run = [&]() {
+                    if (sl == 1) {
+                        HIPX(launch_dbpass_mfma(db_planes(), d_dig[0], d_S[0], nq, d_interm, 2 * C, C, R, C, 0, ccols,
+                                                x, stream));
+                    } else {
+                        HIPX(launch_dbpass_mfma(db_planes(), d_dig[0], d_S[0], nq, d_part, part_stride, ccols, R, C, 0,
+                                                ccols, x, stream));
+                        HIPX(launch_reduce_slices(d_part, part_stride, sl, C, d_interm, nq, 0, ccols, stream));
+                    }
+                };
+                run();
+                HIPX(hipEventRecord(e0, stream));
+                for (int it = 0; it < 3; it++) run();
+                HIPX(hipEventRecord(e1, stream));
+                HIPX(hipEventSynchronize(e1));
+                float ms = 0;
+                HIPX(hipEventElapsedTime(&ms, e0, e1));
+                if (ms < best) {
+                    best = ms;
+                    chunk_slices = sl;
+                    chunk_param = full.param;
+                }
+            }
+        }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -465,8 +501,13 @@ DbPlan GpuServerCtx::plan_for(size_t nq, size_t ncols) const {
 }
 
 DbPlan GpuServerCtx::chunk_plan(size_t nq, size_t ncols) const {
-    return narrow_plan(plan_for(nq, pp.db_cols), nq == 1 ? plan1_target : mfma_target, pp.db_rows, pp.db_cols, ncols,
-                       nq);
+    const DbPlan full = plan_for(nq, pp.db_cols);
+    if (nq > 1 && chunk_slices && full.kind == DbPlan::MFMA && full.param == chunk_param) {
+        const DbPlan x = dbplan_mfma(pp.db_rows, ncols, nq, ncols / dbplan_cols(full) * chunk_slices, mfma_depth,
+                                     mfma_dbtile, mfma_waves);
+        if (x.slices == 1 || x.slices * 2 * ncols <= part_stride) return x;
+    }
+    return narrow_plan(full, nq == 1 ? plan1_target : mfma_target, pp.db_rows, pp.db_cols, ncols, nq);
 }
 
 void GpuServerCtx::dbpass_prep(size_t q0, size_t nq, const DbPlan& p, hipStream_t s, int lane) {

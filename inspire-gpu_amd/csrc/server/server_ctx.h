@@ -39,6 +39,9 @@ struct GpuServerCtx {
     uint32_t mfma_depth = 4;        // register-ring depth of the 9..32-query passes
     bool mfma_dbtile = false;       // 17..32 queries: waves share the DB tile instead of the digit tile
     uint32_t mfma_waves = 4;        // shared-digit passes: waves per block (4 or 8)
+    // row slices of the batched chunk plan, timed at setup for plans with
+    // chunk_param digit tiles (0: the narrowing rule)
+    uint32_t chunk_slices = 0, chunk_param = 0;
     size_t gchunk = 0;              // groups per chunk of the pipelined answer
     // pipeline shape of one-query and many-query calls: 0 serial on the
     // caller's stream, 2 Horner chains beside the passes, 3 collapse beside too

@@ -264,12 +264,14 @@ def test_pair_horner_answers_equal_single(medium8):
     assert np.array_equal(one[0], base[0]) and np.array_equal(one_single[0], base[0])
 
 
-@pytest.mark.parametrize("groups,shape", [(1, 2), (2, 2), (4, 2), (1, 3), (2, 3)])
-def test_pipelined_group_chunks_equal_unchunked(medium8, groups, shape):
+@pytest.mark.parametrize("groups,shape,slices", [(1, 2, 0), (2, 2, 0), (4, 2, 0), (1, 3, 0), (2, 3, 0),
+                                                   (2, 2, 1), (4, 2, 2)])
+def test_pipelined_group_chunks_equal_unchunked(medium8, groups, shape, slices):
     """The answer pipelined over group chunks (Horner chains on a second
     stream, resuming between launches; shape 3 also the collapse on a third)
     equals the serial one-chunk answer, for both Horner kernels, one query
-    and three."""
+    and three; chunk database passes with the narrowing rule's row slices or
+    a set count (one: straight into the chunk's columns)."""
     geo, db, srv = medium8
     qs = [ol.make_query(geo, SEED, idx, 800 + k)[0] for k, idx in enumerate([3, 99999, 524287])]
     base = srv.answer_batch(qs)
@@ -277,6 +279,7 @@ def test_pipelined_group_chunks_equal_unchunked(medium8, groups, shape):
     assert srv.lib.ipir_dbg_set_pipeline(srv.h, 1, shape) == 1
     assert srv.lib.ipir_dbg_set_group_chunk(srv.h, groups) == 0
     assert srv.lib.ipir_dbg_set_pipeline(srv.h, shape, shape) == 0
+    assert srv.lib.ipir_dbg_set_chunk_slices(srv.h, 3, slices) == 0
     try:
         got = srv.answer_batch(qs)
         got1 = srv.answer_batch(qs[:1])
@@ -285,6 +288,7 @@ def test_pipelined_group_chunks_equal_unchunked(medium8, groups, shape):
     finally:
         assert srv.lib.ipir_dbg_set_horner(srv.h, -1) == 0
         assert srv.lib.ipir_dbg_set_pipeline(srv.h, 0, 2) == 0
+        assert srv.lib.ipir_dbg_set_chunk_slices(srv.h, 3, 0) == 0
         assert srv.lib.ipir_dbg_set_group_chunk(srv.h, geo["G"]) == 0
     assert np.array_equal(got, base) and np.array_equal(got1[0], one[0]) and np.array_equal(pair, base)
 
