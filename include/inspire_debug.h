@@ -73,6 +73,10 @@ int ipir_dbg_set_mfma_depth(ipir_server* srv, uint32_t depth, int form);
  * planned like an nq-query batch (0: back to the narrowing rule; a power of
  * two up to 64, used when the partial-sum buffer holds it). */
 int ipir_dbg_set_chunk_slices(ipir_server* srv, size_t nq, uint32_t slices);
+/* Form of the 17..32-query collapse pass (0..5; 1 otherwise), and the one
+ * in use (the setup-time autotuner's pick unless set). */
+int ipir_dbg_set_collapse_variant(ipir_server* srv, int variant);
+int ipir_dbg_get_collapse_variant(const ipir_server* srv);
 /* Set the pipeline shape of one-query and many-query answers: 0 serial, 2
  * Horner chains on a second stream per group chunk, 3 the collapse on a
  * third stream as well; 1 for other values. */

@@ -403,6 +403,15 @@ int ipir_dbg_set_group_chunk(ipir_server* srv, size_t groups) {
     return 0;
 }
 
+int ipir_dbg_set_collapse_variant(ipir_server* srv, int variant) {
+    if (!srv || variant < 0 || variant >= ipx::kCollapseVariants) return 1;
+    std::lock_guard<std::mutex> lk(srv->ctx->mu);
+    srv->ctx->collapse_variant = variant;
+    return 0;
+}
+
+int ipir_dbg_get_collapse_variant(const ipir_server* srv) { return srv ? srv->ctx->collapse_variant : -1; }
+
 int ipir_dbg_set_chunk_slices(ipir_server* srv, size_t nq, uint32_t slices) {
     if (!srv || nq < 2 || nq > ipx::kMfmaMaxQueries || slices > 64 || (slices & (slices - 1))) return 1;
     GpuServerCtx* c = srv->ctx;

@@ -104,8 +104,12 @@ hipError_t launch_collapse_keys(const uint32_t* keys, size_t kstride, size_t nq,
 // Precomp-tensor pass in dot-product form (int8 MFMA, Dd read once) of ng
 // groups whose precomp blocks start at precomp_g0, for nq queries:
 // part2_g0 + q*p2_stride + g*2N holds [limb][N] of group g (relative).
+// 17..32 queries come in kCollapseVariants forms: 0 the two-step
+// prefetch kernel, 1..5 the ring kernel (depth 3, 4 in 4-wave blocks; 3, 4,
+// 5 in 8-wave blocks); fewer queries ignore the variant.
+constexpr int kCollapseVariants = 6;
 hipError_t launch_collapse(const uint32_t* precomp_g0, size_t ng, const int8_t* kdig, const uint32_t* KS, size_t nq,
-                           uint32_t* part2_g0, size_t p2_stride, hipStream_t s);
+                           uint32_t* part2_g0, size_t p2_stride, hipStream_t s, int variant = 0);
 // packed_b[q][g] = NTT(interm[q] group slice) - collapse sums - Df*Kn, for
 // groups [g0, g0 + ng) (absolute indexes into every buffer).
 hipError_t launch_pack_finalize(const uint32_t* interm, size_t C, const uint32_t* part2, size_t part2_stride,

@@ -273,6 +273,99 @@ __global__ __launch_bounds__(256, 2) void k_collapse_mfma(const uint32_t* __rest
     }
 }
 
+// 17..32 queries (8 key-digit tiles) in the load pattern of the batched
+// database pass: every global load of a step (the Dd operands of all 4
+// planes into a register ring of DEPTH slots, the thread's share of the
+// step's key-digit tile into a ring of DEPTH-1 slots, stored to LDS one
+// step ahead) is issued DEPTH-1 steps before its step, so each wait covers
+// one step's loads (vector-memory loads complete in issue order); the
+// block's WAVES waves share the digit tile. Past the last step the loads
+// repeat it (in bounds, unused).
+template <int MT, int NT, int DEPTH, int WAVES>
+__global__ __launch_bounds__(64 * WAVES, 2) void k_collapse_ring(const uint32_t* __restrict__ precomp,
+                                                                 const int8_t* __restrict__ kdig,
+                                                                 const uint32_t* __restrict__ KS,
+                                                                 uint32_t* __restrict__ part2, size_t p2_stride,
+                                                                 uint32_t nq) {
+    static_assert(DEPTH >= 3, "the digit ring needs two slots");
+    constexpr int L = DEPTH - 1;
+    constexpr int P = DEPTH * L;
+    constexpr uint32_t Mp = 16 * MT;
+    constexpr uint32_t NKB = DD_K / 16;
+    constexpr uint32_t TH = 64 * WAVES;
+    constexpr uint32_t AU = Mp * 4 / TH;  // 16-byte units of a step's digit tile per thread
+    constexpr uint32_t nks = DD_K / 64;
+    __shared__ i32x4 atile[2][4 * Mp];
+    const uint32_t tid = threadIdx.x, w = tid >> 6, l = tid & 63;
+    const uint32_t g = blockIdx.z;
+    const int c = (int)blockIdx.y;
+    const uint32_t col0 = (blockIdx.x * WAVES + w) * NT * 16;
+    const i32x4* dd = reinterpret_cast<const i32x4*>(precomp + (size_t)g * PC_GROUP + PC_DD) +
+                      (size_t)c * 4 * NKB * N + (size_t)(l >> 4) * N + col0 + (l & 15);
+    const size_t pstride = (size_t)NKB * N;  // plane, in 16-byte units
+    const size_t bstep = (size_t)4 * N;      // 4 blocks of 16 rows
+    const i32x4* kd = reinterpret_cast<const i32x4*>(kdig) + (size_t)c * NKB * Mp + tid;
+    i32x4 acc[MT][NT][4];
+#pragma unroll
+    for (int a = 0; a < MT; a++)
+#pragma unroll
+        for (int n = 0; n < NT; n++)
+#pragma unroll
+            for (int pl = 0; pl < 4; pl++) acc[a][n][pl] = i32x4{0, 0, 0, 0};
+    i32x4 v[DEPTH][NT][4];
+    i32x4 ar[L][AU];
+    auto issue = [&](int bs, int as, uint32_t step) {
+        const uint32_t k = step < nks ? step : nks - 1;
+#pragma unroll
+        for (int u = 0; u < (int)AU; u++) ar[as][u] = kd[(size_t)k * 4 * Mp + TH * u];
+#pragma unroll
+        for (int n = 0; n < NT; n++)
+#pragma unroll
+            for (int pl = 0; pl < 4; pl++)
+                v[bs][n][pl] = __builtin_nontemporal_load(dd + pl * pstride + k * bstep + n * 16);
+    };
+    auto store_a = [&](int buf, int as) {
+#pragma unroll
+        for (int u = 0; u < (int)AU; u++) atile[buf][tid + TH * u] = ar[as][u];
+    };
+    auto compute = [&](int bs, int buf) {
+#pragma unroll
+        for (int a = 0; a < MT; a++) {
+            const i32x4 Aop = atile[buf][(l >> 4) * Mp + a * 16 + (l & 15)];
+#pragma unroll
+            for (int n = 0; n < NT; n++)
+#pragma unroll
+                for (int pl = 0; pl < 4; pl++)
+                    acc[a][n][pl] = __builtin_amdgcn_mfma_i32_16x16x64_i8(Aop, v[bs][n][pl], acc[a][n][pl], 0, 0, 0);
+        }
+    };
+    // Step s: issue step s+L's loads, compute from slot s%DEPTH and LDS
+    // buffer s&1, store step s+1's digits in buffer (s+1)&1 (its last
+    // reader, step s-1, finished before the previous barrier).
+#pragma unroll
+    for (int j = 0; j < L; j++) issue(j, j, (uint32_t)j);
+    store_a(0, 0);
+    __syncthreads();
+    for (uint32_t ks = 0; ks < nks; ks += P) {
+#pragma unroll
+        for (int j = 0; j < P; j++) {
+            const uint32_t step = ks + (uint32_t)j;
+            if (step >= nks) break;
+            issue((j + L) % DEPTH, (j + L) % L, step + L);
+            compute(j % DEPTH, (int)(step & 1));
+            store_a((int)((step + 1) & 1), (j + 1) % L);
+            __syncthreads();
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < MT; a++) {
+        const uint32_t q = a * 4 + (l >> 4);
+        if (q >= nq) continue;
+#pragma unroll
+        for (int n = 0; n < NT; n++) collapse_store(acc[a][n], q, c, g, col0 + n * 16 + (l & 15), KS, part2, p2_stride);
+    }
+}
+
 }  // namespace
 
 hipError_t launch_dd_relayout(const uint32_t* Dp, const uint32_t* Dm, size_t gstride, size_t ng, uint32_t* precomp_g0,
@@ -297,20 +390,33 @@ hipError_t launch_collapse_keys(const uint32_t* keys, size_t kstride, size_t nq,
 }
 
 hipError_t launch_collapse(const uint32_t* precomp_g0, size_t ng, const int8_t* kdig, const uint32_t* KS, size_t nq,
-                           uint32_t* part2_g0, size_t p2_stride, hipStream_t s) {
-    if (nq == 0 || nq > kCollapseMaxQueries) return hipErrorInvalidValue;
+                           uint32_t* part2_g0, size_t p2_stride, hipStream_t s, int variant) {
+    if (nq == 0 || nq > kCollapseMaxQueries || variant < 0 || variant >= kCollapseVariants) return hipErrorInvalidValue;
     if (ng == 0) return hipSuccess;
     const uint32_t Q = (uint32_t)nq;
 #define COLLAPSE(MT, NT, SH)                                                                                    \
     hipLaunchKernelGGL((k_collapse_mfma<MT, NT, SH>), dim3(N / (64 * NT), 2, (unsigned)ng), dim3(256), 0, s, \
                        precomp_g0, kdig, KS, part2_g0, p2_stride, Q)
+#define RING(D, W)                                                                                            \
+    hipLaunchKernelGGL((k_collapse_ring<8, 1, D, W>), dim3(N / (16 * W), 2, (unsigned)ng), dim3(64 * W), 0, s, \
+                       precomp_g0, kdig, KS, part2_g0, p2_stride, Q)
     switch (collapse_mt(nq)) {
         case 1: COLLAPSE(1, 2, false); break;
         case 2: COLLAPSE(2, 2, false); break;
         case 4: COLLAPSE(4, 2, false); break;
-        default: COLLAPSE(8, 1, true); break;
+        default:
+            switch (variant) {
+                case 1: RING(3, 4); break;
+                case 2: RING(4, 4); break;
+                case 3: RING(3, 8); break;
+                case 4: RING(4, 8); break;
+                case 5: RING(5, 8); break;
+                default: COLLAPSE(8, 1, true); break;
+            }
+            break;
     }
 #undef COLLAPSE
+#undef RING
     return hipGetLastError();
 }
 

@@ -445,6 +445,25 @@ void GpuServerCtx::autotune_dbpass() {
                     }
                 }
             }
+        // the 17..32-query collapse pass form, over all groups (zeroed keys)
+        if (nq > 16) {
+            HIPX(hipMemset(d_keys, 0, nq * keys_stride * 4));
+            HIPX(launch_collapse_keys(d_keys, keys_stride, nq, d_kdig[0], d_KS[0], stream));
+            float cbest = 1e30f;
+            for (int v = 0; v < kCollapseVariants; v++) {
+                auto run = [&]() {
+                    HIPX(launch_collapse(d_precomp, G, d_kdig[0], d_KS[0], nq, d_part2, part2_stride, stream, v));
+                };
+                run();
+                HIPX(hipEventRecord(e0, stream));
+                for (int it = 0; it < 3; it++) run();
+                HIPX(hipEventRecord(e1, stream));
+                HIPX(hipEventSynchronize(e1));
+                float ms = 0;
+                HIPX(hipEventElapsedTime(&ms, e0, e1));
+                if (ms < cbest) { cbest = ms; collapse_variant = v; }
+            }
+        }
         // the pipelined answer's chunk plan: the same kernel over one chunk's
         // columns, its row-slice count timed with the slice reduction (a
         // narrow grid leaves CUs idle; short slices pay ring fill and partials)
@@ -557,7 +576,7 @@ void GpuServerCtx::stage_collapse(size_t count, hipStream_t s, size_t slot0, int
         const size_t nq = slot0 + count - q0 < kCollapseMaxQueries ? slot0 + count - q0 : kCollapseMaxQueries;
         HIPX(launch_collapse_keys(d_keys + q0 * keys_stride, keys_stride, nq, d_kdig[lane], d_KS[lane], s));
         HIPX(launch_collapse(d_precomp, G, d_kdig[lane], d_KS[lane], nq, d_part2 + q0 * part2_stride, part2_stride,
-                             s));
+                             s, collapse_variant));
     }
 }
 
@@ -617,7 +636,7 @@ void GpuServerCtx::enqueue(const uint64_t* d_in, size_t count, uint64_t* d_out, 
             dbpass_prep(q0, nq, p, s, lane);
             dbpass_range(q0, nq, 0, G, p, s, lane);
             HIPX(launch_collapse_keys(keys, keys_stride, nq, d_kdig[lane], d_KS[lane], s));
-            HIPX(launch_collapse(d_precomp, G, d_kdig[lane], d_KS[lane], nq, p2, part2_stride, s));
+            HIPX(launch_collapse(d_precomp, G, d_kdig[lane], d_KS[lane], nq, p2, part2_stride, s, collapse_variant));
             HIPX(launch_pack_finalize(d_interm + q0 * 2 * C, C, p2, part2_stride, d_precomp, 0, G, keys, keys_stride,
                                       packed, packed_stride, nq, nc, s));
             horner_range(nq, out, 0, G, s, q0);
@@ -640,7 +659,7 @@ void GpuServerCtx::enqueue(const uint64_t* d_in, size_t count, uint64_t* d_out, 
             dbpass_range(q0, nq, g_lo, g_hi, p, s, lane);
             HIPX(hipEventRecord(dbe, s));
             HIPX(launch_collapse(d_precomp + g_lo * PC_GROUP, gchunk, d_kdig[lane], d_KS[lane], nq,
-                                 p2 + g_lo * 2 * N, part2_stride, cs));
+                                 p2 + g_lo * 2 * N, part2_stride, cs, collapse_variant));
             HIPX(hipEventRecord(ce, cs));
             HIPX(hipStreamWaitEvent(hs, dbe, 0));
             HIPX(hipStreamWaitEvent(hs, ce, 0));

@@ -42,6 +42,7 @@ struct GpuServerCtx {
     // row slices of the batched chunk plan, timed at setup for plans with
     // chunk_param digit tiles (0: the narrowing rule)
     uint32_t chunk_slices = 0, chunk_param = 0;
+    int collapse_variant = 0;       // 17..32-query collapse pass form (launch_collapse)
     size_t gchunk = 0;              // groups per chunk of the pipelined answer
     // pipeline shape of one-query and many-query calls: 0 serial on the
     // caller's stream, 2 Horner chains beside the passes, 3 collapse beside too

@@ -246,6 +246,30 @@ def test_batch_equals_singles_and_extracts(medium8):
         assert np.array_equal(part, batch[:k])
 
 
+def test_collapse_variants_equal(medium8):
+    """Every form of the 17..32-query collapse pass (two-step prefetch;
+    load rings of depth 3..5 in 4- or 8-wave blocks) gives the same
+    responses for a 20-query batch, and those extract."""
+    geo, db, srv = medium8
+    idxs = [int(i) for i in np.random.default_rng(77).integers(0, 524288, 20)]
+    qs, sts = zip(*[ol.make_query(geo, SEED, idx, 9100 + k) for k, idx in enumerate(idxs)])
+    tuned = srv.lib.ipir_dbg_get_collapse_variant(srv.h)
+    assert 0 <= tuned < 6
+    assert srv.lib.ipir_dbg_set_collapse_variant(srv.h, 6) == 1
+    try:
+        outs = []
+        for v in range(6):
+            assert srv.lib.ipir_dbg_set_collapse_variant(srv.h, v) == 0
+            outs.append(srv.answer_batch(list(qs)))
+    finally:
+        assert srv.lib.ipir_dbg_set_collapse_variant(srv.h, tuned) == 0
+    for v in range(1, 6):
+        assert np.array_equal(outs[v], outs[0]), f"collapse variant {v}"
+    for k in (0, 19):
+        assert np.array_equal(ol.extract(geo, sts[k], outs[0][k].reshape(geo["T"], -1)),
+                              ol.expected_slots(geo, db, idxs[k]))
+
+
 def test_pair_horner_answers_equal_single(medium8):
     """The workgroup-pair Horner gives the same responses as the one-workgroup
     kernel, alone and batched (Horner chain of 7 steps, 10 chains)."""
