@@ -1,4 +1,4 @@
-"""ctypes access to the checkers: the oracle (oracle/liboracle.so) and, where
+"""ctypes access to the test references: the oracle (oracle/liboracle.so) and, where
 it was built, the reference's own CPU library (oracle/_ref/libinspire_ref.so).
 
 Test infrastructure only.
