@@ -362,8 +362,8 @@ void GpuServerCtx::alloc_slots() {
 
 void GpuServerCtx::autotune_dbpass() {
     // INSPIRE_DBPASS forces the single-query plan: "f64,<unroll>,<target>" or
-    // "mfma,<target>"; INSPIRE_MFMA_TARGET sets the batched pass's target;
-    // INSPIRE_AUTOTUNE=0 keeps the defaults.
+    // "mfma,<target>" (the batched forms are still tuned); INSPIRE_MFMA_TARGET
+    // sets the batched pass's target; INSPIRE_AUTOTUNE=0 keeps the defaults.
     const size_t R = pp.db_rows, C = pp.db_cols;
     if (const char* t = std::getenv("INSPIRE_MFMA_TARGET")) {
         const long v = std::atol(t);
@@ -380,16 +380,18 @@ void GpuServerCtx::autotune_dbpass() {
         mfma_dbtile = depth_forced && std::strstr(t, "dbtile") != nullptr;
         mfma_waves = depth_forced && std::strstr(t, "w8") != nullptr ? 8 : 4;
     }
+    bool plan1_forced = false;
     if (const char* force = std::getenv("INSPIRE_DBPASS")) {
         int u = 4, t = 2048;
         if (std::sscanf(force, "f64,%d,%d", &u, &t) == 2 && (u == 2 || u == 4 || u == 8) && t > 0) {
             plan1 = dbplan_f64(R, C, u, (size_t)t);
             plan1_target = (size_t)t;
+            plan1_forced = true;
         } else if (std::sscanf(force, "mfma,%d", &t) == 1 && t > 0) {
             plan1 = dbplan_mfma(R, C, 1, (size_t)t);
             plan1_target = (size_t)t;
+            plan1_forced = true;
         }
-        return;
     }
     const char* at = std::getenv("INSPIRE_AUTOTUNE");
     if (at && at[0] == '0') return;
@@ -416,6 +418,7 @@ void GpuServerCtx::autotune_dbpass() {
     };
     float best = 1e30f;
     for (const DbCand& c : kDbCands) {
+        if (plan1_forced) break;
         const DbPlan x = cand_plan(c, R, C);
         const float ms = time_plan(x, 1);
         if (ms < best) { best = ms; plan1 = x; plan1_target = c.target; }
