@@ -467,41 +467,39 @@ void GpuServerCtx::autotune_dbpass() {
                 if (ms < cbest) { cbest = ms; collapse_variant = v; }
             }
         }
-        // the pipelined answer's chunk plan: the same kernel over one chunk's
-        // columns, its row-slice count timed with the slice reduction (a
-        // narrow grid leaves CUs idle; short slices pay ring fill and partials)
+        // the pipelined answer's chunk plan: the full-width kernel over one
+        // chunk's columns with a row-slice count timed on whole answers of a
+        // synthetic (zero) batch, the Horner chains running beside the
+        // passes as they do when serving (a one-round grid can be held up by
+        // the few CUs the chains take; more slices pay ring fill and partials)
         if (G / gchunk > 1) {
             const size_t ccols = gchunk * N;
             const DbPlan full = plan_for(nq, C);
             const size_t blocks = ccols / dbplan_cols(full);
-            HIPX(launch_mfma_digits(d_bq, nq, R, full, d_dig[0], d_S[0], stream));
+            HIPX(hipMemset(d_q, 0, nq * qlen * 8));
             best = 1e30f;
-            for (uint32_t sl = 1; sl <= 16; sl *= 2) {
-                const DbPlan x = dbplan_mfma(R, ccols, nq, blocks * sl, mfma_depth, mfma_dbtile, mfma_waves);
-                if (x.slices != sl || (sl > 1 && sl * 2 * ccols > part_stride)) continue;
-                auto run = [&]() {
-                    if (sl == 1) {
-                        HIPX(launch_dbpass_mfma(db_planes(), d_dig[0], d_S[0], nq, d_interm, 2 * C, C, R, C, 0, ccols,
-                                                x, stream));
-                    } else {
-                        HIPX(launch_dbpass_mfma(db_planes(), d_dig[0], d_S[0], nq, d_part, part_stride, ccols, R, C, 0,
-                                                ccols, x, stream));
-                        HIPX(launch_reduce_slices(d_part, part_stride, sl, C, d_interm, nq, 0, ccols, stream));
-                    }
-                };
-                run();
+            uint32_t pick = 0;
+            for (uint32_t sl = 0; sl <= 16; sl = sl ? 2 * sl : 1) {  // 0: the narrowing rule
+                if (sl) {
+                    const DbPlan x = dbplan_mfma(R, ccols, nq, blocks * sl, mfma_depth, mfma_dbtile, mfma_waves);
+                    if (x.slices != sl || (sl > 1 && sl * 2 * ccols > part_stride)) continue;
+                }
+                chunk_slices = sl;
+                chunk_param = sl ? full.param : 0;
+                enqueue(d_q, nq, d_resp, stream);  // warm
                 HIPX(hipEventRecord(e0, stream));
-                for (int it = 0; it < 3; it++) run();
+                for (int it = 0; it < 2; it++) enqueue(d_q, nq, d_resp, stream);
                 HIPX(hipEventRecord(e1, stream));
                 HIPX(hipEventSynchronize(e1));
                 float ms = 0;
                 HIPX(hipEventElapsedTime(&ms, e0, e1));
                 if (ms < best) {
                     best = ms;
-                    chunk_slices = sl;
-                    chunk_param = full.param;
+                    pick = sl;
                 }
             }
+            chunk_slices = pick;
+            chunk_param = pick ? full.param : 0;
         }
     }
     (void)hipEventDestroy(e0);
