@@ -66,8 +66,10 @@ int ipir_dbg_set_horner(ipir_server* srv, int mode);
 /* Set the register-ring depth of the batched database passes a server uses
  * and their form: 0 the waves of a 4-wave block share the digit tile (depth
  * 3..5), 1 (17..32 queries) they share the DB tile (3..4), 2 the waves of an
- * 8-wave block share the digit tile (3..7); 9..16-query plans cap the depth
- * at 4. 1 for other values. */
+ * 8-wave block share the digit tile (3..7), 3 and 4 (17..32 queries) the
+ * 8- and 4-wave forms with 7 digit rows per query (3..7, 3..5); 9..16-query
+ * plans cap the depth at 4 and use form 0 or 2 for 3 or 4. 1 for other
+ * values. */
 int ipir_dbg_set_mfma_depth(ipir_server* srv, uint32_t depth, int form);
 /* Row slices of the pipelined answer's database-pass chunks for batches
  * planned like an nq-query batch (0: back to the narrowing rule; a power of

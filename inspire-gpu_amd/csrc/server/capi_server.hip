@@ -422,18 +422,16 @@ int ipir_dbg_set_chunk_slices(ipir_server* srv, size_t nq, uint32_t slices) {
 }
 
 int ipir_dbg_set_mfma_depth(ipir_server* srv, uint32_t depth, int form) {
-    if (!srv || form < 0 || form > 2) return 1;
+    if (!srv || form < 0 || form >= ipx::kMfmaForms) return 1;
     // a depth the 32-query kernel of this form is built with
-    if (ipx::dbplan_mfma(srv->ctx->pp.db_rows, srv->ctx->pp.db_cols, ipx::kMfmaMaxQueries, 1, depth, form == 1,
-                         form == 2 ? 8 : 4)
-            .depth != depth)
+    if (ipx::dbplan_form(srv->ctx->pp.db_rows, srv->ctx->pp.db_cols, ipx::kMfmaMaxQueries, 1, depth, form).depth !=
+        depth)
         return 1;
     for (uint32_t d : ipx::kMfmaDepths)
         if (d == depth) {
             std::lock_guard<std::mutex> lk(srv->ctx->mu);
             srv->ctx->mfma_depth = depth;
-            srv->ctx->mfma_dbtile = form == 1;
-            srv->ctx->mfma_waves = form == 2 ? 8 : 4;
+            srv->ctx->mfma_form = form;
             return 0;
         }
     return 1;

@@ -49,15 +49,20 @@ hipError_t launch_compress(const uint64_t* resp, size_t nct, uint32_t* scratch, 
 // slice's rows of db[r][c]*b_{q,limb}[r] mod q_limb.
 struct DbPlan {
     enum Kind { F64, MFMA } kind;
-    uint32_t param;      // F64: 16-row blocks per iteration (2, 4, 8); MFMA: 16-row digit tiles (1, 2, 4, 8, 16)
+    uint32_t param;      // F64: 16-row blocks per iteration (2, 4, 8); MFMA: 16-row digit tiles (1, 2, 4, 8, 14, 16)
     uint32_t slices;     // row slices (power of two)
     uint32_t depth = 3;  // 8 or 16 digit tiles: register-ring slots of DB operands (kMfmaDepths)
     bool dbtile = false; // 16 digit tiles: the waves share the DB tile (k_dbpass_mfma_bs), not the digit tile
     uint32_t waves = 4;  // 8 or 16 digit tiles, shared digits: waves per block (4, or 8 = one block per CU)
+    bool crt = false;    // 17..32 queries, shared digits: 7 digit rows per query (CRT value), 14 tiles
 };
+// Forms of the 9..32-query pass: 0 shared digits in 4-wave blocks, 1 shared
+// DB tile, 2 shared digits in 8-wave blocks, 3 and 4 the 8- and 4-wave
+// shared-digit forms with 7 digit rows per query (17..32 queries).
+constexpr int kMfmaForms = 5;
 // Ring depths of the 9..32-query MFMA passes; plans cap them at what each
-// kernel is built with (7 for 16 shared-digit tiles in 8-wave blocks, 5 in
-// 4-wave blocks, 4 otherwise).
+// kernel is built with (7 for 14 or 16 shared-digit tiles in 8-wave blocks,
+// 5 in 4-wave blocks, 4 otherwise).
 constexpr uint32_t kMfmaDepths[] = {3, 4, 5, 6, 7};
 // Row-major staging rows (16*nkb of them) -> blocks [kb0, kb0 + nkb) of the planes.
 hipError_t launch_to_planes(const uint16_t* src, uint8_t* dst, size_t C, size_t kb0, size_t nkb, hipStream_t s);
@@ -67,7 +72,11 @@ DbPlan dbplan_f64(size_t R, size_t ncols, int unroll, size_t target_blocks);
 constexpr size_t kMfmaMaxQueries = 32;
 // int8-MFMA plan for nq (1..kMfmaMaxQueries) queries.
 DbPlan dbplan_mfma(size_t R, size_t ncols, size_t nq, size_t target_blocks, uint32_t depth = 4,
-                   bool dbtile = false, uint32_t waves = 4);
+                   bool dbtile = false, uint32_t waves = 4, bool crt = false);
+// The same for a form (0 .. kMfmaForms-1) of the 9..32-query pass.
+DbPlan dbplan_form(size_t R, size_t ncols, size_t nq, size_t target_blocks, uint32_t depth, int form);
+// Queries an MFMA plan's digit tiles hold.
+size_t dbplan_max_queries(const DbPlan& p);
 // Digit rows the MFMA plan stages per 16-row block (int8 scratch: R*Mp bytes, Mp <= 256).
 size_t dbplan_mp(const DbPlan& p);
 // Column granularity of a plan (one block's columns): col0 and ncols are multiples of it.

@@ -154,6 +154,75 @@ __global__ __launch_bounds__(256) void k_digits(const double* __restrict__ bq, u
     for (int k = 0; k < 4; k++) dst[k] = o[k];
 }
 
+// 7-digit rows (17..32 queries): both limbs of a query share one digit
+// expansion, that of the CRT value B_r in [0, Q0*Q1) with B_r = b_{q,c}[r]
+// mod q_c. sum_r v_r B_r is congruent to either limb's sum mod q_c, and
+// Q0*Q1 < 2^53 needs 6 balanced digits in [-128, 127] and a top digit in
+// [0, 32]: 7 rows per query instead of 2 x 4. Rows m = 7q + k.
+constexpr uint32_t pow_mod_c(uint64_t b, uint64_t e, uint64_t m) {
+    uint64_t r = 1;
+    b %= m;
+    for (; e; e >>= 1, b = b * b % m)
+        if (e & 1) r = r * b % m;
+    return (uint32_t)r;
+}
+constexpr uint32_t kQ1InvQ0 = pow_mod_c(Q1, Q0 - 2, Q0);  // Q0 is prime
+
+__global__ __launch_bounds__(256) void k_digits7(const double* __restrict__ bq, uint32_t R, uint32_t nq,
+                                                 uint32_t Mp, int8_t* __restrict__ dig) {
+    const uint32_t nkb = R / 16;
+    const uint32_t x = blockIdx.x * 256 + threadIdx.x;
+    if (x >= nkb * (Mp / 7)) return;
+    const uint32_t q = x / nkb, kb = x % nkb;
+    u32x4 o[7];
+#pragma unroll
+    for (int k = 0; k < 7; k++) o[k] = u32x4{0, 0, 0, 0};
+    if (q < nq) {
+        const double* b0 = bq + (size_t)q * 2 * R + (size_t)kb * 16;  // bq [q][c][R]
+        const double* b1 = b0 + R;
+#pragma unroll
+        for (int e = 0; e < 16; e++) {
+            const uint32_t r0 = (uint32_t)b0[e], r1 = (uint32_t)b1[e];
+            // Garner: B = r1 + Q1 * ((r0 - r1) * Q1^-1 mod Q0)
+            const uint32_t t = mul_mod(sub_mod(r0, r1 % Q0, Q0), kQ1InvQ0, 0);
+            int64_t v = (int64_t)r1 + (int64_t)Q1 * t;
+#pragma unroll
+            for (int k = 0; k < 7; k++) {
+                int64_t d = v;
+                if (k < 6) {
+                    d = ((v & 255) ^ 128) - 128;
+                    v = (v - d) >> 8;
+                }
+                o[k][e >> 2] |= ((uint32_t)d & 255u) << (8 * (e & 3));
+            }
+        }
+    }
+    u32x4* dst = reinterpret_cast<u32x4*>(dig + ((size_t)kb * Mp + (size_t)q * 7) * 16);
+#pragma unroll
+    for (int k = 0; k < 7; k++) dst[k] = o[k];
+}
+
+// Residue mod q_c of one output of the 7-digit form from the dot products
+// t_k = 256*H_k + L_k (|t_k| < 2^38.01, at most 2^16 rows per slice):
+//   sum_k 256^k t_k + 32896*S_c.
+// The terms k < 3 are summed exactly (< 2^54.02 in magnitude, lifted by a
+// multiple of q_c); the others are reduced first and multiplied by
+// 256^k mod q_c (each product < 2^54); the sum stays below 2^57.1.
+__device__ __forceinline__ uint32_t crt7_residue(const int64_t (&t)[7], uint32_t Sc, int c) {
+    const uint64_t qc = c ? Q1 : Q0;
+    constexpr uint32_t w0[4] = {pow_mod_c(256, 3, Q0), pow_mod_c(256, 4, Q0), pow_mod_c(256, 5, Q0),
+                                pow_mod_c(256, 6, Q0)};
+    constexpr uint32_t w1[4] = {pow_mod_c(256, 3, Q1), pow_mod_c(256, 4, Q1), pow_mod_c(256, 5, Q1),
+                                pow_mod_c(256, 6, Q1)};
+    const int64_t lo = t[0] + t[1] * 256 + t[2] * 65536;
+    uint64_t s = (uint64_t)(lo + (int64_t)(qc * ((1ull << 55) / qc))) + (uint64_t)32896 * Sc;
+    const int64_t off40 = (int64_t)(qc * ((1ull << 40) / qc));
+#pragma unroll
+    for (int k = 3; k < 7; k++)
+        s += (uint64_t)red64((uint64_t)(t[k] + off40), c) * (c ? w1[k - 3] : w0[k - 3]);
+    return red64(s, c);
+}
+
 // S[pr] = sum_r b_pr[r] mod q_c, the DB-offset correction. One block per
 // pair; 8 doubles per thread per pass.
 __global__ __launch_bounds__(256) void k_bsum(const double* __restrict__ bq, uint32_t R, uint32_t* __restrict__ S) {
@@ -270,6 +339,12 @@ __global__ __launch_bounds__(256, 2) void k_dbpass_mfma(const uint8_t* __restric
 // with one lead for all loads each wait covers exactly one step's loads. Past
 // the slice end the loads repeat the last step (in bounds, never used), so
 // every step issues the same instructions.
+//
+// MT = 14 is the 7-digit form (rows 7q + k, 32 queries in 224 rows): a
+// query's digits straddle the lanes' 4-row output groups, so the block
+// regroups its accumulators through LDS (the digit ring's memory, free after
+// the last step), half of the queries at a time, and each lane recombines
+// whole queries.
 template <int MT, int NT, int OCC, int DEPTH, int WAVES = 4>
 __global__ __launch_bounds__(64 * WAVES, OCC) void k_dbpass_mfma_sa(const uint8_t* __restrict__ db,
                                                            const int8_t* __restrict__ dig,
@@ -279,10 +354,18 @@ __global__ __launch_bounds__(64 * WAVES, OCC) void k_dbpass_mfma_sa(const uint8_
     static_assert(DEPTH >= 3, "the digit ring needs two slots");
     constexpr int L = DEPTH - 1;            // lead of every load, in steps
     constexpr int P = DEPTH * L;            // period of both rings' slot pattern
+    constexpr bool CRT7 = MT == 14;
+    static_assert(!CRT7 || NT == 1, "the 7-digit epilogue covers one column tile per wave");
     constexpr uint32_t Mp = 16 * MT;
-    constexpr uint32_t TH = 64 * WAVES;    // threads per block
-    constexpr uint32_t AU = Mp * 4 / TH;   // 16-byte units of the step's digit tile per thread
-    __shared__ i32x4 atile[2][4 * Mp];      // [buffer][block of the step][row]
+    constexpr uint32_t TH = 64 * WAVES;              // threads per block
+    constexpr uint32_t AU = (Mp * 4 + TH - 1) / TH;  // 16-byte units of the step's digit tile per thread
+    constexpr bool APART = AU * TH != Mp * 4;        // the last unit exists for some threads only
+    // 7-digit epilogue: per wave E[plane][col][row] int32, 112 rows (16
+    // queries) of a half, rows padded to 116 words (spreads the column reads over banks)
+    constexpr uint32_t EPS = 116, EPW = 2 * 16 * EPS;
+    constexpr uint32_t AT_UNITS = 2 * 4 * Mp, EP_UNITS = CRT7 ? WAVES * EPW / 4 : 0;
+    __shared__ i32x4 smem[AT_UNITS > EP_UNITS ? AT_UNITS : EP_UNITS];
+    i32x4(*atile)[4 * Mp] = reinterpret_cast<i32x4(*)[4 * Mp]>(smem);  // [buffer][block of the step][row]
     const uint32_t tid = threadIdx.x, w = tid >> 6, l = tid & 63;
     const uint32_t slice = blockIdx.y;
     const uint32_t col0 = (blockIdx.x * WAVES + w) * NT * 16;
@@ -291,7 +374,7 @@ __global__ __launch_bounds__(64 * WAVES, OCC) void k_dbpass_mfma_sa(const uint8_
     const i32x4* bsrc = reinterpret_cast<const i32x4*>(db) + (size_t)(kb0 + (l >> 4)) * 2 * C + col0 + (l & 15);
     const size_t bstep = (size_t)8 * C;
     // the step's digit tile is 4*Mp contiguous 16-byte units starting at block 4*ks
-    const i32x4* atile_src = reinterpret_cast<const i32x4*>(dig) + (size_t)kb0 * Mp + tid;
+    const i32x4* atile_src = reinterpret_cast<const i32x4*>(dig) + (size_t)kb0 * Mp;
     const i32x4 flip = {(int32_t)0x80808080, (int32_t)0x80808080, (int32_t)0x80808080, (int32_t)0x80808080};
     i32x4 acc[MT][NT][2];
 #pragma unroll
@@ -303,7 +386,11 @@ __global__ __launch_bounds__(64 * WAVES, OCC) void k_dbpass_mfma_sa(const uint8_
     auto issue = [&](int bs, int as, uint32_t step) {
         const uint32_t k = step < nks ? step : nks - 1;
 #pragma unroll
-        for (int u = 0; u < (int)AU; u++) ar[as][u] = atile_src[(size_t)k * 4 * Mp + TH * u];
+        for (int u = 0; u < (int)AU; u++) {
+            // a thread past the tile's end reloads its last unit (in bounds, not stored)
+            const uint32_t x = !APART || tid + TH * u < 4 * Mp ? tid + TH * u : 4 * Mp - 1;
+            ar[as][u] = atile_src[(size_t)k * 4 * Mp + x];
+        }
 #pragma unroll
         for (int n = 0; n < NT; n++) {
             v[bs][n][0] = __builtin_nontemporal_load(bsrc + k * bstep + n * 16);
@@ -312,7 +399,8 @@ __global__ __launch_bounds__(64 * WAVES, OCC) void k_dbpass_mfma_sa(const uint8_
     };
     auto store_a = [&](int buf, int as) {
 #pragma unroll
-        for (int u = 0; u < (int)AU; u++) atile[buf][tid + TH * u] = ar[as][u];
+        for (int u = 0; u < (int)AU; u++)
+            if (!APART || tid + TH * u < 4 * Mp) atile[buf][tid + TH * u] = ar[as][u];
     };
     auto compute = [&](int bs, int buf) {
 #pragma unroll
@@ -360,12 +448,43 @@ __global__ __launch_bounds__(64 * WAVES, OCC) void k_dbpass_mfma_sa(const uint8_
             __syncthreads();
         }
     }
+    if constexpr (CRT7) {
+        // after the last barrier no wave reads the digit ring again
+        int32_t* ep = reinterpret_cast<int32_t*>(smem) + w * EPW;
+        const uint32_t g = l >> 4, cl = l & 15;
 #pragma unroll
-    for (int a = 0; a < MT; a++)
+        for (int h = 0; h < 2; h++) {
+            // tiles 7h..7h+6 = rows 112h.. = queries 16h..16h+15; lane (g, cl)
+            // holds rows 16a + 4g + i of column cl
 #pragma unroll
-        for (int n = 0; n < NT; n++)
-            store_pair(acc[a][n][0], acc[a][n][1], (uint32_t)a * 4 + (l >> 4), nq, slice == 0, S, out,
-                       out_qstride, slice, OC, col0 + n * 16 + (l & 15));
+            for (int a = 0; a < 7; a++)
+#pragma unroll
+                for (int pl = 0; pl < 2; pl++)
+                    *reinterpret_cast<i32x4*>(ep + (pl * 16 + cl) * EPS + 16 * a + 4 * g) = acc[7 * h + a][0][pl];
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const uint32_t ql = g + 4 * j, q = 16 * h + ql;
+                int64_t t[7];
+#pragma unroll
+                for (int k = 0; k < 7; k++)
+                    t[k] = (int64_t)ep[(16 + cl) * EPS + 7 * ql + k] * 256 + ep[cl * EPS + 7 * ql + k];
+                if (q < nq) {
+                    uint32_t* o = out + (size_t)q * out_qstride + (size_t)slice * 2 * OC + col0 + cl;
+#pragma unroll
+                    for (int c = 0; c < 2; c++) o[(size_t)c * OC] = crt7_residue(t, slice == 0 ? S[2 * q + c] : 0, c);
+                }
+            }
+            __syncthreads();  // the second half reuses the region
+        }
+    } else {
+#pragma unroll
+        for (int a = 0; a < MT; a++)
+#pragma unroll
+            for (int n = 0; n < NT; n++)
+                store_pair(acc[a][n][0], acc[a][n][1], (uint32_t)a * 4 + (l >> 4), nq, slice == 0, S, out,
+                           out_qstride, slice, OC, col0 + n * 16 + (l & 15));
+    }
 }
 
 // 17..32 queries with the DB tile shared instead: each of the block's 4
@@ -497,15 +616,17 @@ DbPlan dbplan_f64(size_t R, size_t ncols, int unroll, size_t target_blocks) {
 }
 
 DbPlan dbplan_mfma(size_t R, size_t ncols, size_t nq, size_t target_blocks, uint32_t depth, bool dbtile,
-                   uint32_t waves) {
+                   uint32_t waves, bool crt) {
     DbPlan p;
     p.kind = DbPlan::MFMA;
     p.param = nq <= 2 ? 1 : nq <= 4 ? 2 : nq <= 8 ? 4 : nq <= 16 ? 8 : 16;  // 2 queries per 16-row digit tile
     p.dbtile = dbtile && p.param == 16;
+    p.crt = crt && !p.dbtile && p.param == 16;
+    if (p.crt) p.param = 14;  // 7 rows per query: 32 queries in 14 tiles
     p.waves = p.param >= 8 && !p.dbtile && waves == 8 ? 8 : 4;
-    // ring depths the kernels are built with (what fits 256 VGPRs): 16 tiles
-    // with shared digits 3..7 (8-wave blocks) or 3..5, otherwise 3..4
-    const uint32_t maxd = p.param == 16 && !p.dbtile ? (p.waves == 8 ? 7 : 5) : 4;
+    // ring depths the kernels are built with (what fits 256 VGPRs): 14 or 16
+    // tiles with shared digits 3..7 (8-wave blocks) or 3..5, otherwise 3..4
+    const uint32_t maxd = p.param >= 14 && !p.dbtile ? (p.waves == 8 ? 7 : 5) : 4;
     p.depth = p.param >= 8 ? (depth < 3 ? 3 : depth > maxd ? maxd : depth) : 3;
     const size_t cols = dbplan_cols(p);
     // slices: whole number of 128-row (8-block) chunks so the step count is even
@@ -516,7 +637,16 @@ DbPlan dbplan_mfma(size_t R, size_t ncols, size_t nq, size_t target_blocks, uint
     return p;
 }
 
+DbPlan dbplan_form(size_t R, size_t ncols, size_t nq, size_t target_blocks, uint32_t depth, int form) {
+    return dbplan_mfma(R, ncols, nq, target_blocks, depth, form == 1, form == 2 || form == 3 ? 8 : 4,
+                       form == 3 || form == 4);
+}
+
 size_t dbplan_mp(const DbPlan& p) { return p.kind == DbPlan::MFMA ? 16 * p.param : 0; }
+
+size_t dbplan_max_queries(const DbPlan& p) {
+    return p.kind != DbPlan::MFMA ? 1 : p.crt ? dbplan_mp(p) / 7 : dbplan_mp(p) / 8;
+}
 
 size_t dbplan_cols(const DbPlan& p) { return p.kind == DbPlan::MFMA ? 16 * mfma_nt(p.param) * p.waves : 256; }
 
@@ -550,10 +680,10 @@ hipError_t launch_mfma_digits(const double* bq, size_t nq, size_t R, const DbPla
                               hipStream_t s) {
     if (p.kind != DbPlan::MFMA || nq == 0 || nq > kMfmaMaxQueries) return hipErrorInvalidValue;
     const uint32_t Mp = (uint32_t)dbplan_mp(p);
-    if (2 * nq > Mp / 4) return hipErrorInvalidValue;
-    const size_t nthr = (R / 16) * (Mp / 4);
-    hipLaunchKernelGGL(k_digits, dim3((unsigned)((nthr + 255) / 256)), dim3(256), 0, s, bq, (uint32_t)R,
-                       (uint32_t)nq, Mp, dig);
+    if (nq > dbplan_max_queries(p)) return hipErrorInvalidValue;
+    const size_t nthr = (R / 16) * (p.crt ? Mp / 7 : Mp / 4);
+    hipLaunchKernelGGL(p.crt ? k_digits7 : k_digits, dim3((unsigned)((nthr + 255) / 256)), dim3(256), 0, s, bq,
+                       (uint32_t)R, (uint32_t)nq, Mp, dig);
     hipLaunchKernelGGL(k_bsum, dim3((unsigned)(2 * nq)), dim3(256), 0, s, bq, (uint32_t)R, S);
     return hipGetLastError();
 }
@@ -565,7 +695,8 @@ hipError_t launch_dbpass_mfma(const uint8_t* db, const int8_t* dig, const uint32
     const uint32_t Mp = (uint32_t)dbplan_mp(p);
     const uint32_t kbps = (uint32_t)(R / 16) / p.slices;
     const size_t cols = dbplan_cols(p);
-    if (2 * nq > Mp / 4 || kbps % 8 || ncols % cols || col0 % cols || col0 + ncols > C || ostride < ncols)
+    if (nq > dbplan_max_queries(p) || (p.crt && p.param != 14) || kbps % 8 || ncols % cols || col0 % cols ||
+        col0 + ncols > C || ostride < ncols)
         return hipErrorInvalidValue;
     const uint32_t Cc = (uint32_t)C, OC = (uint32_t)ostride, Q = (uint32_t)nq;
     const dim3 grid((unsigned)(ncols / cols), p.slices);
@@ -584,6 +715,19 @@ hipError_t launch_dbpass_mfma(const uint8_t* db, const int8_t* dig, const uint32
                 case 44: MFMA(k_dbpass_mfma_sa, 8, 2, 2, 4); break;
                 case 38: MFMA8(k_dbpass_mfma_sa, 8, 2, 2, 3); break;
                 case 48: MFMA8(k_dbpass_mfma_sa, 8, 2, 2, 4); break;
+                default: return hipErrorInvalidValue;
+            }
+            break;
+        case 14:
+            switch (p.depth * 10 + p.waves) {
+                case 34: MFMA(k_dbpass_mfma_sa, 14, 1, 2, 3); break;
+                case 44: MFMA(k_dbpass_mfma_sa, 14, 1, 2, 4); break;
+                case 54: MFMA(k_dbpass_mfma_sa, 14, 1, 2, 5); break;
+                case 38: MFMA8(k_dbpass_mfma_sa, 14, 1, 2, 3); break;
+                case 48: MFMA8(k_dbpass_mfma_sa, 14, 1, 2, 4); break;
+                case 58: MFMA8(k_dbpass_mfma_sa, 14, 1, 2, 5); break;
+                case 68: MFMA8(k_dbpass_mfma_sa, 14, 1, 2, 6); break;
+                case 78: MFMA8(k_dbpass_mfma_sa, 14, 1, 2, 7); break;
                 default: return hipErrorInvalidValue;
             }
             break;

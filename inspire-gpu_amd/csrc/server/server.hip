@@ -263,7 +263,7 @@ DbPlan narrow_plan(const DbPlan& full, size_t target, size_t R, size_t C, size_t
     if (wgs > kChunkWgs) wgs = kChunkWgs;
     const size_t t = wgs > target ? wgs : target;
     return full.kind == DbPlan::F64 ? dbplan_f64(R, ncols, (int)full.param, t)
-                                     : dbplan_mfma(R, ncols, nq, t, full.depth, full.dbtile, full.waves);
+                                     : dbplan_mfma(R, ncols, nq, t, full.depth, full.dbtile, full.waves, full.crt);
 }
 }  // namespace
 
@@ -369,16 +369,16 @@ void GpuServerCtx::autotune_dbpass() {
         const long v = std::atol(t);
         if (v > 0) mfma_target = (size_t)v;
     }
-    // INSPIRE_MFMA_DEPTH=<depth>[,dbtile|,w8] forces the batched passes'
-    // ring depth and the form: shared digits in 4-wave blocks (default),
-    // the shared DB tile (17..32 queries), or shared digits in 8-wave blocks
+    // INSPIRE_MFMA_DEPTH=<depth>[,<form>] forces the batched passes' ring
+    // depth and form (dbplan_form; 0, shared digits in 4-wave blocks, by default)
     bool depth_forced = false;
     if (const char* t = std::getenv("INSPIRE_MFMA_DEPTH")) {
         const long v = std::atol(t);
         for (uint32_t d : kMfmaDepths)
             if (v == (long)d) { mfma_depth = d; depth_forced = true; }
-        mfma_dbtile = depth_forced && std::strstr(t, "dbtile") != nullptr;
-        mfma_waves = depth_forced && std::strstr(t, "w8") != nullptr ? 8 : 4;
+        const char* f = std::strchr(t, ',');
+        const long form = f ? std::atol(f + 1) : 0;
+        mfma_form = depth_forced && form >= 0 && form < kMfmaForms ? (int)form : 0;
     }
     bool plan1_forced = false;
     if (const char* force = std::getenv("INSPIRE_DBPASS")) {
@@ -428,23 +428,23 @@ void GpuServerCtx::autotune_dbpass() {
     if (nq > 1) {
         best = 1e30f;
         const uint32_t param = dbplan_mfma(R, C, nq, mfma_target).param;
-        // forms: 0 shared digits (4-wave blocks), 1 shared DB tile, 2 shared digits (8-wave blocks)
-        const int forced_form = mfma_dbtile ? 1 : mfma_waves == 8 ? 2 : 0;
+        // forms (dbplan_form): the shared DB tile and the 7-digit forms exist for 17..32 queries only
+        const int forced_form = mfma_form;
         for (size_t t : kMfmaTargets)
-            for (int form = 0; form < 3; form++) {
-                if (depth_forced ? form != forced_form : (form == 1 && param != 16) || (form > 0 && param < 8))
+            for (int form = 0; form < kMfmaForms; form++) {
+                if (depth_forced ? form != forced_form
+                                 : (form != 0 && form != 2 && param != 16) || (form > 0 && param < 8))
                     continue;
                 for (uint32_t d : kMfmaDepths) {
                     if ((depth_forced || param < 8) && d != mfma_depth) continue;
-                    const DbPlan x = dbplan_mfma(R, C, nq, t, d, form == 1, form == 2 ? 8 : 4);
+                    const DbPlan x = dbplan_form(R, C, nq, t, d, form);
                     if (x.depth != d && param >= 8) continue;  // a depth this kernel is not built with
                     const float ms = time_plan(x, nq);
                     if (ms < best) {
                         best = ms;
                         mfma_target = t;
                         mfma_depth = d;
-                        mfma_dbtile = form == 1;
-                        mfma_waves = form == 2 ? 8 : 4;
+                        mfma_form = form;
                     }
                 }
             }
@@ -481,7 +481,7 @@ void GpuServerCtx::autotune_dbpass() {
             uint32_t pick = 0;
             for (uint32_t sl = 0; sl <= 16; sl = sl ? 2 * sl : 1) {  // 0: the narrowing rule
                 if (sl) {
-                    const DbPlan x = dbplan_mfma(R, ccols, nq, blocks * sl, mfma_depth, mfma_dbtile, mfma_waves);
+                    const DbPlan x = dbplan_form(R, ccols, nq, blocks * sl, mfma_depth, mfma_form);
                     if (x.slices != sl || (sl > 1 && sl * 2 * ccols > part_stride)) continue;
                 }
                 chunk_slices = sl;
@@ -517,14 +517,14 @@ DbPlan GpuServerCtx::plan_for(size_t nq, size_t ncols) const {
     if (nq == 1)
         return plan1.kind == DbPlan::F64 ? dbplan_f64(R, ncols, (int)plan1.param, plan1_target)
                                          : dbplan_mfma(R, ncols, 1, plan1_target);
-    return dbplan_mfma(R, ncols, nq, mfma_target, mfma_depth, mfma_dbtile, mfma_waves);
+    return dbplan_form(R, ncols, nq, mfma_target, mfma_depth, mfma_form);
 }
 
 DbPlan GpuServerCtx::chunk_plan(size_t nq, size_t ncols) const {
     const DbPlan full = plan_for(nq, pp.db_cols);
     if (nq > 1 && chunk_slices && full.kind == DbPlan::MFMA && full.param == chunk_param) {
-        const DbPlan x = dbplan_mfma(pp.db_rows, ncols, nq, ncols / dbplan_cols(full) * chunk_slices, mfma_depth,
-                                     mfma_dbtile, mfma_waves);
+        const DbPlan x =
+            dbplan_form(pp.db_rows, ncols, nq, ncols / dbplan_cols(full) * chunk_slices, mfma_depth, mfma_form);
         if (x.slices == 1 || x.slices * 2 * ncols <= part_stride) return x;
     }
     return narrow_plan(full, nq == 1 ? plan1_target : mfma_target, pp.db_rows, pp.db_cols, ncols, nq);
@@ -566,7 +566,7 @@ void GpuServerCtx::stage_dbpass(size_t count, hipStream_t s, size_t slot0, int l
         DbPlan p = plan_for(nq, C);
         if (force == 0 && p.kind != DbPlan::F64) p = dbplan_f64(R, C, 4, 2048);
         if (force == 1 && p.kind == DbPlan::F64)
-            p = dbplan_mfma(R, C, nq, mfma_target, mfma_depth, mfma_dbtile, mfma_waves);
+            p = dbplan_form(R, C, nq, mfma_target, mfma_depth, mfma_form);
         dbpass_prep(q0, nq, p, s, lane);
         dbpass_range(q0, nq, 0, G, p, s, lane);
     }

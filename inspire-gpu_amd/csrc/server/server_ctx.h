@@ -37,8 +37,7 @@ struct GpuServerCtx {
     size_t plan1_target = 2048;     // its workgroup target
     size_t mfma_target = 1024;      // workgroups the batched pass aims for
     uint32_t mfma_depth = 4;        // register-ring depth of the 9..32-query passes
-    bool mfma_dbtile = false;       // 17..32 queries: waves share the DB tile instead of the digit tile
-    uint32_t mfma_waves = 4;        // shared-digit passes: waves per block (4 or 8)
+    int mfma_form = 0;              // 9..32 queries: form of the pass (dbplan_form)
     // row slices of the batched chunk plan, timed at setup for plans with
     // chunk_param digit tiles (0: the narrowing rule)
     uint32_t chunk_slices = 0, chunk_param = 0;

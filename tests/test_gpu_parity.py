@@ -138,7 +138,7 @@ def test_noncanonical_inputs_are_reduced(served):
 
 
 # ---------------------------------------------------------------- batch contract
-MB8 = 24  # max_batch of the medium8 server (covers the 17..32-query MFMA variant)
+MB8 = 32  # max_batch of the medium8 server (covers the 17..32-query MFMA passes at full width)
 
 
 @pytest.fixture(scope="module")
@@ -163,7 +163,7 @@ def test_db_pass_paths_match_oracle(medium8, path):
     assert srv.lib.ipir_dbg_db_rows(srv.h, 0, R, ia.ptr(enc)) == 0
     flat = enc.reshape(-1)
     rng = np.random.default_rng(100 + path)
-    for nq in (1, 2, 3, 5, 8, 10, 17, 24):
+    for nq in (1, 2, 3, 5, 8, 10, 17, 24, 32):
         b = np.zeros((nq, 2, R), np.uint64)
         b[:, 0] = rng.integers(0, Q0, (nq, R), dtype=np.uint64)
         b[:, 1] = rng.integers(0, Q1, (nq, R), dtype=np.uint64)
@@ -181,12 +181,14 @@ def test_db_pass_paths_match_oracle(medium8, path):
     assert srv.lib.ipir_dbg_db_pass(srv.h, ia.ptr(bad), 1, path, ia.ptr(np.zeros((1, 2, Cc), np.uint32))) == 1
 
 
-@pytest.mark.parametrize("depth,form", [(3, 0), (4, 0), (5, 0), (3, 1), (4, 1), (3, 2), (5, 2), (6, 2), (7, 2)])
+@pytest.mark.parametrize("depth,form", [(3, 0), (4, 0), (5, 0), (3, 1), (4, 1), (3, 2), (5, 2), (6, 2), (7, 2),
+                                        (3, 3), (4, 3), (5, 3), (6, 3), (7, 3), (3, 4), (4, 4), (5, 4)])
 def test_many_query_ring_depths_match_oracle(medium8, depth, form):
     """Every register-ring depth and form of the batched MFMA passes against
-    the oracle's mat-vec (12 queries: 8 digit tiles, depth capped at 4; 17
-    and 24 queries: 16 tiles; the waves of 4- or 8-wave blocks sharing the
-    digit tile, or the DB tile)."""
+    the oracle's mat-vec (12 queries: 8 digit tiles, depth capped at 4; 17,
+    24 and 32 queries: 16 tiles, or 14 in the forms with 7 digit rows per
+    query; the waves of 4- or 8-wave blocks sharing the digit tile, or the
+    DB tile)."""
     geo, db, srv = medium8
     R, Cc = geo["R"], geo["C"]
     enc = np.zeros((R, Cc), np.uint16)
@@ -195,17 +197,22 @@ def test_many_query_ring_depths_match_oracle(medium8, depth, form):
     rng = np.random.default_rng(400 + depth)
     name = C.create_string_buffer(128)
     assert srv.lib.ipir_dbg_dbpass_kernel(srv.h, MB8, name, 128) == 0
-    kname = name.value.decode()  # k_dbpass_mfma_sa<16, 1, 2, depth[, 8]> or k_dbpass_mfma_bs<16, 4, depth>
+    kname = name.value.decode()  # k_dbpass_mfma_sa<16|14, 1, 2, depth[, 8]> or k_dbpass_mfma_bs<16, 4, depth>
     args = [int(x) for x in kname.split("<")[1].split(">")[0].split(",")]
-    tuned = (args[2], 1) if "_bs<" in kname else (args[3], 2 if len(args) == 5 else 0)
+    if "_bs<" in kname:
+        tuned = (args[2], 1)
+    else:
+        tuned = (args[3], (3 if args[0] == 14 else 2) if len(args) == 5 else (4 if args[0] == 14 else 0))
     assert srv.lib.ipir_dbg_set_mfma_depth(srv.h, 2, 0) == 1
     assert srv.lib.ipir_dbg_set_mfma_depth(srv.h, 6, 0) == 1
     assert srv.lib.ipir_dbg_set_mfma_depth(srv.h, 5, 1) == 1
     assert srv.lib.ipir_dbg_set_mfma_depth(srv.h, 8, 2) == 1
-    assert srv.lib.ipir_dbg_set_mfma_depth(srv.h, 3, 3) == 1
+    assert srv.lib.ipir_dbg_set_mfma_depth(srv.h, 8, 3) == 1
+    assert srv.lib.ipir_dbg_set_mfma_depth(srv.h, 6, 4) == 1
+    assert srv.lib.ipir_dbg_set_mfma_depth(srv.h, 3, 5) == 1
     assert srv.lib.ipir_dbg_set_mfma_depth(srv.h, depth, form) == 0
     try:
-        for nq in (12, 17, MB8):
+        for nq in (12, 17, 24, MB8):
             _ring_case(srv, flat, R, Cc, rng, nq, depth)
     finally:
         assert srv.lib.ipir_dbg_set_mfma_depth(srv.h, *tuned) == 0
