@@ -55,11 +55,13 @@ struct DbPlan {
     bool dbtile = false; // 16 digit tiles: the waves share the DB tile (k_dbpass_mfma_bs), not the digit tile
     uint32_t waves = 4;  // 8 or 16 digit tiles, shared digits: waves per block (4, or 8 = one block per CU)
     bool crt = false;    // 17..32 queries, shared digits: 7 digit rows per query (CRT value), 14 tiles
+    bool pow = false;    // crt digits accumulated by power of 256 (8-wave blocks, two column tiles per wave)
 };
 // Forms of the 9..32-query pass: 0 shared digits in 4-wave blocks, 1 shared
 // DB tile, 2 shared digits in 8-wave blocks, 3 and 4 the 8- and 4-wave
-// shared-digit forms with 7 digit rows per query (17..32 queries).
-constexpr int kMfmaForms = 5;
+// shared-digit forms with 7 digit rows per query, 5 those digits
+// accumulated by power (the last four for 17..32 queries).
+constexpr int kMfmaForms = 6;
 // Ring depths of the 9..32-query MFMA passes; plans cap them at what each
 // kernel is built with (7 for 14 or 16 shared-digit tiles in 8-wave blocks,
 // 5 in 4-wave blocks, 4 otherwise).
@@ -72,7 +74,7 @@ DbPlan dbplan_f64(size_t R, size_t ncols, int unroll, size_t target_blocks);
 constexpr size_t kMfmaMaxQueries = 32;
 // int8-MFMA plan for nq (1..kMfmaMaxQueries) queries.
 DbPlan dbplan_mfma(size_t R, size_t ncols, size_t nq, size_t target_blocks, uint32_t depth = 4,
-                   bool dbtile = false, uint32_t waves = 4, bool crt = false);
+                   bool dbtile = false, uint32_t waves = 4, bool crt = false, bool pow = false);
 // The same for a form (0 .. kMfmaForms-1) of the 9..32-query pass.
 DbPlan dbplan_form(size_t R, size_t ncols, size_t nq, size_t target_blocks, uint32_t depth, int form);
 // Queries an MFMA plan's digit tiles hold.

@@ -168,8 +168,9 @@ constexpr uint32_t pow_mod_c(uint64_t b, uint64_t e, uint64_t m) {
 }
 constexpr uint32_t kQ1InvQ0 = pow_mod_c(Q1, Q0 - 2, Q0);  // Q0 is prime
 
+// pmajor: rows 32k + q (the power-row form) instead of 7q + k.
 __global__ __launch_bounds__(256) void k_digits7(const double* __restrict__ bq, uint32_t R, uint32_t nq,
-                                                 uint32_t Mp, int8_t* __restrict__ dig) {
+                                                 uint32_t Mp, int8_t* __restrict__ dig, uint32_t pmajor) {
     const uint32_t nkb = R / 16;
     const uint32_t x = blockIdx.x * 256 + threadIdx.x;
     if (x >= nkb * (Mp / 7)) return;
@@ -197,9 +198,9 @@ __global__ __launch_bounds__(256) void k_digits7(const double* __restrict__ bq, 
             }
         }
     }
-    u32x4* dst = reinterpret_cast<u32x4*>(dig + ((size_t)kb * Mp + (size_t)q * 7) * 16);
+    u32x4* dst = reinterpret_cast<u32x4*>(dig + (size_t)kb * Mp * 16);
 #pragma unroll
-    for (int k = 0; k < 7; k++) dst[k] = o[k];
+    for (int k = 0; k < 7; k++) dst[pmajor ? 32 * k + q : 7 * q + k] = o[k];
 }
 
 // Residue mod q_c of one output of the 7-digit form from the dot products
@@ -220,6 +221,26 @@ __device__ __forceinline__ uint32_t crt7_residue(const int64_t (&t)[7], uint32_t
 #pragma unroll
     for (int k = 3; k < 7; k++)
         s += (uint64_t)red64((uint64_t)(t[k] + off40), c) * (c ? w1[k - 3] : w0[k - 3]);
+    return red64(s, c);
+}
+
+// Residue mod q_c of one output of the power-row form from p_0..p_7
+// (|p_k| <= 2^30): sum_k 256^k p_k + 32896*S_c. The terms k < 4 are summed
+// exactly (< 2^54.01 in magnitude, lifted by a multiple of q_c); the others
+// are reduced first and multiplied by 256^k mod q_c (each product < 2^54);
+// the sum stays below 2^57.
+__device__ __forceinline__ uint32_t pow_residue(const int32_t (&p)[8], uint32_t Sc, int c) {
+    const uint64_t qc = c ? Q1 : Q0;
+    constexpr uint32_t w0[4] = {pow_mod_c(256, 4, Q0), pow_mod_c(256, 5, Q0), pow_mod_c(256, 6, Q0),
+                                pow_mod_c(256, 7, Q0)};
+    constexpr uint32_t w1[4] = {pow_mod_c(256, 4, Q1), pow_mod_c(256, 5, Q1), pow_mod_c(256, 6, Q1),
+                                pow_mod_c(256, 7, Q1)};
+    const int64_t lo = (int64_t)p[0] + (int64_t)p[1] * 256 + (int64_t)p[2] * 65536 + (int64_t)p[3] * 16777216;
+    uint64_t s = (uint64_t)(lo + (int64_t)(qc * ((1ull << 55) / qc))) + (uint64_t)32896 * Sc;
+    const int64_t off32 = (int64_t)(qc * ((1ull << 32) / qc));
+#pragma unroll
+    for (int k = 4; k < 8; k++)
+        s += (uint64_t)red64((uint64_t)(p[k] + off32), c) * (c ? w1[k - 4] : w0[k - 4]);
     return red64(s, c);
 }
 
@@ -345,7 +366,20 @@ __global__ __launch_bounds__(256, 2) void k_dbpass_mfma(const uint8_t* __restric
 // regroups its accumulators through LDS (the digit ring's memory, free after
 // the last step), half of the queries at a time, and each lane recombines
 // whole queries.
-template <int MT, int NT, int OCC, int DEPTH, int WAVES = 4>
+//
+// POW = 1 (MT = 14, digit rows 32k + q: digit k of query q) accumulates by
+// power of 256 instead of by (digit, plane):
+//   sum_r v_r B_r = sum_{k=0..7} 256^k p_k + 32896 sum_r B_r,
+//   p_k = sum_r h'_r d_{k-1,r} + l'_r d_{k,r}   (d_{-1} = d_7 = 0),
+// so the high-plane MFMA of digit tile a adds into output tile a + 2 (the
+// next power of the same 16 queries) and the low-plane one into tile a:
+// 16 output tiles of one accumulator each instead of 14 of two. The freed
+// registers hold a second column tile (NT = 2), which halves the digit-tile
+// bytes loaded and read from LDS per DB byte. A lane's output rows are
+// powers k = a/2 of queries 16(a%2) + 4(l/16) + i: every lane recombines
+// its own queries from registers. |p_k| <= 2 * 2^14 per row: at most 2^15
+// rows per slice keep it within 2^30.
+template <int MT, int NT, int OCC, int DEPTH, int WAVES = 4, int POW = 0>
 __global__ __launch_bounds__(64 * WAVES, OCC) void k_dbpass_mfma_sa(const uint8_t* __restrict__ db,
                                                            const int8_t* __restrict__ dig,
                                                            const uint32_t* __restrict__ S,
@@ -354,8 +388,10 @@ __global__ __launch_bounds__(64 * WAVES, OCC) void k_dbpass_mfma_sa(const uint8_
     static_assert(DEPTH >= 3, "the digit ring needs two slots");
     constexpr int L = DEPTH - 1;            // lead of every load, in steps
     constexpr int P = DEPTH * L;            // period of both rings' slot pattern
-    constexpr bool CRT7 = MT == 14;
+    static_assert(!POW || MT == 14, "power rows come from the 7-digit expansion");
+    constexpr bool CRT7 = MT == 14 && !POW;
     static_assert(!CRT7 || NT == 1, "the 7-digit epilogue covers one column tile per wave");
+    constexpr int MA = POW ? MT + 2 : MT, PL = POW ? 1 : 2;  // output tiles, accumulators per tile
     constexpr uint32_t Mp = 16 * MT;
     constexpr uint32_t TH = 64 * WAVES;              // threads per block
     constexpr uint32_t AU = (Mp * 4 + TH - 1) / TH;  // 16-byte units of the step's digit tile per thread
@@ -376,11 +412,13 @@ __global__ __launch_bounds__(64 * WAVES, OCC) void k_dbpass_mfma_sa(const uint8_
     // the step's digit tile is 4*Mp contiguous 16-byte units starting at block 4*ks
     const i32x4* atile_src = reinterpret_cast<const i32x4*>(dig) + (size_t)kb0 * Mp;
     const i32x4 flip = {(int32_t)0x80808080, (int32_t)0x80808080, (int32_t)0x80808080, (int32_t)0x80808080};
-    i32x4 acc[MT][NT][2];
+    i32x4 acc[MA][NT][PL];
 #pragma unroll
-    for (int a = 0; a < MT; a++)
+    for (int a = 0; a < MA; a++)
 #pragma unroll
-        for (int n = 0; n < NT; n++) acc[a][n][0] = acc[a][n][1] = i32x4{0, 0, 0, 0};
+        for (int n = 0; n < NT; n++)
+#pragma unroll
+            for (int pl = 0; pl < PL; pl++) acc[a][n][pl] = i32x4{0, 0, 0, 0};
     i32x4 v[DEPTH][NT][2];
     i32x4 ar[L][AU];
     auto issue = [&](int bs, int as, uint32_t step) {
@@ -402,7 +440,40 @@ __global__ __launch_bounds__(64 * WAVES, OCC) void k_dbpass_mfma_sa(const uint8_
         for (int u = 0; u < (int)AU; u++)
             if (!APART || tid + TH * u < 4 * Mp) atile[buf][tid + TH * u] = ar[as][u];
     };
+    auto compute_pow = [&](int bs, int buf) {
+        i32x4 (&B)[NT][2] = v[bs];  // flipped in place: the slot is reloaded before its next use
+#pragma unroll
+        for (int n = 0; n < NT; n++) {
+            B[n][0] ^= flip;
+            B[n][1] ^= flip;
+        }
+        constexpr int AW = 4;  // digit tiles read ahead of their MFMAs
+        i32x4 Aw[AW];
+#pragma unroll
+        for (int a = 0; a < AW; a++) Aw[a] = atile[buf][(l >> 4) * Mp + a * 16 + (l & 15)];
+#pragma unroll
+        for (int a = 0; a < MT; a++) {
+            const i32x4 A = Aw[a % AW];
+            if (a + AW < MT) Aw[a % AW] = atile[buf][(l >> 4) * Mp + (a + AW) * 16 + (l & 15)];
+#pragma unroll
+            for (int n = 0; n < NT; n++) {
+                acc[a][n][0] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A, B[n][0], acc[a][n][0], 0, 0, 0);
+                acc[a + 2][n][0] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A, B[n][1], acc[a + 2][n][0], 0, 0, 0);
+            }
+        }
+        __builtin_amdgcn_sched_group_barrier(0x100, AW, 0);
+#pragma unroll
+        for (int a = 0; a + AW < MT; a++) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 2 * NT, 0);
+            __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+        }
+        __builtin_amdgcn_sched_group_barrier(0x008, 2 * NT * AW, 0);
+    };
     auto compute = [&](int bs, int buf) {
+        if constexpr (POW) {
+            compute_pow(bs, buf);
+            return;
+        }
 #pragma unroll
         for (int n = 0; n < NT; n++) {
             const i32x4 Bl = v[bs][n][0] ^ flip, Bh = v[bs][n][1] ^ flip;
@@ -415,7 +486,7 @@ __global__ __launch_bounds__(64 * WAVES, OCC) void k_dbpass_mfma_sa(const uint8_
                 const i32x4 A = Aw[a % AW];
                 if (a + AW < MT) Aw[a % AW] = atile[buf][(l >> 4) * Mp + (a + AW) * 16 + (l & 15)];
                 acc[a][n][0] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A, Bl, acc[a][n][0], 0, 0, 0);
-                acc[a][n][1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A, Bh, acc[a][n][1], 0, 0, 0);
+                acc[a][n][PL - 1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A, Bh, acc[a][n][PL - 1], 0, 0, 0);
             }
             // keep the digit-tile reads AW tiles ahead of their MFMAs
             __builtin_amdgcn_sched_group_barrier(0x100, AW, 0);
@@ -448,7 +519,25 @@ __global__ __launch_bounds__(64 * WAVES, OCC) void k_dbpass_mfma_sa(const uint8_
             __syncthreads();
         }
     }
-    if constexpr (CRT7) {
+    if constexpr (POW) {
+        const uint32_t g = l >> 4, cl = l & 15;
+#pragma unroll
+        for (int h = 0; h < 2; h++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const uint32_t q = 16 * h + 4 * g + i;
+                if (q >= nq) continue;
+#pragma unroll
+                for (int n = 0; n < NT; n++) {
+                    int32_t pk[8];
+#pragma unroll
+                    for (int k = 0; k < 8; k++) pk[k] = acc[2 * k + h][n][0][i];
+                    uint32_t* o = out + (size_t)q * out_qstride + (size_t)slice * 2 * OC + col0 + n * 16 + cl;
+#pragma unroll
+                    for (int c = 0; c < 2; c++) o[(size_t)c * OC] = pow_residue(pk, slice == 0 ? S[2 * q + c] : 0, c);
+                }
+            }
+    } else if constexpr (CRT7) {
         // after the last barrier no wave reads the digit ring again
         int32_t* ep = reinterpret_cast<int32_t*>(smem) + w * EPW;
         const uint32_t g = l >> 4, cl = l & 15;
@@ -616,30 +705,34 @@ DbPlan dbplan_f64(size_t R, size_t ncols, int unroll, size_t target_blocks) {
 }
 
 DbPlan dbplan_mfma(size_t R, size_t ncols, size_t nq, size_t target_blocks, uint32_t depth, bool dbtile,
-                   uint32_t waves, bool crt) {
+                   uint32_t waves, bool crt, bool pow) {
     DbPlan p;
     p.kind = DbPlan::MFMA;
     p.param = nq <= 2 ? 1 : nq <= 4 ? 2 : nq <= 8 ? 4 : nq <= 16 ? 8 : 16;  // 2 queries per 16-row digit tile
     p.dbtile = dbtile && p.param == 16;
-    p.crt = crt && !p.dbtile && p.param == 16;
+    p.crt = (crt || pow) && !p.dbtile && p.param == 16;
+    p.pow = pow && p.crt;
     if (p.crt) p.param = 14;  // 7 rows per query: 32 queries in 14 tiles
     p.waves = p.param >= 8 && !p.dbtile && waves == 8 ? 8 : 4;
+    if (p.pow) p.waves = 8;
     // ring depths the kernels are built with (what fits 256 VGPRs): 14 or 16
-    // tiles with shared digits 3..7 (8-wave blocks) or 3..5, otherwise 3..4
-    const uint32_t maxd = p.param >= 14 && !p.dbtile ? (p.waves == 8 ? 7 : 5) : 4;
+    // tiles with shared digits 3..7 (8-wave blocks) or 3..5, power rows
+    // (two column tiles) 3, otherwise 3..4
+    const uint32_t maxd = p.pow ? 3 : p.param >= 14 && !p.dbtile ? (p.waves == 8 ? 7 : 5) : 4;
     p.depth = p.param >= 8 ? (depth < 3 ? 3 : depth > maxd ? maxd : depth) : 3;
     const size_t cols = dbplan_cols(p);
     // slices: whole number of 128-row (8-block) chunks so the step count is even
     p.slices = pick_slices(R / 16, ncols / cols, target_blocks, 8);
     // at most 2^16 rows per slice keeps |H_k|, |L_k| <= 2^30 (i32-exact) and
-    // the recombined sum below 2^58.1
-    while (R / p.slices > 65536) p.slices *= 2;
+    // the recombined sum below 2^58.1; the power rows' sums of two products
+    // need 2^15
+    while (R / p.slices > (p.pow ? 32768u : 65536u)) p.slices *= 2;
     return p;
 }
 
 DbPlan dbplan_form(size_t R, size_t ncols, size_t nq, size_t target_blocks, uint32_t depth, int form) {
-    return dbplan_mfma(R, ncols, nq, target_blocks, depth, form == 1, form == 2 || form == 3 ? 8 : 4,
-                       form == 3 || form == 4);
+    return dbplan_mfma(R, ncols, nq, target_blocks, depth, form == 1, form == 2 || form == 3 || form == 5 ? 8 : 4,
+                       form == 3 || form == 4, form == 5);
 }
 
 size_t dbplan_mp(const DbPlan& p) { return p.kind == DbPlan::MFMA ? 16 * p.param : 0; }
@@ -648,13 +741,16 @@ size_t dbplan_max_queries(const DbPlan& p) {
     return p.kind != DbPlan::MFMA ? 1 : p.crt ? dbplan_mp(p) / 7 : dbplan_mp(p) / 8;
 }
 
-size_t dbplan_cols(const DbPlan& p) { return p.kind == DbPlan::MFMA ? 16 * mfma_nt(p.param) * p.waves : 256; }
+size_t dbplan_cols(const DbPlan& p) {
+    return p.kind == DbPlan::MFMA ? 16 * (p.pow ? 2 : mfma_nt(p.param)) * p.waves : 256;
+}
 
 std::string dbplan_kernel(const DbPlan& p) {
     if (p.kind == DbPlan::F64) return "k_dbpass_f64<" + std::to_string(p.param) + ">";
     const std::string mt_nt = std::to_string(p.param) + ", " + std::to_string(mfma_nt(p.param));
     if (p.param < 8) return "k_dbpass_mfma<" + mt_nt + ">";
     if (p.dbtile) return "k_dbpass_mfma_bs<16, 4, " + std::to_string(p.depth) + ">";
+    if (p.pow) return "k_dbpass_mfma_sa<14, 2, 2, " + std::to_string(p.depth) + ", 8, 1>";
     return "k_dbpass_mfma_sa<" + mt_nt + ", 2, " + std::to_string(p.depth) + (p.waves == 8 ? ", 8>" : ">");
 }
 
@@ -682,8 +778,12 @@ hipError_t launch_mfma_digits(const double* bq, size_t nq, size_t R, const DbPla
     const uint32_t Mp = (uint32_t)dbplan_mp(p);
     if (nq > dbplan_max_queries(p)) return hipErrorInvalidValue;
     const size_t nthr = (R / 16) * (p.crt ? Mp / 7 : Mp / 4);
-    hipLaunchKernelGGL(p.crt ? k_digits7 : k_digits, dim3((unsigned)((nthr + 255) / 256)), dim3(256), 0, s, bq,
-                       (uint32_t)R, (uint32_t)nq, Mp, dig);
+    const dim3 grid((unsigned)((nthr + 255) / 256));
+    if (p.crt)
+        hipLaunchKernelGGL(k_digits7, grid, dim3(256), 0, s, bq, (uint32_t)R, (uint32_t)nq, Mp, dig,
+                           (uint32_t)p.pow);
+    else
+        hipLaunchKernelGGL(k_digits, grid, dim3(256), 0, s, bq, (uint32_t)R, (uint32_t)nq, Mp, dig);
     hipLaunchKernelGGL(k_bsum, dim3((unsigned)(2 * nq)), dim3(256), 0, s, bq, (uint32_t)R, S);
     return hipGetLastError();
 }
@@ -705,6 +805,9 @@ hipError_t launch_dbpass_mfma(const uint8_t* db, const int8_t* dig, const uint32
     hipLaunchKernelGGL((K<__VA_ARGS__>), grid, dim3(256), 0, s, d, dig, S, out, out_qstride, Cc, OC, Q, kbps)
 #define MFMA8(K, ...) \
     hipLaunchKernelGGL((K<__VA_ARGS__, 8>), grid, dim3(512), 0, s, d, dig, S, out, out_qstride, Cc, OC, Q, kbps)
+#define MFMA8P(D)                                                                                             \
+    hipLaunchKernelGGL((k_dbpass_mfma_sa<14, 2, 2, D, 8, 1>), grid, dim3(512), 0, s, d, dig, S, out, out_qstride, \
+                       Cc, OC, Q, kbps)
     switch (p.param) {
         case 1: MFMA(k_dbpass_mfma, 1, 4); break;
         case 2: MFMA(k_dbpass_mfma, 2, 4); break;
@@ -719,6 +822,11 @@ hipError_t launch_dbpass_mfma(const uint8_t* db, const int8_t* dig, const uint32
             }
             break;
         case 14:
+            if (p.pow) {
+                if (p.depth != 3) return hipErrorInvalidValue;
+                MFMA8P(3);
+                break;
+            }
             switch (p.depth * 10 + p.waves) {
                 case 34: MFMA(k_dbpass_mfma_sa, 14, 1, 2, 3); break;
                 case 44: MFMA(k_dbpass_mfma_sa, 14, 1, 2, 4); break;
@@ -756,6 +864,7 @@ hipError_t launch_dbpass_mfma(const uint8_t* db, const int8_t* dig, const uint32
     }
 #undef MFMA
 #undef MFMA8
+#undef MFMA8P
     return hipGetLastError();
 }
 

@@ -263,7 +263,8 @@ DbPlan narrow_plan(const DbPlan& full, size_t target, size_t R, size_t C, size_t
     if (wgs > kChunkWgs) wgs = kChunkWgs;
     const size_t t = wgs > target ? wgs : target;
     return full.kind == DbPlan::F64 ? dbplan_f64(R, ncols, (int)full.param, t)
-                                     : dbplan_mfma(R, ncols, nq, t, full.depth, full.dbtile, full.waves, full.crt);
+                                     : dbplan_mfma(R, ncols, nq, t, full.depth, full.dbtile, full.waves, full.crt,
+                                                   full.pow);
 }
 }  // namespace
 

@@ -182,13 +182,13 @@ def test_db_pass_paths_match_oracle(medium8, path):
 
 
 @pytest.mark.parametrize("depth,form", [(3, 0), (4, 0), (5, 0), (3, 1), (4, 1), (3, 2), (5, 2), (6, 2), (7, 2),
-                                        (3, 3), (4, 3), (5, 3), (6, 3), (7, 3), (3, 4), (4, 4), (5, 4)])
+                                        (3, 3), (4, 3), (5, 3), (6, 3), (7, 3), (3, 4), (4, 4), (5, 4), (3, 5)])
 def test_many_query_ring_depths_match_oracle(medium8, depth, form):
     """Every register-ring depth and form of the batched MFMA passes against
     the oracle's mat-vec (12 queries: 8 digit tiles, depth capped at 4; 17,
     24 and 32 queries: 16 tiles, or 14 in the forms with 7 digit rows per
-    query; the waves of 4- or 8-wave blocks sharing the digit tile, or the
-    DB tile)."""
+    query, accumulated per (digit, plane) or per power of 256; the waves of
+    4- or 8-wave blocks sharing the digit tile, or the DB tile)."""
     geo, db, srv = medium8
     R, Cc = geo["R"], geo["C"]
     enc = np.zeros((R, Cc), np.uint16)
@@ -197,10 +197,13 @@ def test_many_query_ring_depths_match_oracle(medium8, depth, form):
     rng = np.random.default_rng(400 + depth)
     name = C.create_string_buffer(128)
     assert srv.lib.ipir_dbg_dbpass_kernel(srv.h, MB8, name, 128) == 0
-    kname = name.value.decode()  # k_dbpass_mfma_sa<16|14, 1, 2, depth[, 8]> or k_dbpass_mfma_bs<16, 4, depth>
+    # k_dbpass_mfma_sa<16|14, 1, 2, depth[, 8]>, k_dbpass_mfma_sa<14, 2, 2, depth, 8, 1> or k_dbpass_mfma_bs<16, 4, depth>
+    kname = name.value.decode()
     args = [int(x) for x in kname.split("<")[1].split(">")[0].split(",")]
     if "_bs<" in kname:
         tuned = (args[2], 1)
+    elif len(args) == 6:
+        tuned = (args[3], 5)
     else:
         tuned = (args[3], (3 if args[0] == 14 else 2) if len(args) == 5 else (4 if args[0] == 14 else 0))
     assert srv.lib.ipir_dbg_set_mfma_depth(srv.h, 2, 0) == 1
@@ -209,7 +212,8 @@ def test_many_query_ring_depths_match_oracle(medium8, depth, form):
     assert srv.lib.ipir_dbg_set_mfma_depth(srv.h, 8, 2) == 1
     assert srv.lib.ipir_dbg_set_mfma_depth(srv.h, 8, 3) == 1
     assert srv.lib.ipir_dbg_set_mfma_depth(srv.h, 6, 4) == 1
-    assert srv.lib.ipir_dbg_set_mfma_depth(srv.h, 3, 5) == 1
+    assert srv.lib.ipir_dbg_set_mfma_depth(srv.h, 4, 5) == 1
+    assert srv.lib.ipir_dbg_set_mfma_depth(srv.h, 3, 6) == 1
     assert srv.lib.ipir_dbg_set_mfma_depth(srv.h, depth, form) == 0
     try:
         for nq in (12, 17, 24, MB8):
