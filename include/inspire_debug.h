@@ -68,7 +68,7 @@ int ipir_dbg_set_horner(ipir_server* srv, int mode);
  * 3..5), 1 (17..32 queries) they share the DB tile (3..4), 2 the waves of an
  * 8-wave block share the digit tile (3..7), 3 and 4 (17..32 queries) the
  * 8- and 4-wave forms with 7 digit rows per query (3..7, 3..5), 5 (17..32
- * queries) those digits accumulated by power of 256 (3); 9..16-query plans
+ * queries) those digits accumulated by power of 256 (3..5); 9..16-query plans
  * cap the depth at 4 and use form 0 for 4 and 2 for 3 and 5. 1 for other
  * values. */
 int ipir_dbg_set_mfma_depth(ipir_server* srv, uint32_t depth, int form);

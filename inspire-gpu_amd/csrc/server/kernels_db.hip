@@ -526,15 +526,18 @@ __global__ __launch_bounds__(64 * WAVES, OCC) void k_dbpass_mfma_sa(const uint8_
 #pragma unroll
             for (int i = 0; i < 4; i++) {
                 const uint32_t q = 16 * h + 4 * g + i;
-                if (q >= nq) continue;
 #pragma unroll
                 for (int n = 0; n < NT; n++) {
                     int32_t pk[8];
 #pragma unroll
                     for (int k = 0; k < 8; k++) pk[k] = acc[2 * k + h][n][0][i];
                     uint32_t* o = out + (size_t)q * out_qstride + (size_t)slice * 2 * OC + col0 + n * 16 + cl;
-#pragma unroll
-                    for (int c = 0; c < 2; c++) o[(size_t)c * OC] = pow_residue(pk, slice == 0 ? S[2 * q + c] : 0, c);
+                    if (q < nq) {
+                        const uint32_t r0 = pow_residue(pk, slice == 0 ? S[2 * q] : 0, 0);
+                        const uint32_t r1 = pow_residue(pk, slice == 0 ? S[2 * q + 1] : 0, 1);
+                        o[0] = r0;
+                        o[OC] = r1;
+                    }
                 }
             }
     } else if constexpr (CRT7) {
@@ -717,8 +720,8 @@ DbPlan dbplan_mfma(size_t R, size_t ncols, size_t nq, size_t target_blocks, uint
     if (p.pow) p.waves = 8;
     // ring depths the kernels are built with (what fits 256 VGPRs): 14 or 16
     // tiles with shared digits 3..7 (8-wave blocks) or 3..5, power rows
-    // (two column tiles) 3, otherwise 3..4
-    const uint32_t maxd = p.pow ? 3 : p.param >= 14 && !p.dbtile ? (p.waves == 8 ? 7 : 5) : 4;
+    // (two column tiles) 3..5, otherwise 3..4
+    const uint32_t maxd = p.pow ? 5 : p.param >= 14 && !p.dbtile ? (p.waves == 8 ? 7 : 5) : 4;
     p.depth = p.param >= 8 ? (depth < 3 ? 3 : depth > maxd ? maxd : depth) : 3;
     const size_t cols = dbplan_cols(p);
     // slices: whole number of 128-row (8-block) chunks so the step count is even
@@ -823,8 +826,12 @@ hipError_t launch_dbpass_mfma(const uint8_t* db, const int8_t* dig, const uint32
             break;
         case 14:
             if (p.pow) {
-                if (p.depth != 3) return hipErrorInvalidValue;
-                MFMA8P(3);
+                switch (p.depth) {
+                    case 3: MFMA8P(3); break;
+                    case 4: MFMA8P(4); break;
+                    case 5: MFMA8P(5); break;
+                    default: return hipErrorInvalidValue;
+                }
                 break;
             }
             switch (p.depth * 10 + p.waves) {

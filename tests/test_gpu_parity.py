@@ -182,7 +182,8 @@ def test_db_pass_paths_match_oracle(medium8, path):
 
 
 @pytest.mark.parametrize("depth,form", [(3, 0), (4, 0), (5, 0), (3, 1), (4, 1), (3, 2), (5, 2), (6, 2), (7, 2),
-                                        (3, 3), (4, 3), (5, 3), (6, 3), (7, 3), (3, 4), (4, 4), (5, 4), (3, 5)])
+                                        (3, 3), (4, 3), (5, 3), (6, 3), (7, 3), (3, 4), (4, 4), (5, 4), (3, 5), (4, 5),
+                                        (5, 5)])
 def test_many_query_ring_depths_match_oracle(medium8, depth, form):
     """Every register-ring depth and form of the batched MFMA passes against
     the oracle's mat-vec (12 queries: 8 digit tiles, depth capped at 4; 17,
@@ -212,7 +213,7 @@ def test_many_query_ring_depths_match_oracle(medium8, depth, form):
     assert srv.lib.ipir_dbg_set_mfma_depth(srv.h, 8, 2) == 1
     assert srv.lib.ipir_dbg_set_mfma_depth(srv.h, 8, 3) == 1
     assert srv.lib.ipir_dbg_set_mfma_depth(srv.h, 6, 4) == 1
-    assert srv.lib.ipir_dbg_set_mfma_depth(srv.h, 4, 5) == 1
+    assert srv.lib.ipir_dbg_set_mfma_depth(srv.h, 6, 5) == 1
     assert srv.lib.ipir_dbg_set_mfma_depth(srv.h, 3, 6) == 1
     assert srv.lib.ipir_dbg_set_mfma_depth(srv.h, depth, form) == 0
     try:
