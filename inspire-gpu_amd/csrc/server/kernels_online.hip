@@ -113,6 +113,9 @@ __global__ __launch_bounds__(256) void k_pack_finalize(const uint32_t* __restric
 // acc (a0, a1, b0, b1 polys, NTT form) lives in registers between steps;
 // per step: 4 INTTs, gadget split, 2 rounds of 4 NTTs, multiply-accumulate
 // against the query's RGSW rows, + packed[i]. L == 1 copies packed[start].
+// Each round's RGSW rows (and the step's packed parts) are loaded one phase
+// ahead, so their latency (HBM latency while a database pass streams
+// through L2) hides behind the transforms instead of adding to every step.
 __global__ __launch_bounds__(1024) void k_horner(const uint32_t* __restrict__ pa_base, size_t pa_stride, uint32_t G, uint32_t Da,
                                                  uint32_t T, const uint32_t* __restrict__ packed_b,
                                                  size_t pstride, const uint32_t* __restrict__ rgsw,
@@ -156,6 +159,21 @@ __global__ __launch_bounds__(1024) void k_horner(const uint32_t* __restrict__ pa
     }
     for (int gs = first; gs >= (int)lo; gs--) {
         const int st = gs - (int)start;
+        const uint32_t gi0 = start + (uint32_t)st;
+        // this step's global operands, issued before the transforms they wait behind
+        uint32_t rA[2][2][2], rB[2][2][2];  // round-0 RGSW [c][e][j]
+        uint32_t pav[2][2], pbv[2][2];      // packed a / b parts [c][e]
+#pragma unroll
+        for (int c = 0; c < 2; c++)
+#pragma unroll
+            for (int e = 0; e < 2; e++) {
+                const uint32_t i = tid + 1024 * e;
+#pragma unroll
+                for (int j = 0; j < 2; j++) {
+                    rA[c][e][j] = rg[(size_t)(j * 2 + c) * N + i];
+                    rB[c][e][j] = rg[(size_t)BLK + (size_t)(j * 2 + c) * N + i];
+                }
+            }
 #pragma unroll
         for (int p = 0; p < 4; p++)
 #pragma unroll
@@ -185,8 +203,6 @@ __global__ __launch_bounds__(1024) void k_horner(const uint32_t* __restrict__ pa
             __syncthreads();
             ntt_fwd_lds(buf[team], t, tw, tlimb);
             // RGSW parts: 0 top_a, 1 top_b, 2 bot_a, 3 bot_b; [part][digit][limb][N]
-            const uint32_t* pA = rg + (size_t)(round ? 2 : 0) * BLK;
-            const uint32_t* pB = rg + (size_t)(round ? 3 : 1) * BLK;
 #pragma unroll
             for (int c = 0; c < 2; c++)
 #pragma unroll
@@ -196,22 +212,34 @@ __global__ __launch_bounds__(1024) void k_horner(const uint32_t* __restrict__ pa
 #pragma unroll
                     for (int j = 0; j < 2; j++) {
                         const uint64_t d = buf[j * 2 + c][ls(i)];
-                        sa += d * pA[(size_t)(j * 2 + c) * N + i];
-                        sb += d * pB[(size_t)(j * 2 + c) * N + i];
+                        sa += d * rA[c][e][j];
+                        sb += d * rB[c][e][j];
                     }
                     ra[c][e] = sa; rb[c][e] = sb;
                 }
+            if (round == 0) {  // round 1's RGSW rows, issued before its transforms
+#pragma unroll
+                for (int c = 0; c < 2; c++)
+#pragma unroll
+                    for (int e = 0; e < 2; e++) {
+                        const uint32_t i = tid + 1024 * e;
+#pragma unroll
+                        for (int j = 0; j < 2; j++) {
+                            rA[c][e][j] = rg[(size_t)2 * BLK + (size_t)(j * 2 + c) * N + i];
+                            rB[c][e][j] = rg[(size_t)3 * BLK + (size_t)(j * 2 + c) * N + i];
+                        }
+                        pav[c][e] = pa_base[(size_t)gi0 * pa_stride + (size_t)c * N + i];
+                        pbv[c][e] = pb[(size_t)gi0 * 2 * N + (size_t)c * N + i];
+                    }
+            }
         }
-        const uint32_t gi = start + (uint32_t)st;
-        const uint32_t* pa = pa_base + (size_t)gi * pa_stride;
 #pragma unroll
         for (int c = 0; c < 2; c++)
 #pragma unroll
             for (int e = 0; e < 2; e++) {
-                const uint32_t i = tid + 1024 * e;
                 const uint32_t q = qof(c);
-                acc[c][e] = add_mod(red64(ra[c][e], c), pa[(size_t)c * N + i], q);
-                acc[2 + c][e] = add_mod(red64(rb[c][e], c), pb[(size_t)gi * 2 * N + (size_t)c * N + i], q);
+                acc[c][e] = add_mod(red64(ra[c][e], c), pav[c][e], q);
+                acc[2 + c][e] = add_mod(red64(rb[c][e], c), pbv[c][e], q);
             }
         __syncthreads();
     }
