@@ -27,9 +27,11 @@ int ipir_dbg_db_rows(const ipir_server* srv, size_t row0, size_t nrows, uint16_t
  * plans of the answer pipeline, 0 = the fp64 pass per query, 1 = the int8
  * MFMA pass. */
 int ipir_dbg_db_pass(ipir_server* srv, const uint64_t* b, size_t nq, int path, uint32_t* interm);
-/* Device time of the database pass alone (the kernels answer_batch
- * launches for it, on the server stream) over the first nq query slots as
- * last staged, averaged over iters passes. */
+/* Device time of the full-width database pass alone (the pass kernel of
+ * the plan answer_batch uses for nq <= 32 queries, and its slice reduction
+ * if the plan has row slices, on the server stream) over the first nq
+ * query slots as last staged, averaged over iters passes; the query digits
+ * are prepared once, outside the timed launches. */
 int ipir_dbg_time_dbpass(ipir_server* srv, size_t nq, int iters, double* ms_per_pass);
 /* The database-pass kernel (profiler name) and row-slice count the server
  * uses for a batch of nq queries, NUL-terminated into out[cap]. */

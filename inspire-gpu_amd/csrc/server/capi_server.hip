@@ -232,7 +232,8 @@ int ipir_dbg_db_pass(ipir_server* srv, const uint64_t* b, size_t nq, int path, u
 }
 
 int ipir_dbg_time_dbpass(ipir_server* srv, size_t nq, int iters, double* ms_per_pass) {
-    if (!srv || nq == 0 || nq > srv->ctx->cfg.max_batch || iters <= 0 || !ms_per_pass) return 1;
+    if (!srv || nq == 0 || nq > srv->ctx->cfg.max_batch || nq > ipx::kMfmaMaxQueries || iters <= 0 || !ms_per_pass)
+        return 1;
     GpuServerCtx* c = srv->ctx;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     int rc = 0;
@@ -240,9 +241,13 @@ int ipir_dbg_time_dbpass(ipir_server* srv, size_t nq, int iters, double* ms_per_
         std::lock_guard<std::mutex> lk(c->mu);
         hip_check(hipEventCreate(&e0), "event");
         hip_check(hipEventCreate(&e1), "event");
-        c->stage_dbpass(nq, c->stream, 0);  // warm-up
+        // the query digits once; then the pass's own launches (the kernel and,
+        // for a plan with row slices, the slice reduction)
+        const ipx::DbPlan p = c->plan_for(nq, c->pp.db_cols);
+        c->dbpass_prep(0, nq, p, c->stream, 0);
+        c->dbpass_range(0, nq, 0, c->G, p, c->stream, 0);  // warm-up
         hip_check(hipEventRecord(e0, c->stream), "record");
-        for (int i = 0; i < iters; i++) c->stage_dbpass(nq, c->stream, 0);
+        for (int i = 0; i < iters; i++) c->dbpass_range(0, nq, 0, c->G, p, c->stream, 0);
         hip_check(hipEventRecord(e1, c->stream), "record");
         hip_check(hipEventSynchronize(e1), "sync");
         float ms = 0;

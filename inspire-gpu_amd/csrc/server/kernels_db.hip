@@ -753,8 +753,9 @@ std::string dbplan_kernel(const DbPlan& p) {
     const std::string mt_nt = std::to_string(p.param) + ", " + std::to_string(mfma_nt(p.param));
     if (p.param < 8) return "k_dbpass_mfma<" + mt_nt + ">";
     if (p.dbtile) return "k_dbpass_mfma_bs<16, 4, " + std::to_string(p.depth) + ">";
-    if (p.pow) return "k_dbpass_mfma_sa<14, 2, 2, " + std::to_string(p.depth) + ", 8, 1>";
-    return "k_dbpass_mfma_sa<" + mt_nt + ", 2, " + std::to_string(p.depth) + (p.waves == 8 ? ", 8>" : ">");
+    // every template argument, defaults included (as profilers demangle it)
+    return "k_dbpass_mfma_sa<" + (p.pow ? std::string("14, 2") : mt_nt) + ", 2, " + std::to_string(p.depth) + ", " +
+           std::to_string(p.waves) + ", " + (p.pow ? "1>" : "0>");
 }
 
 hipError_t launch_dbpass_f64(const uint8_t* db, const double* bq, uint32_t* out, size_t ostride, size_t R, size_t C,
