@@ -659,11 +659,13 @@ void GpuServerCtx::enqueue(const uint64_t* d_in, size_t count, uint64_t* d_out, 
             const size_t g_lo = k * gchunk, g_hi = g_lo + gchunk;
             hipEvent_t dbe = chunk_ev[lane][k % kChunkEvents], ce = coll_ev[lane][k % kChunkEvents];
             dbpass_range(q0, nq, g_lo, g_hi, p, s, lane);
-            HIPX(hipEventRecord(dbe, s));
+            // with the collapse on s its event covers the DB pass too (an
+            // event record costs the stream a few microseconds of idle)
+            if (cs != s) HIPX(hipEventRecord(dbe, s));
             HIPX(launch_collapse(d_precomp + g_lo * PC_GROUP, gchunk, d_kdig[lane], d_KS[lane], nq,
                                  p2 + g_lo * 2 * N, part2_stride, cs, collapse_variant));
             HIPX(hipEventRecord(ce, cs));
-            HIPX(hipStreamWaitEvent(hs, dbe, 0));
+            if (cs != s) HIPX(hipStreamWaitEvent(hs, dbe, 0));
             HIPX(hipStreamWaitEvent(hs, ce, 0));
             HIPX(launch_pack_finalize(d_interm + q0 * 2 * C, C, p2, part2_stride, d_precomp, g_lo, gchunk, keys,
                                       keys_stride, packed, packed_stride, nq, nc, hs));
