@@ -429,17 +429,30 @@ void GpuServerCtx::autotune_dbpass() {
     if (nq > 1) {
         best = 1e30f;
         const uint32_t param = dbplan_mfma(R, C, nq, mfma_target).param;
-        // forms (dbplan_form): the shared DB tile and the 7-digit forms exist for 17..32 queries only
+        // forms (dbplan_form): the shared DB tile and the 7-digit forms exist
+        // for 17..32 queries only; the 7-digit forms regrouped through LDS
+        // (3, 4) never beat the power rows (5) and are timed only when forced
         const int forced_form = mfma_form;
+        std::vector<DbPlan> timed;  // targets that give the same grid are timed once
+        auto seen = [&](const DbPlan& x) {
+            for (const DbPlan& y : timed)
+                if (y.param == x.param && y.slices == x.slices && y.depth == x.depth && y.dbtile == x.dbtile &&
+                    y.waves == x.waves && y.crt == x.crt && y.pow == x.pow)
+                    return true;
+            timed.push_back(x);
+            return false;
+        };
         for (size_t t : kMfmaTargets)
             for (int form = 0; form < kMfmaForms; form++) {
                 if (depth_forced ? form != forced_form
-                                 : (form != 0 && form != 2 && param != 16) || (form > 0 && param < 8))
+                                 : (form != 0 && form != 2 && param != 16) || (form > 0 && param < 8) || form == 3 ||
+                                       form == 4)
                     continue;
                 for (uint32_t d : kMfmaDepths) {
                     if ((depth_forced || param < 8) && d != mfma_depth) continue;
                     const DbPlan x = dbplan_form(R, C, nq, t, d, form);
                     if (x.depth != d && param >= 8) continue;  // a depth this kernel is not built with
+                    if (seen(x)) continue;
                     const float ms = time_plan(x, nq);
                     if (ms < best) {
                         best = ms;
