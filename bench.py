@@ -365,6 +365,10 @@ def main():
     dbB_ms, knameB = dbpass(min(B, 32))
     log(f"[bench] DB pass: B=1 {kname} {dbpass_ms:.3f} ms ({achieved:.0f} GB/s); "
         f"B={min(B, 32)} {knameB} {dbB_ms:.3f} ms")
+    tb = C.create_string_buffer(1024)
+    lib.ipir_dbg_tuning(srv.h, tb, 1024)
+    tuning = tb.value.decode()
+    log(f"[bench] tuning: {tuning}")
     achievedB = db_bytes / (dbB_ms * 1e-3) / 1e9
     # HBM bytes per launch from the committed copy-calibrated counter runs
     # (profiles/pmc_traffic_*.md), for the same kernel on the same workload
@@ -456,6 +460,7 @@ def main():
                                                             / (dbB_ms * 1e-3) / 5.0e15, 4)},
         "batch1_db_pass": {"kernel": kname, "ms": round(dbpass_ms, 3), "hbm_GBs": round(achieved, 1),
                            "frac_of_hbm_peak": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
+        "tuning": tuning,
         "horner_step_us": {"batch1": round(st1[4] * 1e3 / max(1, geo["Da"] - 1), 2),
                            f"batch{B}": round(st32[4] * 1e3 / max(1, geo["Da"] - 1), 2)},
         "ntt_throughput": ntt,

@@ -36,6 +36,11 @@ int ipir_dbg_time_dbpass(ipir_server* srv, size_t nq, int iters, double* ms_per_
 /* The database-pass kernel (profiler name) and row-slice count the server
  * uses for a batch of nq queries, NUL-terminated into out[cap]. */
 int ipir_dbg_dbpass_kernel(const ipir_server* srv, size_t nq, char* out, size_t cap);
+/* The setup autotuner's choices as one line of text (database-pass plans
+ * at batch 1 and at the largest batch, their form and workgroup target,
+ * the pipelined chunks' plans, the collapse variant, the pipeline shapes),
+ * NUL-terminated into out[cap]. */
+int ipir_dbg_tuning(const ipir_server* srv, char* out, size_t cap);
 /* Answer one host flat query and return the intermediate tensors:
  * interm [2][db_cols] (DB pass, coefficient form), packed_b
  * [n_packed][2][N] (after the collapse), resp (flat response). Any output
@@ -70,9 +75,9 @@ int ipir_dbg_set_horner(ipir_server* srv, int mode);
  * 3..5), 1 (17..32 queries) they share the DB tile (3..4), 2 the waves of an
  * 8-wave block share the digit tile (3..7), 3 and 4 (17..32 queries) the
  * 8- and 4-wave forms with 7 digit rows per query (3..7, 3..5), 5 (17..32
- * queries) those digits accumulated by power of 256 (3..5); 9..16-query plans
- * cap the depth at 4 and use form 0 for 4 and 2 for 3 and 5. 1 for other
- * values. */
+ * queries) those digits accumulated by power of 256 (3..5, also for 9..16
+ * queries); other 9..16-query plans cap the depth at 4 and use form 0 for
+ * 1 and 4 and 2 for 3. 1 for other values. */
 int ipir_dbg_set_mfma_depth(ipir_server* srv, uint32_t depth, int form);
 /* Row slices of the pipelined answer's database-pass chunks for batches
  * planned like an nq-query batch (0: back to the narrowing rule; a power of

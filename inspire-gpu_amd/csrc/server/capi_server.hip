@@ -270,6 +270,25 @@ int ipir_dbg_dbpass_kernel(const ipir_server* srv, size_t nq, char* out, size_t 
     return 0;
 }
 
+int ipir_dbg_tuning(const ipir_server* srv, char* out, size_t cap) {
+    if (!srv || !out || cap == 0) return 1;
+    const GpuServerCtx* c = srv->ctx;
+    const size_t nq = c->cfg.max_batch < ipx::kMfmaMaxQueries ? c->cfg.max_batch : ipx::kMfmaMaxQueries;
+    auto plan = [](const ipx::DbPlan& p) {
+        return ipx::dbplan_kernel(p) + " slices=" + std::to_string(p.slices);
+    };
+    std::string s = "batch1: " + plan(c->plan_for(1, c->pp.db_cols)) + "; batch" + std::to_string(nq) + ": " +
+                    plan(c->plan_for(nq, c->pp.db_cols)) + " (form " + std::to_string(c->mfma_form) + ", target " +
+                    std::to_string(c->mfma_target) + ")";
+    if (c->G / c->gchunk > 1)
+        s += "; chunks of " + std::to_string(c->gchunk) + " groups: batch1 " + plan(c->chunk_plan(1, c->gchunk * ipx::N)) +
+             ", batch" + std::to_string(nq) + " " + plan(c->chunk_plan(nq, c->gchunk * ipx::N));
+    s += "; collapse variant " + std::to_string(c->collapse_variant) + "; pipeline shapes " +
+         std::to_string(c->pipe_single) + "/" + std::to_string(c->pipe_batched);
+    std::snprintf(out, cap, "%s", s.c_str());
+    return 0;
+}
+
 int ipir_dbg_answer_taps(ipir_server* srv, const uint64_t* query, uint32_t* interm, uint32_t* packed_b,
                          uint64_t* resp) {
     if (!srv || !query) return 1;

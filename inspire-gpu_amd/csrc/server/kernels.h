@@ -49,18 +49,18 @@ hipError_t launch_compress(const uint64_t* resp, size_t nct, uint32_t* scratch, 
 // slice's rows of db[r][c]*b_{q,limb}[r] mod q_limb.
 struct DbPlan {
     enum Kind { F64, MFMA } kind;
-    uint32_t param;      // F64: 16-row blocks per iteration (2, 4, 8); MFMA: 16-row digit tiles (1, 2, 4, 8, 14, 16)
+    uint32_t param;      // F64: 16-row blocks per iteration (2, 4, 8); MFMA: 16-row digit tiles (1, 2, 4, 7, 8, 14, 16)
     uint32_t slices;     // row slices (power of two)
     uint32_t depth = 3;  // 8 or 16 digit tiles: register-ring slots of DB operands (kMfmaDepths)
     bool dbtile = false; // 16 digit tiles: the waves share the DB tile (k_dbpass_mfma_bs), not the digit tile
     uint32_t waves = 4;  // 8 or 16 digit tiles, shared digits: waves per block (4, or 8 = one block per CU)
-    bool crt = false;    // 17..32 queries, shared digits: 7 digit rows per query (CRT value), 14 tiles
+    bool crt = false;    // shared digits: 7 digit rows per query (CRT value), 14 tiles (7 for power rows, 9..16 queries)
     bool pow = false;    // crt digits accumulated by power of 256 (8-wave blocks, two column tiles per wave)
 };
 // Forms of the 9..32-query pass: 0 shared digits in 4-wave blocks, 1 shared
-// DB tile, 2 shared digits in 8-wave blocks, 3 and 4 the 8- and 4-wave
-// shared-digit forms with 7 digit rows per query, 5 those digits
-// accumulated by power (the last four for 17..32 queries).
+// DB tile (17..32 queries), 2 shared digits in 8-wave blocks, 3 and 4 the
+// 8- and 4-wave shared-digit forms with 7 digit rows per query (17..32
+// queries), 5 those digits accumulated by power (9..32 queries).
 constexpr int kMfmaForms = 6;
 // Ring depths of the 9..32-query MFMA passes; plans cap them at what each
 // kernel is built with (7 for 14 or 16 shared-digit tiles in 8-wave blocks,

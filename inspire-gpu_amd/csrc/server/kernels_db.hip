@@ -168,7 +168,7 @@ constexpr uint32_t pow_mod_c(uint64_t b, uint64_t e, uint64_t m) {
 }
 constexpr uint32_t kQ1InvQ0 = pow_mod_c(Q1, Q0 - 2, Q0);  // Q0 is prime
 
-// pmajor: rows 32k + q (the power-row form) instead of 7q + k.
+// pmajor: rows (Mp/7) k + q (the power-row form) instead of 7q + k.
 __global__ __launch_bounds__(256) void k_digits7(const double* __restrict__ bq, uint32_t R, uint32_t nq,
                                                  uint32_t Mp, int8_t* __restrict__ dig, uint32_t pmajor) {
     const uint32_t nkb = R / 16;
@@ -200,7 +200,7 @@ __global__ __launch_bounds__(256) void k_digits7(const double* __restrict__ bq, 
     }
     u32x4* dst = reinterpret_cast<u32x4*>(dig + (size_t)kb * Mp * 16);
 #pragma unroll
-    for (int k = 0; k < 7; k++) dst[pmajor ? 32 * k + q : 7 * q + k] = o[k];
+    for (int k = 0; k < 7; k++) dst[pmajor ? (Mp / 7) * k + q : 7 * q + k] = o[k];
 }
 
 // Residue mod q_c of one output of the 7-digit form from the dot products
@@ -367,18 +367,19 @@ __global__ __launch_bounds__(256, 2) void k_dbpass_mfma(const uint8_t* __restric
 // the last step), half of the queries at a time, and each lane recombines
 // whole queries.
 //
-// POW = 1 (MT = 14, digit rows 32k + q: digit k of query q) accumulates by
-// power of 256 instead of by (digit, plane):
+// POW = 1 (MT = 7 QT, QT = 1 or 2 tiles of 16 queries, digit rows
+// 16 QT k + q: digit k of query q) accumulates by power of 256 instead of
+// by (digit, plane):
 //   sum_r v_r B_r = sum_{k=0..7} 256^k p_k + 32896 sum_r B_r,
 //   p_k = sum_r h'_r d_{k-1,r} + l'_r d_{k,r}   (d_{-1} = d_7 = 0),
-// so the high-plane MFMA of digit tile a adds into output tile a + 2 (the
+// so the high-plane MFMA of digit tile a adds into output tile a + QT (the
 // next power of the same 16 queries) and the low-plane one into tile a:
-// 16 output tiles of one accumulator each instead of 14 of two. The freed
-// registers hold a second column tile (NT = 2), which halves the digit-tile
-// bytes loaded and read from LDS per DB byte. A lane's output rows are
-// powers k = a/2 of queries 16(a%2) + 4(l/16) + i: every lane recombines
-// its own queries from registers. |p_k| <= 2 * 2^14 per row: at most 2^15
-// rows per slice keep it within 2^30.
+// 8 QT output tiles of one accumulator each instead of 7 QT of two. The
+// freed registers hold a second column tile (NT = 2), which halves the
+// digit-tile bytes loaded and read from LDS per DB byte. A lane's output
+// rows are powers k = a/QT of queries 16(a%QT) + 4(l/16) + i: every lane
+// recombines its own queries from registers. |p_k| <= 2 * 2^14 per row: at
+// most 2^15 rows per slice keep it within 2^30.
 template <int MT, int NT, int OCC, int DEPTH, int WAVES = 4, int POW = 0>
 __global__ __launch_bounds__(64 * WAVES, OCC) void k_dbpass_mfma_sa(const uint8_t* __restrict__ db,
                                                            const int8_t* __restrict__ dig,
@@ -388,10 +389,11 @@ __global__ __launch_bounds__(64 * WAVES, OCC) void k_dbpass_mfma_sa(const uint8_
     static_assert(DEPTH >= 3, "the digit ring needs two slots");
     constexpr int L = DEPTH - 1;            // lead of every load, in steps
     constexpr int P = DEPTH * L;            // period of both rings' slot pattern
-    static_assert(!POW || MT == 14, "power rows come from the 7-digit expansion");
+    static_assert(!POW || MT == 7 || MT == 14, "power rows come from the 7-digit expansion");
     constexpr bool CRT7 = MT == 14 && !POW;
     static_assert(!CRT7 || NT == 1, "the 7-digit epilogue covers one column tile per wave");
-    constexpr int MA = POW ? MT + 2 : MT, PL = POW ? 1 : 2;  // output tiles, accumulators per tile
+    constexpr int QT = POW ? MT / 7 : 1;                      // power rows: tiles per power
+    constexpr int MA = POW ? MT + QT : MT, PL = POW ? 1 : 2;  // output tiles, accumulators per tile
     constexpr uint32_t Mp = 16 * MT;
     constexpr uint32_t TH = 64 * WAVES;              // threads per block
     constexpr uint32_t AU = (Mp * 4 + TH - 1) / TH;  // 16-byte units of the step's digit tile per thread
@@ -458,7 +460,7 @@ __global__ __launch_bounds__(64 * WAVES, OCC) void k_dbpass_mfma_sa(const uint8_
 #pragma unroll
             for (int n = 0; n < NT; n++) {
                 acc[a][n][0] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A, B[n][0], acc[a][n][0], 0, 0, 0);
-                acc[a + 2][n][0] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A, B[n][1], acc[a + 2][n][0], 0, 0, 0);
+                acc[a + QT][n][0] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A, B[n][1], acc[a + QT][n][0], 0, 0, 0);
             }
         }
         __builtin_amdgcn_sched_group_barrier(0x100, AW, 0);
@@ -522,7 +524,7 @@ __global__ __launch_bounds__(64 * WAVES, OCC) void k_dbpass_mfma_sa(const uint8_
     if constexpr (POW) {
         const uint32_t g = l >> 4, cl = l & 15;
 #pragma unroll
-        for (int h = 0; h < 2; h++)
+        for (int h = 0; h < QT; h++)
 #pragma unroll
             for (int i = 0; i < 4; i++) {
                 const uint32_t q = 16 * h + 4 * g + i;
@@ -530,7 +532,7 @@ __global__ __launch_bounds__(64 * WAVES, OCC) void k_dbpass_mfma_sa(const uint8_
                 for (int n = 0; n < NT; n++) {
                     int32_t pk[8];
 #pragma unroll
-                    for (int k = 0; k < 8; k++) pk[k] = acc[2 * k + h][n][0][i];
+                    for (int k = 0; k < 8; k++) pk[k] = acc[QT * k + h][n][0][i];
                     uint32_t* o = out + (size_t)q * out_qstride + (size_t)slice * 2 * OC + col0 + n * 16 + cl;
                     if (q < nq) {
                         const uint32_t r0 = pow_residue(pk, slice == 0 ? S[2 * q] : 0, 0);
@@ -713,16 +715,17 @@ DbPlan dbplan_mfma(size_t R, size_t ncols, size_t nq, size_t target_blocks, uint
     p.kind = DbPlan::MFMA;
     p.param = nq <= 2 ? 1 : nq <= 4 ? 2 : nq <= 8 ? 4 : nq <= 16 ? 8 : 16;  // 2 queries per 16-row digit tile
     p.dbtile = dbtile && p.param == 16;
-    p.crt = (crt || pow) && !p.dbtile && p.param == 16;
+    // 7 rows per query: 32 queries in 14 tiles (16 in 7 with power rows)
+    p.crt = !p.dbtile && ((crt && p.param == 16) || (pow && p.param >= 8));
     p.pow = pow && p.crt;
-    if (p.crt) p.param = 14;  // 7 rows per query: 32 queries in 14 tiles
+    if (p.crt) p.param = p.param == 16 ? 14 : 7;
     p.waves = p.param >= 8 && !p.dbtile && waves == 8 ? 8 : 4;
     if (p.pow) p.waves = 8;
     // ring depths the kernels are built with (what fits 256 VGPRs): 14 or 16
     // tiles with shared digits 3..7 (8-wave blocks) or 3..5, power rows
     // (two column tiles) 3..5, otherwise 3..4
     const uint32_t maxd = p.pow ? 5 : p.param >= 14 && !p.dbtile ? (p.waves == 8 ? 7 : 5) : 4;
-    p.depth = p.param >= 8 ? (depth < 3 ? 3 : depth > maxd ? maxd : depth) : 3;
+    p.depth = p.param >= 7 ? (depth < 3 ? 3 : depth > maxd ? maxd : depth) : 3;
     const size_t cols = dbplan_cols(p);
     // slices: whole number of 128-row (8-block) chunks so the step count is even
     p.slices = pick_slices(R / 16, ncols / cols, target_blocks, 8);
@@ -745,7 +748,7 @@ size_t dbplan_max_queries(const DbPlan& p) {
 }
 
 size_t dbplan_cols(const DbPlan& p) {
-    return p.kind == DbPlan::MFMA ? 16 * (p.pow ? 2 : mfma_nt(p.param)) * p.waves : 256;
+    return p.kind == DbPlan::MFMA ? 16 * (p.pow ? 2 : mfma_nt(p.param)) * p.waves : 256;  // power rows: 2 tiles
 }
 
 std::string dbplan_kernel(const DbPlan& p) {
@@ -754,7 +757,7 @@ std::string dbplan_kernel(const DbPlan& p) {
     if (p.param < 8) return "k_dbpass_mfma<" + mt_nt + ">";
     if (p.dbtile) return "k_dbpass_mfma_bs<16, 4, " + std::to_string(p.depth) + ">";
     // every template argument, defaults included (as profilers demangle it)
-    return "k_dbpass_mfma_sa<" + (p.pow ? std::string("14, 2") : mt_nt) + ", 2, " + std::to_string(p.depth) + ", " +
+    return "k_dbpass_mfma_sa<" + (p.pow ? std::to_string(p.param) + ", 2" : mt_nt) + ", 2, " + std::to_string(p.depth) + ", " +
            std::to_string(p.waves) + ", " + (p.pow ? "1>" : "0>");
 }
 
@@ -799,7 +802,8 @@ hipError_t launch_dbpass_mfma(const uint8_t* db, const int8_t* dig, const uint32
     const uint32_t Mp = (uint32_t)dbplan_mp(p);
     const uint32_t kbps = (uint32_t)(R / 16) / p.slices;
     const size_t cols = dbplan_cols(p);
-    if (nq > dbplan_max_queries(p) || (p.crt && p.param != 14) || kbps % 8 || ncols % cols || col0 % cols ||
+    if (nq > dbplan_max_queries(p) || (p.crt && p.param != 14 && !(p.pow && p.param == 7)) || kbps % 8 ||
+        ncols % cols || col0 % cols ||
         col0 + ncols > C || ostride < ncols)
         return hipErrorInvalidValue;
     const uint32_t Cc = (uint32_t)C, OC = (uint32_t)ostride, Q = (uint32_t)nq;
@@ -809,8 +813,8 @@ hipError_t launch_dbpass_mfma(const uint8_t* db, const int8_t* dig, const uint32
     hipLaunchKernelGGL((K<__VA_ARGS__>), grid, dim3(256), 0, s, d, dig, S, out, out_qstride, Cc, OC, Q, kbps)
 #define MFMA8(K, ...) \
     hipLaunchKernelGGL((K<__VA_ARGS__, 8>), grid, dim3(512), 0, s, d, dig, S, out, out_qstride, Cc, OC, Q, kbps)
-#define MFMA8P(D)                                                                                             \
-    hipLaunchKernelGGL((k_dbpass_mfma_sa<14, 2, 2, D, 8, 1>), grid, dim3(512), 0, s, d, dig, S, out, out_qstride, \
+#define MFMA8P(MT, D)                                                                                          \
+    hipLaunchKernelGGL((k_dbpass_mfma_sa<MT, 2, 2, D, 8, 1>), grid, dim3(512), 0, s, d, dig, S, out, out_qstride, \
                        Cc, OC, Q, kbps)
     switch (p.param) {
         case 1: MFMA(k_dbpass_mfma, 1, 4); break;
@@ -828,9 +832,9 @@ hipError_t launch_dbpass_mfma(const uint8_t* db, const int8_t* dig, const uint32
         case 14:
             if (p.pow) {
                 switch (p.depth) {
-                    case 3: MFMA8P(3); break;
-                    case 4: MFMA8P(4); break;
-                    case 5: MFMA8P(5); break;
+                    case 3: MFMA8P(14, 3); break;
+                    case 4: MFMA8P(14, 4); break;
+                    case 5: MFMA8P(14, 5); break;
                     default: return hipErrorInvalidValue;
                 }
                 break;
@@ -844,6 +848,15 @@ hipError_t launch_dbpass_mfma(const uint8_t* db, const int8_t* dig, const uint32
                 case 58: MFMA8(k_dbpass_mfma_sa, 14, 1, 2, 5); break;
                 case 68: MFMA8(k_dbpass_mfma_sa, 14, 1, 2, 6); break;
                 case 78: MFMA8(k_dbpass_mfma_sa, 14, 1, 2, 7); break;
+                default: return hipErrorInvalidValue;
+            }
+            break;
+        case 7:
+            if (!p.pow) return hipErrorInvalidValue;
+            switch (p.depth) {
+                case 3: MFMA8P(7, 3); break;
+                case 4: MFMA8P(7, 4); break;
+                case 5: MFMA8P(7, 5); break;
                 default: return hipErrorInvalidValue;
             }
             break;

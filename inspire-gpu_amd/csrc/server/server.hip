@@ -399,87 +399,99 @@ void GpuServerCtx::autotune_dbpass() {
     hipEvent_t e0, e1;
     HIPX(hipEventCreate(&e0));
     HIPX(hipEventCreate(&e1));
-    auto time_plan = [&](const DbPlan& x, size_t nq) {
-        if (x.kind == DbPlan::MFMA) HIPX(launch_mfma_digits(d_bq, nq, R, x, d_dig[0], d_S[0], stream));
-        auto run = [&]() {
-            if (x.kind == DbPlan::F64)
-                HIPX(launch_dbpass_f64(db_planes(), d_bq, d_part, C, R, C, 0, C, x, stream));
-            else
-                HIPX(launch_dbpass_mfma(db_planes(), d_dig[0], d_S[0], nq, d_part, part_stride, C, R, C, 0, C, x,
-                                        stream));
-        };
-        run();  // warm
-        HIPX(hipEventRecord(e0, stream));
-        for (int it = 0; it < 3; it++) run();
-        HIPX(hipEventRecord(e1, stream));
-        HIPX(hipEventSynchronize(e1));
-        float ms = 0;
-        HIPX(hipEventElapsedTime(&ms, e0, e1));
-        return ms;
+    // Candidates are timed round-robin: every round runs each candidate once
+    // (after its own preparation, untimed) and the first round only warms
+    // up. Under sustained int8-MFMA work the power limit lowers the clock
+    // within tens of milliseconds, so timing the candidates one after
+    // another would favour whichever ran first.
+    auto fastest = [&](size_t n, auto&& prep, auto&& run) {
+        std::vector<float> tot(n, 0.f);
+        for (int round = 0; round < 4; round++)
+            for (size_t i = 0; i < n; i++) {
+                prep(i);
+                HIPX(hipEventRecord(e0, stream));
+                run(i);
+                HIPX(hipEventRecord(e1, stream));
+                HIPX(hipEventSynchronize(e1));
+                float ms = 0;
+                HIPX(hipEventElapsedTime(&ms, e0, e1));
+                if (round) tot[i] += ms;
+            }
+        size_t b = 0;
+        for (size_t i = 1; i < n; i++)
+            if (tot[i] < tot[b]) b = i;
+        return b;
     };
-    float best = 1e30f;
-    for (const DbCand& c : kDbCands) {
-        if (plan1_forced) break;
-        const DbPlan x = cand_plan(c, R, C);
-        const float ms = time_plan(x, 1);
-        if (ms < best) { best = ms; plan1 = x; plan1_target = c.target; }
+    // one full-width database pass of plan x over nq staged queries (zeroed)
+    auto prep_plan = [&](const DbPlan& x, size_t nq) {
+        if (x.kind == DbPlan::MFMA) HIPX(launch_mfma_digits(d_bq, nq, R, x, d_dig[0], d_S[0], stream));
+    };
+    auto run_plan = [&](const DbPlan& x, size_t nq) {
+        if (x.kind == DbPlan::F64)
+            HIPX(launch_dbpass_f64(db_planes(), d_bq, d_part, C, R, C, 0, C, x, stream));
+        else
+            HIPX(launch_dbpass_mfma(db_planes(), d_dig[0], d_S[0], nq, d_part, part_stride, C, R, C, 0, C, x, stream));
+    };
+    if (!plan1_forced) {
+        std::vector<DbPlan> xs;
+        for (const DbCand& c : kDbCands) xs.push_back(cand_plan(c, R, C));
+        const size_t b = fastest(
+            xs.size(), [&](size_t i) { prep_plan(xs[i], 1); }, [&](size_t i) { run_plan(xs[i], 1); });
+        plan1 = xs[b];
+        plan1_target = kDbCands[b].target;
     }
     // batched pass: the workgroup target at the largest batch this context serves
     const size_t nq = cfg.max_batch < kMfmaMaxQueries ? cfg.max_batch : kMfmaMaxQueries;
     if (nq > 1) {
-        best = 1e30f;
         const uint32_t param = dbplan_mfma(R, C, nq, mfma_target).param;
-        // forms (dbplan_form): the shared DB tile and the 7-digit forms exist
-        // for 17..32 queries only; the 7-digit forms regrouped through LDS
-        // (3, 4) never beat the power rows (5) and are timed only when forced
+        // forms (dbplan_form): the shared DB tile and the 7-digit forms
+        // regrouped through LDS exist for 17..32 queries only, the power rows
+        // for 9..32; the LDS-regrouped forms (3, 4) never beat the power rows
+        // (5) and are timed only when forced
         const int forced_form = mfma_form;
-        std::vector<DbPlan> timed;  // targets that give the same grid are timed once
+        struct Cand {
+            DbPlan x;
+            size_t target;
+            uint32_t depth;
+            int form;
+        };
+        std::vector<Cand> cands;  // targets that give the same grid are timed once
         auto seen = [&](const DbPlan& x) {
-            for (const DbPlan& y : timed)
-                if (y.param == x.param && y.slices == x.slices && y.depth == x.depth && y.dbtile == x.dbtile &&
-                    y.waves == x.waves && y.crt == x.crt && y.pow == x.pow)
+            for (const Cand& y : cands)
+                if (y.x.param == x.param && y.x.slices == x.slices && y.x.depth == x.depth &&
+                    y.x.dbtile == x.dbtile && y.x.waves == x.waves && y.x.crt == x.crt && y.x.pow == x.pow)
                     return true;
-            timed.push_back(x);
             return false;
         };
         for (size_t t : kMfmaTargets)
             for (int form = 0; form < kMfmaForms; form++) {
                 if (depth_forced ? form != forced_form
-                                 : (form != 0 && form != 2 && param != 16) || (form > 0 && param < 8) || form == 3 ||
-                                       form == 4)
+                                 : (form == 1 && param != 16) || (form > 0 && param < 8) || form == 3 || form == 4)
                     continue;
                 for (uint32_t d : kMfmaDepths) {
                     if ((depth_forced || param < 8) && d != mfma_depth) continue;
                     const DbPlan x = dbplan_form(R, C, nq, t, d, form);
                     if (x.depth != d && param >= 8) continue;  // a depth this kernel is not built with
-                    if (seen(x)) continue;
-                    const float ms = time_plan(x, nq);
-                    if (ms < best) {
-                        best = ms;
-                        mfma_target = t;
-                        mfma_depth = d;
-                        mfma_form = form;
-                    }
+                    if (!seen(x)) cands.push_back({x, t, d, form});
                 }
             }
+        if (!cands.empty()) {  // empty: a forced depth the forced form is not built with keeps the defaults
+            const size_t b = fastest(
+                cands.size(), [&](size_t i) { prep_plan(cands[i].x, nq); },
+                [&](size_t i) { run_plan(cands[i].x, nq); });
+            mfma_target = cands[b].target;
+            mfma_depth = cands[b].depth;
+            mfma_form = cands[b].form;
+        }
         // the 17..32-query collapse pass form, over all groups (zeroed keys)
         if (nq > 16) {
             HIPX(hipMemset(d_keys, 0, nq * keys_stride * 4));
             HIPX(launch_collapse_keys(d_keys, keys_stride, nq, d_kdig[0], d_KS[0], stream));
-            float cbest = 1e30f;
-            for (int v = 0; v < kCollapseVariants; v++) {
-                auto run = [&]() {
-                    HIPX(launch_collapse(d_precomp, G, d_kdig[0], d_KS[0], nq, d_part2, part2_stride, stream, v));
-                };
-                run();
-                HIPX(hipEventRecord(e0, stream));
-                for (int it = 0; it < 3; it++) run();
-                HIPX(hipEventRecord(e1, stream));
-                HIPX(hipEventSynchronize(e1));
-                float ms = 0;
-                HIPX(hipEventElapsedTime(&ms, e0, e1));
-                if (ms < cbest) { cbest = ms; collapse_variant = v; }
-            }
+            collapse_variant = (int)fastest(
+                (size_t)kCollapseVariants, [](size_t) {},
+                [&](size_t v) {
+                    HIPX(launch_collapse(d_precomp, G, d_kdig[0], d_KS[0], nq, d_part2, part2_stride, stream, (int)v));
+                });
         }
         // the pipelined answer's chunk plan: the full-width kernel over one
         // chunk's columns with a row-slice count timed on whole answers of a
@@ -491,29 +503,21 @@ void GpuServerCtx::autotune_dbpass() {
             const DbPlan full = plan_for(nq, C);
             const size_t blocks = ccols / dbplan_cols(full);
             HIPX(hipMemset(d_q, 0, nq * qlen * 8));
-            best = 1e30f;
-            uint32_t pick = 0;
+            std::vector<uint32_t> sls;
             for (uint32_t sl = 0; sl <= 16; sl = sl ? 2 * sl : 1) {  // 0: the narrowing rule
                 if (sl) {
                     const DbPlan x = dbplan_form(R, ccols, nq, blocks * sl, mfma_depth, mfma_form);
                     if (x.slices != sl || (sl > 1 && sl * 2 * ccols > part_stride)) continue;
                 }
+                sls.push_back(sl);
+            }
+            auto use = [&](uint32_t sl) {
                 chunk_slices = sl;
                 chunk_param = sl ? full.param : 0;
-                enqueue(d_q, nq, d_resp, stream);  // warm
-                HIPX(hipEventRecord(e0, stream));
-                for (int it = 0; it < 2; it++) enqueue(d_q, nq, d_resp, stream);
-                HIPX(hipEventRecord(e1, stream));
-                HIPX(hipEventSynchronize(e1));
-                float ms = 0;
-                HIPX(hipEventElapsedTime(&ms, e0, e1));
-                if (ms < best) {
-                    best = ms;
-                    pick = sl;
-                }
-            }
-            chunk_slices = pick;
-            chunk_param = pick ? full.param : 0;
+            };
+            const size_t b = fastest(
+                sls.size(), [&](size_t i) { use(sls[i]); }, [&](size_t) { enqueue(d_q, nq, d_resp, stream); });
+            use(sls[b]);
         }
     }
     (void)hipEventDestroy(e0);
