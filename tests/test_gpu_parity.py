@@ -198,15 +198,15 @@ def test_many_query_ring_depths_match_oracle(medium8, depth, form):
     rng = np.random.default_rng(400 + depth)
     name = C.create_string_buffer(128)
     assert srv.lib.ipir_dbg_dbpass_kernel(srv.h, MB8, name, 128) == 0
-    # k_dbpass_mfma_sa<16|14, 1, 2, depth[, 8]>, k_dbpass_mfma_sa<14, 2, 2, depth, 8, 1> or k_dbpass_mfma_bs<16, 4, depth>
+    # k_dbpass_mfma_sa<tiles, col tiles, 2, depth, waves, power rows> or k_dbpass_mfma_bs<16, 4, depth>
     kname = name.value.decode()
     args = [int(x) for x in kname.split("<")[1].split(">")[0].split(",")]
     if "_bs<" in kname:
         tuned = (args[2], 1)
-    elif len(args) == 6:
+    elif args[5]:
         tuned = (args[3], 5)
     else:
-        tuned = (args[3], (3 if args[0] == 14 else 2) if len(args) == 5 else (4 if args[0] == 14 else 0))
+        tuned = (args[3], (3 if args[0] == 14 else 2) if args[4] == 8 else (4 if args[0] == 14 else 0))
     assert srv.lib.ipir_dbg_set_mfma_depth(srv.h, 2, 0) == 1
     assert srv.lib.ipir_dbg_set_mfma_depth(srv.h, 6, 0) == 1
     assert srv.lib.ipir_dbg_set_mfma_depth(srv.h, 5, 1) == 1
