@@ -291,11 +291,12 @@ void GpuServerCtx::alloc_slots() {
         need(narrow_plan(full, c.target, R, C, ccols, 1), ccols);
     }
     for (size_t t : kMfmaTargets)
-        for (size_t nq = 1; nq <= kMfmaMaxQueries; nq++) {
-            const DbPlan full = dbplan_mfma(R, C, nq, t);
-            need(full, C);
-            need(narrow_plan(full, t, R, C, ccols, nq), ccols);
-        }
+        for (size_t nq = 1; nq <= kMfmaMaxQueries; nq++)
+            for (int form = 0; form < kMfmaForms; form++) {  // the slice count depends on the block's columns
+                const DbPlan full = dbplan_form(R, C, nq, t, 3, form);
+                need(full, C);
+                need(narrow_plan(full, t, R, C, ccols, nq), ccols);
+            }
     lay.R = R; lay.C = C; lay.G = G;
     qlen = lay.query_u64s();
     part_stride = part_words;
@@ -432,13 +433,23 @@ void GpuServerCtx::autotune_dbpass() {
         else
             HIPX(launch_dbpass_mfma(db_planes(), d_dig[0], d_S[0], nq, d_part, part_stride, C, R, C, 0, C, x, stream));
     };
+    // a plan whose row-slice partials would not fit their buffer is not run
+    auto fits = [&](const DbPlan& x) { return x.slices == 1 || x.slices * 2 * C <= part_stride; };
     if (!plan1_forced) {
         std::vector<DbPlan> xs;
-        for (const DbCand& c : kDbCands) xs.push_back(cand_plan(c, R, C));
-        const size_t b = fastest(
-            xs.size(), [&](size_t i) { prep_plan(xs[i], 1); }, [&](size_t i) { run_plan(xs[i], 1); });
-        plan1 = xs[b];
-        plan1_target = kDbCands[b].target;
+        std::vector<size_t> ts;
+        for (const DbCand& c : kDbCands) {
+            const DbPlan x = cand_plan(c, R, C);
+            if (!fits(x)) continue;
+            xs.push_back(x);
+            ts.push_back(c.target);
+        }
+        if (!xs.empty()) {
+            const size_t b = fastest(
+                xs.size(), [&](size_t i) { prep_plan(xs[i], 1); }, [&](size_t i) { run_plan(xs[i], 1); });
+            plan1 = xs[b];
+            plan1_target = ts[b];
+        }
     }
     // batched pass: the workgroup target at the largest batch this context serves
     const size_t nq = cfg.max_batch < kMfmaMaxQueries ? cfg.max_batch : kMfmaMaxQueries;
@@ -472,7 +483,7 @@ void GpuServerCtx::autotune_dbpass() {
                     if ((depth_forced || param < 8) && d != mfma_depth) continue;
                     const DbPlan x = dbplan_form(R, C, nq, t, d, form);
                     if (x.depth != d && param >= 8) continue;  // a depth this kernel is not built with
-                    if (!seen(x)) cands.push_back({x, t, d, form});
+                    if (fits(x) && !seen(x)) cands.push_back({x, t, d, form});
                 }
             }
         if (!cands.empty()) {  // empty: a forced depth the forced form is not built with keeps the defaults

@@ -408,6 +408,80 @@ def test_generation_rollover_while_serving(medium8):
     assert np.array_equal(ol.extract(geo, st, r.reshape(geo["T"], -1)), ol.expected_slots(geo, db, 424242))
 
 
+@pytest.fixture(scope="module")
+def tall():
+    """2^16 DB rows (one group) with a workgroup target of 1 and no
+    autotuning: the plans then cut rows into slices only where the exactness
+    limits require it (one slice of 2^16 rows for the 16-tile forms, two of
+    2^15 for the power-row forms)."""
+    import os
+    n, w, rows = 2 ** 21, 120, 65536
+    rc, geo = ol.setup(n, w, rows)
+    assert rc == 0 and geo["R"] == 65536 and geo["C"] == 2048
+    saved = {k: os.environ.get(k) for k in ("INSPIRE_AUTOTUNE", "INSPIRE_MFMA_TARGET")}
+    os.environ["INSPIRE_AUTOTUNE"] = "0"
+    os.environ["INSPIRE_MFMA_TARGET"] = "1"
+    try:
+        say("[tall] GPU create")
+        zero = ia.Server(np.zeros((geo["R"], geo["C"]), np.uint16), n, w, rows, max_batch=32,
+                         crs_seed=bytes(SEED))
+        db = np.random.default_rng(44).integers(0, P, size=(geo["R"], geo["C"]), dtype=np.uint16)
+        rnd = ia.Server(db, n, w, rows, max_batch=32, crs_seed=bytes(SEED))
+    finally:
+        for k, v in saved.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+    yield geo, zero, rnd
+    zero.close()
+    rnd.close()
+
+
+@pytest.mark.parametrize("form", [0, 1, 2, 3, 4, 5])
+def test_slice_limits_hold_at_extreme_values(tall, form):
+    """Largest dot products the int8 passes can meet: an all-zero DB (every
+    plane byte reads as -128) against query rows whose CRT value has six
+    balanced digits -128 and top digit 32 (each power-row sum grows by
+    2 x 2^14 per row: 2^31 over 2^16 rows, past i32), or whose per-limb
+    residues have digits -128, -128, -128, 5 (each per-plane sum grows by
+    2^14 per row). The exact product is 0, so a slice past its limit shows
+    as a nonzero residue. Then a random DB against the oracle's mat-vec."""
+    geo, zero, rnd = tall
+    R, Cc = geo["R"], geo["C"]
+    B = 2 ** 53 - 0x808080808080  # CRT value, balanced digits: six times -128, then 32
+    b4 = 5 * 2 ** 24 - 0x808080   # residue, balanced digits: three times -128, then 5
+    exts = [np.array([B % Q0, B % Q1], np.uint64), np.array([b4, b4], np.uint64)]
+    for srv in (zero, rnd):
+        assert srv.lib.ipir_dbg_set_mfma_depth(srv.h, 3, form) == 0
+    for nq in (16, 32):
+        name = C.create_string_buffer(128)
+        assert zero.lib.ipir_dbg_dbpass_kernel(zero.h, nq, name, 128) == 0
+        slices = int(name.value.decode().split("slices=")[1])
+        assert slices == (2 if form == 5 else 1), name.value  # the limits, not grid filling, cut the rows
+        for ext in exts:
+            b = np.broadcast_to(ext[None, :, None], (nq, 2, R)).copy()
+            got = np.ones((nq, 2, Cc), np.uint32)
+            assert zero.lib.ipir_dbg_db_pass(zero.h, ia.ptr(b), nq, 1, ia.ptr(got)) == 0
+            assert not got.any(), f"form {form} nq {nq}: {np.count_nonzero(got)} nonzero residues"
+    ext = exts[0]
+    enc = np.zeros((R, Cc), np.uint16)
+    assert rnd.lib.ipir_dbg_db_rows(rnd.h, 0, R, ia.ptr(enc)) == 0
+    rng = np.random.default_rng(500 + form)
+    nq = 17
+    b = np.zeros((nq, 2, R), np.uint64)
+    b[:, 0] = rng.integers(0, Q0, (nq, R), dtype=np.uint64)
+    b[:, 1] = rng.integers(0, Q1, (nq, R), dtype=np.uint64)
+    b[-1] = ext[:, None]
+    got = np.zeros((nq, 2, Cc), np.uint32)
+    assert rnd.lib.ipir_dbg_db_pass(rnd.h, ia.ptr(b), nq, 1, ia.ptr(got)) == 0
+    for k in (0, nq - 1):
+        want = np.zeros(2 * Cc, np.uint64)
+        ol.oracle().orc_matvec(enc.reshape(-1), R, Cc, np.ascontiguousarray(b[k, 0]), np.ascontiguousarray(b[k, 1]),
+                               0, Cc, want)
+        assert np.array_equal(got[k].reshape(-1).astype(np.uint64), want), f"form {form} query {k}"
+
+
 def test_batch_validation_codes(medium8):
     geo, db, srv = medium8
     lib = srv.lib
