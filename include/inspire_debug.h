@@ -67,8 +67,8 @@ int ipir_dbg_horner_variant(const uint64_t* cts, size_t L, const uint64_t* rgsw,
  * 1 if the partial-sum buffer cannot hold the resulting plans). */
 int ipir_dbg_set_group_chunk(ipir_server* srv, size_t groups);
 /* Select the Horner kernel a server's answers use: mode as variant above,
- * or -1 (the default) to use the pair only for calls with at most two
- * chains. */
+ * or -1 (the default) to use the pair only for serial answers with at most
+ * two chains. */
 int ipir_dbg_set_horner(ipir_server* srv, int mode);
 /* Set the register-ring depth of the batched database passes a server uses
  * and their form: 0 the waves of a 4-wave block share the digit tile (depth
@@ -89,7 +89,7 @@ int ipir_dbg_set_collapse_variant(ipir_server* srv, int variant);
 int ipir_dbg_get_collapse_variant(const ipir_server* srv);
 /* Set the pipeline shape of one-query and many-query answers: 0 serial, 2
  * Horner chains on a second stream per group chunk, 3 the collapse on a
- * third stream as well; 1 for other values. */
+ * third stream as well (defaults 3 and 2); 1 for other values. */
 int ipir_dbg_set_pipeline(ipir_server* srv, int single, int batched);
 /* Per-stage device time (ms) of the last ipir_dbg_profile_batch run:
  * [unmarshal+key NTT, DB pass, collapse pass, pack finalize, Horner]. */

@@ -616,15 +616,16 @@ void GpuServerCtx::stage_finalize(size_t count, hipStream_t s, size_t slot0) {
                               d_packed_b + slot0 * packed_stride, packed_stride, count, nc, s));
 }
 
-bool GpuServerCtx::use_pair(size_t chains) const {
-    // auto: the pair kernel shortens a lone chain (its per-step exchange costs
-    // more than it saves once many chains share the L2s)
-    return horner_mode == 1 || (horner_mode < 0 && chains <= kPairMaxChains);
+bool GpuServerCtx::use_pair(size_t chains, bool beside_passes) const {
+    // auto: the pair kernel shortens a lone chain run by itself (its per-step
+    // exchange costs more than it saves once many chains share the L2s, or
+    // while database passes stream through them)
+    return horner_mode == 1 || (horner_mode < 0 && chains <= kPairMaxChains && !beside_passes);
 }
 
 void GpuServerCtx::horner_range(size_t count, uint64_t* d_out, size_t g_lo, size_t g_hi, hipStream_t s,
-                                size_t slot0) {
-    if (use_pair(count * T))
+                                size_t slot0, bool beside_passes) {
+    if (use_pair(count * T, beside_passes))
         HIPX(launch_horner_pair(d_precomp + PC_A, PC_GROUP, G, Da, T, d_packed_b + slot0 * packed_stride,
                                 packed_stride, d_rgsw + slot0 * rgsw_stride, rgsw_stride, d_out,
                                 d_hstate + slot0 * T * 4 * N, g_lo, g_hi, count,
@@ -697,7 +698,7 @@ void GpuServerCtx::enqueue(const uint64_t* d_in, size_t count, uint64_t* d_out, 
             HIPX(hipStreamWaitEvent(hs, ce, 0));
             HIPX(launch_pack_finalize(d_interm + q0 * 2 * C, C, p2, part2_stride, d_precomp, g_lo, gchunk, keys,
                                       keys_stride, packed, packed_stride, nq, nc, hs));
-            horner_range(nq, out, g_lo, g_hi, hs, q0);
+            horner_range(nq, out, g_lo, g_hi, hs, q0, true);
         }
         if (cs != s) {  // the next batch's key digits overwrite the lane's scratch
             HIPX(hipEventRecord(prep_ev[lane], cs));

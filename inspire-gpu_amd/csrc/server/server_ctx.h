@@ -45,7 +45,7 @@ struct GpuServerCtx {
     size_t gchunk = 0;              // groups per chunk of the pipelined answer
     // pipeline shape of one-query and many-query calls: 0 serial on the
     // caller's stream, 2 Horner chains beside the passes, 3 collapse beside too
-    int pipe_single = 0, pipe_batched = 2;
+    int pipe_single = 3, pipe_batched = 2;
     size_t qlen = 0, resp_u64s = 0;
     size_t part_stride = 0, part2_stride = 0, keys_stride = 0, rgsw_stride = 0, packed_stride = 0;
     uint64_t* d_q = nullptr;        // flat queries (device staging)
@@ -75,7 +75,8 @@ struct GpuServerCtx {
     hipEvent_t horner_done[kLanes] = {};
     uint32_t* d_packed_b = nullptr;
     // Horner: 0 = one workgroup per chain, 1 = a workgroup pair (one per
-    // limb), -1 = pair when a call has at most kPairMaxChains chains
+    // limb), -1 = pair when a call has at most kPairMaxChains chains and its
+    // answer is serial (beside the passes the pair's exchange is slower)
     int horner_mode = -1;
     static constexpr size_t kPairMaxChains = 2;
     uint32_t* d_hxbuf = nullptr;    // pair exchange buffers [B*T chains]
@@ -110,9 +111,10 @@ struct GpuServerCtx {
     void dbpass_range(size_t q0, size_t nq, size_t g_lo, size_t g_hi, const ipx::DbPlan& p, hipStream_t s, int lane);
     // force: -1 the tuned plans, 0 the fp64 pass per query, 1 the MFMA pass
     void stage_dbpass(size_t count, hipStream_t s, size_t slot0, int lane = 0, int force = -1);
-    bool use_pair(size_t chains) const;
+    bool use_pair(size_t chains, bool beside_passes) const;
     // Horner chains of `count` queries over groups [g_lo, g_hi).
-    void horner_range(size_t count, uint64_t* d_out, size_t g_lo, size_t g_hi, hipStream_t s, size_t slot0);
+    void horner_range(size_t count, uint64_t* d_out, size_t g_lo, size_t g_hi, hipStream_t s, size_t slot0,
+                      bool beside_passes = false);
     void stage_collapse(size_t count, hipStream_t s, size_t slot0, int lane = 0);
     void stage_finalize(size_t count, hipStream_t s, size_t slot0);
     void stage_horner(size_t count, uint64_t* d_out, hipStream_t s, size_t slot0);
