@@ -113,9 +113,10 @@ __global__ __launch_bounds__(256) void k_pack_finalize(const uint32_t* __restric
 // acc (a0, a1, b0, b1 polys, NTT form) lives in registers between steps;
 // per step: 4 INTTs, gadget split, 2 rounds of 4 NTTs, multiply-accumulate
 // against the query's RGSW rows, + packed[i]. L == 1 copies packed[start].
-// Each round's RGSW rows (and the step's packed parts) are loaded one phase
-// ahead, so their latency (HBM latency while a database pass streams
-// through L2) hides behind the transforms instead of adding to every step.
+// The RGSW rows are loop-invariant (the compiler keeps them in registers
+// across steps); the step's packed parts are loaded during round 0, a phase
+// ahead of their use, so HBM latency (long while a database pass streams
+// through L2) hides behind the round-1 transforms.
 __global__ __launch_bounds__(1024) void k_horner(const uint32_t* __restrict__ pa_base, size_t pa_stride, uint32_t G, uint32_t Da,
                                                  uint32_t T, const uint32_t* __restrict__ packed_b,
                                                  size_t pstride, const uint32_t* __restrict__ rgsw,

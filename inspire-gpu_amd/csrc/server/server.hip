@@ -343,6 +343,7 @@ void GpuServerCtx::alloc_slots() {
             (a2 == 0 || a2 == 2 || a2 == 3)) {
             pipe_single = a1;
             pipe_batched = a2;
+            pipe_forced = true;
         }
     }
     // packed-wire path: packed queries, coefficient-form scratch, compressed responses
@@ -529,6 +530,20 @@ void GpuServerCtx::autotune_dbpass() {
             const size_t b = fastest(
                 sls.size(), [&](size_t i) { use(sls[i]); }, [&](size_t) { enqueue(d_q, nq, d_resp, stream); });
             use(sls[b]);
+        }
+    }
+    // the pipeline shape of one-query and of many-query answers: Horner
+    // chains beside the passes (2), or the collapse beside the database
+    // pass as well (3), timed on whole answers of a synthetic batch
+    if (G / gchunk > 1 && !pipe_forced) {
+        HIPX(hipMemset(d_q, 0, (nq > 1 ? nq : 1) * qlen * 8));
+        const int shapes[2] = {2, 3};
+        for (const size_t count : {(size_t)1, nq}) {
+            int* target = count == 1 ? &pipe_single : &pipe_batched;
+            const size_t b = fastest(
+                2, [&](size_t i) { *target = shapes[i]; }, [&](size_t) { enqueue(d_q, count, d_resp, stream); });
+            *target = shapes[b];
+            if (nq == 1) break;
         }
     }
     (void)hipEventDestroy(e0);

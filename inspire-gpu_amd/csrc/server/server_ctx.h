@@ -44,8 +44,10 @@ struct GpuServerCtx {
     int collapse_variant = 0;       // 17..32-query collapse pass form (launch_collapse)
     size_t gchunk = 0;              // groups per chunk of the pipelined answer
     // pipeline shape of one-query and many-query calls: 0 serial on the
-    // caller's stream, 2 Horner chains beside the passes, 3 collapse beside too
+    // caller's stream, 2 Horner chains beside the passes, 3 collapse beside
+    // too (2 or 3 timed at setup)
     int pipe_single = 3, pipe_batched = 2;
+    bool pipe_forced = false;       // INSPIRE_PIPELINE set the shapes (not autotuned)
     size_t qlen = 0, resp_u64s = 0;
     size_t part_stride = 0, part2_stride = 0, keys_stride = 0, rgsw_stride = 0, packed_stride = 0;
     uint64_t* d_q = nullptr;        // flat queries (device staging)
