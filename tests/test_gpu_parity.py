@@ -258,6 +258,34 @@ def test_batch_equals_singles_and_extracts(medium8):
         assert np.array_equal(part, batch[:k])
 
 
+def test_ragged_batch_over_sub_batches(medium8):
+    """A batch of 33 on a server of max_batch 40: a 32-query sub-batch then a
+    one-query sub-batch, each with its own pipeline shape and Horner kernel,
+    queued on the same streams; equals the serial answer and extracts."""
+    geo, db, _ = medium8
+    n, w, rows = 524288, 120, 2048
+    srv = ia.Server(db, n, w, rows, max_batch=40, crs_seed=bytes(SEED))
+    try:
+        idxs = [(37 * k + 5) % n for k in range(33)]
+        qs, sts = [], []
+        for t, idx in enumerate(idxs):
+            q, st = ol.make_query(geo, SEED, idx, 9000 + t)
+            qs.append(q)
+            sts.append(st)
+        assert srv.lib.ipir_dbg_set_group_chunk(srv.h, geo["G"]) == 0
+        serial = srv.answer_batch(qs)
+        assert srv.lib.ipir_dbg_set_group_chunk(srv.h, 2) == 0  # four chunks: pipelined
+        for shapes in ((3, 2), (2, 3), (3, 3)):
+            assert srv.lib.ipir_dbg_set_pipeline(srv.h, *shapes) == 0
+            got = srv.answer_batch(qs)
+            assert np.array_equal(got, serial), f"shapes {shapes}"
+        for i in (0, 31, 32):
+            assert np.array_equal(ol.extract(geo, sts[i], got[i].reshape(geo["T"], -1)),
+                                  ol.expected_slots(geo, db, idxs[i]))
+    finally:
+        srv.close()
+
+
 def test_collapse_variants_equal(medium8):
     """Every form of the 17..32-query collapse pass (two-step prefetch;
     load rings of depth 3..5 in 4- or 8-wave blocks) gives the same
