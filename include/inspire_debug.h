@@ -80,8 +80,9 @@ int ipir_dbg_set_horner(ipir_server* srv, int mode);
  * 1 and 4 and 2 for 3. 1 for other values. */
 int ipir_dbg_set_mfma_depth(ipir_server* srv, uint32_t depth, int form);
 /* Row slices of the pipelined answer's database-pass chunks for batches
- * planned like an nq-query batch (0: back to the narrowing rule; a power of
- * two up to 64, used when the partial-sum buffer holds it). */
+ * planned like an nq-query batch, or for one-query answers when nq is 1
+ * (0: back to the narrowing rule; a power of two up to 64, used when the
+ * plan reaches it and the partial-sum buffer holds it). */
 int ipir_dbg_set_chunk_slices(ipir_server* srv, size_t nq, uint32_t slices);
 /* Form of the 17..32-query collapse pass (0..5; 1 otherwise), and the one
  * in use (the setup-time autotuner's pick unless set). */

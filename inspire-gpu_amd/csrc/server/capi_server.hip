@@ -437,9 +437,13 @@ int ipir_dbg_set_collapse_variant(ipir_server* srv, int variant) {
 int ipir_dbg_get_collapse_variant(const ipir_server* srv) { return srv ? srv->ctx->collapse_variant : -1; }
 
 int ipir_dbg_set_chunk_slices(ipir_server* srv, size_t nq, uint32_t slices) {
-    if (!srv || nq < 2 || nq > ipx::kMfmaMaxQueries || slices > 64 || (slices & (slices - 1))) return 1;
+    if (!srv || nq < 1 || nq > ipx::kMfmaMaxQueries || slices > 64 || (slices & (slices - 1))) return 1;
     GpuServerCtx* c = srv->ctx;
     std::lock_guard<std::mutex> lk(c->mu);
+    if (nq == 1) {
+        c->chunk_slices1 = slices;
+        return 0;
+    }
     c->chunk_slices = slices;
     c->chunk_param = slices ? c->plan_for(nq, c->pp.db_cols).param : 0;
     return 0;

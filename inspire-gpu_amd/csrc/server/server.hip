@@ -406,9 +406,9 @@ void GpuServerCtx::autotune_dbpass() {
     // up. Under sustained int8-MFMA work the power limit lowers the clock
     // within tens of milliseconds, so timing the candidates one after
     // another would favour whichever ran first.
-    auto fastest = [&](size_t n, auto&& prep, auto&& run) {
+    auto fastest = [&](size_t n, auto&& prep, auto&& run, int rounds = 4) {
         std::vector<float> tot(n, 0.f);
-        for (int round = 0; round < 4; round++)
+        for (int round = 0; round < rounds; round++)
             for (size_t i = 0; i < n; i++) {
                 prep(i);
                 HIPX(hipEventRecord(e0, stream));
@@ -505,44 +505,55 @@ void GpuServerCtx::autotune_dbpass() {
                     HIPX(launch_collapse(d_precomp, G, d_kdig[0], d_KS[0], nq, d_part2, part2_stride, stream, (int)v));
                 });
         }
-        // the pipelined answer's chunk plan: the full-width kernel over one
-        // chunk's columns with a row-slice count timed on whole answers of a
-        // synthetic (zero) batch, the Horner chains running beside the
-        // passes as they do when serving (a one-round grid can be held up by
-        // the few CUs the chains take; more slices pay ring fill and partials)
-        if (G / gchunk > 1) {
-            const size_t ccols = gchunk * N;
-            const DbPlan full = plan_for(nq, C);
+    }
+    // The pipelined answer's chunk plans and pipeline shapes, timed together
+    // on whole answers of a synthetic (zero) batch, one query and the
+    // largest batch, with the Horner chains beside the passes as when
+    // serving. A chunk's database pass runs the full-width plan's kernel over
+    // the chunk's columns with a row-slice count (0: the narrowing rule); the
+    // shape puts the collapse beside it (3) or after it (2). The two interact:
+    // a one-slice grid needs no reduction kernel and leaves CUs that only a
+    // collapse beside it uses; more slices fill the device alone but pay
+    // partials and their reduction.
+    if (G / gchunk > 1) {
+        const size_t ccols = gchunk * N;
+        HIPX(hipMemset(d_q, 0, nq * qlen * 8));
+        for (const size_t count : {(size_t)1, nq}) {
+            const DbPlan full = plan_for(count, C);
             const size_t blocks = ccols / dbplan_cols(full);
-            HIPX(hipMemset(d_q, 0, nq * qlen * 8));
-            std::vector<uint32_t> sls;
-            for (uint32_t sl = 0; sl <= 16; sl = sl ? 2 * sl : 1) {  // 0: the narrowing rule
+            struct Pick {
+                uint32_t slices;
+                int shape;
+            };
+            std::vector<Pick> cands;
+            for (uint32_t sl = 0; sl <= 8; sl = sl ? 2 * sl : 1) {
                 if (sl) {
-                    const DbPlan x = dbplan_form(R, ccols, nq, blocks * sl, mfma_depth, mfma_form);
+                    const size_t t = blocks * sl;
+                    const DbPlan x = count > 1 ? dbplan_form(R, ccols, count, t, mfma_depth, mfma_form)
+                                     : full.kind == DbPlan::F64 ? dbplan_f64(R, ccols, (int)full.param, t)
+                                                                : dbplan_mfma(R, ccols, 1, t);
                     if (x.slices != sl || (sl > 1 && sl * 2 * ccols > part_stride)) continue;
                 }
-                sls.push_back(sl);
+                for (const int shape : {2, 3}) {
+                    if (pipe_forced && shape != (count == 1 ? pipe_single : pipe_batched)) continue;
+                    cands.push_back({sl, shape});
+                }
             }
-            auto use = [&](uint32_t sl) {
-                chunk_slices = sl;
-                chunk_param = sl ? full.param : 0;
+            auto use = [&](const Pick& p) {
+                if (count == 1) {
+                    chunk_slices1 = p.slices;
+                    if (!pipe_forced) pipe_single = p.shape;
+                } else {
+                    chunk_slices = p.slices;
+                    chunk_param = p.slices ? full.param : 0;
+                    if (!pipe_forced) pipe_batched = p.shape;
+                }
             };
+            if (cands.empty()) continue;  // a forced serial shape: nothing to time
             const size_t b = fastest(
-                sls.size(), [&](size_t i) { use(sls[i]); }, [&](size_t) { enqueue(d_q, nq, d_resp, stream); });
-            use(sls[b]);
-        }
-    }
-    // the pipeline shape of one-query and of many-query answers: Horner
-    // chains beside the passes (2), or the collapse beside the database
-    // pass as well (3), timed on whole answers of a synthetic batch
-    if (G / gchunk > 1 && !pipe_forced) {
-        HIPX(hipMemset(d_q, 0, (nq > 1 ? nq : 1) * qlen * 8));
-        const int shapes[2] = {2, 3};
-        for (const size_t count : {(size_t)1, nq}) {
-            int* target = count == 1 ? &pipe_single : &pipe_batched;
-            const size_t b = fastest(
-                2, [&](size_t i) { *target = shapes[i]; }, [&](size_t) { enqueue(d_q, count, d_resp, stream); });
-            *target = shapes[b];
+                cands.size(), [&](size_t i) { use(cands[i]); }, [&](size_t) { enqueue(d_q, count, d_resp, stream); },
+                6);
+            use(cands[b]);
             if (nq == 1) break;
         }
     }
@@ -566,6 +577,12 @@ DbPlan GpuServerCtx::plan_for(size_t nq, size_t ncols) const {
 
 DbPlan GpuServerCtx::chunk_plan(size_t nq, size_t ncols) const {
     const DbPlan full = plan_for(nq, pp.db_cols);
+    if (nq == 1 && chunk_slices1) {
+        const size_t t = ncols / dbplan_cols(full) * chunk_slices1;
+        const DbPlan x = full.kind == DbPlan::F64 ? dbplan_f64(pp.db_rows, ncols, (int)full.param, t)
+                                                  : dbplan_mfma(pp.db_rows, ncols, 1, t);
+        if (x.slices == chunk_slices1 && (x.slices == 1 || x.slices * 2 * ncols <= part_stride)) return x;
+    }
     if (nq > 1 && chunk_slices && full.kind == DbPlan::MFMA && full.param == chunk_param) {
         const DbPlan x =
             dbplan_form(pp.db_rows, ncols, nq, ncols / dbplan_cols(full) * chunk_slices, mfma_depth, mfma_form);

@@ -41,6 +41,7 @@ struct GpuServerCtx {
     // row slices of the batched chunk plan, timed at setup for plans with
     // chunk_param digit tiles (0: the narrowing rule)
     uint32_t chunk_slices = 0, chunk_param = 0;
+    uint32_t chunk_slices1 = 0;     // the same for one-query answers (plan1's kind)
     int collapse_variant = 0;       // 17..32-query collapse pass form (launch_collapse)
     size_t gchunk = 0;              // groups per chunk of the pipelined answer
     // pipeline shape of one-query and many-query calls: 0 serial on the

@@ -344,6 +344,7 @@ def test_pipelined_group_chunks_equal_unchunked(medium8, groups, shape, slices):
     assert srv.lib.ipir_dbg_set_group_chunk(srv.h, groups) == 0
     assert srv.lib.ipir_dbg_set_pipeline(srv.h, shape, shape) == 0
     assert srv.lib.ipir_dbg_set_chunk_slices(srv.h, 3, slices) == 0
+    assert srv.lib.ipir_dbg_set_chunk_slices(srv.h, 1, slices) == 0
     try:
         got = srv.answer_batch(qs)
         got1 = srv.answer_batch(qs[:1])
@@ -353,6 +354,7 @@ def test_pipelined_group_chunks_equal_unchunked(medium8, groups, shape, slices):
         assert srv.lib.ipir_dbg_set_horner(srv.h, -1) == 0
         assert srv.lib.ipir_dbg_set_pipeline(srv.h, 3, 2) == 0
         assert srv.lib.ipir_dbg_set_chunk_slices(srv.h, 3, 0) == 0
+        assert srv.lib.ipir_dbg_set_chunk_slices(srv.h, 1, 0) == 0
         assert srv.lib.ipir_dbg_set_group_chunk(srv.h, geo["G"]) == 0
     assert np.array_equal(got, base) and np.array_equal(got1[0], one[0]) and np.array_equal(pair, base)
 
