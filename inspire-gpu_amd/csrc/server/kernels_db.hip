@@ -346,15 +346,16 @@ __global__ __launch_bounds__(256, 2) void k_dbpass_mfma(const uint8_t* __restric
                        slice, OC, col0 + n * 16 + (l & 15));
 }
 
-// Many queries (MT = 8 or 16 digit tiles, up to 16 or 32 queries): as above,
-// but the block's WAVES waves share the step's digit tile (Mp x 64 bytes)
-// through an LDS double buffer (one barrier per step). Every block reads the
-// whole digit tile each step, twice the bytes of its 64 columns' DB operands
-// at 4 waves; 8 waves (both waves of every SIMD of a CU) halve that ratio,
-// and the digit loads are what the vector-memory path spends most on. Every global load is
-// issued L = DEPTH-1 steps before its step: the DB operands into a register
-// ring of DEPTH slots, the thread's share of the digit tile into a ring of L
-// slots (stored to LDS one step ahead). Vector-memory loads complete in issue
+// Many queries (MT = 8 or 16 tiles of per-limb digits, up to 16 or 32
+// queries; MT = 7 or 14 tiles of CRT digits, below): as above, but the
+// block's WAVES waves share the step's digit tile (Mp x 64 bytes) through an
+// LDS double buffer (one barrier per step). Every block reads the whole
+// digit tile each step, twice the bytes of its 64 columns' DB operands at 4
+// waves; 8 waves (both waves of every SIMD of a CU) halve that ratio, and
+// the digit loads are what the vector-memory path spends most on. Every
+// global load is issued L = DEPTH-1 steps before its step: the DB operands
+// into a register ring of DEPTH slots, the thread's share of the digit tile
+// into a ring of L slots (stored to LDS one step ahead). Vector-memory loads complete in issue
 // order, so a load issued closer to its use (a digit prefetch one step ahead)
 // would make the wait for it also wait for every DB operand issued before it;
 // with one lead for all loads each wait covers exactly one step's loads. Past
