@@ -66,6 +66,11 @@ int ipir_dbg_horner_variant(const uint64_t* cts, size_t L, const uint64_t* rgsw,
 /* Set the groups per chunk of the pipelined answer (a divisor of n_packed;
  * 1 if the partial-sum buffer cannot hold the resulting plans). */
 int ipir_dbg_set_group_chunk(ipir_server* srv, size_t groups);
+/* Split the pipelined answer's lowest chunk into chunks of 1/2, 1/4 and 1/4
+ * of its groups (1, the default, where the chunk size divides by 4) or not
+ * (0), so fewer Horner steps are left after the last pass. 1 for other
+ * values. */
+int ipir_dbg_set_tail_split(ipir_server* srv, int on);
 /* Select the Horner kernel a server's answers use: mode as variant above,
  * or -1 (the default) to use the pair only for serial answers with at most
  * two chains. */

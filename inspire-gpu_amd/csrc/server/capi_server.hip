@@ -283,6 +283,7 @@ int ipir_dbg_tuning(const ipir_server* srv, char* out, size_t cap) {
     if (c->G / c->gchunk > 1)
         s += "; chunks of " + std::to_string(c->gchunk) + " groups: batch1 " + plan(c->chunk_plan(1, c->gchunk * ipx::N)) +
              ", batch" + std::to_string(nq) + " " + plan(c->chunk_plan(nq, c->gchunk * ipx::N));
+    s += std::string(c->tail_split && c->chunks.size() > c->G / c->gchunk ? " (lowest split in 1/2, 1/4, 1/4)" : "");
     s += "; collapse variant " + std::to_string(c->collapse_variant) + "; pipeline shapes " +
          std::to_string(c->pipe_single) + "/" + std::to_string(c->pipe_batched);
     std::snprintf(out, cap, "%s", s.c_str());
@@ -424,6 +425,16 @@ int ipir_dbg_set_group_chunk(ipir_server* srv, size_t groups) {
         if (x.slices > 1 && x.slices * 2 * ncols > c->part_stride) return 1;
     }
     c->gchunk = groups;
+    c->set_chunks();
+    return 0;
+}
+
+int ipir_dbg_set_tail_split(ipir_server* srv, int on) {
+    if (!srv || (on != 0 && on != 1)) return 1;
+    GpuServerCtx* c = srv->ctx;
+    std::lock_guard<std::mutex> lk(c->mu);
+    c->tail_split = on != 0;
+    c->set_chunks();
     return 0;
 }
 

@@ -278,6 +278,8 @@ void GpuServerCtx::alloc_slots() {
         const long v = std::atol(gc);
         if (v > 0 && (size_t)v <= G && G % (size_t)v == 0) gchunk = (size_t)v;
     }
+    if (const char* ts = std::getenv("INSPIRE_TAIL_SPLIT")) tail_split = ts[0] != '0';
+    set_chunks();
     // row-slice partials of one query: [slices][2][ncols] words for the
     // widest plan any candidate (full width or narrowed to a chunk) needs
     size_t part_words = 2 * C;
@@ -575,18 +577,37 @@ DbPlan GpuServerCtx::plan_for(size_t nq, size_t ncols) const {
     return dbplan_form(R, ncols, nq, mfma_target, mfma_depth, mfma_form);
 }
 
+void GpuServerCtx::set_chunks() {
+    chunks.clear();
+    const size_t n = G / gchunk;
+    for (size_t k = n; k-- > 1;) chunks.push_back({k * gchunk, (k + 1) * gchunk});
+    if (tail_split && n > 1 && gchunk % 4 == 0) {
+        const size_t h = gchunk / 2, q = gchunk / 4;
+        chunks.push_back({h, gchunk});
+        chunks.push_back({q, h});
+        chunks.push_back({0, q});
+    } else {
+        chunks.push_back({0, gchunk});
+    }
+}
+
+// The chunk plan over ncols (<= gchunk N) columns: a timed slice count sets
+// the workgroup count of a whole chunk, which narrower chunks keep by
+// cutting their rows into more slices.
 DbPlan GpuServerCtx::chunk_plan(size_t nq, size_t ncols) const {
     const DbPlan full = plan_for(nq, pp.db_cols);
+    const size_t whole = gchunk * N;
+    auto fits = [&](const DbPlan& x) { return x.slices == 1 || x.slices * 2 * ncols <= part_stride; };
     if (nq == 1 && chunk_slices1) {
-        const size_t t = ncols / dbplan_cols(full) * chunk_slices1;
+        const size_t t = whole / dbplan_cols(full) * chunk_slices1;
         const DbPlan x = full.kind == DbPlan::F64 ? dbplan_f64(pp.db_rows, ncols, (int)full.param, t)
                                                   : dbplan_mfma(pp.db_rows, ncols, 1, t);
-        if (x.slices == chunk_slices1 && (x.slices == 1 || x.slices * 2 * ncols <= part_stride)) return x;
+        if (fits(x)) return x;
     }
     if (nq > 1 && chunk_slices && full.kind == DbPlan::MFMA && full.param == chunk_param) {
         const DbPlan x =
-            dbplan_form(pp.db_rows, ncols, nq, ncols / dbplan_cols(full) * chunk_slices, mfma_depth, mfma_form);
-        if (x.slices == 1 || x.slices * 2 * ncols <= part_stride) return x;
+            dbplan_form(pp.db_rows, ncols, nq, whole / dbplan_cols(full) * chunk_slices, mfma_depth, mfma_form);
+        if (fits(x)) return x;
     }
     return narrow_plan(full, nq == 1 ? plan1_target : mfma_target, pp.db_rows, pp.db_cols, ncols, nq);
 }
@@ -709,26 +730,26 @@ void GpuServerCtx::enqueue(const uint64_t* d_in, size_t count, uint64_t* d_out, 
         // share the bandwidth worse than back to back); shape 3 runs it beside
         const hipStream_t cs = shape == 3 ? cstream[lane] : s;
         joined = true;
-        const DbPlan p = chunk_plan(nq, gchunk * N);
+        const DbPlan p = chunk_plan(nq, gchunk * N);  // every chunk's plan has this one's kernel and digits
         if (cs != s) {
             HIPX(hipEventRecord(prep_ev[lane], s));  // the keys are in place
             HIPX(hipStreamWaitEvent(cs, prep_ev[lane], 0));
         }
         HIPX(launch_collapse_keys(keys, keys_stride, nq, d_kdig[lane], d_KS[lane], cs));
         dbpass_prep(q0, nq, p, s, lane);
-        for (size_t k = nchunks; k-- > 0;) {
-            const size_t g_lo = k * gchunk, g_hi = g_lo + gchunk;
+        for (size_t k = 0; k < chunks.size(); k++) {
+            const size_t g_lo = chunks[k].first, g_hi = chunks[k].second, ng = g_hi - g_lo;
             hipEvent_t dbe = chunk_ev[lane][k % kChunkEvents], ce = coll_ev[lane][k % kChunkEvents];
-            dbpass_range(q0, nq, g_lo, g_hi, p, s, lane);
+            dbpass_range(q0, nq, g_lo, g_hi, ng == gchunk ? p : chunk_plan(nq, ng * N), s, lane);
             // with the collapse on s its event covers the DB pass too (an
             // event record costs the stream a few microseconds of idle)
             if (cs != s) HIPX(hipEventRecord(dbe, s));
-            HIPX(launch_collapse(d_precomp + g_lo * PC_GROUP, gchunk, d_kdig[lane], d_KS[lane], nq,
+            HIPX(launch_collapse(d_precomp + g_lo * PC_GROUP, ng, d_kdig[lane], d_KS[lane], nq,
                                  p2 + g_lo * 2 * N, part2_stride, cs, collapse_variant));
             HIPX(hipEventRecord(ce, cs));
             if (cs != s) HIPX(hipStreamWaitEvent(hs, dbe, 0));
             HIPX(hipStreamWaitEvent(hs, ce, 0));
-            HIPX(launch_pack_finalize(d_interm + q0 * 2 * C, C, p2, part2_stride, d_precomp, g_lo, gchunk, keys,
+            HIPX(launch_pack_finalize(d_interm + q0 * 2 * C, C, p2, part2_stride, d_precomp, g_lo, ng, keys,
                                       keys_stride, packed, packed_stride, nq, nc, hs));
             horner_range(nq, out, g_lo, g_hi, hs, q0, true);
         }

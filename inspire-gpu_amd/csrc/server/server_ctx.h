@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 
 #include <mutex>
+#include <utility>
+#include <vector>
 
 #include "gpu_protocol.h"
 #include "kernels.h"
@@ -44,6 +46,12 @@ struct GpuServerCtx {
     uint32_t chunk_slices1 = 0;     // the same for one-query answers (plan1's kind)
     int collapse_variant = 0;       // 17..32-query collapse pass form (launch_collapse)
     size_t gchunk = 0;              // groups per chunk of the pipelined answer
+    // the pipelined answer's chunks [lo, hi), highest groups first: chunks of
+    // gchunk groups, the lowest one split into gchunk/2, /4, /4 when
+    // tail_split (its Horner steps are the ones left after every pass)
+    bool tail_split = true;
+    std::vector<std::pair<size_t, size_t>> chunks;
+    void set_chunks();
     // pipeline shape of one-query and many-query calls: 0 serial on the
     // caller's stream, 2 Horner chains beside the passes, 3 collapse beside
     // too (2 or 3 timed at setup)

@@ -83,6 +83,7 @@ SERVER_FUNCS = [
     ("ipir_dbg_db_pass", C.c_int, [VP, VP, SZ, C.c_int, VP]),
     ("ipir_dbg_dbpass_kernel", C.c_int, [VP, SZ, C.c_char_p, SZ]),
     ("ipir_dbg_tuning", C.c_int, [VP, C.c_char_p, SZ]),
+    ("ipir_dbg_set_tail_split", C.c_int, [VP, C.c_int]),
     ("ipir_dbg_time_dbpass", C.c_int, [VP, SZ, C.c_int, P(C.c_double)]),
     ("ipir_dbg_answer_taps", C.c_int, [VP, VP, VP, VP, VP]),
     ("ipir_dbg_ntt", C.c_int, [VP, VP, SZ, C.c_int]),

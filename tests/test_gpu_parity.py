@@ -261,7 +261,8 @@ def test_batch_equals_singles_and_extracts(medium8):
 def test_ragged_batch_over_sub_batches(medium8):
     """A batch of 33 on a server of max_batch 40: a 32-query sub-batch then a
     one-query sub-batch, each with its own pipeline shape and Horner kernel,
-    queued on the same streams; equals the serial answer and extracts."""
+    queued on the same streams, over equal chunks or with the lowest chunk
+    split; equals the serial answer and extracts."""
     geo, db, _ = medium8
     n, w, rows = 524288, 120, 2048
     srv = ia.Server(db, n, w, rows, max_batch=40, crs_seed=bytes(SEED))
@@ -274,11 +275,13 @@ def test_ragged_batch_over_sub_batches(medium8):
             sts.append(st)
         assert srv.lib.ipir_dbg_set_group_chunk(srv.h, geo["G"]) == 0
         serial = srv.answer_batch(qs)
-        assert srv.lib.ipir_dbg_set_group_chunk(srv.h, 2) == 0  # four chunks: pipelined
-        for shapes in ((3, 2), (2, 3), (3, 3)):
-            assert srv.lib.ipir_dbg_set_pipeline(srv.h, *shapes) == 0
-            got = srv.answer_batch(qs)
-            assert np.array_equal(got, serial), f"shapes {shapes}"
+        for groups, split in ((2, 1), (4, 1), (4, 0)):  # chunks of 2; of 4 with the lowest split 2+1+1, or not
+            assert srv.lib.ipir_dbg_set_group_chunk(srv.h, groups) == 0
+            assert srv.lib.ipir_dbg_set_tail_split(srv.h, split) == 0
+            for shapes in ((3, 2), (2, 3), (3, 3)):
+                assert srv.lib.ipir_dbg_set_pipeline(srv.h, *shapes) == 0
+                got = srv.answer_batch(qs)
+                assert np.array_equal(got, serial), f"groups {groups} split {split} shapes {shapes}"
         for i in (0, 31, 32):
             assert np.array_equal(ol.extract(geo, sts[i], got[i].reshape(geo["T"], -1)),
                                   ol.expected_slots(geo, db, idxs[i]))
