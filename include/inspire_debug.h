@@ -82,7 +82,8 @@ int ipir_dbg_set_horner(ipir_server* srv, int mode);
  * 8- and 4-wave forms with 7 digit rows per query (3..7, 3..5), 5 (17..32
  * queries) those digits accumulated by power of 256 (3..5, also for 9..16
  * queries); other 9..16-query plans cap the depth at 4 and use form 0 for
- * 1 and 4 and 2 for 3. 1 for other values. */
+ * 1 and 4 and 2 for 3. The setting applies to every 9..32-query batch
+ * (also where setup timed 9..16 queries apart). 1 for other values. */
 int ipir_dbg_set_mfma_depth(ipir_server* srv, uint32_t depth, int form);
 /* Row slices of the pipelined answer's database-pass chunks for batches
  * planned like an nq-query batch, or for one-query answers when nq is 1

@@ -280,6 +280,9 @@ int ipir_dbg_tuning(const ipir_server* srv, char* out, size_t cap) {
     std::string s = "batch1: " + plan(c->plan_for(1, c->pp.db_cols)) + "; batch" + std::to_string(nq) + ": " +
                     plan(c->plan_for(nq, c->pp.db_cols)) + " (form " + std::to_string(c->mfma_form) + ", target " +
                     std::to_string(c->mfma_target) + ")";
+    if (c->tuned16)
+        s += "; batch16: " + plan(c->plan_for(16, c->pp.db_cols)) + " (form " + std::to_string(c->mfma16_form) +
+             ", target " + std::to_string(c->mfma16_target) + ")";
     if (c->G / c->gchunk > 1)
         s += "; chunks of " + std::to_string(c->gchunk) + " groups: batch1 " + plan(c->chunk_plan(1, c->gchunk * ipx::N)) +
              ", batch" + std::to_string(nq) + " " + plan(c->chunk_plan(nq, c->gchunk * ipx::N));
@@ -471,6 +474,7 @@ int ipir_dbg_set_mfma_depth(ipir_server* srv, uint32_t depth, int form) {
             std::lock_guard<std::mutex> lk(srv->ctx->mu);
             srv->ctx->mfma_depth = depth;
             srv->ctx->mfma_form = form;
+            srv->ctx->tuned16 = false;  // every 9..32-query batch takes the set form
             return 0;
         }
     return 1;

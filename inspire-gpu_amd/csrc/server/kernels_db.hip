@@ -755,7 +755,7 @@ size_t dbplan_cols(const DbPlan& p) {
 std::string dbplan_kernel(const DbPlan& p) {
     if (p.kind == DbPlan::F64) return "k_dbpass_f64<" + std::to_string(p.param) + ">";
     const std::string mt_nt = std::to_string(p.param) + ", " + std::to_string(mfma_nt(p.param));
-    if (p.param < 8) return "k_dbpass_mfma<" + mt_nt + ">";
+    if (p.param <= 4) return "k_dbpass_mfma<" + mt_nt + ">";  // 1, 2 or 4 tiles: the direct kernel
     if (p.dbtile) return "k_dbpass_mfma_bs<16, 4, " + std::to_string(p.depth) + ">";
     // every template argument, defaults included (as profilers demangle it)
     return "k_dbpass_mfma_sa<" + (p.pow ? std::to_string(p.param) + ", 2" : mt_nt) + ", 2, " + std::to_string(p.depth) + ", " +

@@ -454,15 +454,17 @@ void GpuServerCtx::autotune_dbpass() {
             plan1_target = ts[b];
         }
     }
-    // batched pass: the workgroup target at the largest batch this context serves
+    // batched pass: target, depth and form at the largest batch this context
+    // serves, and apart for 9..16 queries when that batch is larger (their
+    // best form differs)
     const size_t nq = cfg.max_batch < kMfmaMaxQueries ? cfg.max_batch : kMfmaMaxQueries;
-    if (nq > 1) {
-        const uint32_t param = dbplan_mfma(R, C, nq, mfma_target).param;
+    auto tune_batched = [&](size_t nqc, size_t& target_out, uint32_t& depth_out, int& form_out) {
+        const uint32_t param = dbplan_mfma(R, C, nqc, target_out).param;
         // forms (dbplan_form): the shared DB tile and the 7-digit forms
         // regrouped through LDS exist for 17..32 queries only, the power rows
         // for 9..32; the LDS-regrouped forms (3, 4) never beat the power rows
         // (5) and are timed only when forced
-        const int forced_form = mfma_form;
+        const int forced_form = form_out;
         struct Cand {
             DbPlan x;
             size_t target;
@@ -483,19 +485,28 @@ void GpuServerCtx::autotune_dbpass() {
                                  : (form == 1 && param != 16) || (form > 0 && param < 8) || form == 3 || form == 4)
                     continue;
                 for (uint32_t d : kMfmaDepths) {
-                    if ((depth_forced || param < 8) && d != mfma_depth) continue;
-                    const DbPlan x = dbplan_form(R, C, nq, t, d, form);
+                    if ((depth_forced || param < 8) && d != depth_out) continue;
+                    const DbPlan x = dbplan_form(R, C, nqc, t, d, form);
                     if (x.depth != d && param >= 8) continue;  // a depth this kernel is not built with
                     if (fits(x) && !seen(x)) cands.push_back({x, t, d, form});
                 }
             }
         if (!cands.empty()) {  // empty: a forced depth the forced form is not built with keeps the defaults
             const size_t b = fastest(
-                cands.size(), [&](size_t i) { prep_plan(cands[i].x, nq); },
-                [&](size_t i) { run_plan(cands[i].x, nq); });
-            mfma_target = cands[b].target;
-            mfma_depth = cands[b].depth;
-            mfma_form = cands[b].form;
+                cands.size(), [&](size_t i) { prep_plan(cands[i].x, nqc); },
+                [&](size_t i) { run_plan(cands[i].x, nqc); });
+            target_out = cands[b].target;
+            depth_out = cands[b].depth;
+            form_out = cands[b].form;
+        }
+    };
+    if (nq > 1) {
+        tune_batched(nq, mfma_target, mfma_depth, mfma_form);
+        if (nq > 16 && !depth_forced) {
+            mfma16_target = mfma_target;
+            mfma16_depth = mfma_depth;
+            tune_batched(16, mfma16_target, mfma16_depth, mfma16_form);
+            tuned16 = true;
         }
         // the 17..32-query collapse pass form, over all groups (zeroed keys)
         if (nq > 16) {
@@ -574,6 +585,7 @@ DbPlan GpuServerCtx::plan_for(size_t nq, size_t ncols) const {
     if (nq == 1)
         return plan1.kind == DbPlan::F64 ? dbplan_f64(R, ncols, (int)plan1.param, plan1_target)
                                          : dbplan_mfma(R, ncols, 1, plan1_target);
+    if (tuned16 && nq >= 9 && nq <= 16) return dbplan_form(R, ncols, nq, mfma16_target, mfma16_depth, mfma16_form);
     return dbplan_form(R, ncols, nq, mfma_target, mfma_depth, mfma_form);
 }
 

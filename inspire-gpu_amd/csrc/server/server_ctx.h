@@ -40,6 +40,12 @@ struct GpuServerCtx {
     size_t mfma_target = 1024;      // workgroups the batched pass aims for
     uint32_t mfma_depth = 4;        // register-ring depth of the 9..32-query passes
     int mfma_form = 0;              // 9..32 queries: form of the pass (dbplan_form)
+    // 9..16 queries when timed apart from a larger largest batch (their
+    // best form differs): target, depth and form
+    bool tuned16 = false;
+    size_t mfma16_target = 1024;
+    uint32_t mfma16_depth = 4;
+    int mfma16_form = 0;
     // row slices of the batched chunk plan, timed at setup for plans with
     // chunk_param digit tiles (0: the narrowing rule)
     uint32_t chunk_slices = 0, chunk_param = 0;
