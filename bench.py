@@ -16,6 +16,7 @@ import argparse
 import ctypes as C
 import json
 import os
+import re
 import sys
 import time
 
@@ -378,8 +379,22 @@ def main():
         t = json.load(open(tfile))
         if t.get("workload") == label:
             traffics = {k: v["traffic_bytes"] for k, v in t.get("kernels", {}).items()}
-    traffic = traffics.get(kname.split(" slices=")[0])
-    trafficB = traffics.get(knameB.split(" slices=")[0])
+    def family(k):  # the kernel without its ring depth (which does not change the bytes it moves)
+        m = re.match(r"(k_dbpass_mfma_(?:sa|bs))<(.*)>$", k)
+        if not m:
+            return k
+        a = [x.strip() for x in m.group(2).split(",")]
+        a[3 if m.group(1).endswith("sa") else 2] = "*"
+        return f"{m.group(1)}<{', '.join(a)}>"
+
+    def traffic_of(k):
+        k = k.split(" slices=")[0]
+        if k in traffics:
+            return traffics[k], k
+        same = sorted(x for x in traffics if family(x) == family(k))
+        return (traffics[same[0]], same[0]) if same else (None, None)
+    traffic, _ = traffic_of(kname)
+    trafficB, traffic_kernel = traffic_of(knameB)
     # NTT throughput at N = 2048, both limbs per polynomial, 1024 polynomials per launch
     ntt = {}
     for inv in (0, 1):
@@ -451,6 +466,7 @@ def main():
         "verified_queries": verified,
         "roofline": {"bound": "hbm", "achieved": round(achievedB, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achievedB / HBM_PEAK_GBS, 4), "traffic": trafficB,
+                     "traffic_profiled_kernel": traffic_kernel,
                      "kernel": f"{knameB} (database pass, batch {min(B, 32)}; {dbB_ms:.3f} ms per launch)"},
         "batched_db_pass": {"kernel": knameB, "queries": min(B, 32), "ms": round(dbB_ms, 3),
                             "hbm_GBs": round(achievedB, 1),
