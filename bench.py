@@ -233,6 +233,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--verify", type=int, default=0,
                     help="also answer and extract this many further random queries (batches of --batch)")
+    ap.add_argument("--serving-inflight", type=int, default=0,
+                    help="serving loop partitions (batches in flight), each of max_batch / N slots; "
+                         "0: as many 32-slot partitions as the batch holds, at most 2")
     ap.add_argument("--serving", action="store_true",
                     help="also run Poisson arrivals through the serving loop at 0.25/0.5/0.9x capacity")
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
@@ -429,7 +432,9 @@ def main():
 
     serve = None
     if args.serving:
-        serve = serving(lib, srv, qs, qps32, resp_u64s)
+        if args.serving_inflight <= 0:  # full 32-query passes in every partition
+            args.serving_inflight = max(1, min(2, B // 32))
+        serve = serving(lib, srv, qs, qps32, resp_u64s, max_inflight=args.serving_inflight)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -484,8 +489,9 @@ def main():
         "cpu_baseline": cpu,
     }
     if serve is not None:
-        out["serving"] = {"loop": "ipir_loop (continuous batching, 2 partitions of max_batch/2 slots)",
-                          "rates": serve}
+        k = args.serving_inflight
+        out["serving"] = {"loop": f"ipir_loop (continuous batching, {k} partition{'s' if k > 1 else ''} of "
+                                  f"{B // k} slots)", "rates": serve}
     for h in hs:
         cl.destroy(h)
     dq.free()
