@@ -343,6 +343,22 @@ def main():
     qps1 = max(args.steps, 10) / tot1
     log(f"[bench] B={B}: {qps32:.1f} q/s ({tot32 / args.steps * 1e3:.2f} ms/batch); "
         f"B=1: {qps1:.1f} q/s ({tot1 / max(args.steps, 10) * 1e3:.2f} ms/query)")
+    # the same answers replayed from captured hipGraphs (a warmup call captures)
+    assert lib.ipir_dbg_set_graph(srv.h, 1) == 0
+    gtot32, glat32 = timed(B, args.steps, max(args.warmup, 1))
+    gtot1, glat1 = timed(1, max(args.steps, 10), max(args.warmup, 1))
+    step(B)
+    lib.ipir_dev_sync()
+    g_equal = bool(np.array_equal(dout.download(np.zeros((B, resp_u64s), np.uint64)), resp))
+    assert lib.ipir_dbg_set_graph(srv.h, 0) == 0
+    graph = {f"batch{B}_qps": round(B * args.steps / gtot32, 2),
+             "batch1_qps": round(max(args.steps, 10) / gtot1, 2),
+             f"batch{B}_p50_ms": round(float(np.percentile(glat32, 50)) * 1e3, 3),
+             "batch1_p50_ms": round(float(np.percentile(glat1, 50)) * 1e3, 3),
+             "responses_equal_launch_by_launch": g_equal}
+    log(f"[bench] graph replay: B={B}: {graph[f'batch{B}_qps']} q/s; B=1: {graph['batch1_qps']} q/s; "
+        f"responses equal: {g_equal}")
+    ok = ok and g_equal
 
     # per-stage device times (HIP events on the server stream)
     st1 = (C.c_double * 5)()
@@ -482,6 +498,7 @@ def main():
         "batch1_db_pass": {"kernel": kname, "ms": round(dbpass_ms, 3), "hbm_GBs": round(achieved, 1),
                            "frac_of_hbm_peak": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
         "tuning": tuning,
+        "graph_replay": graph,
         "horner_step_us": {"batch1": round(st1[4] * 1e3 / max(1, geo["Da"] - 1), 2),
                            f"batch{B}": round(st32[4] * 1e3 / max(1, geo["Da"] - 1), 2)},
         "ntt_throughput": ntt,

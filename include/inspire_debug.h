@@ -75,6 +75,16 @@ int ipir_dbg_set_tail_split(ipir_server* srv, int on);
  * or -1 (the default) to use the pair only for serial answers with at most
  * two chains. */
 int ipir_dbg_set_horner(ipir_server* srv, int mode);
+/* Answer through captured hipGraphs (1) or launch by launch (0, the default
+ * unless INSPIRE_GRAPH=1 at setup). With 1, the first answer of each
+ * (input buffer, output buffer, count) on a stream that is not the null
+ * stream is captured (its kernels, cross-stream events and waits) and later
+ * answers of the same key replay it; a call made while the caller's stream
+ * is capturing is recorded into the caller's graph instead. The
+ * ipir_dbg_set_* calls above drop the captured graphs. No counterpart in
+ * the reference (whose answer launches kernel by kernel). 1 for other
+ * values. */
+int ipir_dbg_set_graph(ipir_server* srv, int on);
 /* Set the register-ring depth of the batched database passes a server uses
  * and their form: 0 the waves of a 4-wave block share the digit tile (depth
  * 3..5), 1 (17..32 queries) they share the DB tile (3..4), 2 the waves of an

@@ -101,7 +101,8 @@ int ipir_answer_batch_device(ipir_server* srv, const uint64_t* d_queries, size_t
     if (count > srv->ctx->cfg.max_batch) return 2;
     try {
         hipStream_t s = hip_stream ? static_cast<hipStream_t>(hip_stream) : srv->ctx->stream;
-        srv->ctx->enqueue(d_queries, count, d_out_resp, s);
+        srv->ctx->run_step(d_queries, d_out_resp, count, 0, 0, 0, s,
+                           [&] { srv->ctx->enqueue(d_queries, count, d_out_resp, s); });
         return 0;
     } catch (...) {
         return 4;
@@ -129,7 +130,8 @@ int ipir_answer_batch_packed_device(ipir_server* srv, const uint8_t* d_packed_qu
     if (count > srv->ctx->cfg.max_batch) return 2;
     try {
         hipStream_t s = hip_stream ? static_cast<hipStream_t>(hip_stream) : srv->ctx->stream;
-        srv->ctx->enqueue_packed(d_packed_queries, count, d_out_compressed, s);
+        srv->ctx->run_step(d_packed_queries, d_out_compressed, count, 0, 0, 1, s,
+                           [&] { srv->ctx->enqueue_packed(d_packed_queries, count, d_out_compressed, s); });
         return 0;
     } catch (...) {
         return 4;
@@ -429,6 +431,7 @@ int ipir_dbg_set_group_chunk(ipir_server* srv, size_t groups) {
     }
     c->gchunk = groups;
     c->set_chunks();
+    c->drop_graphs();  // their kernels carry the old plans
     return 0;
 }
 
@@ -438,6 +441,7 @@ int ipir_dbg_set_tail_split(ipir_server* srv, int on) {
     std::lock_guard<std::mutex> lk(c->mu);
     c->tail_split = on != 0;
     c->set_chunks();
+    c->drop_graphs();  // their kernels carry the old plans
     return 0;
 }
 
@@ -445,6 +449,7 @@ int ipir_dbg_set_collapse_variant(ipir_server* srv, int variant) {
     if (!srv || variant < 0 || variant >= ipx::kCollapseVariants) return 1;
     std::lock_guard<std::mutex> lk(srv->ctx->mu);
     srv->ctx->collapse_variant = variant;
+    srv->ctx->drop_graphs();  // their kernels carry the old plans
     return 0;
 }
 
@@ -456,10 +461,12 @@ int ipir_dbg_set_chunk_slices(ipir_server* srv, size_t nq, uint32_t slices) {
     std::lock_guard<std::mutex> lk(c->mu);
     if (nq == 1) {
         c->chunk_slices1 = slices;
+        c->drop_graphs();  // their kernels carry the old plans
         return 0;
     }
     c->chunk_slices = slices;
     c->chunk_param = slices ? c->plan_for(nq, c->pp.db_cols).param : 0;
+    c->drop_graphs();  // their kernels carry the old plans
     return 0;
 }
 
@@ -475,6 +482,7 @@ int ipir_dbg_set_mfma_depth(ipir_server* srv, uint32_t depth, int form) {
             srv->ctx->mfma_depth = depth;
             srv->ctx->mfma_form = form;
             srv->ctx->tuned16 = false;  // every 9..32-query batch takes the set form
+            srv->ctx->drop_graphs();  // their kernels carry the old plans
             return 0;
         }
     return 1;
@@ -486,6 +494,15 @@ int ipir_dbg_set_pipeline(ipir_server* srv, int single, int batched) {
     std::lock_guard<std::mutex> lk(srv->ctx->mu);
     srv->ctx->pipe_single = single;
     srv->ctx->pipe_batched = batched;
+    srv->ctx->drop_graphs();  // their kernels carry the old plans
+    return 0;
+}
+
+int ipir_dbg_set_graph(ipir_server* srv, int on) {
+    if (!srv || (on != 0 && on != 1)) return 1;
+    std::lock_guard<std::mutex> lk(srv->ctx->mu);
+    srv->ctx->drop_graphs();
+    srv->ctx->graph_mode = on;
     return 0;
 }
 
@@ -493,6 +510,7 @@ int ipir_dbg_set_horner(ipir_server* srv, int mode) {
     if (!srv || mode < -1 || mode > 1) return 1;
     std::lock_guard<std::mutex> lk(srv->ctx->mu);
     srv->ctx->horner_mode = mode;
+    srv->ctx->drop_graphs();  // their kernels carry the old plans
     return 0;
 }
 

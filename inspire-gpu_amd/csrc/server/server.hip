@@ -335,6 +335,8 @@ void GpuServerCtx::alloc_slots() {
     d_hflags = dalloc<uint32_t>(2 * B * T);
     d_status = dalloc<uint32_t>(1);
     HIPX(hipMemset(d_status, 0, 4));
+    // INSPIRE_GRAPH=1: answers replay captured hipGraphs
+    if (const char* gm = std::getenv("INSPIRE_GRAPH")) graph_mode = gm[0] == '1' ? 1 : 0;
     if (const char* h = std::getenv("INSPIRE_HORNER"))
         horner_mode = std::strcmp(h, "pair") == 0 ? 1 : std::strcmp(h, "single") == 0 ? 0 : -1;
     // INSPIRE_PIPELINE=<single>,<batched>: the pipeline shape (0 serial, 2
@@ -776,6 +778,56 @@ void GpuServerCtx::enqueue(const uint64_t* d_in, size_t count, uint64_t* d_out, 
     }
 }
 
+void GpuServerCtx::run_step(const void* in, void* out, size_t count, size_t slot0, int lane, int packed,
+                            hipStream_t s, const std::function<void()>& fn) {
+    if (!graph_mode || !s) {
+        fn();
+        return;
+    }
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    HIPX(hipStreamIsCapturing(s, &cap));
+    if (cap != hipStreamCaptureStatusNone) {  // inside the caller's own capture
+        fn();
+        return;
+    }
+    std::lock_guard<std::mutex> lk(graph_mu);
+    for (const StepGraph& g : graphs)
+        if (g.in == in && g.out == out && g.count == count && g.slot0 == slot0 && g.lane == lane &&
+            g.packed == packed) {
+            HIPX(hipGraphLaunch(g.exec, s));
+            return;
+        }
+    // thread-local: other threads' allocations do not break this capture
+    HIPX(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    hipGraph_t graph = nullptr;
+    try {
+        fn();
+    } catch (...) {
+        (void)hipStreamEndCapture(s, &graph);
+        if (graph) (void)hipGraphDestroy(graph);
+        throw;
+    }
+    HIPX(hipStreamEndCapture(s, &graph));
+    hipGraphExec_t exec = nullptr;
+    const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    HIPX(e);
+    if (graphs.size() >= kMaxGraphs) {
+        (void)hipGraphExecDestroy(graphs.front().exec);
+        graphs.erase(graphs.begin());
+    }
+    graphs.push_back({in, out, count, slot0, lane, packed, exec});
+    HIPX(hipGraphLaunch(exec, s));
+}
+
+void GpuServerCtx::drop_graphs() {
+    std::lock_guard<std::mutex> lk(graph_mu);
+    if (graphs.empty()) return;
+    (void)hipDeviceSynchronize();  // no replay of a dropped graph may still be queued
+    for (const StepGraph& g : graphs) (void)hipGraphExecDestroy(g.exec);
+    graphs.clear();
+}
+
 void GpuServerCtx::enqueue_packed(const uint8_t* d_in, size_t count, uint8_t* d_out, hipStream_t s, size_t slot0,
                                   int lane) {
     uint64_t* dq = d_q + slot0 * qlen;
@@ -791,7 +843,7 @@ void GpuServerCtx::answer_packed_host(const uint8_t* in, size_t count, uint8_t* 
     uint8_t* hr = reinterpret_cast<uint8_t*>(h_resp);  // so do compressed responses
     std::memcpy(hq, in, count * qpack_bytes);
     HIPX(hipMemcpyAsync(d_qpack, hq, count * qpack_bytes, hipMemcpyHostToDevice, s));
-    enqueue_packed(d_qpack, count, d_rcmp, s);
+    run_step(d_qpack, d_rcmp, count, 0, 0, 1, s, [&] { enqueue_packed(d_qpack, count, d_rcmp, s); });
     HIPX(hipMemcpyAsync(hr, d_rcmp, count * T * kCompressedCtBytes, hipMemcpyDeviceToHost, s));
     HIPX(hipStreamSynchronize(s));
     if (horner_mode != 0) check_status(s);
@@ -816,7 +868,7 @@ void GpuServerCtx::answer_host(size_t count, uint64_t* out, hipStream_t s, size_
     uint64_t* dq = d_q + slot0 * qlen;
     uint64_t* dr = d_resp + slot0 * resp_u64s;
     HIPX(hipMemcpyAsync(dq, hq, count * qlen * 8, hipMemcpyHostToDevice, s));
-    enqueue(dq, count, dr, s, slot0);
+    run_step(dq, dr, count, slot0, 0, 0, s, [&] { enqueue(dq, count, dr, s, slot0); });
     HIPX(hipMemcpyAsync(hr, dr, count * resp_u64s * 8, hipMemcpyDeviceToHost, s));
     HIPX(hipStreamSynchronize(s));
     if (horner_mode != 0) check_status(s);
@@ -887,6 +939,7 @@ std::vector<RlweCt> gpu_answer(GpuServerCtx* ctx, const QueryMessage& qry) {
 void gpu_free_server(GpuServerCtx* ctx) {
     if (!ctx) return;
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    ctx->drop_graphs();
     dfree(ctx->d_tw); dfree(ctx->d_db); dfree(ctx->d_precomp);
     dfree(ctx->d_q); dfree(ctx->d_resp); dfree(ctx->d_bq); dfree(ctx->d_keys); dfree(ctx->d_rgsw);
     dfree(ctx->d_part); dfree(ctx->d_interm); dfree(ctx->d_part2); dfree(ctx->d_packed_b);

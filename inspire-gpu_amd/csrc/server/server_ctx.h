@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <functional>
 #include <mutex>
 #include <utility>
 #include <vector>
@@ -112,6 +113,27 @@ struct GpuServerCtx {
 
     size_t resident_bytes = 0;
     std::mutex mu;  // serialises host-staged answers on this context
+
+    // Captured answer steps: with graph_mode on, the first answer of a given
+    // (input, output, count, slots, lane, form) is captured from its stream
+    // into a hipGraph and every later one replays it. Kernel arguments are
+    // frozen at capture, so anything that changes a plan drops the cache.
+    struct StepGraph {
+        const void* in;
+        void* out;
+        size_t count, slot0;
+        int lane, packed;
+        hipGraphExec_t exec;
+    };
+    static constexpr size_t kMaxGraphs = 64;
+    int graph_mode = 0;
+    std::vector<StepGraph> graphs;
+    std::mutex graph_mu;  // the cache (device-side calls take no other lock)
+    // fn enqueues the step on s; captured or replayed as above, run as is when
+    // graph_mode is off or s is the null stream or already capturing.
+    void run_step(const void* in, void* out, size_t count, size_t slot0, int lane, int packed, hipStream_t s,
+                  const std::function<void()>& fn);
+    void drop_graphs();
 
     void alloc_slots();
     // Pick the fastest database-pass variant on this device and geometry.

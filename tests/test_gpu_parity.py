@@ -566,6 +566,51 @@ def test_device_resident_entry_matches_host_path(medium8):
     dout.free()
 
 
+@pytest.mark.parametrize("groups,shape", [(8, 0), (2, 2), (2, 3)])
+def test_graph_replays_equal_eager(medium8, groups, shape):
+    """Answers replayed from captured hipGraphs (serial, and pipelined over
+    group chunks on two and three streams) equal the launch-by-launch ones,
+    on every replay with new queries in the same buffers, with eager work and
+    synchronizations between the replays, for one query, three, twenty (a
+    17..32-query pass), the device-buffer entry and the packed wire form."""
+    geo, db, srv = medium8
+    cl = ia.Client(srv.params, srv.lib)
+    sets = {k: [[ol.make_query(geo, SEED, (97 * r + 131071 * j) % 524288, 5000 + 100 * r + j)[0]
+                 for j in range(k)] for r in range(3)] for k in (1, 3, 20)}
+    pset = [np.stack([cl.pack_query(q) for q in sets[3][r]]) for r in range(3)]
+    assert srv.lib.ipir_dbg_set_group_chunk(srv.h, groups) == 0
+    assert srv.lib.ipir_dbg_set_pipeline(srv.h, shape, shape) == 0
+    try:
+        eager = {k: [srv.answer_batch(v) for v in sets[k]] for k in sets}
+        peager = [srv.answer_packed(x) for x in pset]
+        assert srv.lib.ipir_dbg_set_graph(srv.h, 2) == 1
+        assert srv.lib.ipir_dbg_set_graph(srv.h, 1) == 0
+        src = np.arange(2 * 2048, dtype=np.uint32)  # one polynomial, two limbs of N = 2048
+        ntt_out = np.zeros_like(src)
+        flat = np.ascontiguousarray(np.stack(sets[3][0]))
+        dq = ia.DeviceBuffer(flat.nbytes, srv.lib)
+        dout = ia.DeviceBuffer(eager[3][0].nbytes, srv.lib)
+        for rep in range(2):
+            for r in range(3):
+                say(f"[graph] groups={groups} shape={shape} rep={rep} set={r}")
+                for k in sets:
+                    assert np.array_equal(srv.answer_batch(sets[k][r]), eager[k][r]), f"count {k} set {r}"
+                    # eager device work and a synchronization between replays
+                    assert srv.lib.ipir_dbg_ntt(ia.ptr(src), ia.ptr(ntt_out), 1, 0) == 0
+                assert np.array_equal(srv.answer_packed(pset[r]), peager[r]), f"packed set {r}"
+                dq.upload(np.ascontiguousarray(np.stack(sets[3][r])))
+                assert srv.lib.ipir_answer_batch_device(srv.h, C.c_void_p(dq.p), 3, C.c_void_p(dout.p), None) == 0
+                assert srv.lib.ipir_dev_sync() == 0
+                assert np.array_equal(dout.download(np.zeros_like(eager[3][r])), eager[3][r]), f"device set {r}"
+        dq.free()
+        dout.free()
+    finally:
+        assert srv.lib.ipir_dbg_set_graph(srv.h, 0) == 0
+        assert srv.lib.ipir_dbg_set_pipeline(srv.h, 3, 2) == 0
+        assert srv.lib.ipir_dbg_set_group_chunk(srv.h, geo["G"]) == 0
+    assert np.array_equal(srv.answer_batch(sets[3][1]), eager[3][1])
+
+
 def test_serving_loop_results_equal_alone(medium8):
     geo, db, srv = medium8
     lib = srv.lib
