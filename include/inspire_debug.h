@@ -66,15 +66,22 @@ int ipir_dbg_horner_variant(const uint64_t* cts, size_t L, const uint64_t* rgsw,
 /* Set the groups per chunk of the pipelined answer (a divisor of n_packed;
  * 1 if the partial-sum buffer cannot hold the resulting plans). */
 int ipir_dbg_set_group_chunk(ipir_server* srv, size_t groups);
-/* Split the pipelined answer's lowest chunk into chunks of 1/2, 1/4 and 1/4
- * of its groups (1, the default, where the chunk size divides by 4) or not
- * (0), so fewer Horner steps are left after the last pass. 1 for other
- * values. */
+/* Split chunks of the pipelined answer where the chunk size divides by 4.
+ * Bit 0 of `on` splits the lowest chunk into 1/2, 1/4 and 1/4 of its
+ * groups, so fewer Horner steps are left after the last pass. Bit 1 splits
+ * the highest chunk into 1/4, 1/4 and 1/2, so the Horner chains start after
+ * a quarter chunk's passes. The default is set at setup (INSPIRE_TAIL_SPLIT
+ * overrides it). 1 for values outside 0..3. */
 int ipir_dbg_set_tail_split(ipir_server* srv, int on);
 /* Select the Horner kernel a server's answers use: mode as variant above,
  * or -1 (the default) to use the pair only for serial answers with at most
  * two chains. */
 int ipir_dbg_set_horner(ipir_server* srv, int mode);
+/* Select the pack kernel of pipelined answers' chunks: 1 one 1024-lane
+ * workgroup per query, 0 (the default, or INSPIRE_PACK_WG at setup) a grid
+ * of one 256-lane workgroup per (group, limb, query), -1 one workgroup for
+ * one-query calls. Outputs are equal. 1 for other values. */
+int ipir_dbg_set_pack(ipir_server* srv, int mode);
 /* Answer through captured hipGraphs (1) or launch by launch (0, the default
  * unless INSPIRE_GRAPH=1 at setup). With 1, the first answer of each
  * (input buffer, output buffer, count) on a stream that is not the null

@@ -288,7 +288,10 @@ int ipir_dbg_tuning(const ipir_server* srv, char* out, size_t cap) {
     if (c->G / c->gchunk > 1)
         s += "; chunks of " + std::to_string(c->gchunk) + " groups: batch1 " + plan(c->chunk_plan(1, c->gchunk * ipx::N)) +
              ", batch" + std::to_string(nq) + " " + plan(c->chunk_plan(nq, c->gchunk * ipx::N));
-    s += std::string(c->tail_split && c->chunks.size() > c->G / c->gchunk ? " (lowest split in 1/2, 1/4, 1/4)" : "");
+    if (c->chunks.size() > c->G / c->gchunk) {
+        s += c->tail_split ? " (lowest split in 1/2, 1/4, 1/4" : " (";
+        s += c->head_split ? std::string(c->tail_split ? "; " : "") + "highest split in 1/4, 1/4, 1/2)" : ")";
+    }
     s += "; collapse variant " + std::to_string(c->collapse_variant) + "; pipeline shapes " +
          std::to_string(c->pipe_single) + "/" + std::to_string(c->pipe_batched);
     std::snprintf(out, cap, "%s", s.c_str());
@@ -436,10 +439,11 @@ int ipir_dbg_set_group_chunk(ipir_server* srv, size_t groups) {
 }
 
 int ipir_dbg_set_tail_split(ipir_server* srv, int on) {
-    if (!srv || (on != 0 && on != 1)) return 1;
+    if (!srv || on < 0 || on > 3) return 1;
     GpuServerCtx* c = srv->ctx;
     std::lock_guard<std::mutex> lk(c->mu);
-    c->tail_split = on != 0;
+    c->tail_split = on & 1;
+    c->head_split = (on >> 1) & 1;
     c->set_chunks();
     c->drop_graphs();  // their kernels carry the old plans
     return 0;
@@ -494,6 +498,14 @@ int ipir_dbg_set_pipeline(ipir_server* srv, int single, int batched) {
     std::lock_guard<std::mutex> lk(srv->ctx->mu);
     srv->ctx->pipe_single = single;
     srv->ctx->pipe_batched = batched;
+    srv->ctx->drop_graphs();  // their kernels carry the old plans
+    return 0;
+}
+
+int ipir_dbg_set_pack(ipir_server* srv, int mode) {
+    if (!srv || mode < -1 || mode > 1) return 1;
+    std::lock_guard<std::mutex> lk(srv->ctx->mu);
+    srv->ctx->pack_wg = mode;
     srv->ctx->drop_graphs();  // their kernels carry the old plans
     return 0;
 }

@@ -127,6 +127,12 @@ hipError_t launch_pack_finalize(const uint32_t* interm, size_t C, const uint32_t
                                 const uint32_t* precomp, size_t g0, size_t ng, const uint32_t* keys,
                                 size_t keys_stride, uint32_t* packed_b, size_t packed_stride, size_t nq,
                                 const NttConst& nc, hipStream_t s);
+// The same output from one 1024-lane workgroup per query (for one-query
+// chunks beside database passes, where it starts without waiting for slots).
+hipError_t launch_pack_finalize_wg(const uint32_t* interm, size_t C, const uint32_t* part2, size_t part2_stride,
+                                   const uint32_t* precomp, size_t g0, size_t ng, const uint32_t* keys,
+                                   size_t keys_stride, uint32_t* packed_b, size_t packed_stride, size_t nq,
+                                   const NttConst& nc, hipStream_t s);
 // Horner over each response ciphertext, restricted to the groups in
 // [g_lo, g_hi): a chain whose top group lies in the range starts there,
 // one that started in an earlier launch resumes from hstate

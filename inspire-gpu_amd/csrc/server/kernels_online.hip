@@ -108,6 +108,51 @@ __global__ __launch_bounds__(256) void k_pack_finalize(const uint32_t* __restric
     }
 }
 
+// The same result with one workgroup per query: four 256-lane teams take one
+// (group, limb) polynomial each per round. A one-query chunk's grid of the
+// kernel above (2 workgroups per group) waits for CU slots behind the
+// database-pass workgroups running beside it; one workgroup starts as soon
+// as one CU has room, like the Horner chain it feeds.
+__global__ __launch_bounds__(1024) void k_pack_finalize_wg(const uint32_t* __restrict__ interm, uint32_t C,
+                                                           const uint32_t* __restrict__ part2, size_t p2_stride,
+                                                           const uint32_t* __restrict__ precomp,
+                                                           const uint32_t* __restrict__ keys, size_t kstride,
+                                                           uint32_t* __restrict__ packed_b, size_t pstride,
+                                                           uint32_t g0, uint32_t ng, NttConst nc) {
+    __shared__ uint32_t buf[4][POLY_LDS];
+    __shared__ uint32_t twl[8 * N];
+    const uint32_t tid = threadIdx.x, team = tid >> 8, t = tid & 255, qi = blockIdx.z;
+    const uint32_t npoly = 2 * ng, rounds = (npoly + 3) / 4;
+    uint32_t* b = buf[team];
+    stage_twiddles(twl, nc, tid, 1024);  // the first round's barrier publishes them
+    for (uint32_t r = 0; r < rounds; r++) {  // every lane runs every round (barriers inside)
+        const uint32_t k = 4 * r + team;
+        const bool active = k < npoly;
+        const uint32_t g = g0 + (active ? k >> 1 : 0), c = k & 1;
+        const uint32_t q = qof((int)c);
+        if (active) {
+            const uint32_t* pp = interm + (size_t)qi * 2 * C + (size_t)c * C + (size_t)g * N;
+#pragma unroll
+            for (int e = 0; e < 8; e++) b[ls(t + 256 * e)] = pp[t + 256 * e];
+        }
+        __syncthreads();
+        ntt_fwd_lds(b, t, tw_lds(twl, (int)c), (int)c, active);
+        if (active) {
+            const uint32_t* Df = precomp + (size_t)g * PC_GROUP + PC_DF + (size_t)c * D_EFF * N;
+            const uint32_t* Kn = keys + qi * kstride + BLK + (size_t)c * N;
+            const uint32_t* p2 = part2 + qi * p2_stride + (size_t)g * 2 * N + (size_t)c * N;
+#pragma unroll 2
+            for (int e = 0; e < 8; e++) {
+                const uint32_t i = t + 256 * e;
+                const uint64_t sub = (uint64_t)Df[i] * Kn[i] + (uint64_t)Df[N + i] * Kn[2 * N + i] + p2[i];
+                packed_b[qi * pstride + (size_t)g * 2 * N + (size_t)c * N + i] =
+                    sub_mod(b[ls(i)], red64(sub, (int)c), q);
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // ------------------------------------------------------------ Horner chain
 // One workgroup per (response ciphertext, query); 4 teams of 256 lanes.
 // acc (a0, a1, b0, b1 polys, NTT form) lives in registers between steps;
@@ -308,6 +353,17 @@ hipError_t launch_pack_finalize(const uint32_t* interm, size_t C, const uint32_t
     if (ng == 0 || nq == 0) return hipSuccess;
     hipLaunchKernelGGL(k_pack_finalize, dim3((unsigned)ng, 2, (unsigned)nq), dim3(256), 0, s, interm, (uint32_t)C,
                        part2, part2_stride, precomp, keys, keys_stride, packed_b, packed_stride, (uint32_t)g0, nc);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack_finalize_wg(const uint32_t* interm, size_t C, const uint32_t* part2, size_t part2_stride,
+                                   const uint32_t* precomp, size_t g0, size_t ng, const uint32_t* keys,
+                                   size_t keys_stride, uint32_t* packed_b, size_t packed_stride, size_t nq,
+                                   const NttConst& nc, hipStream_t s) {
+    if (ng == 0 || nq == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_pack_finalize_wg, dim3(1, 1, (unsigned)nq), dim3(1024), 0, s, interm, (uint32_t)C, part2,
+                       part2_stride, precomp, keys, keys_stride, packed_b, packed_stride, (uint32_t)g0, (uint32_t)ng,
+                       nc);
     return hipGetLastError();
 }
 

@@ -278,7 +278,14 @@ void GpuServerCtx::alloc_slots() {
         const long v = std::atol(gc);
         if (v > 0 && (size_t)v <= G && G % (size_t)v == 0) gchunk = (size_t)v;
     }
-    if (const char* ts = std::getenv("INSPIRE_TAIL_SPLIT")) tail_split = ts[0] != '0';
+    // INSPIRE_TAIL_SPLIT=<0..3>: bit 0 splits the lowest chunk, bit 1 the highest
+    if (const char* ts = std::getenv("INSPIRE_TAIL_SPLIT")) {
+        const int v = std::atoi(ts);
+        if (v >= 0 && v <= 3) {
+            tail_split = v & 1;
+            head_split = (v >> 1) & 1;
+        }
+    }
     set_chunks();
     // row-slice partials of one query: [slices][2][ncols] words for the
     // widest plan any candidate (full width or narrowed to a chunk) needs
@@ -335,6 +342,9 @@ void GpuServerCtx::alloc_slots() {
     d_hflags = dalloc<uint32_t>(2 * B * T);
     d_status = dalloc<uint32_t>(1);
     HIPX(hipMemset(d_status, 0, 4));
+    // INSPIRE_PACK_WG=0/1/-1: the pipelined chunks' pack kernel, a grid over
+    // (group, limb), one workgroup per query, or one workgroup for one query
+    if (const char* pw = std::getenv("INSPIRE_PACK_WG")) pack_wg = pw[0] == '1' ? 1 : pw[0] == '0' ? 0 : -1;
     // INSPIRE_GRAPH=1: answers replay captured hipGraphs
     if (const char* gm = std::getenv("INSPIRE_GRAPH")) graph_mode = gm[0] == '1' ? 1 : 0;
     if (const char* h = std::getenv("INSPIRE_HORNER"))
@@ -593,15 +603,21 @@ DbPlan GpuServerCtx::plan_for(size_t nq, size_t ncols) const {
 
 void GpuServerCtx::set_chunks() {
     chunks.clear();
-    const size_t n = G / gchunk;
-    for (size_t k = n; k-- > 1;) chunks.push_back({k * gchunk, (k + 1) * gchunk});
-    if (tail_split && n > 1 && gchunk % 4 == 0) {
-        const size_t h = gchunk / 2, q = gchunk / 4;
-        chunks.push_back({h, gchunk});
-        chunks.push_back({q, h});
-        chunks.push_back({0, q});
-    } else {
-        chunks.push_back({0, gchunk});
+    const size_t n = G / gchunk, h = gchunk / 2, q = gchunk / 4;
+    const bool quarters = n > 1 && gchunk % 4 == 0;
+    for (size_t k = n; k-- > 0;) {
+        const size_t lo = k * gchunk, hi = lo + gchunk;
+        if (quarters && head_split && k == n - 1) {
+            chunks.push_back({hi - q, hi});
+            chunks.push_back({hi - h, hi - q});
+            chunks.push_back({lo, hi - h});
+        } else if (quarters && tail_split && k == 0) {
+            chunks.push_back({lo + h, hi});
+            chunks.push_back({lo + q, lo + h});
+            chunks.push_back({lo, lo + q});
+        } else {
+            chunks.push_back({lo, hi});
+        }
     }
 }
 
@@ -763,8 +779,12 @@ void GpuServerCtx::enqueue(const uint64_t* d_in, size_t count, uint64_t* d_out, 
             HIPX(hipEventRecord(ce, cs));
             if (cs != s) HIPX(hipStreamWaitEvent(hs, dbe, 0));
             HIPX(hipStreamWaitEvent(hs, ce, 0));
-            HIPX(launch_pack_finalize(d_interm + q0 * 2 * C, C, p2, part2_stride, d_precomp, g_lo, ng, keys,
-                                      keys_stride, packed, packed_stride, nq, nc, hs));
+            if (nq == 1 ? pack_wg != 0 : pack_wg == 1)
+                HIPX(launch_pack_finalize_wg(d_interm + q0 * 2 * C, C, p2, part2_stride, d_precomp, g_lo, ng, keys,
+                                             keys_stride, packed, packed_stride, nq, nc, hs));
+            else
+                HIPX(launch_pack_finalize(d_interm + q0 * 2 * C, C, p2, part2_stride, d_precomp, g_lo, ng, keys,
+                                          keys_stride, packed, packed_stride, nq, nc, hs));
             horner_range(nq, out, g_lo, g_hi, hs, q0, true);
         }
         if (cs != s) {  // the next batch's key digits overwrite the lane's scratch

@@ -55,8 +55,10 @@ struct GpuServerCtx {
     size_t gchunk = 0;              // groups per chunk of the pipelined answer
     // the pipelined answer's chunks [lo, hi), highest groups first: chunks of
     // gchunk groups, the lowest one split into gchunk/2, /4, /4 when
-    // tail_split (its Horner steps are the ones left after every pass)
-    bool tail_split = true;
+    // tail_split (its Horner steps are the ones left after every pass), the
+    // highest one into gchunk/4, /4, /2 when head_split (the chains start
+    // after a quarter chunk's passes)
+    bool tail_split = true, head_split = false;
     std::vector<std::pair<size_t, size_t>> chunks;
     void set_chunks();
     // pipeline shape of one-query and many-query calls: 0 serial on the
@@ -97,6 +99,9 @@ struct GpuServerCtx {
     // answer is serial (beside the passes the pair's exchange is slower)
     int horner_mode = -1;
     static constexpr size_t kPairMaxChains = 2;
+    // pack kernel of pipelined chunks: 1 one workgroup per query, 0 a grid
+    // over (group, limb), -1 one workgroup for one-query calls
+    int pack_wg = 0;
     uint32_t* d_hxbuf = nullptr;    // pair exchange buffers [B*T chains]
     uint32_t* d_hflags = nullptr;   // pair step flags [B*T][2]
     uint32_t* d_status = nullptr;   // sticky device fault word (pair wait bound exceeded)

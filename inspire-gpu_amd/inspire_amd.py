@@ -93,6 +93,7 @@ SERVER_FUNCS = [
     ("ipir_dbg_horner_variant", C.c_int, [VP, SZ, VP, VP, C.c_int]),
     ("ipir_dbg_set_horner", C.c_int, [VP, C.c_int]),
     ("ipir_dbg_set_graph", C.c_int, [VP, C.c_int]),
+    ("ipir_dbg_set_pack", C.c_int, [VP, C.c_int]),
     ("ipir_dbg_set_group_chunk", C.c_int, [VP, SZ]),
     ("ipir_dbg_set_mfma_depth", C.c_int, [VP, C.c_uint32, C.c_int]),
     ("ipir_dbg_set_chunk_slices", C.c_int, [VP, SZ, C.c_uint32]),

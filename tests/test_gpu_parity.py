@@ -275,7 +275,9 @@ def test_ragged_batch_over_sub_batches(medium8):
             sts.append(st)
         assert srv.lib.ipir_dbg_set_group_chunk(srv.h, geo["G"]) == 0
         serial = srv.answer_batch(qs)
-        for groups, split in ((2, 1), (4, 1), (4, 0)):  # chunks of 2; of 4 with the lowest split 2+1+1, or not
+        # chunks of 2; of 4 with the lowest split 2+1+1 (bit 0), the highest
+        # 1+1+2 (bit 1), both, or neither
+        for groups, split in ((2, 1), (4, 1), (4, 0), (4, 2), (4, 3)):
             assert srv.lib.ipir_dbg_set_group_chunk(srv.h, groups) == 0
             assert srv.lib.ipir_dbg_set_tail_split(srv.h, split) == 0
             for shapes in ((3, 2), (2, 3), (3, 3)):
@@ -336,9 +338,10 @@ def test_pair_horner_answers_equal_single(medium8):
 def test_pipelined_group_chunks_equal_unchunked(medium8, groups, shape, slices):
     """The answer pipelined over group chunks (Horner chains on a second
     stream, resuming between launches; shape 3 also the collapse on a third)
-    equals the serial one-chunk answer, for both Horner kernels, one query
-    and three; chunk database passes with the narrowing rule's row slices or
-    a set count (one: straight into the chunk's columns)."""
+    equals the serial one-chunk answer, for both Horner kernels and both
+    pack kernels, one query and three; chunk database passes with the
+    narrowing rule's row slices or a set count (one: straight into the
+    chunk's columns)."""
     geo, db, srv = medium8
     qs = [ol.make_query(geo, SEED, idx, 800 + k)[0] for k, idx in enumerate([3, 99999, 524287])]
     base = srv.answer_batch(qs)
@@ -351,15 +354,25 @@ def test_pipelined_group_chunks_equal_unchunked(medium8, groups, shape, slices):
     try:
         got = srv.answer_batch(qs)
         got1 = srv.answer_batch(qs[:1])
+        # the chunks' one-workgroup pack kernel: for three queries, and for
+        # one query alone
+        assert srv.lib.ipir_dbg_set_pack(srv.h, 2) == 1
+        assert srv.lib.ipir_dbg_set_pack(srv.h, 1) == 0
+        got_wg = srv.answer_batch(qs)
+        assert srv.lib.ipir_dbg_set_pack(srv.h, -1) == 0
+        got1_wg = srv.answer_batch(qs[:1])  # -1: one workgroup for one query
+        assert srv.lib.ipir_dbg_set_pack(srv.h, 0) == 0
         assert srv.lib.ipir_dbg_set_horner(srv.h, 1) == 0
         pair = srv.answer_batch(qs)
     finally:
+        assert srv.lib.ipir_dbg_set_pack(srv.h, 0) == 0
         assert srv.lib.ipir_dbg_set_horner(srv.h, -1) == 0
         assert srv.lib.ipir_dbg_set_pipeline(srv.h, 3, 2) == 0
         assert srv.lib.ipir_dbg_set_chunk_slices(srv.h, 3, 0) == 0
         assert srv.lib.ipir_dbg_set_chunk_slices(srv.h, 1, 0) == 0
         assert srv.lib.ipir_dbg_set_group_chunk(srv.h, geo["G"]) == 0
     assert np.array_equal(got, base) and np.array_equal(got1[0], one[0]) and np.array_equal(pair, base)
+    assert np.array_equal(got_wg, base) and np.array_equal(got1_wg[0], one[0])
 
 
 def test_packed_wire_answers_match_host_codecs(medium8):
