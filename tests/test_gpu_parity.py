@@ -284,6 +284,15 @@ def test_ragged_batch_over_sub_batches(medium8):
                 assert srv.lib.ipir_dbg_set_pipeline(srv.h, *shapes) == 0
                 got = srv.answer_batch(qs)
                 assert np.array_equal(got, serial), f"groups {groups} split {split} shapes {shapes}"
+        # both sub-batches captured into one graph, replayed with new inputs
+        assert srv.lib.ipir_dbg_set_graph(srv.h, 1) == 0
+        try:
+            for rep in range(3):
+                order = qs if rep % 2 == 0 else qs[::-1]
+                want = serial if rep % 2 == 0 else serial[::-1]
+                assert np.array_equal(srv.answer_batch(order), want), f"graph replay {rep}"
+        finally:
+            assert srv.lib.ipir_dbg_set_graph(srv.h, 0) == 0
         for i in (0, 31, 32):
             assert np.array_equal(ol.extract(geo, sts[i], got[i].reshape(geo["T"], -1)),
                                   ol.expected_slots(geo, db, idxs[i]))
