@@ -833,6 +833,7 @@ void GpuServerCtx::run_step(const void* in, void* out, size_t count, size_t slot
     (void)hipGraphDestroy(graph);
     HIPX(e);
     if (graphs.size() >= kMaxGraphs) {
+        (void)hipDeviceSynchronize();  // the evicted graph may still run on another stream
         (void)hipGraphExecDestroy(graphs.front().exec);
         graphs.erase(graphs.begin());
     }
