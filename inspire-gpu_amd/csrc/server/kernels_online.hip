@@ -60,21 +60,35 @@ __global__ __launch_bounds__(256) void k_ntt_rns(uint32_t* __restrict__ polys, N
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 // interm[q][limb][col] = sum over row slices of part mod q_limb, for columns
-// col0 + [0, gridDim.x*1024); blockIdx.y = limb, blockIdx.z = query.
+// col0 + [0, gridDim.x*256); blockIdx.y = limb, blockIdx.z = query. A
+// workgroup covers 256 columns (4 per lane of a wave); its 4 waves take every
+// 4th slice, so narrow passes with many slices still spread over many CUs
+// and each lane has few loads in sequence. Exact 64-bit sums (< 2^32 per
+// slice), combined in LDS, then one reduction.
 __global__ __launch_bounds__(256) void k_reduce_slices(const uint32_t* __restrict__ part, size_t part_stride,
                                                        uint32_t S, uint32_t C, uint32_t col0, uint32_t ncols,
                                                        uint32_t* __restrict__ interm) {
-    const uint32_t j = (blockIdx.x * 256 + threadIdx.x) * 4, col = col0 + j;
+    __shared__ uint64_t acc[3][64][4];
+    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t j = blockIdx.x * 256 + lane * 4, col = col0 + j;
     const int limb = (int)blockIdx.y;
     const uint32_t q = blockIdx.z;
     const u32x4* pp = reinterpret_cast<const u32x4*>(part + q * part_stride + (size_t)limb * ncols + j);
+    const size_t sstride = 2 * (size_t)ncols / 4;  // one slice, in u32x4
     uint64_t a[4] = {0, 0, 0, 0};
-    for (uint32_t s = 0; s < S; s++) {
-        const u32x4 v = __builtin_nontemporal_load(pp + (size_t)s * (2 * ncols / 4));
+#pragma unroll 4
+    for (uint32_t s = w; s < S; s += 4) {
+        const u32x4 v = __builtin_nontemporal_load(pp + (size_t)s * sstride);
         a[0] += v[0]; a[1] += v[1]; a[2] += v[2]; a[3] += v[3];
     }
+    if (w > 0)
+#pragma unroll
+        for (int e = 0; e < 4; e++) acc[w - 1][lane][e] = a[e];
+    __syncthreads();
+    if (w > 0) return;
     u32x4 o;
-    o[0] = red64(a[0], limb); o[1] = red64(a[1], limb); o[2] = red64(a[2], limb); o[3] = red64(a[3], limb);
+#pragma unroll
+    for (int e = 0; e < 4; e++) o[e] = red64(a[e] + acc[0][lane][e] + acc[1][lane][e] + acc[2][lane][e], limb);
     *reinterpret_cast<u32x4*>(interm + (size_t)q * 2 * C + (size_t)limb * C + col) = o;
 }
 
@@ -341,7 +355,7 @@ hipError_t launch_reduce_slices(const uint32_t* part, size_t part_stride, uint32
                                 size_t nq, size_t col0, size_t ncols, hipStream_t s) {
     if (ncols % 1024 || col0 % 4 || col0 + ncols > C) return hipErrorInvalidValue;
     if (nq == 0 || ncols == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_reduce_slices, dim3((unsigned)(ncols / 1024), 2, (unsigned)nq), dim3(256), 0, s, part,
+    hipLaunchKernelGGL(k_reduce_slices, dim3((unsigned)(ncols / 256), 2, (unsigned)nq), dim3(256), 0, s, part,
                        part_stride, S, (uint32_t)C, (uint32_t)col0, (uint32_t)ncols, interm);
     return hipGetLastError();
 }
