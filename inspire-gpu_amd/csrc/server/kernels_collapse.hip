@@ -401,7 +401,10 @@ hipError_t launch_collapse(const uint32_t* precomp_g0, size_t ng, const int8_t* 
     hipLaunchKernelGGL((k_collapse_ring<8, 1, D, W>), dim3(N / (16 * W), 2, (unsigned)ng), dim3(64 * W), 0, s, \
                        precomp_g0, kdig, KS, part2_g0, p2_stride, Q)
     switch (collapse_mt(nq)) {
-        case 1: COLLAPSE(1, 2, false); break;
+        case 1:  // few groups: 16-column tiles, twice the workgroups
+            if (ng < 16) COLLAPSE(1, 1, false);
+            else COLLAPSE(1, 2, false);
+            break;
         case 2: COLLAPSE(2, 2, false); break;
         case 4: COLLAPSE(4, 2, false); break;
         default:
