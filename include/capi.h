@@ -15,7 +15,10 @@
  *
  * Thread-safety: one ipir_server is externally serialized for
  * ipir_answer_batch / ipir_answer_batch_device; the serving loop
- * (ipir_loop_*) is the thread-safe, concurrent front end.
+ * (ipir_loop_*) is the thread-safe, concurrent front end. While a loop runs
+ * on a server, the server's other answer entry points return 5 (busy)
+ * instead of blocking. Asynchronous calls on different HIP streams are
+ * ordered by the library: each call's work waits for the previous call's.
  */
 #ifndef INSPIRE_CAPI_H
 #define INSPIRE_CAPI_H
@@ -64,7 +67,8 @@ ipir_server* ipir_server_create(size_t n_entries, size_t entry_bytes, size_t db_
                                 const uint8_t* crs_seed_or_null, ipir_params* out_params);
 void ipir_server_caps(const ipir_server* srv, ipir_caps* out);
 /* 0 ok; 1 null argument; 2 count > max_batch; 3 null view field;
- * 4 device failure. count == 0 returns 0 and writes nothing. */
+ * 4 device failure; 5 busy (a serving loop owns the server; the reference
+ * has no loop and no such code). count == 0 returns 0 and writes nothing. */
 int ipir_answer_batch(ipir_server* srv, const ipir_query_view* queries, size_t count,
                       uint64_t* out_resp);
 void ipir_server_destroy(ipir_server* srv);
@@ -107,7 +111,11 @@ ipir_client_query* ipir_query_build_seeded(const ipir_params* params, uint64_t i
  * memory (count * ipir_query_u64s u64s), writing the flat responses to
  * DEVICE memory, on the given HIP stream (NULL = the server's own stream).
  * Asynchronous: returns after enqueueing. Return codes as
- * ipir_answer_batch. For callers that keep queries resident in HBM. */
+ * ipir_answer_batch (4 only for launch failures: the answer's kernels
+ * report nothing afterwards). The work waits for the server's previous
+ * call if that one was on another stream; inside a caller's stream capture
+ * the caller orders the captured work itself. For callers that keep
+ * queries resident in HBM. */
 int ipir_answer_batch_device(ipir_server* srv, const uint64_t* d_queries, size_t count,
                              uint64_t* d_out_resp, void* hip_stream);
 
@@ -125,19 +133,59 @@ int ipir_answer_batch_packed(ipir_server* srv, const uint8_t* packed_queries, si
 int ipir_answer_batch_packed_device(ipir_server* srv, const uint8_t* d_packed_queries, size_t count,
                                     uint8_t* d_out_compressed, void* hip_stream);
 
+/* Device memory of a server of this geometry (setup()'s rules; 1 if it
+ * rejects the geometry): resident_fixed (DB, precomp, fixed scratch) plus
+ * per_slot for each of max_batch answer slots is what ipir_server_caps
+ * reports as resident_bytes; preprocess_peak is the most gpu_preprocess
+ * holds at once (DB, precomp and its transient buffers). For sizing
+ * max_batch against free device memory before ipir_server_create, as the
+ * reference's caps are meant for (src/gpu_protocol.h:26-33). */
+typedef struct {
+    size_t resident_fixed;
+    size_t per_slot;
+    size_t preprocess_peak;
+} ipir_footprint;
+int ipir_server_footprint(size_t n_entries, size_t entry_bytes, size_t db_rows_or_zero, ipir_footprint* out);
+/* The largest max_batch (at most cap) whose server fits the current
+ * device's free memory with reserve_bytes left over, both while
+ * preprocessing and once set up; 0 if none fits or the geometry is
+ * rejected. */
+size_t ipir_fit_max_batch(size_t n_entries, size_t entry_bytes, size_t db_rows_or_zero, size_t reserve_bytes,
+                          size_t cap);
+
 /* Continuous-batching serving loop over one server. Requests are admitted
  * into free slots as they arrive, batches of different requests overlap on
  * the device, and each request completes independently.
- *   max_inflight: concurrent device batches (1..4); slots are shared.
+ *   max_inflight: concurrent device batches (1..4), each taking
+ *   max_batch / max_inflight slots; 0 sizes them from the server's slots:
+ *   partitions of 32 (one full batched database pass; all slots if fewer),
+ *   as many as max_batch holds, up to 4.
+ * One loop per server: NULL if one runs already (or on bad arguments).
+ * While it runs, the server's other answer entry points return 5.
  * ipir_loop_submit copies the query, returns a ticket (> 0) or 0 on error.
  * ipir_loop_wait blocks until the ticket's response is written to the
  * out_resp given at submit; returns 0 and the request latency (submit to
- * completion) in *latency_us. */
+ * completion) in *latency_us, 4 on a device failure of its batch.
+ * ipir_loop_stop lets admitted work finish, then returns the server to
+ * its other entry points. */
 typedef struct ipir_loop ipir_loop;
 ipir_loop* ipir_loop_start(ipir_server* srv, size_t max_inflight);
 uint64_t ipir_loop_submit(ipir_loop* loop, const ipir_query_view* q, uint64_t* out_resp);
 int ipir_loop_wait(ipir_loop* loop, uint64_t ticket, double* latency_us);
 void ipir_loop_stop(ipir_loop* loop);
+/* Counters of a running loop (a snapshot; 0 ok, 1 null argument). */
+typedef struct {
+    uint64_t partitions, slots_per_partition;
+    uint64_t submitted, completed, failed;
+    uint64_t batches;              /* device batches launched */
+    uint64_t max_batch_size;       /* most requests admitted into one batch */
+    uint64_t max_busy_partitions;  /* most partitions holding admitted requests at once */
+    uint64_t admitted_while_busy;  /* batches admitted while another partition held requests */
+    uint64_t device_overlaps;      /* batches launched while an earlier batch's work was still on the device */
+    uint64_t pending;              /* requests waiting for a free partition now */
+    uint64_t busy_partitions;      /* partitions holding admitted requests now */
+} ipir_loop_stats;
+int ipir_loop_get_stats(ipir_loop* loop, ipir_loop_stats* out);
 
 #ifdef __cplusplus
 }

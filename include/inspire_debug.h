@@ -22,6 +22,10 @@ size_t ipir_dbg_precomp_group_words(void);
 int ipir_dbg_precomp_group(const ipir_server* srv, size_t group, uint32_t* out);
 /* rows [row0, row0+nrows) of the encoded u16 database, returned row-major */
 int ipir_dbg_db_rows(const ipir_server* srv, size_t row0, size_t nrows, uint16_t* out);
+/* The block of rows [row0, row0+nrows) and columns [col0, col0+ncols) of
+ * the encoded u16 database, returned row-major (nrows x ncols). */
+int ipir_dbg_db_block(const ipir_server* srv, size_t row0, size_t nrows, size_t col0, size_t ncols,
+                      uint16_t* out);
 /* Database pass alone over nq (<= max_batch) query vectors b [nq][2][db_rows]
  * (residues mod q0 | q1, u64): interm [nq][2][db_cols]. path -1 = the tuned
  * plans of the answer pipeline, 0 = the fp64 pass per query, 1 = the int8
@@ -75,8 +79,21 @@ int ipir_dbg_set_group_chunk(ipir_server* srv, size_t groups);
 int ipir_dbg_set_tail_split(ipir_server* srv, int on);
 /* Select the Horner kernel a server's answers use: mode as variant above,
  * or -1 (the default) to use the pair only for serial answers with at most
- * two chains. */
+ * two chains. Whatever the mode, the asynchronous device entry points
+ * (ipir_answer_batch_device, ipir_answer_batch_packed_device) never use the
+ * pair: its bounded wait reports through a status word that only the
+ * blocking entry points and the serving loop read. */
 int ipir_dbg_set_horner(ipir_server* srv, int mode);
+/* Bound of the pair kernel's partner wait, in sleeps (default 2^26, ~2 s).
+ * 0 makes every pair chain report the bound exceeded at its first
+ * exchange and stop (all kernels still end normally): the call that ran it
+ * then returns 4, and later calls are unaffected. 5 while a loop runs. */
+int ipir_dbg_set_horner_spin_limit(ipir_server* srv, uint32_t limit);
+/* Keep a serving loop's partitions busy after their device work finished
+ * (on = 1) until released (on = 0): requests admitted meanwhile go to the
+ * other partitions or wait. For tests of admission while batches are in
+ * flight. ipir_loop_stop releases a hold. */
+int ipir_dbg_loop_hold(ipir_loop* loop, int on);
 /* Select the pack kernel of pipelined answers' chunks: 1 one 1024-lane
  * workgroup per query, 0 (the default, or INSPIRE_PACK_WG at setup) a grid
  * of one 256-lane workgroup per (group, limb, query), -1 one workgroup for
@@ -127,6 +144,12 @@ void* ipir_dev_alloc(size_t bytes);
 int ipir_dev_free(void* p);
 int ipir_dev_copy(void* dst, const void* src, size_t bytes, int kind);
 int ipir_dev_sync(void);
+/* A HIP stream (default priority) for the asynchronous entry points, its
+ * synchronisation and destruction; free/total device memory now. */
+void* ipir_dev_stream_create(void);
+int ipir_dev_stream_sync(void* stream);
+int ipir_dev_stream_destroy(void* stream);
+int ipir_dev_mem_info(size_t* free_bytes, size_t* total_bytes);
 
 #ifdef __cplusplus
 }

@@ -19,13 +19,18 @@ namespace inspire {
 // dot-product form (D+ and D- together), D_minus null. When d_precomp_arena
 // is null the per-group pointers own their memory and hold the reference's
 // layout (D+ / D- each [limb][step][digit][coeff]); gpu_setup_server
-// converts them and frees them.
+// converts them and frees them. gpu_setup_server takes ownership only when
+// it succeeds; if it throws, every buffer is still the caller's.
 struct PreprocessData {
     std::vector<uint32_t*> d_precomp_a;        // [n_packed] -> 2*N u32, NTT form
     std::vector<uint32_t*> d_precomp_D_plus;   // [n_packed] (see above)
     std::vector<uint32_t*> d_precomp_D_minus;  // [n_packed] (see above)
     std::vector<uint32_t*> d_precomp_D_final;  // [n_packed] -> 2*D_EFF*N u32
-    // encoded DB, u16 in 16-row blocks: (r, c) at ((r/16)*cols + c)*16 + r%16
+    // encoded DB in the server's resident layout, as gpu_preprocess leaves
+    // it: two byte planes (low, high byte) in 16-row blocks, plane p of
+    // (r, c) at byte ((r/16)*2 + p)*cols*16 + c*16 + r%16 (db_rows*db_cols*2
+    // bytes). gpu_setup_server adopts it as is, also with per-group
+    // precomp buffers (it does not accept the reference's row-major DB).
     uint16_t* d_db_col = nullptr;
     void* d_precomp_arena = nullptr;
 };

@@ -86,6 +86,7 @@ int ipir_answer_batch(ipir_server* srv, const ipir_query_view* queries, size_t c
     }
     try {
         std::lock_guard<std::mutex> lk(srv->ctx->mu);
+        if (srv->ctx->loop_active) return 5;
         stage_views(srv->ctx, queries, count);
         srv->ctx->answer_host(count, out_resp);
         return 0;
@@ -100,9 +101,15 @@ int ipir_answer_batch_device(ipir_server* srv, const uint64_t* d_queries, size_t
     if (count == 0) return 0;
     if (count > srv->ctx->cfg.max_batch) return 2;
     try {
-        hipStream_t s = hip_stream ? static_cast<hipStream_t>(hip_stream) : srv->ctx->stream;
-        srv->ctx->run_step(d_queries, d_out_resp, count, 0, 0, 0, s,
-                           [&] { srv->ctx->enqueue(d_queries, count, d_out_resp, s); });
+        GpuServerCtx* c = srv->ctx;
+        std::lock_guard<std::mutex> lk(c->mu);
+        if (c->loop_active) return 5;
+        hipStream_t s = hip_stream ? static_cast<hipStream_t>(hip_stream) : c->stream;
+        c->order_before(s);
+        // nobody reads a status word after an asynchronous call, so its
+        // answers never take the pair Horner kernel (enqueue's default)
+        c->run_step(d_queries, d_out_resp, count, 0, 0, 0, s, [&] { c->enqueue(d_queries, count, d_out_resp, s); });
+        c->order_after(s);
         return 0;
     } catch (...) {
         return 4;
@@ -116,6 +123,7 @@ int ipir_answer_batch_packed(ipir_server* srv, const uint8_t* packed_queries, si
     if (count > srv->ctx->cfg.max_batch) return 2;
     try {
         std::lock_guard<std::mutex> lk(srv->ctx->mu);
+        if (srv->ctx->loop_active) return 5;
         srv->ctx->answer_packed_host(packed_queries, count, out_compressed);
         return 0;
     } catch (...) {
@@ -129,13 +137,43 @@ int ipir_answer_batch_packed_device(ipir_server* srv, const uint8_t* d_packed_qu
     if (count == 0) return 0;
     if (count > srv->ctx->cfg.max_batch) return 2;
     try {
-        hipStream_t s = hip_stream ? static_cast<hipStream_t>(hip_stream) : srv->ctx->stream;
-        srv->ctx->run_step(d_packed_queries, d_out_compressed, count, 0, 0, 1, s,
-                           [&] { srv->ctx->enqueue_packed(d_packed_queries, count, d_out_compressed, s); });
+        GpuServerCtx* c = srv->ctx;
+        std::lock_guard<std::mutex> lk(c->mu);
+        if (c->loop_active) return 5;
+        hipStream_t s = hip_stream ? static_cast<hipStream_t>(hip_stream) : c->stream;
+        c->order_before(s);
+        c->run_step(d_packed_queries, d_out_compressed, count, 0, 0, 1, s,
+                    [&] { c->enqueue_packed(d_packed_queries, count, d_out_compressed, s); });
+        c->order_after(s);
         return 0;
     } catch (...) {
         return 4;
     }
+}
+
+int ipir_server_footprint(size_t n_entries, size_t entry_bytes, size_t db_rows_or_zero, ipir_footprint* out) {
+    if (!out) return 1;
+    try {
+        const PublicParams pp = setup(n_entries, entry_bytes, db_rows_or_zero ? db_rows_or_zero : 32768);
+        const ServerFootprint f = server_footprint(pp, default_group_chunk(pp.n_packed));
+        out->resident_fixed = f.fixed;
+        out->per_slot = f.per_slot;
+        out->preprocess_peak = f.preprocess_peak;
+        return 0;
+    } catch (...) {
+        return 1;
+    }
+}
+
+size_t ipir_fit_max_batch(size_t n_entries, size_t entry_bytes, size_t db_rows_or_zero, size_t reserve_bytes,
+                          size_t cap) {
+    ipir_footprint f{};
+    if (ipir_server_footprint(n_entries, entry_bytes, db_rows_or_zero, &f)) return 0;
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) return 0;
+    if (fr < reserve_bytes || fr - reserve_bytes < f.preprocess_peak || fr - reserve_bytes < f.resident_fixed) return 0;
+    const size_t b = (fr - reserve_bytes - f.resident_fixed) / f.per_slot;
+    return b < cap ? b : cap;
 }
 
 void ipir_server_destroy(ipir_server* srv) {
@@ -195,22 +233,31 @@ int ipir_dbg_precomp_group(const ipir_server* srv, size_t group, uint32_t* out) 
     return 0;
 }
 
-int ipir_dbg_db_rows(const ipir_server* srv, size_t row0, size_t nrows, uint16_t* out) {
-    if (!srv || !out || row0 + nrows > srv->pp.db_rows) return 1;
-    if (nrows == 0) return 0;
+int ipir_dbg_db_block(const ipir_server* srv, size_t row0, size_t nrows, size_t col0, size_t ncols,
+                      uint16_t* out) {
+    if (!srv || !out || row0 + nrows > srv->pp.db_rows || col0 + ncols > srv->pp.db_cols) return 1;
+    if (nrows == 0 || ncols == 0) return 0;
     const size_t C = srv->pp.db_cols;
-    // resident layout: plane p of (r, c) at byte ((r/16)*2 + p)*C*16 + c*16 + r%16
+    // resident layout: plane p of (r, c) at byte ((r/16)*2 + p)*C*16 + c*16 + r%16;
+    // one 16-row block of both planes at a time, columns [col0, col0+ncols) of each
     const size_t kb0 = row0 / 16, kb1 = (row0 + nrows + 15) / 16;
-    std::vector<uint8_t> blk((kb1 - kb0) * 2 * C * 16);
-    if (hipMemcpy(blk.data(), srv->ctx->db_planes() + kb0 * 2 * C * 16, blk.size(), hipMemcpyDeviceToHost) !=
-        hipSuccess)
-        return 4;
-    for (size_t r = row0; r < row0 + nrows; r++)
-        for (size_t c = 0; c < C; c++) {
-            const size_t base = ((r / 16 - kb0) * 2 * C + c) * 16 + r % 16;
-            out[(r - row0) * C + c] = (uint16_t)(blk[base] | (blk[base + C * 16] << 8));
-        }
+    std::vector<uint8_t> blk(2 * ncols * 16);
+    for (size_t kb = kb0; kb < kb1; kb++) {
+        for (size_t p = 0; p < 2; p++)
+            if (hipMemcpy(blk.data() + p * ncols * 16, srv->ctx->db_planes() + ((kb * 2 + p) * C + col0) * 16,
+                          ncols * 16, hipMemcpyDeviceToHost) != hipSuccess)
+                return 4;
+        for (size_t r = kb * 16 > row0 ? kb * 16 : row0; r < (kb + 1) * 16 && r < row0 + nrows; r++)
+            for (size_t c = 0; c < ncols; c++) {
+                const size_t base = c * 16 + r % 16;
+                out[(r - row0) * ncols + c] = (uint16_t)(blk[base] | (blk[base + ncols * 16] << 8));
+            }
+    }
     return 0;
+}
+
+int ipir_dbg_db_rows(const ipir_server* srv, size_t row0, size_t nrows, uint16_t* out) {
+    return srv ? ipir_dbg_db_block(srv, row0, nrows, 0, srv->pp.db_cols, out) : 1;
 }
 
 int ipir_dbg_db_pass(ipir_server* srv, const uint64_t* b, size_t nq, int path, uint32_t* interm) {
@@ -221,10 +268,13 @@ int ipir_dbg_db_pass(ipir_server* srv, const uint64_t* b, size_t nq, int path, u
         if (b[i] >= ((i / R) % 2 ? ipx::Q1 : ipx::Q0)) return 1;
     try {
         std::lock_guard<std::mutex> lk(c->mu);
+        if (c->loop_active) return 5;
+        c->order_before(c->stream);
         std::vector<double> bd(nq * 2 * R);
         for (size_t i = 0; i < bd.size(); i++) bd[i] = (double)b[i];
         hip_check(hipMemcpy(c->d_bq, bd.data(), bd.size() * 8, hipMemcpyHostToDevice), "dbg db pass upload");
         c->stage_dbpass(nq, c->stream, 0, 0, path);
+        c->order_after(c->stream);
         hip_check(hipStreamSynchronize(c->stream), "dbg db pass");
         hip_check(hipMemcpy(interm, c->d_interm, nq * 2 * C * 4, hipMemcpyDeviceToHost), "dbg db pass download");
         return 0;
@@ -241,6 +291,11 @@ int ipir_dbg_time_dbpass(ipir_server* srv, size_t nq, int iters, double* ms_per_
     int rc = 0;
     try {
         std::lock_guard<std::mutex> lk(c->mu);
+        if (c->loop_active) {
+            rc = 5;
+            throw 0;
+        }
+        c->order_before(c->stream);
         hip_check(hipEventCreate(&e0), "event");
         hip_check(hipEventCreate(&e1), "event");
         // the query digits once; then the pass's own launches (the kernel and,
@@ -255,8 +310,9 @@ int ipir_dbg_time_dbpass(ipir_server* srv, size_t nq, int iters, double* ms_per_
         float ms = 0;
         hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
         *ms_per_pass = ms / iters;
+        c->order_after(c->stream);
     } catch (...) {
-        rc = 4;
+        if (!rc) rc = 4;
     }
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
@@ -304,6 +360,7 @@ int ipir_dbg_answer_taps(ipir_server* srv, const uint64_t* query, uint32_t* inte
     GpuServerCtx* c = srv->ctx;
     try {
         std::lock_guard<std::mutex> lk(c->mu);
+        if (c->loop_active) return 5;
         std::memcpy(c->h_q, query, c->qlen * 8);
         std::vector<uint64_t> r(c->resp_u64s);
         c->answer_host(1, r.data());
@@ -426,6 +483,7 @@ int ipir_dbg_set_group_chunk(ipir_server* srv, size_t groups) {
     if (!srv || groups == 0 || groups > srv->ctx->G || srv->ctx->G % groups) return 1;
     GpuServerCtx* c = srv->ctx;
     std::lock_guard<std::mutex> lk(c->mu);
+    if (c->loop_active) return 5;
     // the partial-sum buffer was sized for the setup-time chunk width
     const size_t ncols = groups * N;
     for (size_t nq = 1; nq <= ipx::kMfmaMaxQueries; nq++) {
@@ -442,6 +500,7 @@ int ipir_dbg_set_tail_split(ipir_server* srv, int on) {
     if (!srv || on < 0 || on > 3) return 1;
     GpuServerCtx* c = srv->ctx;
     std::lock_guard<std::mutex> lk(c->mu);
+    if (c->loop_active) return 5;
     c->tail_split = on & 1;
     c->head_split = (on >> 1) & 1;
     c->set_chunks();
@@ -452,6 +511,7 @@ int ipir_dbg_set_tail_split(ipir_server* srv, int on) {
 int ipir_dbg_set_collapse_variant(ipir_server* srv, int variant) {
     if (!srv || variant < 0 || variant >= ipx::kCollapseVariants) return 1;
     std::lock_guard<std::mutex> lk(srv->ctx->mu);
+    if (srv->ctx->loop_active) return 5;
     srv->ctx->collapse_variant = variant;
     srv->ctx->drop_graphs();  // their kernels carry the old plans
     return 0;
@@ -463,6 +523,7 @@ int ipir_dbg_set_chunk_slices(ipir_server* srv, size_t nq, uint32_t slices) {
     if (!srv || nq < 1 || nq > ipx::kMfmaMaxQueries || slices > 64 || (slices & (slices - 1))) return 1;
     GpuServerCtx* c = srv->ctx;
     std::lock_guard<std::mutex> lk(c->mu);
+    if (c->loop_active) return 5;
     if (nq == 1) {
         c->chunk_slices1 = slices;
         c->drop_graphs();  // their kernels carry the old plans
@@ -483,6 +544,7 @@ int ipir_dbg_set_mfma_depth(ipir_server* srv, uint32_t depth, int form) {
     for (uint32_t d : ipx::kMfmaDepths)
         if (d == depth) {
             std::lock_guard<std::mutex> lk(srv->ctx->mu);
+            if (srv->ctx->loop_active) return 5;
             srv->ctx->mfma_depth = depth;
             srv->ctx->mfma_form = form;
             srv->ctx->tuned16 = false;  // every 9..32-query batch takes the set form
@@ -496,6 +558,7 @@ int ipir_dbg_set_pipeline(ipir_server* srv, int single, int batched) {
     auto ok = [](int v) { return v == 0 || v == 2 || v == 3; };
     if (!srv || !ok(single) || !ok(batched)) return 1;
     std::lock_guard<std::mutex> lk(srv->ctx->mu);
+    if (srv->ctx->loop_active) return 5;
     srv->ctx->pipe_single = single;
     srv->ctx->pipe_batched = batched;
     srv->ctx->drop_graphs();  // their kernels carry the old plans
@@ -505,6 +568,7 @@ int ipir_dbg_set_pipeline(ipir_server* srv, int single, int batched) {
 int ipir_dbg_set_pack(ipir_server* srv, int mode) {
     if (!srv || mode < -1 || mode > 1) return 1;
     std::lock_guard<std::mutex> lk(srv->ctx->mu);
+    if (srv->ctx->loop_active) return 5;
     srv->ctx->pack_wg = mode;
     srv->ctx->drop_graphs();  // their kernels carry the old plans
     return 0;
@@ -513,6 +577,7 @@ int ipir_dbg_set_pack(ipir_server* srv, int mode) {
 int ipir_dbg_set_graph(ipir_server* srv, int on) {
     if (!srv || (on != 0 && on != 1)) return 1;
     std::lock_guard<std::mutex> lk(srv->ctx->mu);
+    if (srv->ctx->loop_active) return 5;
     srv->ctx->drop_graphs();
     srv->ctx->graph_mode = on;
     return 0;
@@ -521,8 +586,18 @@ int ipir_dbg_set_graph(ipir_server* srv, int on) {
 int ipir_dbg_set_horner(ipir_server* srv, int mode) {
     if (!srv || mode < -1 || mode > 1) return 1;
     std::lock_guard<std::mutex> lk(srv->ctx->mu);
+    if (srv->ctx->loop_active) return 5;
     srv->ctx->horner_mode = mode;
     srv->ctx->drop_graphs();  // their kernels carry the old plans
+    return 0;
+}
+
+int ipir_dbg_set_horner_spin_limit(ipir_server* srv, uint32_t limit) {
+    if (!srv) return 1;
+    std::lock_guard<std::mutex> lk(srv->ctx->mu);
+    if (srv->ctx->loop_active) return 5;
+    srv->ctx->horner_spin_limit = limit;
+    srv->ctx->drop_graphs();  // their kernels carry the old bound
     return 0;
 }
 
@@ -571,11 +646,14 @@ int ipir_dbg_profile_batch(ipir_server* srv, const uint64_t* d_queries, size_t c
                            double* stage_ms) {
     if (!srv || !d_queries || !d_out || !stage_ms || count == 0 || count > srv->ctx->cfg.max_batch) return 1;
     GpuServerCtx* c = srv->ctx;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (c->loop_active) return 5;
     hipEvent_t ev[6];
     for (auto& e : ev) (void)hipEventCreate(&e);
     hipStream_t s = c->stream;
     int rc = 0;
     try {
+        c->order_before(s);
         (void)hipEventRecord(ev[0], s);
         c->stage_prep(d_queries, count, s, 0);
         (void)hipEventRecord(ev[1], s);
@@ -585,9 +663,11 @@ int ipir_dbg_profile_batch(ipir_server* srv, const uint64_t* d_queries, size_t c
         (void)hipEventRecord(ev[3], s);
         c->stage_finalize(count, s, 0);
         (void)hipEventRecord(ev[4], s);
-        c->stage_horner(count, d_out, s, 0);
+        c->stage_horner(count, d_out, s, 0, true);
         (void)hipEventRecord(ev[5], s);
+        c->order_after(s);
         hip_check(hipEventSynchronize(ev[5]), "sync");
+        c->check_status(s, 0);
         for (int i = 0; i < 5; i++) {
             float ms = 0;
             (void)hipEventElapsedTime(&ms, ev[i], ev[i + 1]);
@@ -615,5 +695,23 @@ int ipir_dev_copy(void* dst, const void* src, size_t bytes, int kind) {
 }
 
 int ipir_dev_sync(void) { return hipDeviceSynchronize() == hipSuccess ? 0 : 4; }
+
+void* ipir_dev_stream_create(void) {
+    hipStream_t s = nullptr;
+    return hipStreamCreate(&s) == hipSuccess ? static_cast<void*>(s) : nullptr;
+}
+
+int ipir_dev_stream_sync(void* stream) {
+    return hipStreamSynchronize(static_cast<hipStream_t>(stream)) == hipSuccess ? 0 : 4;
+}
+
+int ipir_dev_stream_destroy(void* stream) {
+    return stream && hipStreamDestroy(static_cast<hipStream_t>(stream)) == hipSuccess ? 0 : 1;
+}
+
+int ipir_dev_mem_info(size_t* free_bytes, size_t* total_bytes) {
+    if (!free_bytes || !total_bytes) return 1;
+    return hipMemGetInfo(free_bytes, total_bytes) == hipSuccess ? 0 : 4;
+}
 
 }  // extern "C"

@@ -150,7 +150,8 @@ hipError_t launch_horner(const uint32_t* pa_base, size_t pa_stride, size_t G, si
 // coefficient-form values once per step. xbuf holds
 // horner_pair_xbuf_words(nq*T) words and flags 2*nq*T words (reset by the
 // launch); a partner wait longer than spin_limit sleeps sets *status to 1
-// and the chain's output is then undefined.
+// and the chain's output is then undefined. spin_limit 0 sets it at the
+// first exchange.
 size_t horner_pair_xbuf_words(size_t nchains);
 hipError_t launch_horner_pair(const uint32_t* pa_base, size_t pa_stride, size_t G, size_t Da, size_t T,
                               const uint32_t* packed_b, size_t packed_stride, const uint32_t* rgsw,

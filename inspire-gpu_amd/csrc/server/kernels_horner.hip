@@ -16,7 +16,8 @@
 // is a release store of the step number and a bounded wait for the
 // partner's (acquire at agent scope, so correctness does not depend on the
 // placement); a wait that exceeds the bound sets *status and ends both
-// workgroups, so every wave always reaches the end.
+// workgroups, so every wave always reaches the end (a bound of zero takes
+// that path at the first exchange).
 #include "kernels.h"
 
 namespace ipx {
@@ -111,8 +112,10 @@ __global__ __launch_bounds__(1024) void k_horner_pair(const uint32_t* __restrict
         __syncthreads();
         if (tid == 0) {
             __hip_atomic_store(my_flag, k, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            // a zero bound reports at the first exchange whatever the partner
+            // did (a deterministic way into this failure path)
             uint32_t spins = 0;
-            while (__hip_atomic_load(other_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < k) {
+            while (spin_limit == 0 || __hip_atomic_load(other_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < k) {
                 if (++spins > spin_limit) {
                     __hip_atomic_store(status, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     abort_flag = 1;

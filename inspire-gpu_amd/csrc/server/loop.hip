@@ -2,8 +2,8 @@
 // the reference, whose server is a blocking, externally serialised call
 // (src/gpu_protocol.h:67, src/capi.h:12-14).
 //
-// The context's max_batch slots are split into `max_inflight` partitions,
-// each with its own HIP stream, pinned staging and completion thread. A
+// The context's max_batch slots are split into partitions, each with its
+// own HIP stream, scratch lane, pinned staging and completion thread. A
 // dispatcher thread admits waiting requests into the first free partition
 // (up to its slot count) as soon as one frees up, so a new batch starts
 // while earlier batches are still on the device and their kernels overlap
@@ -11,6 +11,10 @@
 // HBM-bound database pass). Each request completes independently and
 // records its submit-to-completion latency. Results are bit-identical to
 // answering the request alone: the pipeline is exact and per-slot.
+//
+// While a loop runs, the context is marked loop-owned (loop_active, set and
+// cleared under the context's mutex); the other entry points that use the
+// slots return "busy" instead of blocking.
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -20,6 +24,7 @@
 #include <vector>
 
 #include "capi.h"
+#include "inspire_debug.h"
 #include "server_ctx.h"
 
 using namespace inspire;
@@ -39,7 +44,7 @@ struct Req {
 
 struct Partition {
     size_t slot0 = 0, nslots = 0;
-    int lane = 0;  // scratch set of the context this partition uses
+    int lane = 0;  // scratch lane (and status word) of the context this partition uses
     hipStream_t stream = nullptr;
     std::vector<Req*> batch;  // admitted requests; partition busy when non-empty
     bool launched = false;    // batch's device work fully enqueued
@@ -59,8 +64,15 @@ struct ipir_loop {
     std::vector<Partition*> parts;
     std::thread dispatcher;
     bool stop = false;
+    bool hold = false;  // completers keep finished partitions busy (inspection)
     uint64_t next_ticket = 1;
-    std::unique_lock<std::mutex> owner;  // holds ctx->mu while the loop runs
+    ipir_loop_stats st{};
+
+    size_t busy_parts() const {
+        size_t n = 0;
+        for (auto* p : parts) n += !p->batch.empty();
+        return n;
+    }
 
     void dispatch_loop() {
         std::unique_lock<std::mutex> lk(m);
@@ -75,12 +87,21 @@ struct ipir_loop {
             });
             if (stop) return;
             const size_t n = pending.size() < free_p->nslots ? pending.size() : free_p->nslots;
+            if (busy_parts() > 0) st.admitted_while_busy++;
             for (size_t i = 0; i < n; i++) {
                 free_p->batch.push_back(pending.front());
                 pending.pop_front();
             }
+            const size_t busy = busy_parts();
+            if (busy > st.max_busy_partitions) st.max_busy_partitions = busy;
+            if (n > st.max_batch_size) st.max_batch_size = n;
+            st.batches++;
             for (size_t i = 0; i < n; i++)
                 std::memcpy(ctx->h_q + (free_p->slot0 + i) * ctx->qlen, free_p->batch[i]->q.data(), ctx->qlen * 8);
+            // partitions whose device work was launched before this batch
+            std::vector<hipStream_t> others;
+            for (auto* p : parts)
+                if (p != free_p && p->launched) others.push_back(p->stream);
             lk.unlock();
             bool ok = true;
             try {
@@ -88,13 +109,16 @@ struct ipir_loop {
                 hip_check(hipMemcpyAsync(ctx->d_q + s0 * ctx->qlen, ctx->h_q + s0 * ctx->qlen, n * ctx->qlen * 8,
                                          hipMemcpyHostToDevice, free_p->stream), "loop h2d");
                 ctx->enqueue(ctx->d_q + s0 * ctx->qlen, n, ctx->d_resp + s0 * ctx->resp_u64s, free_p->stream, s0,
-                             free_p->lane);
+                             free_p->lane, true);
                 hip_check(hipMemcpyAsync(ctx->h_resp + s0 * ctx->resp_u64s, ctx->d_resp + s0 * ctx->resp_u64s,
                                          n * ctx->resp_u64s * 8, hipMemcpyDeviceToHost, free_p->stream), "loop d2h");
             } catch (...) {
                 ok = false;
             }
+            bool overlap = false;
+            for (hipStream_t o : others) overlap |= hipStreamQuery(o) == hipErrorNotReady;
             lk.lock();
+            if (overlap) st.device_overlaps++;
             if (!ok)
                 for (Req* r : free_p->batch) r->failed = true;
             free_p->launched = true;
@@ -109,15 +133,16 @@ struct ipir_loop {
             if (!p->launched) return;  // stopping with nothing in flight
             lk.unlock();
             bool ok = hipStreamSynchronize(p->stream) == hipSuccess;
-            if (ok && ctx->horner_mode != 0) {
+            if (ok) {
                 try {
-                    ctx->check_status(p->stream);
+                    ctx->check_status(p->stream, p->lane);
                 } catch (...) {
                     ok = false;
                 }
             }
-            const auto now = Clock::now();
             lk.lock();
+            p->cv.wait(lk, [&] { return !hold; });
+            const auto now = Clock::now();
             for (size_t i = 0; i < p->batch.size(); i++) {
                 Req* r = p->batch[i];
                 if (ok && !r->failed)
@@ -126,6 +151,8 @@ struct ipir_loop {
                     r->failed = true;
                 r->lat_us = std::chrono::duration<double, std::micro>(now - r->t0).count();
                 r->done = true;
+                st.completed++;
+                st.failed += r->failed;
             }
             p->batch.clear();
             p->launched = false;
@@ -138,15 +165,29 @@ struct ipir_loop {
 extern "C" {
 
 ipir_loop* ipir_loop_start(ipir_server* srv, size_t max_inflight) {
-    if (!srv || max_inflight == 0 || max_inflight > 4) return nullptr;
+    if (!srv || max_inflight > (size_t)GpuServerCtx::kLanes) return nullptr;
     GpuServerCtx* ctx = srv->ctx;
-    if (max_inflight > ctx->cfg.max_batch) max_inflight = ctx->cfg.max_batch;
+    const size_t B = ctx->cfg.max_batch;
+    size_t per = 0, nparts = 0;
+    if (max_inflight == 0) {
+        // full batched passes in every partition: 32-query partitions (the
+        // whole batch if smaller), as many as the slots and lanes allow
+        per = B < ipx::kMfmaMaxQueries ? B : ipx::kMfmaMaxQueries;
+        nparts = B / per;
+        if (nparts > (size_t)GpuServerCtx::kLanes) nparts = GpuServerCtx::kLanes;
+    } else {
+        nparts = max_inflight > B ? B : max_inflight;
+        per = B / nparts;
+    }
+    {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        if (ctx->loop_active) return nullptr;  // one loop per server
+        ctx->loop_active = true;
+    }
     auto* L = new ipir_loop;
     L->srv = srv;
     L->ctx = ctx;
-    L->owner = std::unique_lock<std::mutex>(ctx->mu);
-    const size_t per = ctx->cfg.max_batch / max_inflight;
-    for (size_t i = 0; i < max_inflight; i++) {
+    for (size_t i = 0; i < nparts; i++) {
         auto* p = new Partition;
         p->slot0 = i * per;
         p->lane = (int)i;
@@ -159,7 +200,21 @@ ipir_loop* ipir_loop_start(ipir_server* srv, size_t max_inflight) {
     }
     if (L->parts.empty()) {
         delete L;
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        ctx->loop_active = false;
         return nullptr;
+    }
+    L->st.partitions = L->parts.size();
+    L->st.slots_per_partition = per;
+    {
+        // work the context's other entry points left on their streams comes first
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        for (auto* p : L->parts) {
+            try {
+                ctx->order_before(p->stream);
+            } catch (...) {
+            }
+        }
     }
     for (auto* p : L->parts) p->completer = std::thread([L, p] { L->complete_loop(p); });
     L->dispatcher = std::thread([L] { L->dispatch_loop(); });
@@ -187,6 +242,7 @@ uint64_t ipir_loop_submit(ipir_loop* L, const ipir_query_view* v, uint64_t* out_
     r->t0 = Clock::now();
     L->reqs[r->ticket] = r;
     L->pending.push_back(r);
+    L->st.submitted++;
     L->cv_work.notify_one();
     return r->ticket;
 }
@@ -205,11 +261,31 @@ int ipir_loop_wait(ipir_loop* L, uint64_t ticket, double* latency_us) {
     return rc;
 }
 
+int ipir_loop_get_stats(ipir_loop* L, ipir_loop_stats* out) {
+    if (!L || !out) return 1;
+    std::lock_guard<std::mutex> lk(L->m);
+    *out = L->st;
+    out->pending = L->pending.size();
+    out->busy_partitions = L->busy_parts();
+    return 0;
+}
+
+int ipir_dbg_loop_hold(ipir_loop* L, int on) {
+    if (!L || (on != 0 && on != 1)) return 1;
+    std::lock_guard<std::mutex> lk(L->m);
+    L->hold = on != 0;
+    if (!L->hold)
+        for (auto* p : L->parts) p->cv.notify_all();
+    return 0;
+}
+
 void ipir_loop_stop(ipir_loop* L) {
     if (!L) return;
     {
         // let admitted work finish, then stop
         std::unique_lock<std::mutex> lk(L->m);
+        L->hold = false;
+        for (auto* p : L->parts) p->cv.notify_all();
         L->cv_done.wait(lk, [&] {
             if (!L->pending.empty()) return false;
             for (auto* p : L->parts)
@@ -221,13 +297,19 @@ void ipir_loop_stop(ipir_loop* L) {
     L->cv_work.notify_all();
     for (auto* p : L->parts) p->cv.notify_all();
     L->dispatcher.join();
+    GpuServerCtx* ctx = L->ctx;
     for (auto* p : L->parts) {
         p->completer.join();
+        (void)hipStreamSynchronize(p->stream);  // every completer synchronised already
         (void)hipStreamDestroy(p->stream);
         delete p;
     }
     for (auto& kv : L->reqs) delete kv.second;
-    L->owner.unlock();
+    {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        ctx->order_stream_set = false;  // the loop's work has finished and its streams are gone
+        ctx->loop_active = false;
+    }
     delete L;
 }
 

@@ -190,50 +190,60 @@ GpuServerCtx* gpu_setup_server(const PublicParams& pp, const PreprocessData& pre
     if (!precomp.d_db_col || precomp.d_precomp_a.size() != G || precomp.d_precomp_D_plus.size() != G ||
         precomp.d_precomp_D_minus.size() != G || precomp.d_precomp_D_final.size() != G)
         throw std::invalid_argument("gpu_setup_server: precomp does not match the geometry");
+    const bool per_group = !precomp.d_precomp_arena;
+    if (per_group)
+        for (size_t g = 0; g < G; g++)
+            if (!precomp.d_precomp_a[g] || !precomp.d_precomp_D_plus[g] || !precomp.d_precomp_D_minus[g] ||
+                !precomp.d_precomp_D_final[g])
+                throw std::invalid_argument("gpu_setup_server: missing per-group precomp buffer");
     auto* ctx = new GpuServerCtx;
     ctx->pp = pp;
     ctx->cfg = cfg;
     ctx->G = G;
     ctx->Da = pp.D < G ? pp.D : G;
     ctx->T = pp.num_cts;
+    // The caller's buffers stay the caller's until everything below has
+    // succeeded: on failure the context frees only what it allocated.
+    ctx->d_db = precomp.d_db_col;
     try {
         HIPX(hipGetDevice(&ctx->device));
         HIPX(hipStreamCreate(&ctx->stream));
         ctx->d_tw = upload_tables(ctx->nc, ctx->cc);
-        ctx->d_db = precomp.d_db_col;
-        if (precomp.d_precomp_arena) {
+        if (!per_group) {
             ctx->d_precomp = static_cast<uint32_t*>(precomp.d_precomp_arena);
         } else {
             // per-group buffers in the reference layout: gather a / D_final
             // into one arena and convert D+/D- into the dot-product form
             ctx->d_precomp = dalloc<uint32_t>(G * PC_GROUP);
-            for (size_t g = 0; g < G; g++)
-                if (!precomp.d_precomp_a[g] || !precomp.d_precomp_D_plus[g] || !precomp.d_precomp_D_minus[g] ||
-                    !precomp.d_precomp_D_final[g])
-                    throw std::invalid_argument("gpu_setup_server: missing per-group precomp buffer");
             for (size_t g = 0; g < G; g++) {
                 uint32_t* b = ctx->d_precomp + g * PC_GROUP;
-                HIPX(hipMemcpy(b + PC_A, precomp.d_precomp_a[g], 2 * N * 4, hipMemcpyDeviceToDevice));
-                HIPX(hipMemcpy(b + PC_DF, precomp.d_precomp_D_final[g], 2 * D_EFF * N * 4, hipMemcpyDeviceToDevice));
+                HIPX(hipMemcpyAsync(b + PC_A, precomp.d_precomp_a[g], 2 * N * 4, hipMemcpyDeviceToDevice,
+                                    ctx->stream));
+                HIPX(hipMemcpyAsync(b + PC_DF, precomp.d_precomp_D_final[g], 2 * D_EFF * N * 4,
+                                    hipMemcpyDeviceToDevice, ctx->stream));
                 HIPX(launch_dd_relayout(precomp.d_precomp_D_plus[g], precomp.d_precomp_D_minus[g], 0, 1, b,
                                         ctx->stream));
             }
             HIPX(hipStreamSynchronize(ctx->stream));
-            for (size_t g = 0; g < G; g++) {
-                dfree(precomp.d_precomp_a[g]); dfree(precomp.d_precomp_D_plus[g]);
-                dfree(precomp.d_precomp_D_minus[g]); dfree(precomp.d_precomp_D_final[g]);
-            }
         }
-        // ownership transfer (src/gpu_protocol.cu:856-885)
-        auto& pd = const_cast<PreprocessData&>(precomp);
-        pd.d_db_col = nullptr;
-        pd.d_precomp_arena = nullptr;
-        pd.d_precomp_a.clear(); pd.d_precomp_D_plus.clear(); pd.d_precomp_D_minus.clear(); pd.d_precomp_D_final.clear();
         ctx->alloc_slots();
     } catch (...) {
+        ctx->d_db = nullptr;                        // still the caller's
+        if (!per_group) ctx->d_precomp = nullptr;  // likewise
         gpu_free_server(ctx);
         throw;
     }
+    // ownership transfer (src/gpu_protocol.cu:856-885); converted per-group
+    // buffers are released now that the arena holds them
+    if (per_group)
+        for (size_t g = 0; g < G; g++) {
+            dfree(precomp.d_precomp_a[g]); dfree(precomp.d_precomp_D_plus[g]);
+            dfree(precomp.d_precomp_D_minus[g]); dfree(precomp.d_precomp_D_final[g]);
+        }
+    auto& pd = const_cast<PreprocessData&>(precomp);
+    pd.d_db_col = nullptr;
+    pd.d_precomp_arena = nullptr;
+    pd.d_precomp_a.clear(); pd.d_precomp_D_plus.clear(); pd.d_precomp_D_minus.clear(); pd.d_precomp_D_final.clear();
     return ctx;
 }
 
@@ -268,25 +278,14 @@ DbPlan narrow_plan(const DbPlan& full, size_t target, size_t R, size_t C, size_t
 }
 }  // namespace
 
-void GpuServerCtx::alloc_slots() {
-    const size_t R = pp.db_rows, C = pp.db_cols, B = cfg.max_batch;
-    plan1 = dbplan_f64(R, C, 4, 2048);
-    plan1_target = 2048;
+size_t default_group_chunk(size_t G) {
     // group chunks of the pipelined answer: about 8, of at least 8 groups
-    gchunk = G / 8 >= 8 ? G / 8 : (G < 8 ? G : 8);
-    if (const char* gc = std::getenv("INSPIRE_GROUP_CHUNK")) {
-        const long v = std::atol(gc);
-        if (v > 0 && (size_t)v <= G && G % (size_t)v == 0) gchunk = (size_t)v;
-    }
-    // INSPIRE_TAIL_SPLIT=<0..3>: bit 0 splits the lowest chunk, bit 1 the highest
-    if (const char* ts = std::getenv("INSPIRE_TAIL_SPLIT")) {
-        const int v = std::atoi(ts);
-        if (v >= 0 && v <= 3) {
-            tail_split = v & 1;
-            head_split = (v >> 1) & 1;
-        }
-    }
-    set_chunks();
+    return G / 8 >= 8 ? G / 8 : (G < 8 ? G : 8);
+}
+
+ServerFootprint server_footprint(const PublicParams& pp, size_t gchunk) {
+    const size_t R = pp.db_rows, C = pp.db_cols, G = pp.n_packed, T = pp.num_cts;
+    ServerFootprint f;
     // row-slice partials of one query: [slices][2][ncols] words for the
     // widest plan any candidate (full width or narrowed to a chunk) needs
     size_t part_words = 2 * C;
@@ -306,9 +305,50 @@ void GpuServerCtx::alloc_slots() {
                 need(full, C);
                 need(narrow_plan(full, t, R, C, ccols, nq), ccols);
             }
+    f.part_words = part_words;
+    SlotLayout lay{R, C, G};
+    const size_t qlen = lay.query_u64s(), resp = T * 4 * N;
+    f.per_slot = qlen * 8 + resp * 8 + 2 * R * 8 +
+                 (lay.keys_words() + lay.rgsw_words() + part_words + 2 * C + 2 * G * 2 * N) * 4 +
+                 (horner_pair_xbuf_words(T) + 2 * T + T * 4 * N) * 4 +
+                 query_packed_bytes(R) + T * 4 * N * 4 + T * GpuServerCtx::kCompressedCtBytes;
+    f.fixed = R * C * 2 + G * PC_GROUP * 4 + 8 * N * 4 +
+              GpuServerCtx::kLanes * (R * 256 + GpuServerCtx::kKdigBytes + 2 * 64 * 4 + 4);
+    // gpu_preprocess: the DB with its upload slab, then the DB, the arena and
+    // one chunk's ring-embed table, chain states and D+/D- in reference order
+    size_t slab = ((size_t)1 << 27) / C / 16 * 16;
+    if (slab < 16) slab = 16;
+    if (slab > R) slab = R;
+    const size_t chunk = G < 128 ? G : 128;
+    const size_t trans = (chunk * (2 * (size_t)N * N + 4 * (size_t)N + 2 * RD_SZ) + (R / N) * 2 * N +
+                          4 * (size_t)D_EFF * N + 8 * (size_t)N) * 4;
+    const size_t stage = slab * C * 2, after = G * PC_GROUP * 4 + trans;
+    f.preprocess_peak = R * C * 2 + (stage > after ? stage : after);
+    return f;
+}
+
+void GpuServerCtx::alloc_slots() {
+    const size_t R = pp.db_rows, C = pp.db_cols, B = cfg.max_batch;
+    plan1 = dbplan_f64(R, C, 4, 2048);
+    plan1_target = 2048;
+    gchunk = default_group_chunk(G);
+    if (const char* gc = std::getenv("INSPIRE_GROUP_CHUNK")) {
+        const long v = std::atol(gc);
+        if (v > 0 && (size_t)v <= G && G % (size_t)v == 0) gchunk = (size_t)v;
+    }
+    // INSPIRE_TAIL_SPLIT=<0..3>: bit 0 splits the lowest chunk, bit 1 the highest
+    if (const char* ts = std::getenv("INSPIRE_TAIL_SPLIT")) {
+        const int v = std::atoi(ts);
+        if (v >= 0 && v <= 3) {
+            tail_split = v & 1;
+            head_split = (v >> 1) & 1;
+        }
+    }
+    set_chunks();
+    const ServerFootprint fp = server_footprint(pp, gchunk);
     lay.R = R; lay.C = C; lay.G = G;
     qlen = lay.query_u64s();
-    part_stride = part_words;
+    part_stride = fp.part_words;
     part2_stride = G * 2 * N;
     keys_stride = lay.keys_words();
     rgsw_stride = lay.rgsw_words();
@@ -340,8 +380,10 @@ void GpuServerCtx::alloc_slots() {
     }
     d_hxbuf = dalloc<uint32_t>(horner_pair_xbuf_words(B * T));
     d_hflags = dalloc<uint32_t>(2 * B * T);
-    d_status = dalloc<uint32_t>(1);
-    HIPX(hipMemset(d_status, 0, 4));
+    d_status = dalloc<uint32_t>(kLanes);
+    HIPX(hipMemset(d_status, 0, kLanes * 4));
+    horner_spin_limit = kHornerSpinLimit;
+    HIPX(hipEventCreateWithFlags(&order_ev, hipEventDisableTiming));
     // INSPIRE_PACK_WG=0/1/-1: the pipelined chunks' pack kernel, a grid over
     // (group, limb), one workgroup per query, or one workgroup for one query
     if (const char* pw = std::getenv("INSPIRE_PACK_WG")) pack_wg = pw[0] == '1' ? 1 : pw[0] == '0' ? 0 : -1;
@@ -368,12 +410,7 @@ void GpuServerCtx::alloc_slots() {
     HIPX(hipMemset(d_bq, 0, B * 2 * R * sizeof(double)));
     HIPX(hipHostMalloc((void**)&h_q, B * qlen * 8, hipHostMallocDefault));
     HIPX(hipHostMalloc((void**)&h_resp, B * resp_u64s * 8, hipHostMallocDefault));
-    resident_bytes = R * C * 2 + G * PC_GROUP * 4 + 8 * N * 4 +
-                     B * (qlen * 8 + resp_u64s * 8 + 2 * R * 8 +
-                          (keys_stride + rgsw_stride + part_stride + 2 * C + part2_stride + packed_stride) * 4) +
-                     kLanes * (R * 256 + kKdigBytes + 128 * 4) +
-                     (horner_pair_xbuf_words(B * T) + 2 * B * T + B * T * 4 * N) * 4 +
-                     B * (qpack_bytes + T * 4 * N * 4 + T * kCompressedCtBytes);
+    resident_bytes = fp.fixed + B * fp.per_slot;
     autotune_dbpass();
 }
 
@@ -699,33 +736,35 @@ void GpuServerCtx::stage_finalize(size_t count, hipStream_t s, size_t slot0) {
                               d_packed_b + slot0 * packed_stride, packed_stride, count, nc, s));
 }
 
-bool GpuServerCtx::use_pair(size_t chains, bool beside_passes) const {
+bool GpuServerCtx::use_pair(size_t chains, bool beside_passes, bool status_read) const {
+    // never where nobody reads the status its bounded wait reports through;
     // auto: the pair kernel shortens a lone chain run by itself (its per-step
     // exchange costs more than it saves once many chains share the L2s, or
     // while database passes stream through them)
+    if (!status_read) return false;
     return horner_mode == 1 || (horner_mode < 0 && chains <= kPairMaxChains && !beside_passes);
 }
 
 void GpuServerCtx::horner_range(size_t count, uint64_t* d_out, size_t g_lo, size_t g_hi, hipStream_t s,
-                                size_t slot0, bool beside_passes) {
-    if (use_pair(count * T, beside_passes))
+                                size_t slot0, int lane, bool status_read, bool beside_passes) {
+    if (use_pair(count * T, beside_passes, status_read))
         HIPX(launch_horner_pair(d_precomp + PC_A, PC_GROUP, G, Da, T, d_packed_b + slot0 * packed_stride,
                                 packed_stride, d_rgsw + slot0 * rgsw_stride, rgsw_stride, d_out,
                                 d_hstate + slot0 * T * 4 * N, g_lo, g_hi, count,
-                                d_hxbuf + horner_pair_xbuf_words(slot0 * T), d_hflags + 2 * slot0 * T, d_status,
-                                kHornerSpinLimit, nc, cc, s));
+                                d_hxbuf + horner_pair_xbuf_words(slot0 * T), d_hflags + 2 * slot0 * T,
+                                d_status + lane, horner_spin_limit, nc, cc, s));
     else
         HIPX(launch_horner(d_precomp + PC_A, PC_GROUP, G, Da, T, d_packed_b + slot0 * packed_stride, packed_stride,
                            d_rgsw + slot0 * rgsw_stride, rgsw_stride, d_out, d_hstate + slot0 * T * 4 * N, g_lo, g_hi,
                            count, nc, cc, s));
 }
 
-void GpuServerCtx::stage_horner(size_t count, uint64_t* d_out, hipStream_t s, size_t slot0) {
-    horner_range(count, d_out, 0, G, s, slot0);
+void GpuServerCtx::stage_horner(size_t count, uint64_t* d_out, hipStream_t s, size_t slot0, bool status_read) {
+    horner_range(count, d_out, 0, G, s, slot0, 0, status_read);
 }
 
 void GpuServerCtx::enqueue(const uint64_t* d_in, size_t count, uint64_t* d_out, hipStream_t s, size_t slot0,
-                           int lane) {
+                           int lane, bool status_read) {
     if (lane < 0 || lane >= kLanes) throw std::invalid_argument("enqueue: scratch lane out of range");
     stage_prep(d_in, count, s, slot0);
     const size_t C = pp.db_cols;
@@ -753,7 +792,7 @@ void GpuServerCtx::enqueue(const uint64_t* d_in, size_t count, uint64_t* d_out, 
             HIPX(launch_collapse(d_precomp, G, d_kdig[lane], d_KS[lane], nq, p2, part2_stride, s, collapse_variant));
             HIPX(launch_pack_finalize(d_interm + q0 * 2 * C, C, p2, part2_stride, d_precomp, 0, G, keys, keys_stride,
                                       packed, packed_stride, nq, nc, s));
-            horner_range(nq, out, 0, G, s, q0);
+            horner_range(nq, out, 0, G, s, q0, lane, status_read);
             continue;
         }
         // shape 2 keeps the collapse on s (two HBM-bound passes side by side
@@ -785,7 +824,7 @@ void GpuServerCtx::enqueue(const uint64_t* d_in, size_t count, uint64_t* d_out, 
             else
                 HIPX(launch_pack_finalize(d_interm + q0 * 2 * C, C, p2, part2_stride, d_precomp, g_lo, ng, keys,
                                           keys_stride, packed, packed_stride, nq, nc, hs));
-            horner_range(nq, out, g_lo, g_hi, hs, q0, true);
+            horner_range(nq, out, g_lo, g_hi, hs, q0, lane, status_read, true);
         }
         if (cs != s) {  // the next batch's key digits overwrite the lane's scratch
             HIPX(hipEventRecord(prep_ev[lane], cs));
@@ -850,11 +889,11 @@ void GpuServerCtx::drop_graphs() {
 }
 
 void GpuServerCtx::enqueue_packed(const uint8_t* d_in, size_t count, uint8_t* d_out, hipStream_t s, size_t slot0,
-                                  int lane) {
+                                  int lane, bool status_read) {
     uint64_t* dq = d_q + slot0 * qlen;
     uint64_t* dr = d_resp + slot0 * resp_u64s;
     HIPX(launch_query_unpack(d_in, count, pp.db_rows, dq, qlen, s));
-    enqueue(dq, count, dr, s, slot0, lane);
+    enqueue(dq, count, dr, s, slot0, lane, status_read);
     HIPX(launch_compress(dr, count * T, d_cmp + slot0 * T * 4 * N, d_out, nc, cc, s));
 }
 
@@ -863,23 +902,41 @@ void GpuServerCtx::answer_packed_host(const uint8_t* in, size_t count, uint8_t* 
     uint8_t* hq = reinterpret_cast<uint8_t*>(h_q);     // packed queries fit the flat staging
     uint8_t* hr = reinterpret_cast<uint8_t*>(h_resp);  // so do compressed responses
     std::memcpy(hq, in, count * qpack_bytes);
+    order_before(s);
     HIPX(hipMemcpyAsync(d_qpack, hq, count * qpack_bytes, hipMemcpyHostToDevice, s));
-    run_step(d_qpack, d_rcmp, count, 0, 0, 1, s, [&] { enqueue_packed(d_qpack, count, d_rcmp, s); });
+    run_step(d_qpack, d_rcmp, count, 0, 0, 1, s, [&] { enqueue_packed(d_qpack, count, d_rcmp, s, 0, 0, true); });
     HIPX(hipMemcpyAsync(hr, d_rcmp, count * T * kCompressedCtBytes, hipMemcpyDeviceToHost, s));
+    order_after(s);
     HIPX(hipStreamSynchronize(s));
-    if (horner_mode != 0) check_status(s);
+    check_status(s, 0);
     std::memcpy(out, hr, count * T * kCompressedCtBytes);
 }
 
-void GpuServerCtx::check_status(hipStream_t s) {
+void GpuServerCtx::check_status(hipStream_t s, int lane) {
     uint32_t st = 0;
-    HIPX(hipMemcpyAsync(&st, d_status, 4, hipMemcpyDeviceToHost, s));
+    HIPX(hipMemcpyAsync(&st, d_status + lane, 4, hipMemcpyDeviceToHost, s));
     HIPX(hipStreamSynchronize(s));
     if (st) {
-        HIPX(hipMemsetAsync(d_status, 0, 4, s));
+        HIPX(hipMemsetAsync(d_status + lane, 0, 4, s));
         HIPX(hipStreamSynchronize(s));
         throw std::runtime_error("Horner pair exchange exceeded its wait bound; responses of this call are invalid");
     }
+}
+
+void GpuServerCtx::order_before(hipStream_t s) {
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    HIPX(hipStreamIsCapturing(s, &cap));
+    if (cap != hipStreamCaptureStatusNone) return;  // a caller's capture orders its own work
+    if (order_stream_set && order_stream != s) HIPX(hipStreamWaitEvent(s, order_ev, 0));
+}
+
+void GpuServerCtx::order_after(hipStream_t s) {
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    HIPX(hipStreamIsCapturing(s, &cap));
+    if (cap != hipStreamCaptureStatusNone) return;
+    HIPX(hipEventRecord(order_ev, s));
+    order_stream = s;
+    order_stream_set = true;
 }
 
 void GpuServerCtx::answer_host(size_t count, uint64_t* out, hipStream_t s, size_t slot0) {
@@ -888,11 +945,13 @@ void GpuServerCtx::answer_host(size_t count, uint64_t* out, hipStream_t s, size_
     uint64_t* hr = h_resp + slot0 * resp_u64s;
     uint64_t* dq = d_q + slot0 * qlen;
     uint64_t* dr = d_resp + slot0 * resp_u64s;
+    order_before(s);
     HIPX(hipMemcpyAsync(dq, hq, count * qlen * 8, hipMemcpyHostToDevice, s));
-    run_step(dq, dr, count, slot0, 0, 0, s, [&] { enqueue(dq, count, dr, s, slot0); });
+    run_step(dq, dr, count, slot0, 0, 0, s, [&] { enqueue(dq, count, dr, s, slot0, 0, true); });
     HIPX(hipMemcpyAsync(hr, dr, count * resp_u64s * 8, hipMemcpyDeviceToHost, s));
+    order_after(s);
     HIPX(hipStreamSynchronize(s));
-    if (horner_mode != 0) check_status(s);
+    check_status(s, 0);
     if (out) std::memcpy(out, hr, count * resp_u64s * 8);
 }
 
@@ -937,6 +996,7 @@ std::vector<std::vector<RlweCt>> gpu_answer_batch(GpuServerCtx* ctx, const Query
     if (count == 0) return {};
     if (count > ctx->cfg.max_batch) throw std::invalid_argument("gpu_answer_batch: count exceeds max_batch");
     std::lock_guard<std::mutex> lk(ctx->mu);
+    if (ctx->loop_active) throw std::runtime_error("gpu_answer_batch: a serving loop owns this server");
     for (size_t i = 0; i < count; i++) flatten_query(queries[i], ctx->pp.db_rows, ctx->h_q + i * ctx->qlen);
     std::vector<uint64_t> flat(count * ctx->resp_u64s);
     ctx->answer_host(count, flat.data());
@@ -960,6 +1020,10 @@ std::vector<RlweCt> gpu_answer(GpuServerCtx* ctx, const QueryMessage& qry) {
 void gpu_free_server(GpuServerCtx* ctx) {
     if (!ctx) return;
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->order_ev) {
+        (void)hipEventSynchronize(ctx->order_ev);  // work of a caller stream
+        (void)hipEventDestroy(ctx->order_ev);
+    }
     ctx->drop_graphs();
     dfree(ctx->d_tw); dfree(ctx->d_db); dfree(ctx->d_precomp);
     dfree(ctx->d_q); dfree(ctx->d_resp); dfree(ctx->d_bq); dfree(ctx->d_keys); dfree(ctx->d_rgsw);

@@ -104,7 +104,10 @@ struct GpuServerCtx {
     int pack_wg = 0;
     uint32_t* d_hxbuf = nullptr;    // pair exchange buffers [B*T chains]
     uint32_t* d_hflags = nullptr;   // pair step flags [B*T][2]
-    uint32_t* d_status = nullptr;   // sticky device fault word (pair wait bound exceeded)
+    // per scratch lane: nonzero once a pair chain of that lane's calls gave
+    // up its bounded wait (read and cleared by check_status)
+    uint32_t* d_status = nullptr;   // [kLanes]
+    uint32_t horner_spin_limit = 0; // the pair kernel's wait bound in sleeps (0: report at once)
     uint32_t* d_hstate = nullptr;   // Horner chain state between chunk launches [B*T][4][N]
     uint64_t* h_q = nullptr;        // pinned host staging
     uint64_t* h_resp = nullptr;
@@ -117,7 +120,19 @@ struct GpuServerCtx {
     uint8_t* d_rcmp = nullptr;      // compressed response staging [B*T][12288]
 
     size_t resident_bytes = 0;
-    std::mutex mu;  // serialises host-staged answers on this context
+    // mu serialises every entry point that touches the slots, scratch lanes
+    // or plans; loop_active (under mu) marks a serving loop that owns them
+    // until it stops, and those entry points then return "busy".
+    std::mutex mu;
+    bool loop_active = false;
+    // Calls on lane 0 and slots [0, count) from different streams are
+    // ordered: each waits for the previous one's work (order_ev, recorded on
+    // order_stream) before enqueueing its own.
+    hipEvent_t order_ev = nullptr;
+    hipStream_t order_stream = nullptr;
+    bool order_stream_set = false;
+    void order_before(hipStream_t s);
+    void order_after(hipStream_t s);
 
     // Captured answer steps: with graph_mode on, the first answer of a given
     // (input, output, count, slots, lane, form) is captured from its stream
@@ -155,33 +170,50 @@ struct GpuServerCtx {
     void dbpass_range(size_t q0, size_t nq, size_t g_lo, size_t g_hi, const ipx::DbPlan& p, hipStream_t s, int lane);
     // force: -1 the tuned plans, 0 the fp64 pass per query, 1 the MFMA pass
     void stage_dbpass(size_t count, hipStream_t s, size_t slot0, int lane = 0, int force = -1);
-    bool use_pair(size_t chains, bool beside_passes) const;
+    // The pair kernel only where the caller reads the lane's status after
+    // the call (status_read): its bounded wait reports through it.
+    bool use_pair(size_t chains, bool beside_passes, bool status_read) const;
     // Horner chains of `count` queries over groups [g_lo, g_hi).
     void horner_range(size_t count, uint64_t* d_out, size_t g_lo, size_t g_hi, hipStream_t s, size_t slot0,
-                      bool beside_passes = false);
+                      int lane, bool status_read, bool beside_passes = false);
     void stage_collapse(size_t count, hipStream_t s, size_t slot0, int lane = 0);
     void stage_finalize(size_t count, hipStream_t s, size_t slot0);
-    void stage_horner(size_t count, uint64_t* d_out, hipStream_t s, size_t slot0);
-    // Throw (and clear) if a kernel reported a fault through d_status.
-    void check_status(hipStream_t s);
+    void stage_horner(size_t count, uint64_t* d_out, hipStream_t s, size_t slot0, bool status_read);
+    // Throw (and clear) if a pair chain of a call on `lane` reported a fault
+    // through its status word; s is the stream the call ran on.
+    void check_status(hipStream_t s, int lane);
     // ~2 s of 64-cycle sleeps: far above any partner skew, bounded so a
     // chain whose partner never runs still ends.
     static constexpr uint32_t kHornerSpinLimit = 1u << 26;
     // Enqueue the answer of `count` flat queries at d_in (device) into d_out
     // (device) on stream s, using slots [slot0, slot0 + count) and scratch
     // set `lane` (work that may overlap on the device uses distinct lanes).
+    // status_read: the caller reads the lane's status word after the call
+    // (check_status), so the pair Horner kernel may be used.
     void enqueue(const uint64_t* d_in, size_t count, uint64_t* d_out, hipStream_t s, size_t slot0 = 0,
-                 int lane = 0);
+                 int lane = 0, bool status_read = false);
     // Host-staged round trip through the pinned staging of slots
     // [slot0, slot0+count): h_q -> device -> h_resp, then copy to out.
     void answer_host(size_t count, uint64_t* out, hipStream_t s = nullptr, size_t slot0 = 0);
     // enqueue() behind the device wire codecs: count packed queries at d_in
     // (device, qpack_bytes each) -> count*T compressed cts at d_out (device).
     void enqueue_packed(const uint8_t* d_in, size_t count, uint8_t* d_out, hipStream_t s, size_t slot0 = 0,
-                        int lane = 0);
+                        int lane = 0, bool status_read = false);
     // Host round trip of the packed forms through the pinned staging.
     void answer_packed_host(const uint8_t* in, size_t count, uint8_t* out);
 };
+
+// Device bytes of a server: the resident DB and precomp with the fixed
+// scratch (twiddles, lanes), what each of max_batch answer slots adds, and
+// the peak of gpu_preprocess (DB, precomp and its transient buffers).
+// resident_bytes of a server with max_batch B is fixed + B * per_slot.
+struct ServerFootprint {
+    size_t fixed = 0, per_slot = 0, preprocess_peak = 0;
+    size_t part_words = 0;  // row-slice partials per slot (u32 words)
+};
+ServerFootprint server_footprint(const PublicParams& pp, size_t gchunk);
+// Groups per chunk of the pipelined answer by default (about 8 chunks).
+size_t default_group_chunk(size_t G);
 
 }  // namespace inspire
 

@@ -34,6 +34,17 @@ class ipir_caps(C.Structure):
                 ("device_free_bytes", C.c_size_t), ("num_cts", C.c_size_t)]
 
 
+class ipir_footprint(C.Structure):
+    _fields_ = [("resident_fixed", C.c_size_t), ("per_slot", C.c_size_t), ("preprocess_peak", C.c_size_t)]
+
+
+class ipir_loop_stats(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in ("partitions", "slots_per_partition", "submitted", "completed", "failed",
+                                          "batches", "max_batch_size", "max_busy_partitions",
+                                          "admitted_while_busy", "device_overlaps", "pending",
+                                          "busy_partitions")]
+
+
 class ipir_query_view(C.Structure):
     _fields_ = [("lwe_b_limb0", C.c_void_p), ("lwe_b_limb1", C.c_void_p), ("ksk5_b", C.c_void_p),
                 ("kskneg1_b", C.c_void_p), ("rgsw", C.c_void_p)]
@@ -77,6 +88,16 @@ SERVER_FUNCS = [
     ("ipir_loop_submit", U64, [VP, P(ipir_query_view), VP]),
     ("ipir_loop_wait", C.c_int, [VP, U64, P(C.c_double)]),
     ("ipir_loop_stop", None, [VP]),
+    ("ipir_loop_get_stats", C.c_int, [VP, P(ipir_loop_stats)]),
+    ("ipir_server_footprint", C.c_int, [SZ, SZ, SZ, P(ipir_footprint)]),
+    ("ipir_fit_max_batch", SZ, [SZ, SZ, SZ, SZ, SZ]),
+    ("ipir_dbg_db_block", C.c_int, [VP, SZ, SZ, SZ, SZ, VP]),
+    ("ipir_dbg_set_horner_spin_limit", C.c_int, [VP, C.c_uint32]),
+    ("ipir_dbg_loop_hold", C.c_int, [VP, C.c_int]),
+    ("ipir_dev_stream_create", VP, []),
+    ("ipir_dev_stream_sync", C.c_int, [VP]),
+    ("ipir_dev_stream_destroy", C.c_int, [VP]),
+    ("ipir_dev_mem_info", C.c_int, [P(C.c_size_t), P(C.c_size_t)]),
     ("ipir_dbg_precomp_group_words", SZ, []),
     ("ipir_dbg_precomp_group", C.c_int, [VP, SZ, VP]),
     ("ipir_dbg_db_rows", C.c_int, [VP, SZ, SZ, VP]),
@@ -247,6 +268,23 @@ def geometry(n_entries, entry_bytes, db_rows=0, lib=None):
                 cpe=(entry_bytes * 8 + 14) // 15)
 
 
+def footprint(n_entries, entry_bytes, db_rows=0, lib=None):
+    """Device bytes of a server of this geometry: dict(resident_fixed,
+    per_slot, preprocess_peak), or ValueError if setup() rejects it."""
+    f = ipir_footprint()
+    if (lib or load("server")).ipir_server_footprint(n_entries, entry_bytes, db_rows, C.byref(f)):
+        raise ValueError("setup rejects this geometry")
+    return dict(resident_fixed=f.resident_fixed, per_slot=f.per_slot, preprocess_peak=f.preprocess_peak)
+
+
+def mem_info(lib=None):
+    """(free, total) device bytes now."""
+    fr, tot = C.c_size_t(), C.c_size_t()
+    if (lib or load("server")).ipir_dev_mem_info(C.byref(fr), C.byref(tot)):
+        raise RuntimeError("ipir_dev_mem_info failed")
+    return fr.value, tot.value
+
+
 def make_params(n_entries, entry_bytes, db_rows, seed, R, Cc, G, D, T):
     raw = ipir_params()
     raw.n_entries, raw.entry_bytes, raw.db_rows, raw.db_cols = n_entries, entry_bytes, R, Cc
@@ -299,6 +337,25 @@ class Server:
         if rc:
             raise RuntimeError(f"ipir_answer_batch rc={rc}")
         return out
+
+    def block(self, row0, nrows, col0, ncols):
+        """Encoded DB rows [row0, row0+nrows) x columns [col0, col0+ncols), row-major u16."""
+        out = np.zeros((nrows, ncols), np.uint16)
+        if self.lib.ipir_dbg_db_block(self.h, row0, nrows, col0, ncols, ptr(out)):
+            raise RuntimeError("ipir_dbg_db_block failed")
+        return out
+
+    def precomp_group(self, g):
+        """Group g's precomp in the reference layout: dict(a, Dp, Dm, Df) as u64
+        arrays shaped like the oracle's ([2][N], [2][N/2-1][2][N], ..., [2][2][N])."""
+        words = self.lib.ipir_dbg_precomp_group_words()
+        blk = np.zeros(words, np.uint32)
+        if self.lib.ipir_dbg_precomp_group(self.h, g, ptr(blk)):
+            raise RuntimeError("ipir_dbg_precomp_group failed")
+        dsz = 2 * (N // 2 - 1) * D_EFF * N
+        return dict(a=blk[:2 * N].astype(np.uint64), Dp=blk[2 * N:2 * N + dsz].astype(np.uint64),
+                    Dm=blk[2 * N + dsz:2 * N + 2 * dsz].astype(np.uint64),
+                    Df=blk[2 * N + 2 * dsz:].astype(np.uint64))
 
     def answer_packed(self, packed):
         """Packed wire queries (ipir_query_pack bytes, one row per query) ->

@@ -155,3 +155,49 @@ def test_hostile_params_and_bad_args_rejected(small):
         assert not L.ipir_query_build(C.byref(bad), 0, ia.ptr(out)), field
     assert L.ipir_query_pack(p.ref(), None, None) == 1
     assert L.ipir_extract(p.ref(), None, ia.ptr(out), ia.ptr(out)) == 1
+
+
+# ---------------------------------------------------------------- product setup()
+GEOS = [((2**20, 60, 32768), 0), ((2**23, 120, 32768), 0), ((2**27, 120, 32768), 0),
+        ((2**29, 120, 32768), 0), ((2**25, 3840, 32768), 0), ((256, 120, 2048), 0), ((256, 120, 0), 0),
+        # the literal benchmark shapes: ceil(8w/15) does not divide N = 2048
+        ((2**20, 32, 32768), 1), ((2**24, 64, 32768), 1), ((2**28, 64, 32768), 1), ((2**25, 4096, 32768), 1),
+        # n_packed not a power of two (D_a must divide 2N); db_rows not a multiple of N
+        ((3 * 2**20, 120, 32768), 1), ((2**20, 60, 1000), 1)]
+
+
+@pytest.mark.parametrize("args,rc", GEOS)
+def test_product_geometry_follows_setup_rules(args, rc):
+    """The product's setup() (through ipir_geometry) accepts and rejects the
+    shapes the reference's setup() does (src/params.cpp:46-60), with the
+    same geometry as the oracle and, where built, the reference library."""
+    raw = ia.ipir_params()
+    got = ia.load("client").ipir_geometry(*args, C.byref(raw))
+    assert got == rc, args
+    orc, geo = ol.setup(args[0], args[1], args[2] or 32768)
+    assert (orc != 0) == (rc != 0)  # the oracle names the rule that failed
+    if ol.have_ref():
+        out = np.zeros(5, np.uint64)
+        assert (ol.ref().ref_setup(args[0], args[1], args[2] or 32768, out) != 0) == (rc != 0)
+        if rc == 0:
+            assert tuple(int(v) for v in out) == (geo["R"], geo["C"], geo["G"], geo["D"], geo["T"])
+    if rc == 0:
+        assert (raw.db_rows, raw.db_cols, raw.n_packed, raw.interp_d, raw.num_cts) == \
+            (geo["R"], geo["C"], geo["G"], geo["D"], geo["T"])
+
+
+@pytest.mark.parametrize("n,w,rows", [(2**20, 60, 32768), (2**27, 120, 32768), (2**25, 3840, 32768)])
+def test_server_footprint_arithmetic(n, w, rows):
+    """ipir_server_footprint: the resident DB (2 bytes per value) and
+    precomp (8198 N words per group) are in the fixed part; a slot holds at
+    least its flat query and response staging; preprocessing peaks above
+    the resident DB and precomp. Rejected geometries return 1."""
+    f = ia.footprint(n, w, rows)
+    rc, geo = ol.setup(n, w, rows)
+    db_bytes = geo["R"] * geo["C"] * 2
+    pc_bytes = geo["G"] * 8198 * N * 4
+    assert db_bytes + pc_bytes < f["resident_fixed"] < db_bytes + pc_bytes + 2**30
+    assert f["per_slot"] >= (2 * geo["R"] + 6 * ol.BLK) * 8 + geo["T"] * 4 * N * 8
+    assert f["preprocess_peak"] > db_bytes + pc_bytes
+    with pytest.raises(ValueError):
+        ia.footprint(2**24, 64, 32768)

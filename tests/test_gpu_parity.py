@@ -633,28 +633,185 @@ def test_graph_replays_equal_eager(medium8, groups, shape):
     assert np.array_equal(srv.answer_batch(sets[3][1]), eager[3][1])
 
 
-def test_serving_loop_results_equal_alone(medium8):
+def _loop_stats(lib, loop):
+    st = ia.ipir_loop_stats()
+    assert lib.ipir_loop_get_stats(loop, C.byref(st)) == 0
+    return st
+
+
+def _wait_until(pred, what, seconds=30.0):
+    import time
+    t0 = time.time()
+    while not pred():
+        assert time.time() - t0 < seconds, f"timed out waiting for {what}"
+        time.sleep(0.002)
+
+
+def test_serving_loop_admits_while_batches_in_flight(medium8):
+    """Continuous batching, deterministically: with the loop's partitions
+    held busy after their device work, a first request occupies partition
+    0; a second wave submitted while it is in flight is admitted into
+    partition 1 and launched; a third waits for a free partition. Once
+    released, every result equals the same query answered alone and
+    extracts to its entry. While the loop runs, the server's other entry
+    points answer 5 (busy) and a second loop cannot start; after it stops
+    they work again."""
     geo, db, srv = medium8
     lib = srv.lib
-    idxs = list(range(0, 524288, 524288 // 13))[:13]
-    qs = [ol.make_query(geo, SEED, idx, 900 + k)[0] for k, idx in enumerate(idxs)]
+    n = 524288
+    idxs = [(104729 * k + 17) % n for k in range(9)]
+    built = [ol.make_query(geo, SEED, idx, 900 + k) for k, idx in enumerate(idxs)]
+    qs = [q for q, _ in built]
     alone = [srv.answer_batch([q])[0] for q in qs]
-    loop = lib.ipir_loop_start(srv.h, 2)
+    loop = lib.ipir_loop_start(srv.h, 2)  # two partitions of 16 slots
     assert loop
+    views = [srv.views([q]) for q in qs]
     outs = [np.zeros(geo["T"] * 4 * N, np.uint64) for _ in qs]
     tickets = []
-    for q, o in zip(qs, outs):
-        v = srv.views([q])
-        tickets.append(lib.ipir_loop_submit(loop, v, ia.ptr(o)))
-        tickets[-1] = (tickets[-1], v, q)  # keep the views alive
-    lat = C.c_double()
-    for (t, _, _) in tickets:
-        assert t > 0
-        assert lib.ipir_loop_wait(loop, t, C.byref(lat)) == 0
-        assert lat.value > 0
-    lib.ipir_loop_stop(loop)
+    try:
+        assert lib.ipir_loop_start(srv.h, 2) is None, "one loop per server"
+        out = np.zeros((1, geo["T"] * 4 * N), np.uint64)
+        assert lib.ipir_answer_batch(srv.h, views[0], 1, ia.ptr(out)) == 5
+        dq = ia.DeviceBuffer(qs[0].nbytes, lib)
+        assert lib.ipir_answer_batch_device(srv.h, C.c_void_p(dq.p), 1, C.c_void_p(dq.p), None) == 5
+        assert lib.ipir_dbg_set_pipeline(srv.h, 3, 2) == 5
+        assert lib.ipir_dbg_loop_hold(loop, 1) == 0
+        tickets.append(lib.ipir_loop_submit(loop, views[0], ia.ptr(outs[0])))
+        _wait_until(lambda: _loop_stats(lib, loop).batches == 1, "the first batch")
+        for k in range(1, 6):  # second wave while partition 0 is in flight
+            tickets.append(lib.ipir_loop_submit(loop, views[k], ia.ptr(outs[k])))
+        _wait_until(lambda: _loop_stats(lib, loop).busy_partitions == 2, "the second partition")
+        for k in range(6, 9):  # third wave: both partitions busy
+            tickets.append(lib.ipir_loop_submit(loop, views[k], ia.ptr(outs[k])))
+        _wait_until(lambda: _loop_stats(lib, loop).submitted == 9, "the third wave")
+        import time
+        time.sleep(0.05)
+        st = _loop_stats(lib, loop)
+        say(f"[loop] held: batches {st.batches}, pending {st.pending}, busy {st.busy_partitions}, "
+            f"completed {st.completed}")
+        assert st.max_busy_partitions == 2 and st.admitted_while_busy >= 1 and st.batches == 2
+        assert st.completed == 0, "held partitions complete nothing"
+        assert 3 <= st.pending <= 7, st.pending  # the third wave (and what partition 1 did not take) waits
+        assert lib.ipir_dbg_loop_hold(loop, 0) == 0
+        lat = C.c_double()
+        for t in tickets:
+            assert t > 0
+            assert lib.ipir_loop_wait(loop, t, C.byref(lat)) == 0
+            assert lat.value > 0
+        st = _loop_stats(lib, loop)
+        assert st.completed == 9 and st.failed == 0 and st.batches >= 3
+        say(f"[loop] batches {st.batches}, largest {st.max_batch_size}, admitted while busy "
+            f"{st.admitted_while_busy}, device overlaps {st.device_overlaps}")
+    finally:
+        lib.ipir_loop_stop(loop)
     for i in range(len(qs)):
         assert np.array_equal(outs[i], alone[i]), f"loop result {i} differs from alone"
+        assert np.array_equal(ol.extract(geo, built[i][1], outs[i].reshape(geo["T"], -1)),
+                              ol.expected_slots(geo, db, idxs[i]))
+    assert np.array_equal(srv.answer_batch(qs[:2]), np.stack(alone[:2])), "entry points work after stop"
+    assert lib.ipir_dbg_set_pipeline(srv.h, 3, 2) == 0
+
+
+def test_serving_loop_sizes_partitions_from_the_slots(medium8):
+    """ipir_loop_start(srv, 0): partitions of 32 slots (the whole batch if
+    fewer), as many as max_batch holds; a burst larger than one partition
+    comes back bit-identical to the alone answers."""
+    geo, db, srv = medium8
+    lib = srv.lib
+    qs = [ol.make_query(geo, SEED, (7919 * k) % 524288, 950 + k)[0] for k in range(40)]
+    alone = srv.answer_batch(qs[:32])
+    alone = np.concatenate([alone, srv.answer_batch(qs[32:])])
+    loop = lib.ipir_loop_start(srv.h, 0)
+    assert loop
+    outs = np.zeros((40, geo["T"] * 4 * N), np.uint64)
+    views = [srv.views([q]) for q in qs]
+    try:
+        st = _loop_stats(lib, loop)
+        assert (st.partitions, st.slots_per_partition) == (1, MB8)
+        tickets = [lib.ipir_loop_submit(loop, views[k], ia.ptr(outs[k])) for k in range(40)]
+        for t in tickets:
+            assert lib.ipir_loop_wait(loop, t, None) == 0
+        st = _loop_stats(lib, loop)
+        assert st.completed == 40 and st.max_batch_size <= MB8
+    finally:
+        lib.ipir_loop_stop(loop)
+    assert np.array_equal(outs, alone)
+
+
+def test_pair_overrun_reported_on_the_right_call(medium8):
+    """The pair Horner kernel's bounded wait, forced to give up at once
+    (spin limit 0; every kernel still ends normally): the blocking call
+    that ran it returns 4, the serving loop reports 4 for its request, the
+    asynchronous device entry point never uses the pair and answers
+    correctly, and once the bound is restored the next call is not blamed
+    for the earlier failure."""
+    geo, db, srv = medium8
+    lib = srv.lib
+    q, st = ol.make_query(geo, SEED, 4242, 31)
+    want = srv.answer_batch([q])[0]
+    out = np.zeros((1, geo["T"] * 4 * N), np.uint64)
+    views = srv.views([q])
+    assert lib.ipir_dbg_set_horner(srv.h, 1) == 0
+    assert lib.ipir_dbg_set_horner_spin_limit(srv.h, 0) == 0
+    try:
+        assert lib.ipir_answer_batch(srv.h, views, 1, ia.ptr(out)) == 4
+        dq = ia.DeviceBuffer(q.nbytes, lib)
+        dq.upload(q)
+        dout = ia.DeviceBuffer(want.nbytes, lib)
+        assert lib.ipir_answer_batch_device(srv.h, C.c_void_p(dq.p), 1, C.c_void_p(dout.p), None) == 0
+        assert lib.ipir_dev_sync() == 0
+        assert np.array_equal(dout.download(np.zeros_like(want)), want), "device entry point (no pair)"
+        loop = lib.ipir_loop_start(srv.h, 2)
+        assert loop
+        try:
+            o2 = np.zeros(geo["T"] * 4 * N, np.uint64)
+            t = lib.ipir_loop_submit(loop, views, ia.ptr(o2))
+            assert lib.ipir_loop_wait(loop, t, None) == 4
+        finally:
+            lib.ipir_loop_stop(loop)
+    finally:
+        assert lib.ipir_dbg_set_horner_spin_limit(srv.h, 1 << 26) == 0
+    try:
+        assert lib.ipir_answer_batch(srv.h, views, 1, ia.ptr(out)) == 0, "a later call is not blamed"
+        assert np.array_equal(out[0], want)
+    finally:
+        assert lib.ipir_dbg_set_horner(srv.h, -1) == 0
+
+
+def test_device_calls_on_two_streams_are_ordered(medium8):
+    """Asynchronous calls share the server's slots: one on stream A, the
+    next on stream B, then a blocking host-staged call, repeated without
+    synchronising in between; each call's responses equal its queries
+    answered alone (the library orders each call after the previous one)."""
+    geo, db, srv = medium8
+    lib = srv.lib
+    sets = [[ol.make_query(geo, SEED, (99991 * (3 * r + j)) % 524288, 1300 + 10 * r + j)[0] for j in range(3)]
+            for r in range(3)]
+    alone = [srv.answer_batch(x) for x in sets]
+    sa, sb = lib.ipir_dev_stream_create(), lib.ipir_dev_stream_create()
+    assert sa and sb
+    bufs = []
+    try:
+        for r in range(2):
+            dqa = ia.DeviceBuffer(np.stack(sets[0]).nbytes, lib)
+            dqb = ia.DeviceBuffer(np.stack(sets[1]).nbytes, lib)
+            da = ia.DeviceBuffer(alone[0].nbytes, lib)
+            db_ = ia.DeviceBuffer(alone[1].nbytes, lib)
+            bufs += [dqa, dqb, da, db_]
+            dqa.upload(np.stack(sets[0]))
+            dqb.upload(np.stack(sets[1]))
+            assert lib.ipir_answer_batch_device(srv.h, C.c_void_p(dqa.p), 3, C.c_void_p(da.p), sa) == 0
+            assert lib.ipir_answer_batch_device(srv.h, C.c_void_p(dqb.p), 3, C.c_void_p(db_.p), sb) == 0
+            host = srv.answer_batch(sets[2])
+            assert lib.ipir_dev_stream_sync(sa) == 0 and lib.ipir_dev_stream_sync(sb) == 0
+            assert np.array_equal(da.download(np.zeros_like(alone[0])), alone[0]), f"stream A, round {r}"
+            assert np.array_equal(db_.download(np.zeros_like(alone[1])), alone[1]), f"stream B, round {r}"
+            assert np.array_equal(host, alone[2]), f"host-staged, round {r}"
+    finally:
+        for b in bufs:
+            b.free()
+        lib.ipir_dev_stream_destroy(sa)
+        lib.ipir_dev_stream_destroy(sb)
 
 
 # ---------------------------------------------------------------- several response ciphertexts
