@@ -41,15 +41,20 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def gen_db(R, Cc, seed=0):
-    """Uniform u16 in [0, 65535), generated in row chunks from one seed."""
+def gen_db(R, Cc, seed=0, threads=16):
+    """Uniform u16 in [0, 65535), row chunks from SeedSequence(seed).spawn,
+    generated on host threads (numpy's generators release the GIL)."""
+    import concurrent.futures as cf
     db = np.empty((R, Cc), np.uint16)
-    ss = np.random.SeedSequence(seed)
-    chunk = max(1, (1 << 28) // Cc)  # rows per chunk (~512 MiB)
-    kids = ss.spawn((R + chunk - 1) // chunk)
-    for k, r0 in enumerate(range(0, R, chunk)):
+    rows = max(1, (1 << 26) // Cc)  # ~128 MiB per chunk
+    starts = list(range(0, R, rows))
+    kids = np.random.SeedSequence(seed).spawn(len(starts))
+
+    def fill(k):
         g = np.random.default_rng(kids[k])
-        db[r0:r0 + chunk] = g.integers(0, 65535, size=(min(chunk, R - r0), Cc), dtype=np.uint16)
+        db[starts[k]:starts[k] + rows] = g.integers(0, 65535, size=(min(rows, R - starts[k]), Cc), dtype=np.uint16)
+    with cf.ThreadPoolExecutor(min(threads, os.cpu_count() or 1)) as ex:
+        list(ex.map(fill, range(len(starts))))
     return db
 
 
@@ -61,6 +66,45 @@ def cpu_model():
     except OSError:
         pass
     return "unknown"
+
+
+def host_cpus():
+    """(logical CPUs of the machine, CPUs this process may run on)."""
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        allowed = os.cpu_count() or 1
+    return os.cpu_count() or 1, allowed
+
+
+def ref_primitives_us():
+    """The reference's own CPU primitives (its ntt.cpp / ring.cpp /
+    crypto.cpp compiled from source into oracle/_ref when the build saw the
+    reference tree), one thread, microseconds per call; None if absent."""
+    import oracle_lib as ol
+    if not ol.have_ref():
+        return None
+    r = ol.ref()
+    N = ol.N
+    rng = np.random.default_rng(6)
+    poly = ol.rand_poly(rng).reshape(-1)
+
+    def us(fn, reps):
+        fn()
+        t = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        return round((time.perf_counter() - t) / reps * 1e6, 1)
+    a = poly[:N].copy()
+    out = {"ntt_fwd_q0": us(lambda: r.ref_ntt_fwd(a, 0), 50), "ntt_fwd_q1": us(lambda: r.ref_ntt_fwd(a, 1), 50),
+           "ntt_inv_q0": us(lambda: r.ref_ntt_inv(a, 0), 50), "ntt_inv_q1": us(lambda: r.ref_ntt_inv(a, 1), 50)}
+    dec = np.zeros(2 * 2 * N, np.uint64)
+    out["decomp"] = us(lambda: r.ref_decomp(poly, dec), 50)
+    rg = ol.rand_poly(rng, (4, 2)).reshape(-1)
+    ct = ol.rand_poly(rng, (2,)).reshape(-1)
+    res = np.zeros(4 * N, np.uint64)
+    out["ext_prod"] = us(lambda: r.ref_ext_prod(rg, ct, res), 10)
+    return out
 
 
 def cpu_baseline(geo, n_groups_sample=8, sample_cols=65536, horner_steps=16, threads=None):
@@ -143,14 +187,21 @@ def cpu_baseline(geo, n_groups_sample=8, sample_cols=65536, horner_steps=16, thr
     prim[f"horner_eval_Da{Da}"] = round(step * max(0, Da - 1) * 1e6, 1)
     total = t_mv + t_pack + t_h
     total1 = t_mv1 + t_pack1 + t_h1
+    ncpu, allowed = host_cpus()
+    refp = ref_primitives_us()
     return {
         "value": 1.0 / total, "unit": "queries/s", "cores": threads, "kind": "port",
-        "sample": (f"oracle (scalar C++ restatement of the reference CPU primitives) on {threads} host threads "
-                   f"({cpu_model()}): mat-vec over {cw} of {Cc} columns, collapse+pack of {ng} groups, "
-                   f"{horner_steps} of {Da - 1} Horner steps; scaled to one whole query (mat-vec {t_mv:.2f} s, "
-                   f"collapse {t_pack:.2f} s, Horner {t_h:.2f} s)"),
+        "host": {"cpu_model": cpu_model(), "logical_cpus": ncpu, "cpus_allowed": allowed, "threads_used": threads},
+        "sample": (f"EXTRAPOLATED from a bounded sample: the oracle (scalar C++ restatement of the reference CPU "
+                   f"primitives) on {threads} worker threads of {ncpu} logical CPUs ({cpu_model()}) times the "
+                   f"mat-vec over {cw} of {Cc} columns, collapse+pack of {ng} of {G} groups and {horner_steps} of "
+                   f"{Da - 1} Horner steps, each scaled to one whole query (mat-vec {t_mv:.2f} s, collapse "
+                   f"{t_pack:.2f} s, Horner {t_h:.2f} s)"),
         "single_thread_qps": 1.0 / total1,
         "primitives_us": prim,
+        "reference_primitives_us": refp,
+        "reference_primitives_note": ("the reference's own ntt.cpp/ring.cpp/crypto.cpp compiled from source, one "
+                                      "thread on this host" if refp else "reference CPU library not built"),
     }
 
 
@@ -184,6 +235,8 @@ def serving(lib, srv, qs, capacity_qps, resp_u64s, rates=(0.25, 0.5, 0.9), secon
         gaps = rng.exponential(1.0 / lam, n)
         loop = lib.ipir_loop_start(srv.h, max_inflight)
         assert loop
+        st0 = ia.ipir_loop_stats()
+        lib.ipir_loop_get_stats(loop, C.byref(st0))
         tickets = [None] * n
         lat = np.zeros(n)
         done_t = [0.0]
@@ -213,12 +266,19 @@ def serving(lib, srv, qs, capacity_qps, resp_u64s, rates=(0.25, 0.5, 0.9), secon
             assert tickets[k]
             ready.release()
         th.join()
+        st = ia.ipir_loop_stats()
+        lib.ipir_loop_get_stats(loop, C.byref(st))
         lib.ipir_loop_stop(loop)
         wall = done_t[0] - t0
         results.append({"offered_qps": round(lam, 1), "achieved_qps": round(n / wall, 1), "requests": n,
                         "p50_ms": round(float(np.percentile(lat, 50)), 3),
                         "p99_ms": round(float(np.percentile(lat, 99)), 3),
-                        "p999_ms": round(float(np.percentile(lat, 99.9)), 3)})
+                        "p999_ms": round(float(np.percentile(lat, 99.9)), 3),
+                        "partitions": int(st0.partitions), "slots_per_partition": int(st0.slots_per_partition),
+                        "batches": int(st.batches), "mean_batch": round(n / max(1, st.batches), 2),
+                        "max_busy_partitions": int(st.max_busy_partitions),
+                        "batches_admitted_while_another_in_flight": int(st.admitted_while_busy),
+                        "batches_launched_beside_device_work": int(st.device_overlaps)})
         log(f"[bench] serving at {rate:.2f}x capacity: {results[-1]}")
     return results
 
@@ -229,13 +289,15 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="c3", choices=sorted(CONFIGS))
-    ap.add_argument("--batch", type=int, default=32)
+    ap.add_argument("--batch", type=int, default=32,
+                    help="queries per batch (max_batch of the server); 0: the most slots that fit the device's free "
+                         "memory (ipir_fit_max_batch, 4 GiB kept free), at most 128")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--verify", type=int, default=0,
                     help="also answer and extract this many further random queries (batches of --batch)")
     ap.add_argument("--serving-inflight", type=int, default=0,
                     help="serving loop partitions (batches in flight), each of max_batch / N slots; "
-                         "0: as many 32-slot partitions as the batch holds, at most 2")
+                         "0: as many 32-slot partitions as the batch holds, at most 4")
     ap.add_argument("--serving", action="store_true",
                     help="also run Poisson arrivals through the serving loop at 0.25/0.5/0.9x capacity")
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
@@ -262,6 +324,17 @@ def main():
     R, Cc, G, T = geo["R"], geo["C"], geo["G"], geo["T"]
     geo["Da"] = min(geo["D"], G)
     B = args.batch
+    fitted = None
+    if B <= 0:
+        B = int(ia.load("server").ipir_fit_max_batch(n, w, rows, 4 * 2 ** 30, 128))
+        fp = ia.footprint(n, w, rows)
+        fitted = {"max_batch": B, "free_bytes_at_start": ia.mem_info()[0], "reserve_bytes": 4 * 2 ** 30,
+                  "resident_fixed": fp["resident_fixed"], "per_slot": fp["per_slot"],
+                  "preprocess_peak": fp["preprocess_peak"]}
+        log(f"[bench] max_batch {B} fits the free device memory ({fitted})")
+        if B < 1:
+            print(json.dumps({"error": "the server does not fit the device's free memory"}))
+            sys.exit(3)
     log(f"[bench] {label}: R={R} C={Cc} G={G} T={T}; generating DB ({R * Cc * 2 / 1e9:.2f} GB)")
     t0 = time.time()
     db = gen_db(R, Cc, 0)
@@ -274,11 +347,18 @@ def main():
     lib = srv.lib
     idx = np.random.default_rng(1).integers(0, n, B)
     cl = ia.Client(srv.params, lib)
-    qs, hs = [], []
+    qs, hs, build_ms = [], [], []
     for k, i in enumerate(idx):
+        t = time.perf_counter()
         h, q = cl.build(int(i), 2 * 1000003 + k)
+        build_ms.append((time.perf_counter() - t) * 1e3)
         qs.append(q)
         hs.append(h)
+    client = {"query_build_ms_first": round(build_ms[0], 2),
+              "query_build_ms_cached_p50": round(float(np.median(build_ms[1:])) if B > 1 else build_ms[0], 2),
+              "note": "client CPU, one thread: the first query of a process expands the seed's CRS polynomials, "
+                      "later ones reuse them (not part of the server metric)"}
+    log(f"[bench] client query build: {client}")
     flat = np.ascontiguousarray(np.stack(qs))
     qlen = flat.shape[1]
     resp_u64s = T * 4 * ia.N
@@ -448,9 +528,8 @@ def main():
 
     serve = None
     if args.serving:
-        if args.serving_inflight <= 0:  # full 32-query passes in every partition
-            args.serving_inflight = max(1, min(2, B // 32))
-        serve = serving(lib, srv, qs, qps32, resp_u64s, max_inflight=args.serving_inflight)
+        # 0: partitions sized by the loop itself (32 slots each, as many as max_batch holds, up to 4)
+        serve = serving(lib, srv, qs, qps32, resp_u64s, max_inflight=max(0, args.serving_inflight))
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -498,6 +577,8 @@ def main():
         "batch1_db_pass": {"kernel": kname, "ms": round(dbpass_ms, 3), "hbm_GBs": round(achieved, 1),
                            "frac_of_hbm_peak": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
         "tuning": tuning,
+        "client": client,
+        "max_batch_fitted": fitted,
         "graph_replay": graph,
         "horner_step_us": {"batch1": round(st1[4] * 1e3 / max(1, geo["Da"] - 1), 2),
                            f"batch{B}": round(st32[4] * 1e3 / max(1, geo["Da"] - 1), 2)},
@@ -506,9 +587,9 @@ def main():
         "cpu_baseline": cpu,
     }
     if serve is not None:
-        k = args.serving_inflight
+        k, per = serve[0]["partitions"], serve[0]["slots_per_partition"]
         out["serving"] = {"loop": f"ipir_loop (continuous batching, {k} partition{'s' if k > 1 else ''} of "
-                                  f"{B // k} slots)", "rates": serve}
+                                  f"{per} slots)", "rates": serve}
     for h in hs:
         cl.destroy(h)
     dq.free()

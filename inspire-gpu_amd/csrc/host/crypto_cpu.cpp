@@ -9,6 +9,11 @@
 // horner_eval (:264-279), compress_ct / dec_compressed (:285-352).
 #include <cassert>
 #include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
 
 #include "crypto.h"
 
@@ -27,12 +32,40 @@ uint64_t pow_base(int digit) {
     for (int t = 0; t < digit; t++) b *= BASE;
     return b;
 }
-RnsPoly times_s(const RnsPoly& a_coeff, const RnsPoly& s_ntt) {
-    RnsPoly t = a_coeff;
-    t.to_ntt();
+// a * s in coefficient form, a given in NTT form
+RnsPoly times_s(const RnsPoly& a_ntt, const RnsPoly& s_ntt) {
+    RnsPoly t = a_ntt;
     t.mul_inplace(s_ntt);
     t.to_coeff();
     return t;
+}
+
+// The CRS polynomials a client multiplies by its secret are fixed by the
+// public seed: expand_seed(seed, domain, index) in NTT form, computed once
+// per (seed, domain, index) and shared by every later query of the process
+// (the reference expands them again for each query; its docs note they are
+// cacheable, docs/ARCHITECTURE.md:118-120). At most kCrsCacheEntries are
+// kept; a full cache is cleared, so memory stays bounded for clients that
+// talk to many servers.
+constexpr size_t kCrsCacheEntries = 1 << 12;
+
+std::shared_ptr<const RnsPoly> crs_ntt(const uint8_t seed[SEED_BYTES], const char* domain, uint64_t index) {
+    static std::mutex mu;
+    static std::map<std::string, std::shared_ptr<const RnsPoly>> cache;
+    std::string key(reinterpret_cast<const char*>(seed), SEED_BYTES);
+    key += domain;
+    key.push_back('\0');
+    key.append(reinterpret_cast<const char*>(&index), sizeof index);
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = cache.find(key);
+        if (it != cache.end()) return it->second;
+    }
+    auto p = std::make_shared<RnsPoly>(expand_seed(seed, domain, index));
+    p->to_ntt();
+    std::lock_guard<std::mutex> lk(mu);
+    if (cache.size() >= kCrsCacheEntries) cache.clear();
+    return cache.emplace(key, p).first->second;
 }
 }  // namespace
 
@@ -64,7 +97,7 @@ LweQuery lwe_enc_b(const RnsPoly& s, size_t i_star, size_t ell, const uint8_t se
     RnsPoly s_ntt = s;
     s_ntt.to_ntt();
     for (size_t r = 0; r < ell / N; r++) {
-        RnsPoly b = times_s(expand_seed(seed, "rlwe", r), s_ntt);
+        RnsPoly b = times_s(*crs_ntt(seed, "rlwe", r), s_ntt);
         b.add_inplace(sample_error(rng));
         if (r == i_star / N) {
             RnsPoly m;
@@ -85,7 +118,7 @@ Ksk ksk_gen_b(const RnsPoly& s, uint64_t k, const uint8_t seed[SEED_BYTES], std:
     const RnsPoly s_auto = automorphism(s, k);
     for (int j = 0; j < D_EFF; j++) {
         const int digit = j + 1;
-        RnsPoly b = times_s(expand_seed(seed, "ksk", k * 100 + digit), s_ntt);
+        RnsPoly b = times_s(*crs_ntt(seed, "ksk", k * 100 + digit), s_ntt);
         b.add_inplace(sample_error(rng));
         RnsPoly sk = s_auto;
         const uint64_t bj = pow_base(digit);
@@ -114,19 +147,19 @@ RgswCt rgsw_enc_b(const RnsPoly& s, const RnsPoly& mu, const uint8_t seed[SEED_B
         RnsPoly b_mu = mu;  // B^j * mu
         b_mu.scalar_mul_inplace(bj % Q0, bj % Q1);
 
-        RnsPoly a = expand_seed(seed, "rgsw0", digit);
-        RnsPoly b = times_s(a, s_ntt);
+        const auto a0 = crs_ntt(seed, "rgsw0", digit);
+        RnsPoly b = times_s(*a0, s_ntt);
         b.add_inplace(sample_error(rng));
         b.sub_inplace(s_mu);
-        a.to_ntt(); b.to_ntt();
-        g.top.a_parts[j] = a; g.top.b_parts[j] = b;
+        b.to_ntt();
+        g.top.a_parts[j] = *a0; g.top.b_parts[j] = b;
 
-        a = expand_seed(seed, "rgsw1", digit);
-        b = times_s(a, s_ntt);
+        const auto a1 = crs_ntt(seed, "rgsw1", digit);
+        b = times_s(*a1, s_ntt);
         b.add_inplace(sample_error(rng));
         b.add_inplace(b_mu);
-        a.to_ntt(); b.to_ntt();
-        g.bottom.a_parts[j] = a; g.bottom.b_parts[j] = b;
+        b.to_ntt();
+        g.bottom.a_parts[j] = *a1; g.bottom.b_parts[j] = b;
     }
     return g;
 }

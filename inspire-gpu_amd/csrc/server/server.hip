@@ -19,6 +19,7 @@
 #include "gpu_protocol.h"
 #include "kernels.h"
 #include "server_ctx.h"
+#include "trace.h"
 
 using namespace ipx;
 
@@ -92,8 +93,10 @@ PreprocessData gpu_preprocess(const PublicParams& pp, const uint16_t* db) {
         if (all) { dfree(d_db); dfree(arena); }
         if (s) (void)hipStreamDestroy(s);
     };
+    TraceRange tr_all("gpu_preprocess %zu x %zu, %zu groups", R, C, G);
     try {
         HIPX(hipStreamCreate(&s));
+        trace_name_stream(s, "preprocess");
         d_tw = upload_tables(nc, cc);
         // The host DB is row-major. It goes up in slabs of whole rows
         // (~256 MiB): copy, encode in place (row-local), then scatter into
@@ -103,6 +106,7 @@ PreprocessData gpu_preprocess(const PublicParams& pp, const uint16_t* db) {
         if (slab < 16) slab = 16;
         if (slab > R) slab = R;
         d_stage = dalloc<uint16_t>(slab * C);
+        TraceRange tr_up("preprocess: upload, encode, byte planes");
         for (size_t r0 = 0; r0 < R; r0 += slab) {
             const size_t nrows = R - r0 < slab ? R - r0 : slab;
             HIPX(hipMemcpyAsync(d_stage, db + r0 * C, nrows * C * sizeof(uint16_t), hipMemcpyHostToDevice, s));
@@ -149,6 +153,8 @@ PreprocessData gpu_preprocess(const PublicParams& pp, const uint16_t* db) {
         d_dtmp = dalloc<uint32_t>(chunk * 2 * RD_SZ);
         for (size_t g0 = 0; g0 < G; g0 += chunk) {
             const size_t ng = G - g0 < chunk ? G - g0 : chunk;
+            TraceRange tr_g("preprocess: groups %zu..%zu (a-side, ring embed, collapse chains, relayout)", g0,
+                            g0 + ng - 1);
             uint32_t* pc = arena + g0 * PC_GROUP;
             HIPX(launch_aside(reinterpret_cast<const uint8_t*>(d_db), R, C, g0, ng, d_crs, d_a32, nc, s));
             HIPX(launch_ring_embed(d_a32, ng, nc, s));
@@ -370,9 +376,14 @@ void GpuServerCtx::alloc_slots() {
         d_KS[l] = dalloc<uint32_t>(64);
     }
     d_hstate = dalloc<uint32_t>(B * T * 4 * N);
+    trace_name_stream(stream, "server");
     for (int l = 0; l < kLanes; l++) {
         HIPX(hipStreamCreate(&hstream[l]));
         HIPX(hipStreamCreate(&cstream[l]));
+        static const char* hn[kLanes] = {"horner lane 0", "horner lane 1", "horner lane 2", "horner lane 3"};
+        static const char* cn[kLanes] = {"collapse lane 0", "collapse lane 1", "collapse lane 2", "collapse lane 3"};
+        trace_name_stream(hstream[l], hn[l]);
+        trace_name_stream(cstream[l], cn[l]);
         for (auto& e : chunk_ev[l]) HIPX(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         for (auto& e : coll_ev[l]) HIPX(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         HIPX(hipEventCreateWithFlags(&prep_ev[l], hipEventDisableTiming));
@@ -449,6 +460,7 @@ void GpuServerCtx::autotune_dbpass() {
     }
     const char* at = std::getenv("INSPIRE_AUTOTUNE");
     if (at && at[0] == '0') return;
+    TraceRange tr("setup: autotune");
     hipEvent_t e0, e1;
     HIPX(hipEventCreate(&e0));
     HIPX(hipEventCreate(&e1));
@@ -766,6 +778,7 @@ void GpuServerCtx::stage_horner(size_t count, uint64_t* d_out, hipStream_t s, si
 void GpuServerCtx::enqueue(const uint64_t* d_in, size_t count, uint64_t* d_out, hipStream_t s, size_t slot0,
                            int lane, bool status_read) {
     if (lane < 0 || lane >= kLanes) throw std::invalid_argument("enqueue: scratch lane out of range");
+    TraceRange tr("answer: %zu queries, slots %zu.., lane %d", count, slot0, lane);
     stage_prep(d_in, count, s, slot0);
     const size_t C = pp.db_cols;
     const size_t nchunks = G / gchunk;
@@ -808,6 +821,7 @@ void GpuServerCtx::enqueue(const uint64_t* d_in, size_t count, uint64_t* d_out, 
         dbpass_prep(q0, nq, p, s, lane);
         for (size_t k = 0; k < chunks.size(); k++) {
             const size_t g_lo = chunks[k].first, g_hi = chunks[k].second, ng = g_hi - g_lo;
+            TraceRange trk("chunk %zu: groups %zu..%zu (DB pass, collapse, pack, Horner)", k, g_lo, g_hi - 1);
             hipEvent_t dbe = chunk_ev[lane][k % kChunkEvents], ce = coll_ev[lane][k % kChunkEvents];
             dbpass_range(q0, nq, g_lo, g_hi, ng == gchunk ? p : chunk_plan(nq, ng * N), s, lane);
             // with the collapse on s its event covers the DB pass too (an
@@ -945,6 +959,7 @@ void GpuServerCtx::answer_host(size_t count, uint64_t* out, hipStream_t s, size_
     uint64_t* hr = h_resp + slot0 * resp_u64s;
     uint64_t* dq = d_q + slot0 * qlen;
     uint64_t* dr = d_resp + slot0 * resp_u64s;
+    TraceRange tr("host-staged answer: %zu queries", count);
     order_before(s);
     HIPX(hipMemcpyAsync(dq, hq, count * qlen * 8, hipMemcpyHostToDevice, s));
     run_step(dq, dr, count, slot0, 0, 0, s, [&] { enqueue(dq, count, dr, s, slot0, 0, true); });
