@@ -67,6 +67,19 @@ int ipir_dbg_horner(const uint64_t* cts, size_t L, const uint64_t* rgsw, uint64_
 /* horner with a chosen kernel: variant 0 = one workgroup per chain, 1 = a
  * workgroup pair (one per limb). */
 int ipir_dbg_horner_variant(const uint64_t* cts, size_t L, const uint64_t* rgsw, uint64_t* out, int variant);
+/* ipir_dbg_horner over DEVICE buffers, allocating nothing: for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops). No counterpart in the reference (its gpu_horner_eval,
+ * src/gpu_online.cu:474-540, allocates per call). d_cts [L][a|b][limb][N]
+ * u64 (NTT form), d_rgsw 4*d_eff*2*N u64, d_out [a|b][limb][N] u64; enqueued
+ * on hip_stream (NULL: the null stream), asynchronous. Its scratch is one
+ * static array per GPU (8.3 MiB, held until the process ends), set up at
+ * the first call on a device; calls that overlap on one GPU share it and
+ * race, so callers order them. 0 ok; 1 null pointer or L == 0; 2 L above
+ * 256 (the longest chain of any geometry, the array's limit), nothing
+ * launched; 4 launch failure. Outputs equal ipir_dbg_horner's. */
+int ipir_horner_noalloc(const uint64_t* d_cts, size_t L, const uint64_t* d_rgsw, uint64_t* d_out,
+                        void* hip_stream);
 /* Set the groups per chunk of the pipelined answer (a divisor of n_packed;
  * 1 if the partial-sum buffer cannot hold the resulting plans). */
 int ipir_dbg_set_group_chunk(ipir_server* srv, size_t groups);

@@ -112,6 +112,7 @@ SERVER_FUNCS = [
     ("ipir_dbg_copy_bench", C.c_int, [SZ, C.c_int, P(C.c_double)]),
     ("ipir_dbg_horner", C.c_int, [VP, SZ, VP, VP]),
     ("ipir_dbg_horner_variant", C.c_int, [VP, SZ, VP, VP, C.c_int]),
+    ("ipir_horner_noalloc", C.c_int, [VP, SZ, VP, VP, VP]),
     ("ipir_dbg_set_horner", C.c_int, [VP, C.c_int]),
     ("ipir_dbg_set_graph", C.c_int, [VP, C.c_int]),
     ("ipir_dbg_set_pack", C.c_int, [VP, C.c_int]),
