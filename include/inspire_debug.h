@@ -110,7 +110,9 @@ int ipir_dbg_loop_hold(ipir_loop* loop, int on);
 /* Select the pack kernel of pipelined answers' chunks: 1 one 1024-lane
  * workgroup per query, 0 (the default, or INSPIRE_PACK_WG at setup) a grid
  * of one 256-lane workgroup per (group, limb, query), -1 one workgroup for
- * one-query calls. Outputs are equal. 1 for other values. */
+ * one-query calls, 2 no kernel of its own: each one-workgroup Horner chain
+ * packs its own groups before its steps (the two-workgroup chain keeps a
+ * pack kernel). Outputs are equal. 1 for other values. */
 int ipir_dbg_set_pack(ipir_server* srv, int mode);
 /* Answer through captured hipGraphs (1) or launch by launch (0, the default
  * unless INSPIRE_GRAPH=1 at setup). With 1, the first answer of each
@@ -137,13 +139,24 @@ int ipir_dbg_set_mfma_depth(ipir_server* srv, uint32_t depth, int form);
  * (0: back to the narrowing rule; a power of two up to 64, used when the
  * plan reaches it and the partial-sum buffer holds it). */
 int ipir_dbg_set_chunk_slices(ipir_server* srv, size_t nq, uint32_t slices);
+/* Row slices of the pipelined answer's database-pass chunks from chunk
+ * index from_chunk on (chunks go from the highest groups down), for batches
+ * planned like an nq-query batch or for one-query answers when nq is 1:
+ * the chunks that run after the collapse beside them has finished can use
+ * the whole device. 0 slices: as the earlier chunks (the default unless
+ * the setup autotuner picked otherwise). 1 for bad values. */
+int ipir_dbg_set_late_slices(ipir_server* srv, size_t nq, size_t from_chunk, uint32_t slices);
 /* Form of the 17..32-query collapse pass (0..5; 1 otherwise), and the one
  * in use (the setup-time autotuner's pick unless set). */
 int ipir_dbg_set_collapse_variant(ipir_server* srv, int variant);
 int ipir_dbg_get_collapse_variant(const ipir_server* srv);
 /* Set the pipeline shape of one-query and many-query answers: 0 serial, 2
  * Horner chains on a second stream per group chunk, 3 the collapse on a
- * third stream as well (defaults 3 and 2); 1 for other values. */
+ * third stream as well, 4 the collapse of all groups first and then the
+ * database-pass chunks with the Horner chains beside them, 5 as 3 with the
+ * collapse stream a chunk ahead of the database pass (the setup autotuner
+ * picks 2 or 3; 4 and 5 are tested alternatives that never won at 16 GB);
+ * 1 for other values. */
 int ipir_dbg_set_pipeline(ipir_server* srv, int single, int batched);
 /* Per-stage device time (ms) of the last ipir_dbg_profile_batch run:
  * [unmarshal+key NTT, DB pass, collapse pass, pack finalize, Horner]. */

@@ -535,6 +535,22 @@ int ipir_dbg_set_chunk_slices(ipir_server* srv, size_t nq, uint32_t slices) {
     return 0;
 }
 
+int ipir_dbg_set_late_slices(ipir_server* srv, size_t nq, size_t from_chunk, uint32_t slices) {
+    if (!srv || nq < 1 || nq > ipx::kMfmaMaxQueries || slices > 64 || (slices & (slices - 1))) return 1;
+    GpuServerCtx* c = srv->ctx;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (c->loop_active) return 5;
+    if (nq == 1) {
+        c->late_from1 = (uint32_t)from_chunk;
+        c->late_slices1 = slices;
+    } else {
+        c->late_from = (uint32_t)from_chunk;
+        c->late_slices = slices;
+    }
+    c->drop_graphs();  // their kernels carry the old plans
+    return 0;
+}
+
 int ipir_dbg_set_mfma_depth(ipir_server* srv, uint32_t depth, int form) {
     if (!srv || form < 0 || form >= ipx::kMfmaForms) return 1;
     // a depth the 32-query kernel of this form is built with
@@ -555,7 +571,7 @@ int ipir_dbg_set_mfma_depth(ipir_server* srv, uint32_t depth, int form) {
 }
 
 int ipir_dbg_set_pipeline(ipir_server* srv, int single, int batched) {
-    auto ok = [](int v) { return v == 0 || v == 2 || v == 3; };
+    auto ok = [](int v) { return v == 0 || (v >= 2 && v <= 5); };
     if (!srv || !ok(single) || !ok(batched)) return 1;
     std::lock_guard<std::mutex> lk(srv->ctx->mu);
     if (srv->ctx->loop_active) return 5;
@@ -566,7 +582,7 @@ int ipir_dbg_set_pipeline(ipir_server* srv, int single, int batched) {
 }
 
 int ipir_dbg_set_pack(ipir_server* srv, int mode) {
-    if (!srv || mode < -1 || mode > 1) return 1;
+    if (!srv || mode < -1 || mode > 2) return 1;
     std::lock_guard<std::mutex> lk(srv->ctx->mu);
     if (srv->ctx->loop_active) return 5;
     srv->ctx->pack_wg = mode;

@@ -141,10 +141,23 @@ hipError_t launch_pack_finalize_wg(const uint32_t* interm, size_t C, const uint3
 // [q][ct][a|b][limb][N] (launches go from the highest group range down).
 // Group g's packed a-part (shared by all queries) is at pa_base +
 // g*pa_stride; its b-part at packed_b + q*packed_stride + g*2N.
+// With pack.interm set, each chain first computes the packed b-parts of
+// its own groups in [g_lo, g_hi) (launch_pack_finalize's output, written
+// to packed_b) in its own workgroup, so a chunk's pack needs no grid of its
+// own beside the database passes.
+struct FusedPack {
+    const uint32_t* interm = nullptr;  // [q][limb][C] (query 0 of the launch)
+    size_t C = 0;
+    const uint32_t* part2 = nullptr;   // collapse sums, [q][g][limb][N]
+    size_t part2_stride = 0;
+    const uint32_t* precomp = nullptr; // arena base (Df of every group)
+    const uint32_t* keys = nullptr;    // [q][key][digit][limb][N]
+    size_t keys_stride = 0;
+};
 hipError_t launch_horner(const uint32_t* pa_base, size_t pa_stride, size_t G, size_t Da, size_t T,
-                         const uint32_t* packed_b, size_t packed_stride, const uint32_t* rgsw, size_t rgsw_stride,
+                         uint32_t* packed_b, size_t packed_stride, const uint32_t* rgsw, size_t rgsw_stride,
                          uint64_t* out, uint32_t* hstate, size_t g_lo, size_t g_hi, size_t nq, const NttConst& nc,
-                         const CrtConst& cc, hipStream_t s);
+                         const CrtConst& cc, hipStream_t s, const FusedPack& pack = FusedPack{});
 
 // The same chains with two workgroups per chain (one per limb) exchanging
 // coefficient-form values once per step. xbuf holds

@@ -177,11 +177,11 @@ __global__ __launch_bounds__(1024) void k_pack_finalize_wg(const uint32_t* __res
 // ahead of their use, so HBM latency (long while a database pass streams
 // through L2) hides behind the round-1 transforms.
 __global__ __launch_bounds__(1024) void k_horner(const uint32_t* __restrict__ pa_base, size_t pa_stride, uint32_t G, uint32_t Da,
-                                                 uint32_t T, const uint32_t* __restrict__ packed_b,
+                                                 uint32_t T, uint32_t* packed_b,
                                                  size_t pstride, const uint32_t* __restrict__ rgsw,
                                                  size_t rstride, uint64_t* __restrict__ out,
                                                  uint32_t* __restrict__ hstate, uint32_t g_lo, uint32_t g_hi,
-                                                 NttConst nc, CrtConst cc) {
+                                                 NttConst nc, CrtConst cc, FusedPack fp) {
     __shared__ uint32_t buf[4][POLY_LDS];
     __shared__ uint32_t twl[8 * N];
     const uint32_t ct = blockIdx.x, qi = blockIdx.y, tid = threadIdx.x;
@@ -196,6 +196,41 @@ __global__ __launch_bounds__(1024) void k_horner(const uint32_t* __restrict__ pa
     const int tlimb = (int)(team & 1);
     const Tw tw = tw_lds(twl, tlimb);
     const uint32_t inv_n = nc.inv_n[tlimb], inv_n_p = nc.inv_n_p[tlimb];
+    if (fp.interm) {
+        // this chain's packed b-parts of groups [lo, hi): team k%4 takes
+        // (group lo + k/2, limb k%2), the limb of its own twiddles (as
+        // k_pack_finalize_wg); written to packed_b and read back below
+        // after the block's barrier
+        const uint32_t npoly = 2 * (hi - lo), rounds = (npoly + 3) / 4;
+        uint32_t* b = buf[team];
+        for (uint32_t r = 0; r < rounds; r++) {  // every lane runs every round (barriers inside)
+            const uint32_t k = 4 * r + team;
+            const bool active = k < npoly;
+            const uint32_t g = lo + (active ? k >> 1 : 0), c = k & 1;
+            __syncthreads();  // twiddles staged, the previous round's reads done
+            if (active) {
+                const uint32_t* src = fp.interm + (size_t)qi * 2 * fp.C + (size_t)c * fp.C + (size_t)g * N;
+#pragma unroll
+                for (int e = 0; e < 8; e++) b[ls(t + 256 * e)] = src[t + 256 * e];
+            }
+            __syncthreads();
+            ntt_fwd_lds(b, t, tw, (int)c, active);
+            if (active) {
+                const uint32_t q = qof((int)c);
+                const uint32_t* Df = fp.precomp + (size_t)g * PC_GROUP + PC_DF + (size_t)c * D_EFF * N;
+                const uint32_t* Kn = fp.keys + qi * fp.keys_stride + BLK + (size_t)c * N;
+                const uint32_t* p2 = fp.part2 + qi * fp.part2_stride + (size_t)g * 2 * N + (size_t)c * N;
+#pragma unroll 2
+                for (int e = 0; e < 8; e++) {
+                    const uint32_t i = t + 256 * e;
+                    const uint64_t sub = (uint64_t)Df[i] * Kn[i] + (uint64_t)Df[N + i] * Kn[2 * N + i] + p2[i];
+                    packed_b[qi * pstride + (size_t)g * 2 * N + (size_t)c * N + i] =
+                        sub_mod(b[ls(i)], red64(sub, (int)c), q);
+                }
+            }
+        }
+        __syncthreads();  // the block's packed parts are in place
+    }
     const uint32_t* pb = packed_b + qi * pstride;
     const uint32_t* rg = rgsw + qi * rstride;
     uint32_t* hs = hstate + ((size_t)qi * T + ct) * 4 * N;  // chain state between launches
@@ -382,13 +417,13 @@ hipError_t launch_pack_finalize_wg(const uint32_t* interm, size_t C, const uint3
 }
 
 hipError_t launch_horner(const uint32_t* pa_base, size_t pa_stride, size_t G, size_t Da, size_t T,
-                         const uint32_t* packed_b, size_t packed_stride, const uint32_t* rgsw, size_t rgsw_stride,
+                         uint32_t* packed_b, size_t packed_stride, const uint32_t* rgsw, size_t rgsw_stride,
                          uint64_t* out, uint32_t* hstate, size_t g_lo, size_t g_hi, size_t nq, const NttConst& nc,
-                         const CrtConst& cc, hipStream_t s) {
+                         const CrtConst& cc, hipStream_t s, const FusedPack& pack) {
     if (nq == 0 || g_lo >= g_hi) return hipSuccess;
     hipLaunchKernelGGL(k_horner, dim3((unsigned)T, (unsigned)nq), dim3(1024), 0, s, pa_base, pa_stride, (uint32_t)G,
                        (uint32_t)Da, (uint32_t)T, packed_b, packed_stride, rgsw, rgsw_stride, out, hstate,
-                       (uint32_t)g_lo, (uint32_t)g_hi, nc, cc);
+                       (uint32_t)g_lo, (uint32_t)g_hi, nc, cc, pack);
     return hipGetLastError();
 }
 

@@ -397,17 +397,19 @@ void GpuServerCtx::alloc_slots() {
     HIPX(hipEventCreateWithFlags(&order_ev, hipEventDisableTiming));
     // INSPIRE_PACK_WG=0/1/-1: the pipelined chunks' pack kernel, a grid over
     // (group, limb), one workgroup per query, or one workgroup for one query
-    if (const char* pw = std::getenv("INSPIRE_PACK_WG")) pack_wg = pw[0] == '1' ? 1 : pw[0] == '0' ? 0 : -1;
+    if (const char* pw = std::getenv("INSPIRE_PACK_WG"))
+        pack_wg = pw[0] == '1' ? 1 : pw[0] == '0' ? 0 : pw[0] == '2' ? 2 : -1;
     // INSPIRE_GRAPH=1: answers replay captured hipGraphs
     if (const char* gm = std::getenv("INSPIRE_GRAPH")) graph_mode = gm[0] == '1' ? 1 : 0;
     if (const char* h = std::getenv("INSPIRE_HORNER"))
         horner_mode = std::strcmp(h, "pair") == 0 ? 1 : std::strcmp(h, "single") == 0 ? 0 : -1;
     // INSPIRE_PIPELINE=<single>,<batched>: the pipeline shape (0 serial, 2
-    // two streams, 3 three streams) of one-query and many-query calls
+    // two streams, 3 three streams, 4 the collapse first, 5 three streams
+    // with the collapse a chunk ahead) of one-query and many-query calls
     if (const char* pl = std::getenv("INSPIRE_PIPELINE")) {
         int a1 = -1, a2 = -1;
-        if (std::sscanf(pl, "%d,%d", &a1, &a2) == 2 && (a1 == 0 || a1 == 2 || a1 == 3) &&
-            (a2 == 0 || a2 == 2 || a2 == 3)) {
+        auto ok = [](int v) { return v == 0 || (v >= 2 && v <= 5); };
+        if (std::sscanf(pl, "%d,%d", &a1, &a2) == 2 && ok(a1) && ok(a2)) {
             pipe_single = a1;
             pipe_batched = a2;
             pipe_forced = true;
@@ -585,7 +587,9 @@ void GpuServerCtx::autotune_dbpass() {
     // largest batch, with the Horner chains beside the passes as when
     // serving. A chunk's database pass runs the full-width plan's kernel over
     // the chunk's columns with a row-slice count (0: the narrowing rule); the
-    // shape puts the collapse beside it (3) or after it (2). The two interact:
+    // shape puts the collapse beside it (3), or after it (2); the forms with
+    // the collapse a chunk ahead (5) or all of it first (4) are timed only
+    // when forced. The two interact:
     // a one-slice grid needs no reduction kernel and leaves CUs that only a
     // collapse beside it uses; more slices fill the device alone but pay
     // partials and their reduction.
@@ -608,7 +612,9 @@ void GpuServerCtx::autotune_dbpass() {
                                                                 : dbplan_mfma(R, ccols, 1, t);
                     if (x.slices != sl || (sl > 1 && sl * 2 * ccols > part_stride)) continue;
                 }
-                for (const int shape : {2, 3}) {
+                // shapes 4 and 5 never won at 16 GB and take setup time: forced only
+                for (const int shape : {2, 3, 4, 5}) {
+                    if (!pipe_forced && shape > 3) continue;
                     if (pipe_forced && shape != (count == 1 ? pipe_single : pipe_batched)) continue;
                     cands.push_back({sl, shape});
                 }
@@ -691,6 +697,21 @@ DbPlan GpuServerCtx::chunk_plan(size_t nq, size_t ncols) const {
     return narrow_plan(full, nq == 1 ? plan1_target : mfma_target, pp.db_rows, pp.db_cols, ncols, nq);
 }
 
+DbPlan GpuServerCtx::chunk_plan_at(size_t nq, size_t k, size_t ng, const DbPlan& whole) const {
+    const uint32_t from = nq == 1 ? late_from1 : late_from, sl = nq == 1 ? late_slices1 : late_slices;
+    if (sl && k >= from) {
+        // the chunk's columns at sl row slices per column block (a narrower
+        // chunk multiplies them to keep a whole chunk's workgroup count)
+        const size_t ncols = ng * N, t = gchunk * N / dbplan_cols(whole) * sl;
+        const DbPlan x = whole.kind == DbPlan::F64
+                             ? dbplan_f64(pp.db_rows, ncols, (int)whole.param, t)
+                             : dbplan_mfma(pp.db_rows, ncols, nq, t, whole.depth, whole.dbtile, whole.waves, whole.crt,
+                                           whole.pow);
+        if (x.param == whole.param && (x.slices == 1 || x.slices * 2 * ncols <= part_stride)) return x;
+    }
+    return ng == gchunk ? whole : chunk_plan(nq, ng * N);
+}
+
 void GpuServerCtx::dbpass_prep(size_t q0, size_t nq, const DbPlan& p, hipStream_t s, int lane) {
     if (p.kind == DbPlan::MFMA)
         HIPX(launch_mfma_digits(d_bq + q0 * 2 * pp.db_rows, nq, pp.db_rows, p, d_dig[lane], d_S[lane], s));
@@ -758,7 +779,7 @@ bool GpuServerCtx::use_pair(size_t chains, bool beside_passes, bool status_read)
 }
 
 void GpuServerCtx::horner_range(size_t count, uint64_t* d_out, size_t g_lo, size_t g_hi, hipStream_t s,
-                                size_t slot0, int lane, bool status_read, bool beside_passes) {
+                                size_t slot0, int lane, bool status_read, bool beside_passes, const FusedPack* fp) {
     if (use_pair(count * T, beside_passes, status_read))
         HIPX(launch_horner_pair(d_precomp + PC_A, PC_GROUP, G, Da, T, d_packed_b + slot0 * packed_stride,
                                 packed_stride, d_rgsw + slot0 * rgsw_stride, rgsw_stride, d_out,
@@ -768,7 +789,7 @@ void GpuServerCtx::horner_range(size_t count, uint64_t* d_out, size_t g_lo, size
     else
         HIPX(launch_horner(d_precomp + PC_A, PC_GROUP, G, Da, T, d_packed_b + slot0 * packed_stride, packed_stride,
                            d_rgsw + slot0 * rgsw_stride, rgsw_stride, d_out, d_hstate + slot0 * T * 4 * N, g_lo, g_hi,
-                           count, nc, cc, s));
+                           count, nc, cc, s, fp ? *fp : FusedPack{}));
 }
 
 void GpuServerCtx::stage_horner(size_t count, uint64_t* d_out, hipStream_t s, size_t slot0, bool status_read) {
@@ -809,8 +830,16 @@ void GpuServerCtx::enqueue(const uint64_t* d_in, size_t count, uint64_t* d_out, 
             continue;
         }
         // shape 2 keeps the collapse on s (two HBM-bound passes side by side
-        // share the bandwidth worse than back to back); shape 3 runs it beside
-        const hipStream_t cs = shape == 3 ? cstream[lane] : s;
+        // share the bandwidth worse than back to back); shape 3 runs it
+        // beside; shape 4 runs it for every group on s before the first
+        // database-pass chunk, so the chunks then have the device to
+        // themselves (and the Horner stream) at their full grid
+        // shape 5 is shape 3 with the first database-pass chunk waiting for
+        // the first collapse chunk: the collapse stream then stays a chunk
+        // ahead, so each chunk's pack and Horner steps wait for its database
+        // pass only, and the last ones follow the last pass at once
+        const hipStream_t cs = shape == 3 || shape == 5 ? cstream[lane] : s;
+        const bool first = shape == 4, ahead = shape == 5;
         joined = true;
         const DbPlan p = chunk_plan(nq, gchunk * N);  // every chunk's plan has this one's kernel and digits
         if (cs != s) {
@@ -818,27 +847,51 @@ void GpuServerCtx::enqueue(const uint64_t* d_in, size_t count, uint64_t* d_out, 
             HIPX(hipStreamWaitEvent(cs, prep_ev[lane], 0));
         }
         HIPX(launch_collapse_keys(keys, keys_stride, nq, d_kdig[lane], d_KS[lane], cs));
+        if (first) {
+            TraceRange trc("collapse of all %zu groups", G);
+            HIPX(launch_collapse(d_precomp, G, d_kdig[lane], d_KS[lane], nq, p2, part2_stride, s, collapse_variant));
+        }
         dbpass_prep(q0, nq, p, s, lane);
+        auto collapse_chunk = [&](size_t k) {
+            const size_t g_lo = chunks[k].first, ng = chunks[k].second - g_lo;
+            HIPX(launch_collapse(d_precomp + g_lo * PC_GROUP, ng, d_kdig[lane], d_KS[lane], nq, p2 + g_lo * 2 * N,
+                                 part2_stride, cs, collapse_variant));
+            HIPX(hipEventRecord(coll_ev[lane][k % kChunkEvents], cs));
+        };
+        if (ahead) {
+            collapse_chunk(0);
+            HIPX(hipStreamWaitEvent(s, coll_ev[lane][0], 0));
+        }
         for (size_t k = 0; k < chunks.size(); k++) {
             const size_t g_lo = chunks[k].first, g_hi = chunks[k].second, ng = g_hi - g_lo;
             TraceRange trk("chunk %zu: groups %zu..%zu (DB pass, collapse, pack, Horner)", k, g_lo, g_hi - 1);
             hipEvent_t dbe = chunk_ev[lane][k % kChunkEvents], ce = coll_ev[lane][k % kChunkEvents];
-            dbpass_range(q0, nq, g_lo, g_hi, ng == gchunk ? p : chunk_plan(nq, ng * N), s, lane);
+            dbpass_range(q0, nq, g_lo, g_hi, chunk_plan_at(nq, k, ng, p), s, lane);
             // with the collapse on s its event covers the DB pass too (an
             // event record costs the stream a few microseconds of idle)
             if (cs != s) HIPX(hipEventRecord(dbe, s));
-            HIPX(launch_collapse(d_precomp + g_lo * PC_GROUP, ng, d_kdig[lane], d_KS[lane], nq,
-                                 p2 + g_lo * 2 * N, part2_stride, cs, collapse_variant));
-            HIPX(hipEventRecord(ce, cs));
+            if (ahead) {
+                if (k + 1 < chunks.size()) collapse_chunk(k + 1);  // ce of chunk k was recorded a chunk earlier
+            } else {
+                if (!first)
+                    HIPX(launch_collapse(d_precomp + g_lo * PC_GROUP, ng, d_kdig[lane], d_KS[lane], nq,
+                                         p2 + g_lo * 2 * N, part2_stride, cs, collapse_variant));
+                HIPX(hipEventRecord(ce, cs));
+            }
             if (cs != s) HIPX(hipStreamWaitEvent(hs, dbe, 0));
             HIPX(hipStreamWaitEvent(hs, ce, 0));
-            if (nq == 1 ? pack_wg != 0 : pack_wg == 1)
+            // the pack: fused into the one-workgroup Horner chains, or a kernel of its own
+            const bool fuse = pack_wg == 2 && !use_pair(nq * T, true, status_read);
+            const FusedPack fp{d_interm + q0 * 2 * C, C, p2, part2_stride, d_precomp, keys, keys_stride};
+            if (fuse)
+                ;
+            else if (nq == 1 ? pack_wg != 0 : pack_wg == 1)
                 HIPX(launch_pack_finalize_wg(d_interm + q0 * 2 * C, C, p2, part2_stride, d_precomp, g_lo, ng, keys,
                                              keys_stride, packed, packed_stride, nq, nc, hs));
             else
                 HIPX(launch_pack_finalize(d_interm + q0 * 2 * C, C, p2, part2_stride, d_precomp, g_lo, ng, keys,
                                           keys_stride, packed, packed_stride, nq, nc, hs));
-            horner_range(nq, out, g_lo, g_hi, hs, q0, lane, status_read, true);
+            horner_range(nq, out, g_lo, g_hi, hs, q0, lane, status_read, true, fuse ? &fp : nullptr);
         }
         if (cs != s) {  // the next batch's key digits overwrite the lane's scratch
             HIPX(hipEventRecord(prep_ev[lane], cs));

@@ -51,6 +51,12 @@ struct GpuServerCtx {
     // chunk_param digit tiles (0: the narrowing rule)
     uint32_t chunk_slices = 0, chunk_param = 0;
     uint32_t chunk_slices1 = 0;     // the same for one-query answers (plan1's kind)
+    // chunks from index late_from on (the collapse beside them has finished)
+    // take late_slices row slices instead (0: as the chunks before them);
+    // one-query answers use the *1 pair
+    uint32_t late_from = 0, late_slices = 0, late_from1 = 0, late_slices1 = 0;
+    // The plan of a chunk of ng groups at index k of the pipelined answer.
+    ipx::DbPlan chunk_plan_at(size_t nq, size_t k, size_t ng, const ipx::DbPlan& whole) const;
     int collapse_variant = 0;       // 17..32-query collapse pass form (launch_collapse)
     size_t gchunk = 0;              // groups per chunk of the pipelined answer
     // the pipelined answer's chunks [lo, hi), highest groups first: chunks of
@@ -63,7 +69,8 @@ struct GpuServerCtx {
     void set_chunks();
     // pipeline shape of one-query and many-query calls: 0 serial on the
     // caller's stream, 2 Horner chains beside the passes, 3 collapse beside
-    // too (2 or 3 timed at setup)
+    // too, 4 the collapse of all groups before the database-pass chunks, 5
+    // collapse beside and a chunk ahead (2 or 3 timed at setup)
     int pipe_single = 3, pipe_batched = 2;
     bool pipe_forced = false;       // INSPIRE_PIPELINE set the shapes (not autotuned)
     size_t qlen = 0, resp_u64s = 0;
@@ -100,7 +107,8 @@ struct GpuServerCtx {
     int horner_mode = -1;
     static constexpr size_t kPairMaxChains = 2;
     // pack kernel of pipelined chunks: 1 one workgroup per query, 0 a grid
-    // over (group, limb), -1 one workgroup for one-query calls
+    // over (group, limb), -1 one workgroup for one-query calls, 2 fused into
+    // the Horner chains' own workgroups (where the one-workgroup Horner runs)
     int pack_wg = 0;
     uint32_t* d_hxbuf = nullptr;    // pair exchange buffers [B*T chains]
     uint32_t* d_hflags = nullptr;   // pair step flags [B*T][2]
@@ -174,8 +182,9 @@ struct GpuServerCtx {
     // the call (status_read): its bounded wait reports through it.
     bool use_pair(size_t chains, bool beside_passes, bool status_read) const;
     // Horner chains of `count` queries over groups [g_lo, g_hi).
+    // fp: the chains compute their groups' packed b-parts themselves first
     void horner_range(size_t count, uint64_t* d_out, size_t g_lo, size_t g_hi, hipStream_t s, size_t slot0,
-                      int lane, bool status_read, bool beside_passes = false);
+                      int lane, bool status_read, bool beside_passes = false, const ipx::FusedPack* fp = nullptr);
     void stage_collapse(size_t count, hipStream_t s, size_t slot0, int lane = 0);
     void stage_finalize(size_t count, hipStream_t s, size_t slot0);
     void stage_horner(size_t count, uint64_t* d_out, hipStream_t s, size_t slot0, bool status_read);

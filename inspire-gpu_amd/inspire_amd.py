@@ -119,6 +119,7 @@ SERVER_FUNCS = [
     ("ipir_dbg_set_group_chunk", C.c_int, [VP, SZ]),
     ("ipir_dbg_set_mfma_depth", C.c_int, [VP, C.c_uint32, C.c_int]),
     ("ipir_dbg_set_chunk_slices", C.c_int, [VP, SZ, C.c_uint32]),
+    ("ipir_dbg_set_late_slices", C.c_int, [VP, SZ, SZ, C.c_uint32]),
     ("ipir_dbg_set_collapse_variant", C.c_int, [VP, C.c_int]),
     ("ipir_dbg_get_collapse_variant", C.c_int, [VP]),
     ("ipir_dbg_set_pipeline", C.c_int, [VP, C.c_int, C.c_int]),

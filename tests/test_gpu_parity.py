@@ -280,7 +280,7 @@ def test_ragged_batch_over_sub_batches(medium8):
         for groups, split in ((2, 1), (4, 1), (4, 0), (4, 2), (4, 3)):
             assert srv.lib.ipir_dbg_set_group_chunk(srv.h, groups) == 0
             assert srv.lib.ipir_dbg_set_tail_split(srv.h, split) == 0
-            for shapes in ((3, 2), (2, 3), (3, 3)):
+            for shapes in ((3, 2), (2, 3), (3, 3), (4, 4), (4, 2), (5, 5), (3, 5)):
                 assert srv.lib.ipir_dbg_set_pipeline(srv.h, *shapes) == 0
                 got = srv.answer_batch(qs)
                 assert np.array_equal(got, serial), f"groups {groups} split {split} shapes {shapes}"
@@ -343,10 +343,13 @@ def test_pair_horner_answers_equal_single(medium8):
 
 
 @pytest.mark.parametrize("groups,shape,slices", [(1, 2, 0), (2, 2, 0), (4, 2, 0), (1, 3, 0), (2, 3, 0),
-                                                   (2, 2, 1), (4, 2, 2)])
+                                                   (2, 2, 1), (4, 2, 2), (1, 4, 0), (2, 4, 0), (2, 4, 2),
+                                                   (1, 5, 0), (2, 5, 1)])
 def test_pipelined_group_chunks_equal_unchunked(medium8, groups, shape, slices):
     """The answer pipelined over group chunks (Horner chains on a second
-    stream, resuming between launches; shape 3 also the collapse on a third)
+    stream, resuming between launches; shape 3 also the collapse on a third,
+    5 with that stream a chunk ahead; shape 4 the collapse of all groups
+    before the chunks)
     equals the serial one-chunk answer, for both Horner kernels and both
     pack kernels, one query and three; chunk database passes with the
     narrowing rule's row slices or a set count (one: straight into the
@@ -365,11 +368,14 @@ def test_pipelined_group_chunks_equal_unchunked(medium8, groups, shape, slices):
         got1 = srv.answer_batch(qs[:1])
         # the chunks' one-workgroup pack kernel: for three queries, and for
         # one query alone
-        assert srv.lib.ipir_dbg_set_pack(srv.h, 2) == 1
+        assert srv.lib.ipir_dbg_set_pack(srv.h, 3) == 1
         assert srv.lib.ipir_dbg_set_pack(srv.h, 1) == 0
         got_wg = srv.answer_batch(qs)
         assert srv.lib.ipir_dbg_set_pack(srv.h, -1) == 0
         got1_wg = srv.answer_batch(qs[:1])  # -1: one workgroup for one query
+        assert srv.lib.ipir_dbg_set_pack(srv.h, 2) == 0  # fused into the Horner chains
+        got_fused = srv.answer_batch(qs)
+        got1_fused = srv.answer_batch(qs[:1])
         assert srv.lib.ipir_dbg_set_pack(srv.h, 0) == 0
         assert srv.lib.ipir_dbg_set_horner(srv.h, 1) == 0
         pair = srv.answer_batch(qs)
@@ -382,6 +388,7 @@ def test_pipelined_group_chunks_equal_unchunked(medium8, groups, shape, slices):
         assert srv.lib.ipir_dbg_set_group_chunk(srv.h, geo["G"]) == 0
     assert np.array_equal(got, base) and np.array_equal(got1[0], one[0]) and np.array_equal(pair, base)
     assert np.array_equal(got_wg, base) and np.array_equal(got1_wg[0], one[0])
+    assert np.array_equal(got_fused, base) and np.array_equal(got1_fused[0], one[0])
 
 
 def test_packed_wire_answers_match_host_codecs(medium8):
@@ -588,7 +595,7 @@ def test_device_resident_entry_matches_host_path(medium8):
     dout.free()
 
 
-@pytest.mark.parametrize("groups,shape", [(8, 0), (2, 2), (2, 3)])
+@pytest.mark.parametrize("groups,shape", [(8, 0), (2, 2), (2, 3), (2, 4), (2, 5)])
 def test_graph_replays_equal_eager(medium8, groups, shape):
     """Answers replayed from captured hipGraphs (serial, and pipelined over
     group chunks on two and three streams) equal the launch-by-launch ones,
