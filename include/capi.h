@@ -181,7 +181,8 @@ typedef struct {
     uint64_t max_batch_size;       /* most requests admitted into one batch */
     uint64_t max_busy_partitions;  /* most partitions holding admitted requests at once */
     uint64_t admitted_while_busy;  /* batches admitted while another partition held requests */
-    uint64_t device_overlaps;      /* batches launched while an earlier batch's work was still on the device */
+    uint64_t device_overlaps;      /* batches whose device work (first to last command, by GPU timestamps)
+                                      overlapped another partition's batch */
     uint64_t pending;              /* requests waiting for a free partition now */
     uint64_t busy_partitions;      /* partitions holding admitted requests now */
 } ipir_loop_stats;

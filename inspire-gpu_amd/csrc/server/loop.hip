@@ -46,6 +46,11 @@ struct Partition {
     size_t slot0 = 0, nslots = 0;
     int lane = 0;  // scratch lane (and status word) of the context this partition uses
     hipStream_t stream = nullptr;
+    // device timestamps of the batch in flight (its first and last command),
+    // and [start, end) in ms after the loop's reference event of the last
+    // completed batch
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    float last0 = -1, last1 = -1;
     std::vector<Req*> batch;  // admitted requests; partition busy when non-empty
     bool launched = false;    // batch's device work fully enqueued
     std::thread completer;
@@ -65,6 +70,7 @@ struct ipir_loop {
     std::thread dispatcher;
     bool stop = false;
     bool hold = false;  // completers keep finished partitions busy (inspection)
+    hipEvent_t ev_ref = nullptr;  // time origin of the batches' device intervals
     uint64_t next_ticket = 1;
     ipir_loop_stats st{};
 
@@ -98,27 +104,22 @@ struct ipir_loop {
             st.batches++;
             for (size_t i = 0; i < n; i++)
                 std::memcpy(ctx->h_q + (free_p->slot0 + i) * ctx->qlen, free_p->batch[i]->q.data(), ctx->qlen * 8);
-            // partitions whose device work was launched before this batch
-            std::vector<hipStream_t> others;
-            for (auto* p : parts)
-                if (p != free_p && p->launched) others.push_back(p->stream);
             lk.unlock();
             bool ok = true;
             try {
                 const size_t s0 = free_p->slot0;
+                hip_check(hipEventRecord(free_p->ev0, free_p->stream), "loop event");
                 hip_check(hipMemcpyAsync(ctx->d_q + s0 * ctx->qlen, ctx->h_q + s0 * ctx->qlen, n * ctx->qlen * 8,
                                          hipMemcpyHostToDevice, free_p->stream), "loop h2d");
                 ctx->enqueue(ctx->d_q + s0 * ctx->qlen, n, ctx->d_resp + s0 * ctx->resp_u64s, free_p->stream, s0,
                              free_p->lane, true);
                 hip_check(hipMemcpyAsync(ctx->h_resp + s0 * ctx->resp_u64s, ctx->d_resp + s0 * ctx->resp_u64s,
                                          n * ctx->resp_u64s * 8, hipMemcpyDeviceToHost, free_p->stream), "loop d2h");
+                hip_check(hipEventRecord(free_p->ev1, free_p->stream), "loop event");
             } catch (...) {
                 ok = false;
             }
-            bool overlap = false;
-            for (hipStream_t o : others) overlap |= hipStreamQuery(o) == hipErrorNotReady;
             lk.lock();
-            if (overlap) st.device_overlaps++;
             if (!ok)
                 for (Req* r : free_p->batch) r->failed = true;
             free_p->launched = true;
@@ -140,7 +141,23 @@ struct ipir_loop {
                     ok = false;
                 }
             }
+            float t0 = -1, t1 = -1;
+            if (ok && hipEventElapsedTime(&t0, ev_ref, p->ev0) == hipSuccess &&
+                hipEventElapsedTime(&t1, ev_ref, p->ev1) == hipSuccess) {
+            } else {
+                t0 = t1 = -1;
+            }
             lk.lock();
+            if (t0 >= 0) {
+                // on the device beside another partition's batch: the two
+                // batches' [first command, last command] intervals intersect
+                bool beside = false;
+                for (auto* o : parts)
+                    if (o != p && o->last1 > t0 && o->last0 < t1) beside = true;
+                if (beside) st.device_overlaps++;
+                p->last0 = t0;
+                p->last1 = t1;
+            }
             p->cv.wait(lk, [&] { return !hold; });
             const auto now = Clock::now();
             for (size_t i = 0; i < p->batch.size(); i++) {
@@ -192,13 +209,27 @@ ipir_loop* ipir_loop_start(ipir_server* srv, size_t max_inflight) {
         p->slot0 = i * per;
         p->lane = (int)i;
         p->nslots = per;
-        if (hipStreamCreate(&p->stream) != hipSuccess) {
+        if (hipStreamCreate(&p->stream) != hipSuccess || hipEventCreate(&p->ev0) != hipSuccess ||
+            hipEventCreate(&p->ev1) != hipSuccess) {
+            if (p->ev0) (void)hipEventDestroy(p->ev0);
+            if (p->stream) (void)hipStreamDestroy(p->stream);
             delete p;
             break;
         }
         L->parts.push_back(p);
     }
+    if (!L->parts.empty() &&
+        (hipEventCreate(&L->ev_ref) != hipSuccess || hipEventRecord(L->ev_ref, L->parts[0]->stream) != hipSuccess)) {
+        for (auto* p : L->parts) {
+            (void)hipEventDestroy(p->ev0);
+            (void)hipEventDestroy(p->ev1);
+            (void)hipStreamDestroy(p->stream);
+            delete p;
+        }
+        L->parts.clear();
+    }
     if (L->parts.empty()) {
+        if (L->ev_ref) (void)hipEventDestroy(L->ev_ref);
         delete L;
         std::lock_guard<std::mutex> lk(ctx->mu);
         ctx->loop_active = false;
@@ -301,9 +332,12 @@ void ipir_loop_stop(ipir_loop* L) {
     for (auto* p : L->parts) {
         p->completer.join();
         (void)hipStreamSynchronize(p->stream);  // every completer synchronised already
+        (void)hipEventDestroy(p->ev0);
+        (void)hipEventDestroy(p->ev1);
         (void)hipStreamDestroy(p->stream);
         delete p;
     }
+    (void)hipEventDestroy(L->ev_ref);
     for (auto& kv : L->reqs) delete kv.second;
     {
         std::lock_guard<std::mutex> lk(ctx->mu);

@@ -391,6 +391,31 @@ def test_pipelined_group_chunks_equal_unchunked(medium8, groups, shape, slices):
     assert np.array_equal(got_fused, base) and np.array_equal(got1_fused[0], one[0])
 
 
+def test_late_chunk_slices_equal_unchunked(medium8):
+    """Chunks from a given index on with their own row-slice count (one
+    query and three, both pipeline shapes with the collapse beside) give the
+    serial answer."""
+    geo, db, srv = medium8
+    qs = [ol.make_query(geo, SEED, idx, 850 + k)[0] for k, idx in enumerate([11, 262144, 524000])]
+    base = srv.answer_batch(qs)
+    one = srv.answer_batch(qs[:1])
+    assert srv.lib.ipir_dbg_set_late_slices(srv.h, 3, 1, 3) == 1  # not a power of two
+    assert srv.lib.ipir_dbg_set_group_chunk(srv.h, 2) == 0
+    try:
+        for shape in (3, 2):
+            assert srv.lib.ipir_dbg_set_pipeline(srv.h, shape, shape) == 0
+            for frm, sl in ((1, 2), (2, 4), (0, 1)):
+                assert srv.lib.ipir_dbg_set_late_slices(srv.h, 3, frm, sl) == 0
+                assert srv.lib.ipir_dbg_set_late_slices(srv.h, 1, frm, sl) == 0
+                assert np.array_equal(srv.answer_batch(qs), base), f"shape {shape} late {frm},{sl}"
+                assert np.array_equal(srv.answer_batch(qs[:1])[0], one[0]), f"one query, shape {shape} late {frm},{sl}"
+    finally:
+        assert srv.lib.ipir_dbg_set_late_slices(srv.h, 3, 0, 0) == 0
+        assert srv.lib.ipir_dbg_set_late_slices(srv.h, 1, 0, 0) == 0
+        assert srv.lib.ipir_dbg_set_pipeline(srv.h, 3, 2) == 0
+        assert srv.lib.ipir_dbg_set_group_chunk(srv.h, geo["G"]) == 0
+
+
 def test_packed_wire_answers_match_host_codecs(medium8):
     """Queries in the 53-bit packed wire form, unpacked on the device, and
     responses compressed on the device give the bytes of the client
