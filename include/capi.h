@@ -115,7 +115,11 @@ ipir_client_query* ipir_query_build_seeded(const ipir_params* params, uint64_t i
  * report nothing afterwards). The work waits for the server's previous
  * call if that one was on another stream; inside a caller's stream capture
  * the caller orders the captured work itself. For callers that keep
- * queries resident in HBM. */
+ * queries resident in HBM. Answers of the device's streaming server may
+ * run their Horner chains beside a database pass they wait for inside the
+ * kernel; such a wait is bounded (about a second or more), and a chain
+ * that gives up writes all-ones words as its response (no residue has
+ * that value; it can happen only if the pass gets no CU for that long). */
 int ipir_answer_batch_device(ipir_server* srv, const uint64_t* d_queries, size_t count,
                              uint64_t* d_out_resp, void* hip_stream);
 
@@ -129,7 +133,8 @@ int ipir_answer_batch_device(ipir_server* srv, const uint64_t* d_queries, size_t
 int ipir_answer_batch_packed(ipir_server* srv, const uint8_t* packed_queries, size_t count,
                              uint8_t* out_compressed);
 /* The same over DEVICE buffers, asynchronous on the given HIP stream (NULL
- * = the server's own stream), like ipir_answer_batch_device. */
+ * = the server's own stream), like ipir_answer_batch_device (a chain that
+ * gives up its bounded wait leaves its compressed response undefined). */
 int ipir_answer_batch_packed_device(ipir_server* srv, const uint8_t* d_packed_queries, size_t count,
                                     uint8_t* d_out_compressed, void* hip_stream);
 

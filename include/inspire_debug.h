@@ -102,6 +102,10 @@ int ipir_dbg_set_horner(ipir_server* srv, int mode);
  * exchange and stop (all kernels still end normally): the call that ran it
  * then returns 4, and later calls are unaffected. 5 while a loop runs. */
 int ipir_dbg_set_horner_spin_limit(ipir_server* srv, uint32_t limit);
+/* The same bound for Horner chains waiting for a streamed database pass's
+ * groups (default 2^23; 0: give up at once and report, as above; the pass
+ * itself runs to its end). */
+int ipir_dbg_set_stream_spin_limit(ipir_server* srv, uint32_t limit);
 /* Keep a serving loop's partitions busy after their device work finished
  * (on = 1) until released (on = 0): requests admitted meanwhile go to the
  * other partitions or wait. For tests of admission while batches are in
@@ -154,10 +158,38 @@ int ipir_dbg_get_collapse_variant(const ipir_server* srv);
  * Horner chains on a second stream per group chunk, 3 the collapse on a
  * third stream as well, 4 the collapse of all groups first and then the
  * database-pass chunks with the Horner chains beside them, 5 as 3 with the
- * collapse stream a chunk ahead of the database pass (the setup autotuner
- * picks 2 or 3; 4 and 5 are tested alternatives that never won at 16 GB);
- * 1 for other values. */
+ * collapse stream a chunk ahead of the database pass; streamed: one
+ * full-width database pass that hands each group over as its columns are
+ * final, the Horner chains waiting for the groups inside the kernel (a
+ * bounded wait, reported like the pair kernel's), 6 after the collapse of
+ * all groups, 7 with the collapse beside it. The setup autotuner picks 2,
+ * 3, 6 or 7 (4 and 5 are tested alternatives that never won at 16 GB).
+ * Answers under stream capture, and every answer of a server that is not
+ * the device's first live one, take the chunked shape the autotuner picked
+ * instead of a streamed one. 1 for other values. */
 int ipir_dbg_set_pipeline(ipir_server* srv, int single, int batched);
+/* Row slices (a power of two <= 16) of the streamed pass of one-query
+ * (nq_class 1) or many-query (other values) answers; raised to what the
+ * exactness bounds need, and ignored (chunked shape) where the partials
+ * would not fit their buffer. */
+int ipir_dbg_set_stream_slices(ipir_server* srv, size_t nq_class, uint32_t slices);
+/* 1 if this server may take the streamed shapes (the device's first live
+ * server), else 0. */
+int ipir_dbg_stream_ok(const ipir_server* srv);
+/* Make srv the device's streaming server; the one that was (if any) then
+ * answers with its chunked shapes. Its own mutex is taken: do not call
+ * this while that server is inside another call on another thread that
+ * itself waits for srv. */
+int ipir_dbg_take_streaming(ipir_server* srv);
+/* Timings (ms, mean of 5 after a warm-up) of the streamed pieces for nq
+ * zero queries at `slices` row slices: out[0] the streamed database pass
+ * alone, out[1] the same plan on its 2-D grid with the slice reduction,
+ * out[2] the streamed pack alone with every group handed over, out[3] the
+ * streamed Horner chains alone with every group published, out[4] the
+ * plain chains over every group (5 doubles). */
+int ipir_dbg_stream_parts(ipir_server* srv, size_t nq, uint32_t slices, double* out_ms);
+/* Streamed 32-query sub-batches this server has enqueued (a test counter). */
+size_t ipir_dbg_streamed_count(const ipir_server* srv);
 /* Per-stage device time (ms) of the last ipir_dbg_profile_batch run:
  * [unmarshal+key NTT, DB pass, collapse pass, pack finalize, Horner]. */
 int ipir_dbg_profile_batch(ipir_server* srv, const uint64_t* d_queries, size_t count, uint64_t* d_out,

@@ -350,6 +350,15 @@ int ipir_dbg_tuning(const ipir_server* srv, char* out, size_t cap) {
     }
     s += "; collapse variant " + std::to_string(c->collapse_variant) + "; pipeline shapes " +
          std::to_string(c->pipe_single) + "/" + std::to_string(c->pipe_batched);
+    if (c->pipe_single >= 6 || c->pipe_batched >= 6) {
+        auto sp = [&](size_t n, uint32_t sl) {
+            const ipx::DbPlan p = c->stream_plan(n, sl);
+            return p.kind == ipx::DbPlan::MFMA ? plan(p) : std::string("none");
+        };
+        s += " (streamed: batch1 " + sp(1, c->stream_slices1) + ", batch" + std::to_string(nq) + " " +
+             sp(nq, c->stream_slices) + "; chunked fallback " + std::to_string(c->pipe_single_chunked) + "/" +
+             std::to_string(c->pipe_batched_chunked) + ")";
+    }
     std::snprintf(out, cap, "%s", s.c_str());
     return 0;
 }
@@ -570,8 +579,125 @@ int ipir_dbg_set_mfma_depth(ipir_server* srv, uint32_t depth, int form) {
     return 1;
 }
 
+int ipir_dbg_set_stream_slices(ipir_server* srv, size_t nq_class, uint32_t slices) {
+    if (!srv || slices == 0 || slices > 16 || (slices & (slices - 1))) return 1;
+    std::lock_guard<std::mutex> lk(srv->ctx->mu);
+    if (srv->ctx->loop_active) return 5;
+    (nq_class == 1 ? srv->ctx->stream_slices1 : srv->ctx->stream_slices) = slices;
+    srv->ctx->drop_graphs();
+    return 0;
+}
+
+int ipir_dbg_stream_ok(const ipir_server* srv) { return srv && srv->ctx->stream_ok ? 1 : 0; }
+
+int ipir_dbg_stream_parts(ipir_server* srv, size_t nq, uint32_t slices, double* out_ms) {
+    if (!srv || !out_ms || nq == 0 || nq > ipx::kMfmaMaxQueries || nq > srv->ctx->cfg.max_batch) return 1;
+    GpuServerCtx* c = srv->ctx;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (c->loop_active) return 5;
+    const ipx::DbPlan p = c->stream_plan(nq, slices);
+    if (p.kind != ipx::DbPlan::MFMA) return 1;
+    hipStream_t s = c->stream;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = 0;
+    try {
+        hip_check(hipEventCreate(&e0), "event");
+        hip_check(hipEventCreate(&e1), "event");
+        const size_t C = c->pp.db_cols, G = c->G;
+        uint32_t* pass_done = c->d_sflags[0];
+        uint32_t* packed_done = pass_done + G;
+        const size_t nblk = (nq + ipx::kPackQueries - 1) / ipx::kPackQueries;
+        const uint32_t target = (uint32_t)(ipx::N / ipx::dbplan_cols(p) * p.slices);
+        const bool direct = p.slices == 1;
+        uint32_t* part = direct ? c->d_interm : c->d_part;
+        const size_t part_qstride = direct ? 2 * C : c->part_stride;
+        c->order_before(s);
+        hip_check(hipMemsetAsync(c->d_bq, 0, nq * 2 * c->pp.db_rows * sizeof(double), s), "zero");
+        c->dbpass_prep(0, nq, p, s, 0);
+        auto timed = [&](const std::function<void()>& fn) {
+            fn();  // warm
+            hip_check(hipEventRecord(e0, s), "event");
+            for (int r = 0; r < 5; r++) fn();
+            hip_check(hipEventRecord(e1, s), "event");
+            hip_check(hipEventSynchronize(e1), "sync");
+            float ms = 0;
+            hip_check(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+            return (double)ms / 5;
+        };
+        // 0: the streamed pass alone (nobody waiting)
+        out_ms[0] = timed([&] {
+            hip_check(ipx::launch_fill_u32(pass_done, G, 0, s), "fill");
+            ipx::PassStream pst;
+            pst.group_done = pass_done;
+            hip_check(ipx::launch_dbpass_mfma(c->db_planes(), c->d_dig[0], c->d_S[0], nq, part, part_qstride, C,
+                                              c->pp.db_rows, C, 0, C, p, s, &pst),
+                      "streamed pass");
+        });
+        // 1: the same plan on its 2-D grid, with the slice reduction kernel
+        out_ms[1] = timed([&] { c->dbpass_range(0, nq, 0, G, p, s, 0); });
+        // 2: the streamed pack alone, every group handed over
+        ipx::PackStream pk;
+        pk.part = part;
+        pk.part_qstride = part_qstride;
+        pk.slices = p.slices;
+        pk.C = (uint32_t)C;
+        pk.pass_done = pass_done;
+        pk.pass_target = target;
+        pk.packed_done = packed_done;
+        pk.spin_limit = c->stream_spin_limit;
+        pk.status = c->d_status;
+        out_ms[2] = timed([&] {
+            hip_check(ipx::launch_fill_u32(pass_done, G, target, s), "fill");
+            hip_check(ipx::launch_fill_u32(packed_done, nblk * G, 0, s), "fill");
+            hip_check(ipx::launch_pack_stream(pk, c->d_part2, c->part2_stride, c->d_precomp, c->d_keys,
+                                              c->keys_stride, c->d_packed_b, c->packed_stride, G, nq, c->nc, s),
+                      "streamed pack");
+        });
+        // 3: the streamed chains alone, every group published
+        ipx::StreamWait sw;
+        sw.done = packed_done;
+        sw.queries = ipx::kPackQueries;
+        sw.stride = (uint32_t)G;
+        sw.spin_limit = c->stream_spin_limit;
+        sw.status = c->d_status;
+        hip_check(ipx::launch_fill_u32(packed_done, nblk * G, 1, s), "fill");
+        out_ms[3] = timed([&] {
+            hip_check(ipx::launch_horner(c->d_precomp + ipx::PC_A, ipx::PC_GROUP, G, c->Da, c->T, c->d_packed_b,
+                                         c->packed_stride, c->d_rgsw, c->rgsw_stride, c->d_resp, c->d_hstate, 0, G,
+                                         nq, c->nc, c->cc, s, ipx::FusedPack{}, sw),
+                      "streamed chains");
+        });
+        // 4: the plain chains over every group
+        out_ms[4] = timed([&] {
+            hip_check(ipx::launch_horner(c->d_precomp + ipx::PC_A, ipx::PC_GROUP, G, c->Da, c->T, c->d_packed_b,
+                                         c->packed_stride, c->d_rgsw, c->rgsw_stride, c->d_resp, c->d_hstate, 0, G,
+                                         nq, c->nc, c->cc, s),
+                      "chains");
+        });
+        c->order_after(s);
+        hip_check(hipStreamSynchronize(s), "sync");
+        c->check_status(s, 0);
+    } catch (...) {
+        rc = 4;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    return rc;
+}
+
+size_t ipir_dbg_streamed_count(const ipir_server* srv) { return srv ? srv->ctx->n_streamed.load() : 0; }
+
+int ipir_dbg_take_streaming(ipir_server* srv) {
+    if (!srv) return 1;
+    std::lock_guard<std::mutex> lk(srv->ctx->mu);
+    if (srv->ctx->loop_active) return 5;
+    srv->ctx->take_streaming();
+    srv->ctx->drop_graphs();
+    return 0;
+}
+
 int ipir_dbg_set_pipeline(ipir_server* srv, int single, int batched) {
-    auto ok = [](int v) { return v == 0 || (v >= 2 && v <= 5); };
+    auto ok = [](int v) { return v == 0 || (v >= 2 && v <= 7); };
     if (!srv || !ok(single) || !ok(batched)) return 1;
     std::lock_guard<std::mutex> lk(srv->ctx->mu);
     if (srv->ctx->loop_active) return 5;
@@ -613,6 +739,15 @@ int ipir_dbg_set_horner_spin_limit(ipir_server* srv, uint32_t limit) {
     std::lock_guard<std::mutex> lk(srv->ctx->mu);
     if (srv->ctx->loop_active) return 5;
     srv->ctx->horner_spin_limit = limit;
+    srv->ctx->drop_graphs();  // their kernels carry the old bound
+    return 0;
+}
+
+int ipir_dbg_set_stream_spin_limit(ipir_server* srv, uint32_t limit) {
+    if (!srv) return 1;
+    std::lock_guard<std::mutex> lk(srv->ctx->mu);
+    if (srv->ctx->loop_active) return 5;
+    srv->ctx->stream_spin_limit = limit;
     srv->ctx->drop_graphs();  // their kernels carry the old bound
     return 0;
 }
@@ -670,6 +805,7 @@ int ipir_dbg_profile_batch(ipir_server* srv, const uint64_t* d_queries, size_t c
     int rc = 0;
     try {
         c->order_before(s);
+        c->clear_status(s, 0);
         (void)hipEventRecord(ev[0], s);
         c->stage_prep(d_queries, count, s, 0);
         (void)hipEventRecord(ev[1], s);

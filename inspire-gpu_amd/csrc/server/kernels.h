@@ -98,9 +98,30 @@ hipError_t launch_mfma_digits(const double* bq, size_t nq, size_t R, const DbPla
                               hipStream_t s);
 // The int8-MFMA pass of nq queries over columns [col0, col0 + ncols), from
 // launch_mfma_digits' output for a plan with the same digit tile count.
+//
+// A streamed pass (ps set; full width, direct kernels and shared-digit
+// forms only) hands its groups over while it runs: its 1-D grid takes the
+// column tiles from the highest down (the order the Horner chains consume
+// groups), each tile's row slices in a row; every workgroup, once its
+// stores are complete at agent scope (one release per workgroup), adds 1 to
+// ps.group_done[column / N]. A group's columns are in place (out: one
+// slice's final values, or [q][slice][limb][OC] partials) when its word
+// reaches (N / dbplan_cols(p)) * p.slices. group_done must be zero before
+// the pass. The pass itself never waits for anything, so whatever waits
+// beside it for its groups cannot stall it beyond taking CUs.
+struct PassStream {
+    uint32_t* group_done = nullptr;
+    uint32_t tiles = 0;        // column tiles of the pass (C / dbplan_cols(p))
+    uint32_t slice_shift = 0;  // log2 of the plan's row slices
+    uint32_t tiles_per_group = 0;
+};
 hipError_t launch_dbpass_mfma(const uint8_t* db, const int8_t* dig, const uint32_t* S, size_t nq, uint32_t* out,
                               size_t out_qstride, size_t ostride, size_t R, size_t C, size_t col0, size_t ncols,
-                              const DbPlan& p, hipStream_t s);
+                              const DbPlan& p, hipStream_t s, const PassStream* ps = nullptr);
+// Whether launch_dbpass_mfma streams plan p over C columns.
+bool dbplan_streams(const DbPlan& p, size_t C);
+// n words at p set to v (a kernel, not a memset: graph-capture safe).
+hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s);
 // interm[q][limb][col0 + j] = sum over the S row slices of part (per query
 // stride part_stride; [slice][limb][j], j < ncols), for j < ncols.
 hipError_t launch_reduce_slices(const uint32_t* part, size_t part_stride, uint32_t S, size_t C, uint32_t* interm,
@@ -154,10 +175,46 @@ struct FusedPack {
     const uint32_t* keys = nullptr;    // [q][key][digit][limb][N]
     size_t keys_stride = 0;
 };
+// With wait.done set (beside a streamed pack), chain q waits before the
+// step of group g until done[(q / wait.queries) * wait.stride + g] is
+// nonzero (the group's packed b-parts are published). A wait longer than
+// wait.spin_limit sleeps (0: report at once) sets *wait.status to 1, writes
+// all-ones response words (no residue has that value) and ends the chain.
+struct StreamWait {
+    const uint32_t* done = nullptr;
+    uint32_t queries = 1, stride = 0;
+    uint32_t spin_limit = 0;
+    uint32_t* status = nullptr;
+};
+// The streamed pack beside a streamed database pass: one 1024-lane
+// workgroup per kPackQueries queries takes the groups from the top down;
+// for each it waits until pass_done[g] reaches pass_target, then packs the
+// group's b-parts of its queries (the pass's row-slice partials summed, NTT,
+// minus the collapse sums and Df x Kn: launch_pack_finalize's output) into
+// packed_b and publishes them: packed_done[block * G + g] = 1 after an
+// agent-scope release. packed_done must be zero before the launch. A wait
+// longer than spin_limit sleeps sets *status to 1 and ends the workgroup
+// (its chains then give up their own waits).
+constexpr uint32_t kPackQueries = 4;
+struct PackStream {
+    const uint32_t* part = nullptr;  // the pass's output: [q][slice][limb][C] partials
+    size_t part_qstride = 0;         //   (one slice: its values, [q][limb][C])
+    uint32_t slices = 1;
+    uint32_t C = 0;
+    const uint32_t* pass_done = nullptr;
+    uint32_t pass_target = 0;
+    uint32_t* packed_done = nullptr;
+    uint32_t spin_limit = 0;
+    uint32_t* status = nullptr;
+};
+hipError_t launch_pack_stream(const PackStream& ps, const uint32_t* part2, size_t part2_stride,
+                              const uint32_t* precomp, const uint32_t* keys, size_t keys_stride, uint32_t* packed_b,
+                              size_t packed_stride, size_t G, size_t nq, const NttConst& nc, hipStream_t s);
 hipError_t launch_horner(const uint32_t* pa_base, size_t pa_stride, size_t G, size_t Da, size_t T,
                          uint32_t* packed_b, size_t packed_stride, const uint32_t* rgsw, size_t rgsw_stride,
                          uint64_t* out, uint32_t* hstate, size_t g_lo, size_t g_hi, size_t nq, const NttConst& nc,
-                         const CrtConst& cc, hipStream_t s, const FusedPack& pack = FusedPack{});
+                         const CrtConst& cc, hipStream_t s, const FusedPack& pack = FusedPack{},
+                         const StreamWait& wait = StreamWait{});
 
 // The same chains with two workgroups per chain (one per limb) exchanging
 // coefficient-form values once per step. xbuf holds

@@ -282,6 +282,32 @@ __device__ __forceinline__ void store_pair(const i32x4& al, const i32x4& ah, uin
         red64((uint64_t)(s + (int64_t)off), c);
 }
 
+// Column tile and row slice of this workgroup: the 2-D grid's, or in a
+// streamed pass the 1-D index read as (tile counted from the top, slice).
+__device__ __forceinline__ void pass_coords(const PassStream& ps, uint32_t& xt, uint32_t& slice) {
+    if (ps.group_done) {
+        slice = blockIdx.x & ((1u << ps.slice_shift) - 1);
+        xt = ps.tiles - 1 - (blockIdx.x >> ps.slice_shift);
+    } else {
+        xt = blockIdx.x;
+        slice = blockIdx.y;
+    }
+}
+
+// Streamed pass, after the workgroup's stores of column tile xt: count the
+// workgroup into its group. One agent-scope release per workgroup (it
+// writes back the L2): every wave first waits for its own stores, the
+// barrier orders them before lane 0's release.
+__device__ __forceinline__ void pass_handover(const PassStream& ps, uint32_t xt) {
+    __builtin_amdgcn_s_waitcnt(0);  // this wave's stores are complete
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __hip_atomic_fetch_add(ps.group_done + xt / ps.tiles_per_group, 1u, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
 // Up to 2*MT queries (Mp = 16*MT digit rows): the 4 waves of a block own
 // disjoint column ranges (NT 16-column tiles each) and load their DB operands
 // straight from HBM into registers, one 64-row step ahead; every wave reads
@@ -291,11 +317,13 @@ __global__ __launch_bounds__(256, 2) void k_dbpass_mfma(const uint8_t* __restric
                                                         const int8_t* __restrict__ dig,
                                                         const uint32_t* __restrict__ S,
                                                         uint32_t* __restrict__ out, size_t out_qstride,
-                                                        uint32_t C, uint32_t OC, uint32_t nq, uint32_t kbps) {
+                                                        uint32_t C, uint32_t OC, uint32_t nq, uint32_t kbps,
+                                                        PassStream ps) {
     constexpr uint32_t Mp = 16 * MT;
     const uint32_t w = threadIdx.x >> 6, l = threadIdx.x & 63;
-    const uint32_t slice = blockIdx.y;
-    const uint32_t col0 = (blockIdx.x * 4 + w) * NT * 16;
+    uint32_t xt, slice;
+    pass_coords(ps, xt, slice);
+    const uint32_t col0 = (xt * 4 + w) * NT * 16;
     const uint32_t kb0 = slice * kbps;
     const uint32_t nks = kbps / 4;
     // lane's low-plane unit of block kb0 + l/16, column col0 + l%16; the high plane is C units later
@@ -344,6 +372,7 @@ __global__ __launch_bounds__(256, 2) void k_dbpass_mfma(const uint8_t* __restric
         for (int n = 0; n < NT; n++)
             store_pair(acc[a][n][0], acc[a][n][1], (uint32_t)a * 4 + (l >> 4), nq, slice == 0, S, out, out_qstride,
                        slice, OC, col0 + n * 16 + (l & 15));
+    if (ps.group_done) pass_handover(ps, xt);
 }
 
 // Many queries (MT = 8 or 16 tiles of per-limb digits, up to 16 or 32
@@ -386,7 +415,8 @@ __global__ __launch_bounds__(64 * WAVES, OCC) void k_dbpass_mfma_sa(const uint8_
                                                            const int8_t* __restrict__ dig,
                                                            const uint32_t* __restrict__ S,
                                                            uint32_t* __restrict__ out, size_t out_qstride,
-                                                           uint32_t C, uint32_t OC, uint32_t nq, uint32_t kbps) {
+                                                           uint32_t C, uint32_t OC, uint32_t nq, uint32_t kbps,
+                                                           PassStream ps) {
     static_assert(DEPTH >= 3, "the digit ring needs two slots");
     constexpr int L = DEPTH - 1;            // lead of every load, in steps
     constexpr int P = DEPTH * L;            // period of both rings' slot pattern
@@ -406,8 +436,9 @@ __global__ __launch_bounds__(64 * WAVES, OCC) void k_dbpass_mfma_sa(const uint8_
     __shared__ i32x4 smem[AT_UNITS > EP_UNITS ? AT_UNITS : EP_UNITS];
     i32x4(*atile)[4 * Mp] = reinterpret_cast<i32x4(*)[4 * Mp]>(smem);  // [buffer][block of the step][row]
     const uint32_t tid = threadIdx.x, w = tid >> 6, l = tid & 63;
-    const uint32_t slice = blockIdx.y;
-    const uint32_t col0 = (blockIdx.x * WAVES + w) * NT * 16;
+    uint32_t xt, slice;
+    pass_coords(ps, xt, slice);
+    const uint32_t col0 = (xt * WAVES + w) * NT * 16;
     const uint32_t kb0 = slice * kbps;
     const uint32_t nks = kbps / 4;
     const i32x4* bsrc = reinterpret_cast<const i32x4*>(db) + (size_t)(kb0 + (l >> 4)) * 2 * C + col0 + (l & 15);
@@ -580,6 +611,7 @@ __global__ __launch_bounds__(64 * WAVES, OCC) void k_dbpass_mfma_sa(const uint8_
                 store_pair(acc[a][n][0], acc[a][n][1], (uint32_t)a * 4 + (l >> 4), nq, slice == 0, S, out,
                            out_qstride, slice, OC, col0 + n * 16 + (l & 15));
     }
+    if (ps.group_done) pass_handover(ps, xt);
 }
 
 // 17..32 queries with the DB tile shared instead: each of the block's 4
@@ -598,7 +630,8 @@ __global__ __launch_bounds__(256, 2) void k_dbpass_mfma_bs(const uint8_t* __rest
                                                            const int8_t* __restrict__ dig,
                                                            const uint32_t* __restrict__ S,
                                                            uint32_t* __restrict__ out, size_t out_qstride,
-                                                           uint32_t C, uint32_t OC, uint32_t nq, uint32_t kbps) {
+                                                           uint32_t C, uint32_t OC, uint32_t nq, uint32_t kbps,
+                                                           PassStream) {  // never streamed
     static_assert(DEPTH >= 3 && MT % 4 == 0, "shape");
     constexpr int L = DEPTH - 1;
     constexpr int P = DEPTH * L;
@@ -796,9 +829,15 @@ hipError_t launch_mfma_digits(const double* bq, size_t nq, size_t R, const DbPla
     return hipGetLastError();
 }
 
+bool dbplan_streams(const DbPlan& p, size_t C) {
+    const size_t cols = dbplan_cols(p);
+    return p.kind == DbPlan::MFMA && !p.dbtile && p.slices > 0 && (p.slices & (p.slices - 1)) == 0 &&
+           N % cols == 0 && C % N == 0;
+}
+
 hipError_t launch_dbpass_mfma(const uint8_t* db, const int8_t* dig, const uint32_t* S, size_t nq, uint32_t* out,
                               size_t out_qstride, size_t ostride, size_t R, size_t C, size_t col0, size_t ncols,
-                              const DbPlan& p, hipStream_t s) {
+                              const DbPlan& p, hipStream_t s, const PassStream* ps) {
     if (p.kind != DbPlan::MFMA || nq == 0 || nq > kMfmaMaxQueries) return hipErrorInvalidValue;
     const uint32_t Mp = (uint32_t)dbplan_mp(p);
     const uint32_t kbps = (uint32_t)(R / 16) / p.slices;
@@ -807,16 +846,30 @@ hipError_t launch_dbpass_mfma(const uint8_t* db, const int8_t* dig, const uint32
         ncols % cols || col0 % cols ||
         col0 + ncols > C || ostride < ncols)
         return hipErrorInvalidValue;
+    PassStream st{};
+    dim3 grid((unsigned)(ncols / cols), p.slices);
+    if (ps) {
+        // the whole width, one 1-D grid (slices of a tile in a row, the highest tiles first)
+        st = *ps;
+        uint32_t sh = 0;
+        while ((1u << sh) < p.slices) sh++;
+        st.slice_shift = sh;
+        st.tiles = (uint32_t)(C / cols);
+        st.tiles_per_group = (uint32_t)(N / cols);
+        if (!dbplan_streams(p, C) || col0 != 0 || ncols != C || !st.group_done || ostride != C ||
+            (size_t)st.tiles * p.slices > 0x7fffffffu)
+            return hipErrorInvalidValue;
+        grid = dim3(st.tiles * p.slices, 1);
+    }
     const uint32_t Cc = (uint32_t)C, OC = (uint32_t)ostride, Q = (uint32_t)nq;
-    const dim3 grid((unsigned)(ncols / cols), p.slices);
     const uint8_t* d = db + col0 * 16;
 #define MFMA(K, ...) \
-    hipLaunchKernelGGL((K<__VA_ARGS__>), grid, dim3(256), 0, s, d, dig, S, out, out_qstride, Cc, OC, Q, kbps)
+    hipLaunchKernelGGL((K<__VA_ARGS__>), grid, dim3(256), 0, s, d, dig, S, out, out_qstride, Cc, OC, Q, kbps, st)
 #define MFMA8(K, ...) \
-    hipLaunchKernelGGL((K<__VA_ARGS__, 8>), grid, dim3(512), 0, s, d, dig, S, out, out_qstride, Cc, OC, Q, kbps)
+    hipLaunchKernelGGL((K<__VA_ARGS__, 8>), grid, dim3(512), 0, s, d, dig, S, out, out_qstride, Cc, OC, Q, kbps, st)
 #define MFMA8P(MT, D)                                                                                          \
     hipLaunchKernelGGL((k_dbpass_mfma_sa<MT, 2, 2, D, 8, 1>), grid, dim3(512), 0, s, d, dig, S, out, out_qstride, \
-                       Cc, OC, Q, kbps)
+                       Cc, OC, Q, kbps, st)
     switch (p.param) {
         case 1: MFMA(k_dbpass_mfma, 1, 4); break;
         case 2: MFMA(k_dbpass_mfma, 2, 4); break;

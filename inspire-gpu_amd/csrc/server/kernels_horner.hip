@@ -209,4 +209,12 @@ hipError_t launch_horner_pair(const uint32_t* pa_base, size_t pa_stride, size_t 
     return hipGetLastError();
 }
 
+hipError_t launch_fill_u32(uint32_t* p, size_t n, uint32_t v, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 1024) blocks = 1024;
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)blocks), dim3(256), 0, s, p, n, v);
+    return hipGetLastError();
+}
+
 }  // namespace ipx

@@ -172,18 +172,21 @@ __global__ __launch_bounds__(1024) void k_pack_finalize_wg(const uint32_t* __res
 // acc (a0, a1, b0, b1 polys, NTT form) lives in registers between steps;
 // per step: 4 INTTs, gadget split, 2 rounds of 4 NTTs, multiply-accumulate
 // against the query's RGSW rows, + packed[i]. L == 1 copies packed[start].
-// The RGSW rows are loop-invariant (the compiler keeps them in registers
-// across steps); the step's packed parts are loaded during round 0, a phase
-// ahead of their use, so HBM latency (long while a database pass streams
-// through L2) hides behind the round-1 transforms.
+// The RGSW rows are loaded once and kept in registers across steps; the
+// step's packed parts are loaded during round 0, a phase ahead of their
+// use, so HBM latency (long while a database pass streams through L2) hides
+// behind the round-1 transforms. STREAMED: beside a streamed pack, each
+// step first waits for its group's packed b-part (StreamWait).
+template <bool STREAMED>
 __global__ __launch_bounds__(1024) void k_horner(const uint32_t* __restrict__ pa_base, size_t pa_stride, uint32_t G, uint32_t Da,
                                                  uint32_t T, uint32_t* packed_b,
                                                  size_t pstride, const uint32_t* __restrict__ rgsw,
                                                  size_t rstride, uint64_t* __restrict__ out,
                                                  uint32_t* __restrict__ hstate, uint32_t g_lo, uint32_t g_hi,
-                                                 NttConst nc, CrtConst cc, FusedPack fp) {
+                                                 NttConst nc, CrtConst cc, FusedPack fp, StreamWait sw) {
     __shared__ uint32_t buf[4][POLY_LDS];
     __shared__ uint32_t twl[8 * N];
+    __shared__ uint32_t sw_abort;
     const uint32_t ct = blockIdx.x, qi = blockIdx.y, tid = threadIdx.x;
     const uint32_t team = tid >> 8, t = tid & 255;
     const uint32_t start = ct * Da;
@@ -196,6 +199,37 @@ __global__ __launch_bounds__(1024) void k_horner(const uint32_t* __restrict__ pa
     const int tlimb = (int)(team & 1);
     const Tw tw = tw_lds(twl, tlimb);
     const uint32_t inv_n = nc.inv_n[tlimb], inv_n_p = nc.inv_n_p[tlimb];
+    // Beside a streamed pack: wait until this chain's packed b-part of
+    // group g is published (lane 0 spins, then one agent-scope acquire for
+    // the block, ordered before every lane's loads by the barrier). False:
+    // the wait exceeded its bound (every lane returns the same).
+    const uint32_t* sw_done = sw.done + (size_t)(qi / sw.queries) * sw.stride;
+    auto wait_packed = [&](uint32_t g) -> bool {
+        if (tid == 0) {
+            uint32_t spins = 0, ab = 0;
+            while (sw.spin_limit == 0 ||
+                   __hip_atomic_load(sw_done + g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+                if (++spins > sw.spin_limit) {
+                    __hip_atomic_store(sw.status, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    ab = 1;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(2);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            sw_abort = ab;
+        }
+        __syncthreads();
+        return !sw_abort;
+    };
+    constexpr bool streamed = STREAMED;
+    uint64_t* o = out + ((size_t)qi * T + ct) * 4 * N;
+    auto poison = [&]() {  // an invalid response: no residue is all-ones
+#pragma unroll
+        for (int p = 0; p < 4; p++)
+#pragma unroll
+            for (int e = 0; e < 2; e++) o[(size_t)p * N + tid + 1024 * e] = ~0ull;
+    };
     if (fp.interm) {
         // this chain's packed b-parts of groups [lo, hi): team k%4 takes
         // (group lo + k/2, limb k%2), the limb of its own twiddles (as
@@ -237,6 +271,11 @@ __global__ __launch_bounds__(1024) void k_horner(const uint32_t* __restrict__ pa
     uint32_t acc[4][2];
     int first;
     if (top < g_hi) {  // the chain starts here: acc = packed[top]
+        if constexpr (streamed)
+            if (!wait_packed(top)) {
+                poison();
+                return;
+            }
         const uint32_t* pa = pa_base + (size_t)top * pa_stride;
 #pragma unroll
         for (int e = 0; e < 2; e++) {
@@ -252,12 +291,12 @@ __global__ __launch_bounds__(1024) void k_horner(const uint32_t* __restrict__ pa
             for (int e = 0; e < 2; e++) acc[p][e] = hs[(size_t)p * N + tid + 1024 * e];
         first = (int)hi - 1;
     }
-    for (int gs = first; gs >= (int)lo; gs--) {
-        const int st = gs - (int)start;
-        const uint32_t gi0 = start + (uint32_t)st;
-        // this step's global operands, issued before the transforms they wait behind
-        uint32_t rA[2][2][2], rB[2][2][2];  // round-0 RGSW [c][e][j]
-        uint32_t pav[2][2], pbv[2][2];      // packed a / b parts [c][e]
+    // the query's RGSW rows [round][c][e][j] (round 0: top_a / top_b, round
+    // 1: bot_a / bot_b), the same for every step: loaded once, kept in
+    // registers across the steps
+    uint32_t rA[2][2][2][2], rB[2][2][2][2];
+#pragma unroll
+    for (int rd = 0; rd < 2; rd++)
 #pragma unroll
         for (int c = 0; c < 2; c++)
 #pragma unroll
@@ -265,10 +304,19 @@ __global__ __launch_bounds__(1024) void k_horner(const uint32_t* __restrict__ pa
                 const uint32_t i = tid + 1024 * e;
 #pragma unroll
                 for (int j = 0; j < 2; j++) {
-                    rA[c][e][j] = rg[(size_t)(j * 2 + c) * N + i];
-                    rB[c][e][j] = rg[(size_t)BLK + (size_t)(j * 2 + c) * N + i];
+                    rA[rd][c][e][j] = rg[(size_t)(2 * rd) * BLK + (size_t)(j * 2 + c) * N + i];
+                    rB[rd][c][e][j] = rg[(size_t)(2 * rd + 1) * BLK + (size_t)(j * 2 + c) * N + i];
                 }
             }
+    for (int gs = first; gs >= (int)lo; gs--) {
+        if constexpr (streamed)
+            if (!wait_packed((uint32_t)gs)) {
+                poison();
+                return;
+            }
+        const int st = gs - (int)start;
+        const uint32_t gi0 = start + (uint32_t)st;
+        uint32_t pav[2][2], pbv[2][2];  // packed a / b parts [c][e], loaded a phase ahead
 #pragma unroll
         for (int p = 0; p < 4; p++)
 #pragma unroll
@@ -307,22 +355,17 @@ __global__ __launch_bounds__(1024) void k_horner(const uint32_t* __restrict__ pa
 #pragma unroll
                     for (int j = 0; j < 2; j++) {
                         const uint64_t d = buf[j * 2 + c][ls(i)];
-                        sa += d * rA[c][e][j];
-                        sb += d * rB[c][e][j];
+                        sa += d * rA[round][c][e][j];
+                        sb += d * rB[round][c][e][j];
                     }
                     ra[c][e] = sa; rb[c][e] = sb;
                 }
-            if (round == 0) {  // round 1's RGSW rows, issued before its transforms
+            if (round == 0) {  // the step's packed parts, issued before round 1's transforms
 #pragma unroll
                 for (int c = 0; c < 2; c++)
 #pragma unroll
                     for (int e = 0; e < 2; e++) {
                         const uint32_t i = tid + 1024 * e;
-#pragma unroll
-                        for (int j = 0; j < 2; j++) {
-                            rA[c][e][j] = rg[(size_t)2 * BLK + (size_t)(j * 2 + c) * N + i];
-                            rB[c][e][j] = rg[(size_t)3 * BLK + (size_t)(j * 2 + c) * N + i];
-                        }
                         pav[c][e] = pa_base[(size_t)gi0 * pa_stride + (size_t)c * N + i];
                         pbv[c][e] = pb[(size_t)gi0 * 2 * N + (size_t)c * N + i];
                     }
@@ -345,11 +388,98 @@ __global__ __launch_bounds__(1024) void k_horner(const uint32_t* __restrict__ pa
             for (int e = 0; e < 2; e++) hs[(size_t)p * N + tid + 1024 * e] = acc[p][e];
         return;
     }
-    uint64_t* o = out + ((size_t)qi * T + ct) * 4 * N;
 #pragma unroll
     for (int p = 0; p < 4; p++)
 #pragma unroll
         for (int e = 0; e < 2; e++) o[(size_t)p * N + tid + 1024 * e] = acc[p][e];
+}
+
+// ------------------------------------------------------------ streamed pack
+// Beside a streamed database pass (launch_pack_stream): workgroup b packs
+// queries [b*kPackQueries, ...) group by group from the top; team k of
+// round r takes polynomial 4r + k = (query, limb), the limb of its own
+// twiddles. The subtrahend of round 0 (collapse sums + Df x Kn, in place
+// before the launch) is loaded while lane 0 waits for the group.
+__global__ __launch_bounds__(1024) void k_pack_stream(PackStream ps, const uint32_t* __restrict__ part2,
+                                                      size_t p2_stride, const uint32_t* __restrict__ precomp,
+                                                      const uint32_t* __restrict__ keys, size_t kstride,
+                                                      uint32_t* __restrict__ packed_b, size_t pstride, uint32_t G,
+                                                      uint32_t nq, NttConst nc) {
+    __shared__ uint32_t buf[4][POLY_LDS];
+    __shared__ uint32_t twl[8 * N];
+    __shared__ uint32_t abort_flag;
+    const uint32_t tid = threadIdx.x, team = tid >> 8, t = tid & 255, blk = blockIdx.x;
+    const uint32_t q0 = blk * kPackQueries;
+    const uint32_t nqb = nq - q0 < kPackQueries ? nq - q0 : kPackQueries;
+    const uint32_t npoly = 2 * nqb, rounds = (npoly + 3) / 4;
+    const int c = (int)(team & 1);
+    const uint32_t q = qof(c);
+    const Tw tw = tw_lds(twl, c);
+    const size_t sstride = 2 * (size_t)ps.C;  // one row slice of the pass's partials
+    stage_twiddles(twl, nc, tid, 1024);        // published by the first barrier below
+    for (int g = (int)G - 1; g >= 0; g--) {
+        for (uint32_t r = 0; r < rounds; r++) {  // every lane runs every round (barriers inside)
+            const uint32_t k = 4 * r + team;
+            const bool active = k < npoly;
+            const uint32_t qi = q0 + (active ? k >> 1 : 0);
+            uint32_t sub[8];
+            if (active) {
+                const uint32_t* Df = precomp + (size_t)g * PC_GROUP + PC_DF + (size_t)c * D_EFF * N;
+                const uint32_t* Kn = keys + qi * kstride + BLK + (size_t)c * N;
+                const uint32_t* p2 = part2 + qi * p2_stride + (size_t)g * 2 * N + (size_t)c * N;
+#pragma unroll
+                for (int e = 0; e < 8; e++) {
+                    const uint32_t x = t + 256 * e;
+                    sub[e] = red64((uint64_t)Df[x] * Kn[x] + (uint64_t)Df[N + x] * Kn[2 * N + x] + p2[x], c);
+                }
+            }
+            if (r == 0) {
+                if (tid == 0) {
+                    uint32_t spins = 0, ab = 0;
+                    while (ps.spin_limit == 0 || __hip_atomic_load(ps.pass_done + g, __ATOMIC_RELAXED,
+                                                                   __HIP_MEMORY_SCOPE_AGENT) < ps.pass_target) {
+                        if (++spins > ps.spin_limit) {
+                            __hip_atomic_store(ps.status, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            ab = 1;
+                            break;
+                        }
+                        __builtin_amdgcn_s_sleep(2);
+                    }
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // the pass's partials, for every lane
+                    abort_flag = ab;
+                }
+                __syncthreads();
+                if (abort_flag) return;
+            }
+            if (active) {
+                // partials are residues (< 2^27) and at most 16 are summed:
+                // 32-bit sums, the 8 loads of a slice in flight together
+                const uint32_t* src = ps.part + qi * ps.part_qstride + (size_t)c * ps.C + (size_t)g * N + t;
+                uint32_t a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+                for (uint32_t sl = 0; sl < ps.slices; sl++) {
+#pragma unroll
+                    for (int e = 0; e < 8; e++) a[e] += src[sl * sstride + 256 * e];
+                }
+#pragma unroll
+                for (int e = 0; e < 8; e++) buf[team][ls(t + 256 * e)] = red64(a[e], c);
+            }
+            __syncthreads();
+            ntt_fwd_lds(buf[team], t, tw, c, active);
+            if (active) {
+                uint32_t* dst = packed_b + qi * pstride + (size_t)g * 2 * N + (size_t)c * N;
+#pragma unroll
+                for (int e = 0; e < 8; e++) dst[t + 256 * e] = sub_mod(buf[team][ls(t + 256 * e)], sub[e], q);
+            }
+            __syncthreads();  // the buffers are free for the next round
+        }
+        // publish the group: every wave's stores complete, then one release
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        if (tid == 0) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            __hip_atomic_store(ps.packed_done + (size_t)blk * G + g, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
 }
 
 }  // namespace
@@ -416,14 +546,33 @@ hipError_t launch_pack_finalize_wg(const uint32_t* interm, size_t C, const uint3
     return hipGetLastError();
 }
 
+hipError_t launch_pack_stream(const PackStream& ps, const uint32_t* part2, size_t part2_stride,
+                              const uint32_t* precomp, const uint32_t* keys, size_t keys_stride, uint32_t* packed_b,
+                              size_t packed_stride, size_t G, size_t nq, const NttConst& nc, hipStream_t s) {
+    if (nq == 0 || G == 0) return hipSuccess;
+    if (!ps.part || !ps.pass_done || !ps.packed_done || !ps.status || ps.slices == 0 || ps.slices > 16 ||
+        ps.C < G * N)  // at most 16 partials below 2^27 keep the 32-bit sums exact
+        return hipErrorInvalidValue;
+    const unsigned blocks = (unsigned)((nq + kPackQueries - 1) / kPackQueries);
+    hipLaunchKernelGGL(k_pack_stream, dim3(blocks), dim3(1024), 0, s, ps, part2, part2_stride, precomp, keys,
+                       keys_stride, packed_b, packed_stride, (uint32_t)G, (uint32_t)nq, nc);
+    return hipGetLastError();
+}
+
 hipError_t launch_horner(const uint32_t* pa_base, size_t pa_stride, size_t G, size_t Da, size_t T,
                          uint32_t* packed_b, size_t packed_stride, const uint32_t* rgsw, size_t rgsw_stride,
                          uint64_t* out, uint32_t* hstate, size_t g_lo, size_t g_hi, size_t nq, const NttConst& nc,
-                         const CrtConst& cc, hipStream_t s, const FusedPack& pack) {
+                         const CrtConst& cc, hipStream_t s, const FusedPack& pack, const StreamWait& wait) {
     if (nq == 0 || g_lo >= g_hi) return hipSuccess;
-    hipLaunchKernelGGL(k_horner, dim3((unsigned)T, (unsigned)nq), dim3(1024), 0, s, pa_base, pa_stride, (uint32_t)G,
-                       (uint32_t)Da, (uint32_t)T, packed_b, packed_stride, rgsw, rgsw_stride, out, hstate,
-                       (uint32_t)g_lo, (uint32_t)g_hi, nc, cc, pack);
+    if (wait.done && (!wait.status || wait.queries == 0)) return hipErrorInvalidValue;
+    if (wait.done)
+        hipLaunchKernelGGL(k_horner<true>, dim3((unsigned)T, (unsigned)nq), dim3(1024), 0, s, pa_base, pa_stride,
+                           (uint32_t)G, (uint32_t)Da, (uint32_t)T, packed_b, packed_stride, rgsw, rgsw_stride, out,
+                           hstate, (uint32_t)g_lo, (uint32_t)g_hi, nc, cc, pack, wait);
+    else
+        hipLaunchKernelGGL(k_horner<false>, dim3((unsigned)T, (unsigned)nq), dim3(1024), 0, s, pa_base, pa_stride,
+                           (uint32_t)G, (uint32_t)Da, (uint32_t)T, packed_b, packed_stride, rgsw, rgsw_stride, out,
+                           hstate, (uint32_t)g_lo, (uint32_t)g_hi, nc, cc, pack, wait);
     return hipGetLastError();
 }
 

@@ -201,6 +201,13 @@ ipir_loop* ipir_loop_start(ipir_server* srv, size_t max_inflight) {
         if (ctx->loop_active) return nullptr;  // one loop per server
         ctx->loop_active = true;
     }
+    // status words left by earlier asynchronous calls are not this loop's
+    if (hipDeviceSynchronize() != hipSuccess ||
+        hipMemset(ctx->d_status, 0, GpuServerCtx::kLanes * sizeof(uint32_t)) != hipSuccess) {
+        std::lock_guard<std::mutex> lk(ctx->mu);
+        ctx->loop_active = false;
+        return nullptr;
+    }
     auto* L = new ipir_loop;
     L->srv = srv;
     L->ctx = ctx;

@@ -254,6 +254,21 @@ GpuServerCtx* gpu_setup_server(const PublicParams& pp, const PreprocessData& pre
 }
 
 namespace {
+// The server of each device that may take the streamed shapes (at most one).
+std::mutex g_stream_mu;
+constexpr size_t kMaxDevices = 64;
+GpuServerCtx* g_streaming[kMaxDevices] = {};
+
+// Row slices of a streamed pass over C columns: up to 16, with partials of
+// at most 32 MiB per query.
+constexpr uint32_t kStreamSlicesMax = 16;
+constexpr size_t kStreamPartWords = (size_t)1 << 23;
+uint32_t stream_slices_cap(size_t C) {
+    uint32_t s = 1;
+    while (s < kStreamSlicesMax && (size_t)s * 2 * 2 * C <= kStreamPartWords) s *= 2;
+    return s;
+}
+
 // Single-query database-pass candidates: (MFMA?, fp64 unroll, target workgroups).
 struct DbCand {
     bool mfma;
@@ -311,6 +326,9 @@ ServerFootprint server_footprint(const PublicParams& pp, size_t gchunk) {
                 need(full, C);
                 need(narrow_plan(full, t, R, C, ccols, nq), ccols);
             }
+    // streamed passes' partials: the whole width at their largest slice count
+    const uint32_t scap = stream_slices_cap(C);
+    if (scap > 1 && (size_t)scap * 2 * C > part_words) part_words = (size_t)scap * 2 * C;
     f.part_words = part_words;
     SlotLayout lay{R, C, G};
     const size_t qlen = lay.query_u64s(), resp = T * 4 * N;
@@ -320,6 +338,10 @@ ServerFootprint server_footprint(const PublicParams& pp, size_t gchunk) {
                  query_packed_bytes(R) + T * 4 * N * 4 + T * GpuServerCtx::kCompressedCtBytes;
     f.fixed = R * C * 2 + G * PC_GROUP * 4 + 8 * N * 4 +
               GpuServerCtx::kLanes * (R * 256 + GpuServerCtx::kKdigBytes + 2 * 64 * 4 + 4);
+    // streamed answers' flags: one set per lane and 32 slots (rounded up)
+    const size_t set_bytes = GpuServerCtx::kStreamSetWords * G * 4;
+    f.fixed += GpuServerCtx::kLanes * set_bytes;
+    f.per_slot += (GpuServerCtx::kLanes * set_bytes + kMfmaMaxQueries - 1) / kMfmaMaxQueries;
     // gpu_preprocess: the DB with its upload slab, then the DB, the arena and
     // one chunk's ring-embed table, chain states and D+/D- in reference order
     size_t slab = ((size_t)1 << 27) / C / 16 * 16;
@@ -389,11 +411,26 @@ void GpuServerCtx::alloc_slots() {
         HIPX(hipEventCreateWithFlags(&prep_ev[l], hipEventDisableTiming));
         HIPX(hipEventCreateWithFlags(&horner_done[l], hipEventDisableTiming));
     }
+    stream_sets = (B + kMfmaMaxQueries - 1) / kMfmaMaxQueries;
+    for (int l = 0; l < kLanes; l++) {
+        const size_t words = stream_sets * kStreamSetWords * G;
+        d_sflags[l] = dalloc<uint32_t>(words);
+        HIPX(hipMemset(d_sflags[l], 0, words * 4));
+    }
+    // INSPIRE_STREAM=0: no streamed shapes; otherwise the device's first
+    // live server takes them (see stream_ok)
+    const char* se = std::getenv("INSPIRE_STREAM");
+    if (!(se && se[0] == '0')) {
+        std::lock_guard<std::mutex> lk(g_stream_mu);
+        stream_ok = (size_t)device < kMaxDevices && !g_streaming[device];
+        if (stream_ok) g_streaming[device] = this;
+    }
     d_hxbuf = dalloc<uint32_t>(horner_pair_xbuf_words(B * T));
     d_hflags = dalloc<uint32_t>(2 * B * T);
     d_status = dalloc<uint32_t>(kLanes);
     HIPX(hipMemset(d_status, 0, kLanes * 4));
     horner_spin_limit = kHornerSpinLimit;
+    stream_spin_limit = kStreamSpinLimit;
     HIPX(hipEventCreateWithFlags(&order_ev, hipEventDisableTiming));
     // INSPIRE_PACK_WG=0/1/-1: the pipelined chunks' pack kernel, a grid over
     // (group, limb), one workgroup per query, or one workgroup for one query
@@ -405,10 +442,11 @@ void GpuServerCtx::alloc_slots() {
         horner_mode = std::strcmp(h, "pair") == 0 ? 1 : std::strcmp(h, "single") == 0 ? 0 : -1;
     // INSPIRE_PIPELINE=<single>,<batched>: the pipeline shape (0 serial, 2
     // two streams, 3 three streams, 4 the collapse first, 5 three streams
-    // with the collapse a chunk ahead) of one-query and many-query calls
+    // with the collapse a chunk ahead, 6 and 7 streamed after or beside the
+    // collapse) of one-query and many-query calls
     if (const char* pl = std::getenv("INSPIRE_PIPELINE")) {
         int a1 = -1, a2 = -1;
-        auto ok = [](int v) { return v == 0 || (v >= 2 && v <= 5); };
+        auto ok = [](int v) { return v == 0 || (v >= 2 && v <= 7); };
         if (std::sscanf(pl, "%d,%d", &a1, &a2) == 2 && ok(a1) && ok(a2)) {
             pipe_single = a1;
             pipe_batched = a2;
@@ -592,8 +630,11 @@ void GpuServerCtx::autotune_dbpass() {
     // when forced. The two interact:
     // a one-slice grid needs no reduction kernel and leaves CUs that only a
     // collapse beside it uses; more slices fill the device alone but pay
-    // partials and their reduction.
-    if (G / gchunk > 1) {
+    // partials and their reduction. The best of these (or, with a single
+    // chunk, the serial answer) is then timed against the streamed shapes
+    // at each row-slice count their full-width pass can take.
+    const bool chunked_ok = G / gchunk > 1;
+    if (chunked_ok || can_stream(stream)) {
         const size_t ccols = gchunk * N;
         HIPX(hipMemset(d_q, 0, nq * qlen * 8));
         for (const size_t count : {(size_t)1, nq}) {
@@ -604,7 +645,9 @@ void GpuServerCtx::autotune_dbpass() {
                 int shape;
             };
             std::vector<Pick> cands;
-            for (uint32_t sl = 0; sl <= 8; sl = sl ? 2 * sl : 1) {
+            if (!chunked_ok && (!pipe_forced || (count == 1 ? pipe_single : pipe_batched) < 6))
+                cands.push_back({0, 0});  // one chunk: the serial answer against the streamed ones
+            for (uint32_t sl = 0; chunked_ok && sl <= 8; sl = sl ? 2 * sl : 1) {
                 if (sl) {
                     const size_t t = blocks * sl;
                     const DbPlan x = count > 1 ? dbplan_form(R, ccols, count, t, mfma_depth, mfma_form)
@@ -619,21 +662,48 @@ void GpuServerCtx::autotune_dbpass() {
                     cands.push_back({sl, shape});
                 }
             }
+            // streamed shapes, at each row-slice count a streamed plan takes
+            // (the slices field is then the streamed pass's)
+            const int forced = count == 1 ? pipe_single : pipe_batched;
+            if (can_stream(stream))
+                for (uint32_t sl = 1; sl <= stream_slices_cap(C); sl *= 2) {
+                    const DbPlan x = stream_plan(count, sl);
+                    if (x.kind != DbPlan::MFMA || x.slices != sl) continue;
+                    // shape 7 (the collapse beside) never won at 16 GB: forced only
+                    for (const int shape : {6, 7})
+                        if (pipe_forced ? shape == forced : shape == 6) cands.push_back({sl, shape});
+                }
             auto use = [&](const Pick& p) {
+                const bool st = p.shape >= 6;
                 if (count == 1) {
-                    chunk_slices1 = p.slices;
+                    if (st) stream_slices1 = p.slices; else chunk_slices1 = p.slices;
                     if (!pipe_forced) pipe_single = p.shape;
                 } else {
-                    chunk_slices = p.slices;
-                    chunk_param = p.slices ? full.param : 0;
+                    if (st) {
+                        stream_slices = p.slices;
+                    } else {
+                        chunk_slices = p.slices;
+                        chunk_param = p.slices ? full.param : 0;
+                    }
                     if (!pipe_forced) pipe_batched = p.shape;
                 }
             };
-            if (cands.empty()) continue;  // a forced serial shape: nothing to time
-            const size_t b = fastest(
-                cands.size(), [&](size_t i) { use(cands[i]); }, [&](size_t) { enqueue(d_q, count, d_resp, stream); },
-                6);
-            use(cands[b]);
+            // the chunked shapes first (the fallback where an answer cannot
+            // stream), then the best of them against the streamed ones
+            std::vector<Pick> chunked, streamed;
+            for (const Pick& c : cands) (c.shape >= 6 ? streamed : chunked).push_back(c);
+            auto time_picks = [&](std::vector<Pick>& v) {
+                const size_t b = fastest(
+                    v.size(), [&](size_t i) { use(v[i]); }, [&](size_t) { enqueue(d_q, count, d_resp, stream); }, 6);
+                use(v[b]);
+                return v[b];
+            };
+            if (!chunked.empty()) {
+                const Pick c = time_picks(chunked);
+                (count == 1 ? pipe_single_chunked : pipe_batched_chunked) = c.shape;
+                if (!streamed.empty()) streamed.insert(streamed.begin(), c);
+            }
+            if (!streamed.empty()) time_picks(streamed);
             if (nq == 1) break;
         }
     }
@@ -710,6 +780,45 @@ DbPlan GpuServerCtx::chunk_plan_at(size_t nq, size_t k, size_t ng, const DbPlan&
         if (x.param == whole.param && (x.slices == 1 || x.slices * 2 * ncols <= part_stride)) return x;
     }
     return ng == gchunk ? whole : chunk_plan(nq, ng * N);
+}
+
+DbPlan GpuServerCtx::stream_plan(size_t nq, uint32_t slices) const {
+    const size_t R = pp.db_rows, C = pp.db_cols;
+    DbPlan p = plan_for(nq, C);
+    if (p.kind != DbPlan::MFMA || p.dbtile)  // a kernel that cannot stream: the direct or 8-wave form
+        p = nq == 1 ? dbplan_mfma(R, C, 1, plan1_target) : dbplan_form(R, C, nq, mfma_target, mfma_depth, 2);
+    // at least the slices the exactness bounds need (2^15 rows for power rows,
+    // else 2^16), at most kStreamSlicesMax (the pack's 32-bit partial sums)
+    uint32_t sl = slices ? slices : 1;
+    while (R / sl > (p.pow ? 32768u : 65536u)) sl *= 2;
+    p.slices = sl;
+    const size_t kbps = R / 16 / sl;
+    if (!dbplan_streams(p, C) || sl > kStreamSlicesMax || kbps < 8 || kbps % 8 ||
+        (sl > 1 && (size_t)sl * 2 * C > part_stride))
+        p.kind = DbPlan::F64;  // no streamed plan
+    return p;
+}
+
+void GpuServerCtx::take_streaming() {
+    std::lock_guard<std::mutex> lk(g_stream_mu);
+    if ((size_t)device >= kMaxDevices) return;
+    GpuServerCtx* prev = g_streaming[device];
+    if (prev == this) return;
+    if (prev) {
+        std::lock_guard<std::mutex> pl(prev->mu);  // its calls see the change between answers
+        prev->stream_ok = false;
+    }
+    g_streaming[device] = this;
+    stream_ok = true;
+}
+
+bool GpuServerCtx::can_stream(hipStream_t s) const {
+    if (!stream_ok) return false;
+    // a graph replays its branches one after another, so chains captured
+    // beside a streamed pass could wait for a pass that runs after them
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    if (s && hipStreamIsCapturing(s, &cap) != hipSuccess) return false;
+    return cap == hipStreamCaptureStatusNone;
 }
 
 void GpuServerCtx::dbpass_prep(size_t q0, size_t nq, const DbPlan& p, hipStream_t s, int lane) {
@@ -811,13 +920,83 @@ void GpuServerCtx::enqueue(const uint64_t* d_in, size_t count, uint64_t* d_out, 
     // wait for both halves of that chunk. s finally waits for the chains,
     // and through their waits for every collapse.
     bool joined = false;
+    const bool streamable = can_stream(s);
     for (size_t q0 = slot0; q0 < slot0 + count; q0 += kMfmaMaxQueries) {
         const size_t nq = slot0 + count - q0 < kMfmaMaxQueries ? slot0 + count - q0 : kMfmaMaxQueries;
-        const int shape = nchunks == 1 ? 0 : nq == 1 ? pipe_single : pipe_batched;
+        int shape = nq == 1 ? pipe_single : pipe_batched;
+        if (shape < 6 && nchunks == 1) shape = 0;  // one chunk: nothing to pipeline
         uint32_t* p2 = d_part2 + q0 * part2_stride;
         const uint32_t* keys = d_keys + q0 * keys_stride;
         uint32_t* packed = d_packed_b + q0 * packed_stride;
         uint64_t* out = d_out + (q0 - slot0) * resp_u64s;
+        DbPlan sp;
+        if (shape >= 6) {
+            sp = stream_plan(nq, nq == 1 ? stream_slices1 : stream_slices);
+            // a forced pair Horner cannot wait for a streamed pass: the chunked shape then
+            if (!streamable || sp.kind != DbPlan::MFMA || horner_mode == 1)
+                shape = nchunks == 1 ? 0 : nq == 1 ? pipe_single_chunked : pipe_batched_chunked;
+        }
+        if (shape >= 6) {
+            // Streamed: one full-width database pass on s hands each group
+            // over as soon as its columns are in place (highest first); the
+            // pack kernel on the lane's collapse stream packs each group as
+            // it arrives and publishes it to the Horner chains on the Horner
+            // stream. The collapse runs on s before the pass (6) or on the
+            // collapse stream beside it (7); the pack and the chains start
+            // after it. Every consumer is enqueued after its producer, so
+            // wherever two of these streams share a hardware queue the
+            // producer runs first.
+            TraceRange trs("streamed pass: %zu queries, %u row slices, shape %d", nq, sp.slices, shape);
+            n_streamed.fetch_add(1, std::memory_order_relaxed);
+            const hipStream_t ps_s = cstream[lane], cs = shape == 7 ? ps_s : s;
+            joined = true;
+            uint32_t* pass_done = d_sflags[lane] + ((q0 - slot0) / kMfmaMaxQueries) * kStreamSetWords * G;
+            uint32_t* packed_done = pass_done + G;
+            const size_t nblk = (nq + kPackQueries - 1) / kPackQueries;
+            HIPX(launch_fill_u32(pass_done, (1 + nblk) * G, 0, s));  // this set's last readers are done
+            if (cs != s) {
+                HIPX(hipEventRecord(prep_ev[lane], s));
+                HIPX(hipStreamWaitEvent(cs, prep_ev[lane], 0));
+            }
+            HIPX(launch_collapse_keys(keys, keys_stride, nq, d_kdig[lane], d_KS[lane], cs));
+            HIPX(launch_collapse(d_precomp, G, d_kdig[lane], d_KS[lane], nq, p2, part2_stride, cs, collapse_variant));
+            HIPX(hipEventRecord(coll_ev[lane][0], cs));
+            dbpass_prep(q0, nq, sp, s, lane);
+            uint32_t* interm = d_interm + q0 * 2 * C;
+            const bool direct = sp.slices == 1;
+            uint32_t* part = direct ? interm : d_part + q0 * part_stride;
+            const size_t part_qstride = direct ? 2 * C : part_stride;
+            PassStream pst;
+            pst.group_done = pass_done;
+            HIPX(launch_dbpass_mfma(db_planes(), d_dig[lane], d_S[lane], nq, part, part_qstride, C, pp.db_rows, C, 0,
+                                    C, sp, s, &pst));
+            if (ps_s != cs) HIPX(hipStreamWaitEvent(ps_s, coll_ev[lane][0], 0));
+            PackStream pk;
+            pk.part = part;
+            pk.part_qstride = part_qstride;
+            pk.slices = sp.slices;
+            pk.C = (uint32_t)C;
+            pk.pass_done = pass_done;
+            pk.pass_target = (uint32_t)(N / dbplan_cols(sp) * sp.slices);
+            pk.packed_done = packed_done;
+            pk.spin_limit = stream_spin_limit;
+            pk.status = d_status + lane;
+            HIPX(launch_pack_stream(pk, p2, part2_stride, d_precomp, keys, keys_stride, packed, packed_stride, G, nq,
+                                    nc, ps_s));
+            HIPX(hipStreamWaitEvent(hs, coll_ev[lane][0], 0));
+            StreamWait sw;
+            sw.done = packed_done;
+            sw.queries = kPackQueries;
+            sw.stride = (uint32_t)G;
+            sw.spin_limit = stream_spin_limit;
+            sw.status = d_status + lane;
+            HIPX(launch_horner(d_precomp + PC_A, PC_GROUP, G, Da, T, packed, packed_stride, d_rgsw + q0 * rgsw_stride,
+                               rgsw_stride, out, d_hstate + q0 * T * 4 * N, 0, G, nq, nc, cc, hs, FusedPack{}, sw));
+            // the next sub-batch's key digits and flags wait for the collapse and the pack
+            HIPX(hipEventRecord(prep_ev[lane], ps_s));
+            HIPX(hipStreamWaitEvent(s, prep_ev[lane], 0));
+            continue;
+        }
         if (shape == 0) {
             const DbPlan p = plan_for(nq, C);
             dbpass_prep(q0, nq, p, s, lane);
@@ -970,6 +1149,7 @@ void GpuServerCtx::answer_packed_host(const uint8_t* in, size_t count, uint8_t* 
     uint8_t* hr = reinterpret_cast<uint8_t*>(h_resp);  // so do compressed responses
     std::memcpy(hq, in, count * qpack_bytes);
     order_before(s);
+    clear_status(s, 0);
     HIPX(hipMemcpyAsync(d_qpack, hq, count * qpack_bytes, hipMemcpyHostToDevice, s));
     run_step(d_qpack, d_rcmp, count, 0, 0, 1, s, [&] { enqueue_packed(d_qpack, count, d_rcmp, s, 0, 0, true); });
     HIPX(hipMemcpyAsync(hr, d_rcmp, count * T * kCompressedCtBytes, hipMemcpyDeviceToHost, s));
@@ -986,8 +1166,15 @@ void GpuServerCtx::check_status(hipStream_t s, int lane) {
     if (st) {
         HIPX(hipMemsetAsync(d_status + lane, 0, 4, s));
         HIPX(hipStreamSynchronize(s));
-        throw std::runtime_error("Horner pair exchange exceeded its wait bound; responses of this call are invalid");
+        throw std::runtime_error(
+            "a Horner chain exceeded its wait bound (pair exchange or streamed database pass); responses of this "
+            "call are invalid");
     }
+}
+
+void GpuServerCtx::clear_status(hipStream_t s, int lane) {
+    // a kernel, not a memset: the call may be captured
+    HIPX(launch_fill_u32(d_status + lane, 1, 0, s));
 }
 
 void GpuServerCtx::order_before(hipStream_t s) {
@@ -1014,6 +1201,7 @@ void GpuServerCtx::answer_host(size_t count, uint64_t* out, hipStream_t s, size_
     uint64_t* dr = d_resp + slot0 * resp_u64s;
     TraceRange tr("host-staged answer: %zu queries", count);
     order_before(s);
+    clear_status(s, 0);
     HIPX(hipMemcpyAsync(dq, hq, count * qlen * 8, hipMemcpyHostToDevice, s));
     run_step(dq, dr, count, slot0, 0, 0, s, [&] { enqueue(dq, count, dr, s, slot0, 0, true); });
     HIPX(hipMemcpyAsync(hr, dr, count * resp_u64s * 8, hipMemcpyDeviceToHost, s));
@@ -1097,6 +1285,11 @@ void gpu_free_server(GpuServerCtx* ctx) {
     dfree(ctx->d_q); dfree(ctx->d_resp); dfree(ctx->d_bq); dfree(ctx->d_keys); dfree(ctx->d_rgsw);
     dfree(ctx->d_part); dfree(ctx->d_interm); dfree(ctx->d_part2); dfree(ctx->d_packed_b);
     dfree(ctx->d_hxbuf); dfree(ctx->d_hflags); dfree(ctx->d_status); dfree(ctx->d_hstate);
+    for (auto& f : ctx->d_sflags) dfree(f);
+    {
+        std::lock_guard<std::mutex> lk(g_stream_mu);
+        if ((size_t)ctx->device < kMaxDevices && g_streaming[ctx->device] == ctx) g_streaming[ctx->device] = nullptr;
+    }
     dfree(ctx->d_qpack); dfree(ctx->d_cmp); dfree(ctx->d_rcmp);
     for (int l = 0; l < GpuServerCtx::kLanes; l++) {
         if (ctx->hstream[l]) (void)hipStreamSynchronize(ctx->hstream[l]);

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <functional>
+#include <atomic>
 #include <mutex>
 #include <utility>
 #include <vector>
@@ -70,9 +71,27 @@ struct GpuServerCtx {
     // pipeline shape of one-query and many-query calls: 0 serial on the
     // caller's stream, 2 Horner chains beside the passes, 3 collapse beside
     // too, 4 the collapse of all groups before the database-pass chunks, 5
-    // collapse beside and a chunk ahead (2 or 3 timed at setup)
+    // collapse beside and a chunk ahead; streamed (one full-width database
+    // pass handing groups over as it runs, the chains waiting for them in
+    // the kernel): 6 after the collapse of all groups, 7 with the collapse
+    // beside (2, 3, 6 and 7 timed at setup)
     int pipe_single = 3, pipe_batched = 2;
+    int pipe_single_chunked = 3, pipe_batched_chunked = 3;  // where a streamed shape cannot run
     bool pipe_forced = false;       // INSPIRE_PIPELINE set the shapes (not autotuned)
+    // row slices of the streamed pass (one query, many), and whether this
+    // server may stream at all: the chains of a streamed answer hold their
+    // CUs while they wait, so only one server per device streams (at most
+    // kLanes x 32 waiting chains, half the CUs at most)
+    uint32_t stream_slices1 = 4, stream_slices = 4;
+    bool stream_ok = false;
+    // The streamed pass's plan for nq queries (the tuned kernel, the streamed
+    // slice count, at least the slices exactness needs); kind F64 if none fits.
+    ipx::DbPlan stream_plan(size_t nq, uint32_t slices) const;
+    // Whether an answer on s may take a streamed shape now (not under capture).
+    bool can_stream(hipStream_t s) const;
+    // Make this the device's streaming server (the previous one, if any,
+    // then answers with chunked shapes). The caller holds this server's mu.
+    void take_streaming();
     size_t qlen = 0, resp_u64s = 0;
     size_t part_stride = 0, part2_stride = 0, keys_stride = 0, rgsw_stride = 0, packed_stride = 0;
     uint64_t* d_q = nullptr;        // flat queries (device staging)
@@ -91,6 +110,12 @@ struct GpuServerCtx {
     static constexpr size_t kKdigBytes = 2 * (2 * ipx::N / 16) * 128 * 16;
     int8_t* d_kdig[kLanes] = {};
     uint32_t* d_KS[kLanes] = {};
+    // per lane, one set per 32-query sub-batch of a call: the streamed
+    // pass's per-group completion counts [G], then the streamed pack's
+    // per-(workgroup, group) publication flags [32 / kPackQueries][G]
+    static constexpr size_t kStreamSetWords = 1 + ipx::kMfmaMaxQueries / ipx::kPackQueries;
+    uint32_t* d_sflags[kLanes] = {};
+    size_t stream_sets = 0;
     // per lane: the streams the collapse and the Horner chains run on beside
     // the database pass, and the events that hand each group chunk over
     static constexpr int kChunkEvents = 16;
@@ -116,6 +141,7 @@ struct GpuServerCtx {
     // up its bounded wait (read and cleared by check_status)
     uint32_t* d_status = nullptr;   // [kLanes]
     uint32_t horner_spin_limit = 0; // the pair kernel's wait bound in sleeps (0: report at once)
+    uint32_t stream_spin_limit = 0; // the same for chains waiting for a streamed pass's groups
     uint32_t* d_hstate = nullptr;   // Horner chain state between chunk launches [B*T][4][N]
     uint64_t* h_q = nullptr;        // pinned host staging
     uint64_t* h_resp = nullptr;
@@ -191,9 +217,17 @@ struct GpuServerCtx {
     // Throw (and clear) if a pair chain of a call on `lane` reported a fault
     // through its status word; s is the stream the call ran on.
     void check_status(hipStream_t s, int lane);
-    // ~2 s of 64-cycle sleeps: far above any partner skew, bounded so a
-    // chain whose partner never runs still ends.
+    // Zero the lane's status word on s (before a call whose status is read,
+    // so an earlier asynchronous call's report is not taken for its own).
+    void clear_status(hipStream_t s, int lane);
+    std::atomic<size_t> n_streamed{0};  // streamed sub-batches enqueued (a test counter)
+    // Seconds of sleeps: far above any partner skew or any streamed pass's
+    // time to reach a group, bounded so a chain whose partner or pass never
+    // runs still ends.
     static constexpr uint32_t kHornerSpinLimit = 1u << 26;
+    // The streamed chains' bound: ~1 s or more of sleeps, far above a
+    // database pass's time even queued behind other batches on the device.
+    static constexpr uint32_t kStreamSpinLimit = 1u << 23;
     // Enqueue the answer of `count` flat queries at d_in (device) into d_out
     // (device) on stream s, using slots [slot0, slot0 + count) and scratch
     // set `lane` (work that may overlap on the device uses distinct lanes).

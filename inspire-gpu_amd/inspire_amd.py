@@ -93,6 +93,7 @@ SERVER_FUNCS = [
     ("ipir_fit_max_batch", SZ, [SZ, SZ, SZ, SZ, SZ]),
     ("ipir_dbg_db_block", C.c_int, [VP, SZ, SZ, SZ, SZ, VP]),
     ("ipir_dbg_set_horner_spin_limit", C.c_int, [VP, C.c_uint32]),
+    ("ipir_dbg_set_stream_spin_limit", C.c_int, [VP, C.c_uint32]),
     ("ipir_dbg_loop_hold", C.c_int, [VP, C.c_int]),
     ("ipir_dev_stream_create", VP, []),
     ("ipir_dev_stream_sync", C.c_int, [VP]),
@@ -123,6 +124,11 @@ SERVER_FUNCS = [
     ("ipir_dbg_set_collapse_variant", C.c_int, [VP, C.c_int]),
     ("ipir_dbg_get_collapse_variant", C.c_int, [VP]),
     ("ipir_dbg_set_pipeline", C.c_int, [VP, C.c_int, C.c_int]),
+    ("ipir_dbg_set_stream_slices", C.c_int, [VP, SZ, C.c_uint32]),
+    ("ipir_dbg_stream_ok", C.c_int, [VP]),
+    ("ipir_dbg_take_streaming", C.c_int, [VP]),
+    ("ipir_dbg_streamed_count", SZ, [VP]),
+    ("ipir_dbg_stream_parts", C.c_int, [VP, SZ, C.c_uint32, P(C.c_double)]),
     ("ipir_dbg_profile_batch", C.c_int, [VP, VP, SZ, VP, VP]),
     ("ipir_dev_alloc", VP, [SZ]),
     ("ipir_dev_free", C.c_int, [VP]),

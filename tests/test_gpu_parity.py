@@ -274,6 +274,7 @@ def test_ragged_batch_over_sub_batches(medium8):
             qs.append(q)
             sts.append(st)
         assert srv.lib.ipir_dbg_set_group_chunk(srv.h, geo["G"]) == 0
+        assert srv.lib.ipir_dbg_set_pipeline(srv.h, 0, 0) == 0
         serial = srv.answer_batch(qs)
         # chunks of 2; of 4 with the lowest split 2+1+1 (bit 0), the highest
         # 1+1+2 (bit 1), both, or neither
@@ -284,7 +285,23 @@ def test_ragged_batch_over_sub_batches(medium8):
                 assert srv.lib.ipir_dbg_set_pipeline(srv.h, *shapes) == 0
                 got = srv.answer_batch(qs)
                 assert np.array_equal(got, serial), f"groups {groups} split {split} shapes {shapes}"
+        # streamed shapes: each sub-batch with its own set of per-group
+        # completion words, the one-query one behind the 32-query one on the
+        # same streams; mixed with a chunked shape
+        assert srv.lib.ipir_dbg_take_streaming(srv.h) == 0
+        assert srv.lib.ipir_dbg_set_group_chunk(srv.h, 2) == 0
+        for shapes in ((6, 6), (7, 7), (6, 3), (3, 7)):
+            assert srv.lib.ipir_dbg_set_pipeline(srv.h, *shapes) == 0
+            for sl in (1, 4):
+                assert srv.lib.ipir_dbg_set_stream_slices(srv.h, 1, sl) == 0
+                assert srv.lib.ipir_dbg_set_stream_slices(srv.h, 32, sl) == 0
+                n0 = srv.lib.ipir_dbg_streamed_count(srv.h)
+                got = srv.answer_batch(qs)
+                assert np.array_equal(got, serial), f"streamed shapes {shapes} slices {sl}"
+                assert srv.lib.ipir_dbg_streamed_count(srv.h) - n0 == (shapes[0] >= 6) + (shapes[1] >= 6)
         # both sub-batches captured into one graph, replayed with new inputs
+        # (a captured answer takes the chunked shapes instead of the streamed ones)
+        assert srv.lib.ipir_dbg_set_pipeline(srv.h, 7, 7) == 0
         assert srv.lib.ipir_dbg_set_graph(srv.h, 1) == 0
         try:
             for rep in range(3):
@@ -389,6 +406,95 @@ def test_pipelined_group_chunks_equal_unchunked(medium8, groups, shape, slices):
     assert np.array_equal(got, base) and np.array_equal(got1[0], one[0]) and np.array_equal(pair, base)
     assert np.array_equal(got_wg, base) and np.array_equal(got1_wg[0], one[0])
     assert np.array_equal(got_fused, base) and np.array_equal(got1_fused[0], one[0])
+
+
+@pytest.mark.parametrize("shape", [6, 7])
+def test_streamed_pass_equals_serial(medium8, shape):
+    """One full-width database pass that hands each group over as soon as
+    its columns are final, the Horner chains waiting for the groups inside
+    the kernel and packing each one themselves; after the collapse of all
+    groups (6) or with the collapse beside (7); at 1, 2, 4 and 8 row slices
+    (one: straight into interm; more: a tile's last slice workgroup sums
+    the partials). One query, three and twenty (the power-row pass) equal
+    the serial answers, host-staged and through the device entry point."""
+    geo, db, srv = medium8
+    lib = srv.lib
+    idxs = [int(i) for i in np.random.default_rng(606).integers(0, 524288, 20)]
+    qs = [ol.make_query(geo, SEED, idx, 6100 + k)[0] for k, idx in enumerate(idxs)]
+    assert lib.ipir_dbg_take_streaming(srv.h) == 0
+    assert lib.ipir_dbg_stream_ok(srv.h) == 1
+    assert lib.ipir_dbg_set_stream_slices(srv.h, 1, 3) == 1  # not a power of two
+    assert lib.ipir_dbg_set_pipeline(srv.h, 0, 0) == 0
+    try:
+        base = {k: srv.answer_batch(qs[:k]) for k in (1, 3, 20)}
+        assert lib.ipir_dbg_set_pipeline(srv.h, shape, shape) == 0
+        for sl in (1, 2, 4, 8):
+            assert lib.ipir_dbg_set_stream_slices(srv.h, 1, sl) == 0
+            assert lib.ipir_dbg_set_stream_slices(srv.h, 20, sl) == 0
+            n0 = lib.ipir_dbg_streamed_count(srv.h)
+            for k in (1, 3, 20):
+                say(f"[streamed] shape={shape} slices={sl} queries={k}")
+                assert np.array_equal(srv.answer_batch(qs[:k]), base[k]), f"slices {sl}, {k} queries"
+            assert lib.ipir_dbg_streamed_count(srv.h) == n0 + 3, "every answer streamed"
+        flat = np.ascontiguousarray(np.stack(qs))
+        dq = ia.DeviceBuffer(flat.nbytes, lib)
+        dq.upload(flat)
+        dout = ia.DeviceBuffer(base[20].nbytes, lib)
+        for _ in range(2):
+            assert lib.ipir_answer_batch_device(srv.h, C.c_void_p(dq.p), 20, C.c_void_p(dout.p), None) == 0
+        assert lib.ipir_dev_sync() == 0
+        assert np.array_equal(dout.download(np.zeros_like(base[20])), base[20]), "device entry point"
+        dq.free()
+        dout.free()
+    finally:
+        assert lib.ipir_dbg_set_stream_slices(srv.h, 1, 4) == 0
+        assert lib.ipir_dbg_set_stream_slices(srv.h, 20, 4) == 0
+        assert lib.ipir_dbg_set_pipeline(srv.h, 3, 2) == 0
+
+
+def test_streamed_wait_overrun_reported(medium8):
+    """The streamed chains' bounded wait, forced to give up at once (spin
+    limit 0; the pass and every kernel still end normally): the blocking
+    call returns 4; the asynchronous device entry point returns 0 with
+    all-ones response words (no residue has that value); the serving loop
+    reports 4 for its request; once the bound is restored the next call is
+    not blamed for the earlier failures."""
+    geo, db, srv = medium8
+    lib = srv.lib
+    q, st = ol.make_query(geo, SEED, 777, 41)
+    assert lib.ipir_dbg_take_streaming(srv.h) == 0
+    assert lib.ipir_dbg_set_pipeline(srv.h, 0, 0) == 0
+    want = srv.answer_batch([q])[0]
+    out = np.zeros((1, geo["T"] * 4 * N), np.uint64)
+    views = srv.views([q])
+    assert lib.ipir_dbg_set_pipeline(srv.h, 7, 7) == 0
+    assert lib.ipir_dbg_set_stream_spin_limit(srv.h, 0) == 0
+    try:
+        assert lib.ipir_answer_batch(srv.h, views, 1, ia.ptr(out)) == 4
+        dq = ia.DeviceBuffer(q.nbytes, lib)
+        dq.upload(q)
+        dout = ia.DeviceBuffer(want.nbytes, lib)
+        assert lib.ipir_answer_batch_device(srv.h, C.c_void_p(dq.p), 1, C.c_void_p(dout.p), None) == 0
+        assert lib.ipir_dev_sync() == 0
+        got = dout.download(np.zeros_like(want))
+        assert np.all(got == np.uint64(2**64 - 1)), "invalid responses are all-ones"
+        dq.free()
+        dout.free()
+        loop = lib.ipir_loop_start(srv.h, 2)
+        assert loop
+        try:
+            o2 = np.zeros(geo["T"] * 4 * N, np.uint64)
+            t = lib.ipir_loop_submit(loop, views, ia.ptr(o2))
+            assert lib.ipir_loop_wait(loop, t, None) == 4
+        finally:
+            lib.ipir_loop_stop(loop)
+    finally:
+        assert lib.ipir_dbg_set_stream_spin_limit(srv.h, 1 << 23) == 0
+    try:
+        assert lib.ipir_answer_batch(srv.h, views, 1, ia.ptr(out)) == 0, "a later call is not blamed"
+        assert np.array_equal(out[0], want)
+    finally:
+        assert lib.ipir_dbg_set_pipeline(srv.h, 3, 2) == 0
 
 
 def test_late_chunk_slices_equal_unchunked(medium8):
@@ -620,7 +726,7 @@ def test_device_resident_entry_matches_host_path(medium8):
     dout.free()
 
 
-@pytest.mark.parametrize("groups,shape", [(8, 0), (2, 2), (2, 3), (2, 4), (2, 5)])
+@pytest.mark.parametrize("groups,shape", [(8, 0), (2, 2), (2, 3), (2, 4), (2, 5), (8, 6), (2, 7)])
 def test_graph_replays_equal_eager(medium8, groups, shape):
     """Answers replayed from captured hipGraphs (serial, and pipelined over
     group chunks on two and three streams) equal the launch-by-launch ones,
@@ -632,6 +738,8 @@ def test_graph_replays_equal_eager(medium8, groups, shape):
     sets = {k: [[ol.make_query(geo, SEED, (97 * r + 131071 * j) % 524288, 5000 + 100 * r + j)[0]
                  for j in range(k)] for r in range(3)] for k in (1, 3, 20)}
     pset = [np.stack([cl.pack_query(q) for q in sets[3][r]]) for r in range(3)]
+    if shape >= 6:  # eager answers stream; captured ones take the chunked (or serial) shape
+        assert srv.lib.ipir_dbg_take_streaming(srv.h) == 0
     assert srv.lib.ipir_dbg_set_group_chunk(srv.h, groups) == 0
     assert srv.lib.ipir_dbg_set_pipeline(srv.h, shape, shape) == 0
     try:
