@@ -173,6 +173,11 @@ int ipir_dbg_set_pipeline(ipir_server* srv, int single, int batched);
  * exactness bounds need, and ignored (chunked shape) where the partials
  * would not fit their buffer. */
 int ipir_dbg_set_stream_slices(ipir_server* srv, size_t nq_class, uint32_t slices);
+/* Split the streamed pass of one-query (nq_class 1) or many-query answers
+ * into two launches: the top `eighths` eighths of the groups first (0..7;
+ * 0: one launch), the Horner chains starting only after that launch (they
+ * hold their CUs from then on). */
+int ipir_dbg_set_stream_split(ipir_server* srv, size_t nq_class, uint32_t eighths);
 /* 1 if this server may take the streamed shapes (the device's first live
  * server), else 0. */
 int ipir_dbg_stream_ok(const ipir_server* srv);

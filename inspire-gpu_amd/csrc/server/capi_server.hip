@@ -351,13 +351,15 @@ int ipir_dbg_tuning(const ipir_server* srv, char* out, size_t cap) {
     s += "; collapse variant " + std::to_string(c->collapse_variant) + "; pipeline shapes " +
          std::to_string(c->pipe_single) + "/" + std::to_string(c->pipe_batched);
     if (c->pipe_single >= 6 || c->pipe_batched >= 6) {
-        auto sp = [&](size_t n, uint32_t sl) {
+        auto sp = [&](size_t n, uint32_t sl, uint32_t split) {
             const ipx::DbPlan p = c->stream_plan(n, sl);
-            return p.kind == ipx::DbPlan::MFMA ? plan(p) : std::string("none");
+            return p.kind == ipx::DbPlan::MFMA
+                       ? plan(p) + (split ? " split " + std::to_string(split) + "/8" : std::string())
+                       : std::string("none");
         };
-        s += " (streamed: batch1 " + sp(1, c->stream_slices1) + ", batch" + std::to_string(nq) + " " +
-             sp(nq, c->stream_slices) + "; chunked fallback " + std::to_string(c->pipe_single_chunked) + "/" +
-             std::to_string(c->pipe_batched_chunked) + ")";
+        s += " (streamed: batch1 " + sp(1, c->stream_slices1, c->stream_split1) + ", batch" + std::to_string(nq) +
+             " " + sp(nq, c->stream_slices, c->stream_split) + "; chunked fallback " +
+             std::to_string(c->pipe_single_chunked) + "/" + std::to_string(c->pipe_batched_chunked) + ")";
     }
     std::snprintf(out, cap, "%s", s.c_str());
     return 0;
@@ -584,6 +586,15 @@ int ipir_dbg_set_stream_slices(ipir_server* srv, size_t nq_class, uint32_t slice
     std::lock_guard<std::mutex> lk(srv->ctx->mu);
     if (srv->ctx->loop_active) return 5;
     (nq_class == 1 ? srv->ctx->stream_slices1 : srv->ctx->stream_slices) = slices;
+    srv->ctx->drop_graphs();
+    return 0;
+}
+
+int ipir_dbg_set_stream_split(ipir_server* srv, size_t nq_class, uint32_t eighths) {
+    if (!srv || eighths > 7) return 1;
+    std::lock_guard<std::mutex> lk(srv->ctx->mu);
+    if (srv->ctx->loop_active) return 5;
+    (nq_class == 1 ? srv->ctx->stream_split1 : srv->ctx->stream_split) = eighths;
     srv->ctx->drop_graphs();
     return 0;
 }

@@ -109,9 +109,12 @@ hipError_t launch_mfma_digits(const double* bq, size_t nq, size_t R, const DbPla
 // reaches (N / dbplan_cols(p)) * p.slices. group_done must be zero before
 // the pass. The pass itself never waits for anything, so whatever waits
 // beside it for its groups cannot stall it beyond taking CUs.
+// A streamed launch may cover a whole number of groups [col0 / N, ...)
+// (a pass split into launches): its counts go to group_done[col0 / N + ...].
 struct PassStream {
     uint32_t* group_done = nullptr;
-    uint32_t tiles = 0;        // column tiles of the pass (C / dbplan_cols(p))
+    uint32_t group_base = 0;   // first group of the launch
+    uint32_t tiles = 0;        // column tiles of the launch (ncols / dbplan_cols(p))
     uint32_t slice_shift = 0;  // log2 of the plan's row slices
     uint32_t tiles_per_group = 0;
 };

@@ -303,7 +303,7 @@ __device__ __forceinline__ void pass_handover(const PassStream& ps, uint32_t xt)
     __syncthreads();
     if (threadIdx.x == 0) {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-        __hip_atomic_fetch_add(ps.group_done + xt / ps.tiles_per_group, 1u, __ATOMIC_RELAXED,
+        __hip_atomic_fetch_add(ps.group_done + ps.group_base + xt / ps.tiles_per_group, 1u, __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
     }
 }
@@ -854,9 +854,10 @@ hipError_t launch_dbpass_mfma(const uint8_t* db, const int8_t* dig, const uint32
         uint32_t sh = 0;
         while ((1u << sh) < p.slices) sh++;
         st.slice_shift = sh;
-        st.tiles = (uint32_t)(C / cols);
+        st.tiles = (uint32_t)(ncols / cols);
         st.tiles_per_group = (uint32_t)(N / cols);
-        if (!dbplan_streams(p, C) || col0 != 0 || ncols != C || !st.group_done || ostride != C ||
+        st.group_base = (uint32_t)(col0 / N);
+        if (!dbplan_streams(p, C) || col0 % N || ncols % N || ncols == 0 || !st.group_done || ostride != C ||
             (size_t)st.tiles * p.slices > 0x7fffffffu)
             return hipErrorInvalidValue;
         grid = dim3(st.tiles * p.slices, 1);

@@ -409,6 +409,7 @@ void GpuServerCtx::alloc_slots() {
         for (auto& e : chunk_ev[l]) HIPX(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         for (auto& e : coll_ev[l]) HIPX(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         HIPX(hipEventCreateWithFlags(&prep_ev[l], hipEventDisableTiming));
+        HIPX(hipEventCreateWithFlags(&pass_ev[l], hipEventDisableTiming));
         HIPX(hipEventCreateWithFlags(&horner_done[l], hipEventDisableTiming));
     }
     stream_sets = (B + kMfmaMaxQueries - 1) / kMfmaMaxQueries;
@@ -703,6 +704,8 @@ void GpuServerCtx::autotune_dbpass() {
                 (count == 1 ? pipe_single_chunked : pipe_batched_chunked) = c.shape;
                 if (!streamed.empty()) streamed.insert(streamed.begin(), c);
             }
+            // (a split streamed pass, the chains released after its first
+            // launch, never won at 16 GB: set by hand only)
             if (!streamed.empty()) time_picks(streamed);
             if (nq == 1) break;
         }
@@ -966,10 +969,22 @@ void GpuServerCtx::enqueue(const uint64_t* d_in, size_t count, uint64_t* d_out, 
             const bool direct = sp.slices == 1;
             uint32_t* part = direct ? interm : d_part + q0 * part_stride;
             const size_t part_qstride = direct ? 2 * C : part_stride;
+            // split: the top eighths of the groups in a first launch, the
+            // chains released after it (they hold their CUs only from then)
+            const uint32_t split = nq == 1 ? stream_split1 : stream_split;
+            const size_t g_split = split ? G - G * split / 8 : 0;
             PassStream pst;
             pst.group_done = pass_done;
-            HIPX(launch_dbpass_mfma(db_planes(), d_dig[lane], d_S[lane], nq, part, part_qstride, C, pp.db_rows, C, 0,
-                                    C, sp, s, &pst));
+            if (g_split > 0 && g_split < G) {
+                HIPX(launch_dbpass_mfma(db_planes(), d_dig[lane], d_S[lane], nq, part + g_split * N, part_qstride, C,
+                                        pp.db_rows, C, g_split * N, (G - g_split) * N, sp, s, &pst));
+                HIPX(hipEventRecord(pass_ev[lane], s));
+                HIPX(launch_dbpass_mfma(db_planes(), d_dig[lane], d_S[lane], nq, part, part_qstride, C, pp.db_rows,
+                                        C, 0, g_split * N, sp, s, &pst));
+            } else {
+                HIPX(launch_dbpass_mfma(db_planes(), d_dig[lane], d_S[lane], nq, part, part_qstride, C, pp.db_rows,
+                                        C, 0, C, sp, s, &pst));
+            }
             if (ps_s != cs) HIPX(hipStreamWaitEvent(ps_s, coll_ev[lane][0], 0));
             PackStream pk;
             pk.part = part;
@@ -984,6 +999,7 @@ void GpuServerCtx::enqueue(const uint64_t* d_in, size_t count, uint64_t* d_out, 
             HIPX(launch_pack_stream(pk, p2, part2_stride, d_precomp, keys, keys_stride, packed, packed_stride, G, nq,
                                     nc, ps_s));
             HIPX(hipStreamWaitEvent(hs, coll_ev[lane][0], 0));
+            if (g_split > 0 && g_split < G) HIPX(hipStreamWaitEvent(hs, pass_ev[lane], 0));
             StreamWait sw;
             sw.done = packed_done;
             sw.queries = kPackQueries;
@@ -1299,6 +1315,7 @@ void gpu_free_server(GpuServerCtx* ctx) {
         for (auto& e : ctx->coll_ev[l])
             if (e) (void)hipEventDestroy(e);
         if (ctx->prep_ev[l]) (void)hipEventDestroy(ctx->prep_ev[l]);
+        if (ctx->pass_ev[l]) (void)hipEventDestroy(ctx->pass_ev[l]);
         if (ctx->horner_done[l]) (void)hipEventDestroy(ctx->horner_done[l]);
         if (ctx->hstream[l]) (void)hipStreamDestroy(ctx->hstream[l]);
         if (ctx->cstream[l]) (void)hipStreamDestroy(ctx->cstream[l]);

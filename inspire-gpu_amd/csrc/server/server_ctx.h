@@ -83,6 +83,9 @@ struct GpuServerCtx {
     // CUs while they wait, so only one server per device streams (at most
     // kLanes x 32 waiting chains, half the CUs at most)
     uint32_t stream_slices1 = 4, stream_slices = 4;
+    // eighths of the groups (from the top) in a streamed pass's first
+    // launch, the chains starting after it (0: one launch, chains at once)
+    uint32_t stream_split1 = 0, stream_split = 0;
     bool stream_ok = false;
     // The streamed pass's plan for nq queries (the tuned kernel, the streamed
     // slice count, at least the slices exactness needs); kind F64 if none fits.
@@ -124,6 +127,7 @@ struct GpuServerCtx {
     hipEvent_t chunk_ev[kLanes][kChunkEvents] = {};  // database pass of a chunk done
     hipEvent_t coll_ev[kLanes][kChunkEvents] = {};   // collapse of a chunk done
     hipEvent_t prep_ev[kLanes] = {};                 // keys of a batch in place
+    hipEvent_t pass_ev[kLanes] = {};                 // a split streamed pass's first launch done
     hipEvent_t horner_done[kLanes] = {};
     uint32_t* d_packed_b = nullptr;
     // Horner: 0 = one workgroup per chain, 1 = a workgroup pair (one per

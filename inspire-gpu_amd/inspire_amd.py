@@ -126,6 +126,7 @@ SERVER_FUNCS = [
     ("ipir_dbg_set_pipeline", C.c_int, [VP, C.c_int, C.c_int]),
     ("ipir_dbg_set_stream_slices", C.c_int, [VP, SZ, C.c_uint32]),
     ("ipir_dbg_stream_ok", C.c_int, [VP]),
+    ("ipir_dbg_set_stream_split", C.c_int, [VP, SZ, C.c_uint32]),
     ("ipir_dbg_take_streaming", C.c_int, [VP]),
     ("ipir_dbg_streamed_count", SZ, [VP]),
     ("ipir_dbg_stream_parts", C.c_int, [VP, SZ, C.c_uint32, P(C.c_double)]),

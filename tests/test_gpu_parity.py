@@ -414,9 +414,10 @@ def test_streamed_pass_equals_serial(medium8, shape):
     its columns are final, the Horner chains waiting for the groups inside
     the kernel and packing each one themselves; after the collapse of all
     groups (6) or with the collapse beside (7); at 1, 2, 4 and 8 row slices
-    (one: straight into interm; more: a tile's last slice workgroup sums
-    the partials). One query, three and twenty (the power-row pass) equal
-    the serial answers, host-staged and through the device entry point."""
+    (one: straight into interm; more: the pack sums the partials), in one
+    launch or split in two (the chains released after the first). One
+    query, three and twenty (the power-row pass) equal the serial answers,
+    host-staged and through the device entry point."""
     geo, db, srv = medium8
     lib = srv.lib
     idxs = [int(i) for i in np.random.default_rng(606).integers(0, 524288, 20)]
@@ -428,13 +429,16 @@ def test_streamed_pass_equals_serial(medium8, shape):
     try:
         base = {k: srv.answer_batch(qs[:k]) for k in (1, 3, 20)}
         assert lib.ipir_dbg_set_pipeline(srv.h, shape, shape) == 0
-        for sl in (1, 2, 4, 8):
+        assert lib.ipir_dbg_set_stream_split(srv.h, 1, 8) == 1
+        for sl, split in ((1, 0), (2, 0), (4, 0), (8, 0), (16, 0), (2, 3), (8, 4), (1, 7)):
             assert lib.ipir_dbg_set_stream_slices(srv.h, 1, sl) == 0
             assert lib.ipir_dbg_set_stream_slices(srv.h, 20, sl) == 0
+            assert lib.ipir_dbg_set_stream_split(srv.h, 1, split) == 0
+            assert lib.ipir_dbg_set_stream_split(srv.h, 20, split) == 0
             n0 = lib.ipir_dbg_streamed_count(srv.h)
             for k in (1, 3, 20):
-                say(f"[streamed] shape={shape} slices={sl} queries={k}")
-                assert np.array_equal(srv.answer_batch(qs[:k]), base[k]), f"slices {sl}, {k} queries"
+                say(f"[streamed] shape={shape} slices={sl} split={split} queries={k}")
+                assert np.array_equal(srv.answer_batch(qs[:k]), base[k]), f"slices {sl} split {split}, {k} queries"
             assert lib.ipir_dbg_streamed_count(srv.h) == n0 + 3, "every answer streamed"
         flat = np.ascontiguousarray(np.stack(qs))
         dq = ia.DeviceBuffer(flat.nbytes, lib)
@@ -449,6 +453,8 @@ def test_streamed_pass_equals_serial(medium8, shape):
     finally:
         assert lib.ipir_dbg_set_stream_slices(srv.h, 1, 4) == 0
         assert lib.ipir_dbg_set_stream_slices(srv.h, 20, 4) == 0
+        assert lib.ipir_dbg_set_stream_split(srv.h, 1, 0) == 0
+        assert lib.ipir_dbg_set_stream_split(srv.h, 20, 0) == 0
         assert lib.ipir_dbg_set_pipeline(srv.h, 3, 2) == 0
 
 
