@@ -119,7 +119,8 @@ ipir_client_query* ipir_query_build_seeded(const ipir_params* params, uint64_t i
  * run their Horner chains beside a database pass they wait for inside the
  * kernel; such a wait is bounded (about a second or more), and a chain
  * that gives up writes all-ones words as its response (no residue has
- * that value; it can happen only if the pass gets no CU for that long). */
+ * that value; it can happen only if the pass gets no CU for that long)
+ * and is reported by ipir_device_status. */
 int ipir_answer_batch_device(ipir_server* srv, const uint64_t* d_queries, size_t count,
                              uint64_t* d_out_resp, void* hip_stream);
 
@@ -137,6 +138,13 @@ int ipir_answer_batch_packed(ipir_server* srv, const uint8_t* packed_queries, si
  * gives up its bounded wait leaves its compressed response undefined). */
 int ipir_answer_batch_packed_device(ipir_server* srv, const uint8_t* d_packed_queries, size_t count,
                                     uint8_t* d_out_compressed, void* hip_stream);
+
+/* For callers of the asynchronous entry points (no counterpart in the
+ * reference): waits for the server's last call and returns 4 if a Horner
+ * chain of an asynchronous call gave up a bounded wait since the last
+ * blocking call or status query (those responses are invalid), else 0;
+ * then clears the report. 5 while a serving loop owns the server. */
+int ipir_device_status(ipir_server* srv);
 
 /* Device memory of a server of this geometry (setup()'s rules; 1 if it
  * rejects the geometry): resident_fixed (DB, precomp, fixed scratch) plus

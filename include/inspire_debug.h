@@ -162,8 +162,10 @@ int ipir_dbg_get_collapse_variant(const ipir_server* srv);
  * full-width database pass that hands each group over as its columns are
  * final, the Horner chains waiting for the groups inside the kernel (a
  * bounded wait, reported like the pair kernel's), 6 after the collapse of
- * all groups, 7 with the collapse beside it. The setup autotuner picks 2,
- * 3, 6 or 7 (4 and 5 are tested alternatives that never won at 16 GB).
+ * all groups, 7 with the collapse beside it, 8 with the collapse inside the
+ * pass's launch, each group's collapse and pass work items together (up to
+ * 8 queries; 6 for more). The setup autotuner picks 2, 3, 6 or 8 (4, 5
+ * and 7 are tested alternatives that never won at 16 GB).
  * Answers under stream capture, and every answer of a server that is not
  * the device's first live one, take the chunked shape the autotuner picked
  * instead of a streamed one. 1 for other values. */

@@ -116,6 +116,21 @@ int ipir_answer_batch_device(ipir_server* srv, const uint64_t* d_queries, size_t
     }
 }
 
+int ipir_device_status(ipir_server* srv) {
+    if (!srv) return 1;
+    try {
+        GpuServerCtx* c = srv->ctx;
+        std::lock_guard<std::mutex> lk(c->mu);
+        if (c->loop_active) return 5;
+        hipStream_t s = c->stream;
+        c->order_before(s);  // the last call's work, whatever stream it ran on
+        c->check_status(s, 0);
+        return 0;
+    } catch (...) {
+        return 4;
+    }
+}
+
 int ipir_answer_batch_packed(ipir_server* srv, const uint8_t* packed_queries, size_t count,
                              uint8_t* out_compressed) {
     if (!srv || !packed_queries || !out_compressed) return 1;
@@ -350,7 +365,7 @@ int ipir_dbg_tuning(const ipir_server* srv, char* out, size_t cap) {
     }
     s += "; collapse variant " + std::to_string(c->collapse_variant) + "; pipeline shapes " +
          std::to_string(c->pipe_single) + "/" + std::to_string(c->pipe_batched);
-    if (c->pipe_single >= 6 || c->pipe_batched >= 6) {
+    if (c->pipe_single >= 6 || c->pipe_batched >= 6) {  // streamed shapes
         auto sp = [&](size_t n, uint32_t sl, uint32_t split) {
             const ipx::DbPlan p = c->stream_plan(n, sl);
             return p.kind == ipx::DbPlan::MFMA
@@ -708,7 +723,7 @@ int ipir_dbg_take_streaming(ipir_server* srv) {
 }
 
 int ipir_dbg_set_pipeline(ipir_server* srv, int single, int batched) {
-    auto ok = [](int v) { return v == 0 || (v >= 2 && v <= 7); };
+    auto ok = [](int v) { return v == 0 || (v >= 2 && v <= 8); };
     if (!srv || !ok(single) || !ok(batched)) return 1;
     std::lock_guard<std::mutex> lk(srv->ctx->mu);
     if (srv->ctx->loop_active) return 5;

@@ -145,6 +145,18 @@ hipError_t launch_collapse_keys(const uint32_t* keys, size_t kstride, size_t nq,
 constexpr int kCollapseVariants = 6;
 hipError_t launch_collapse(const uint32_t* precomp_g0, size_t ng, const int8_t* kdig, const uint32_t* KS, size_t nq,
                            uint32_t* part2_g0, size_t p2_stride, hipStream_t s, int variant = 0);
+// A streamed pass of up to 8 queries with the collapse in the same launch:
+// per group from the top, the collapse (all groups' precomp, part2 as in
+// launch_collapse) and the database pass of plan p (a direct-kernel plan,
+// 256-column tiles; out as a streamed pass's) as 256-lane work items;
+// group_done[g] counts them (zero before the launch) and reaches
+// pass_collapse_target(p, nq, G) when both halves of group g are in place.
+// The key digits (launch_collapse_keys) and query digits must be ready.
+uint32_t pass_collapse_target(const DbPlan& p, size_t nq, size_t G);
+hipError_t launch_pass_collapse_stream(const uint8_t* db, const int8_t* dig, const uint32_t* S, size_t nq,
+                                       uint32_t* out, size_t out_qstride, size_t R, size_t C, const DbPlan& p,
+                                       const uint32_t* precomp, const int8_t* kdig, const uint32_t* KS,
+                                       uint32_t* part2, size_t p2_stride, uint32_t* group_done, hipStream_t s);
 // packed_b[q][g] = NTT(interm[q] group slice) - collapse sums - Df*Kn, for
 // groups [g0, g0 + ng) (absolute indexes into every buffer).
 hipError_t launch_pack_finalize(const uint32_t* interm, size_t C, const uint32_t* part2, size_t part2_stride,

@@ -12,13 +12,13 @@
 // per limb a (2N x 2N-row) by N matrix-vector product per query, which a
 // batch turns into one int8 MFMA contraction that reads Dd once.
 #include "kernels.h"
+#include "dbpass_direct.h"
 
 namespace ipx {
 
 namespace {
 
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-typedef int32_t i32x4 __attribute__((ext_vector_type(4)));
 
 // e with 5^e = x mod 2N, for x = 1 mod 4 (bit by bit: 5^(2^k) = 1 + 2^(k+2)
 // mod 2^(k+3), so the k-th bit is set iff x*5^-e is not 1 mod 2^(k+3)).
@@ -178,19 +178,19 @@ __device__ __forceinline__ void collapse_store(const i32x4 (&acc)[4], uint32_t q
 // ahead). The key digits of the step (Mp x 64 bytes) are read by each wave
 // from L2 (SHARED = false) or staged once per block in an LDS double
 // buffer (SHARED = true, for large MT).
+// (the body, for slot tile `it`, limb c and group g; the kernel's grid is
+// (tile, limb, group))
 template <int MT, int NT, bool SHARED>
-__global__ __launch_bounds__(256, 2) void k_collapse_mfma(const uint32_t* __restrict__ precomp,
-                                                          const int8_t* __restrict__ kdig,
-                                                          const uint32_t* __restrict__ KS,
-                                                          uint32_t* __restrict__ part2, size_t p2_stride,
-                                                          uint32_t nq) {
+__device__ __forceinline__ void collapse_mfma_body(const uint32_t* __restrict__ precomp,
+                                                   const int8_t* __restrict__ kdig,
+                                                   const uint32_t* __restrict__ KS,
+                                                   uint32_t* __restrict__ part2, size_t p2_stride, uint32_t nq,
+                                                   uint32_t it, int c, uint32_t g) {
     constexpr uint32_t Mp = 16 * MT;
     constexpr uint32_t NKB = DD_K / 16;
     constexpr uint32_t AU = SHARED ? Mp * 4 / 256 : 1;  // 16-byte units of a step's digit tile per thread
     __shared__ i32x4 atile[SHARED ? 2 : 1][SHARED ? 4 * Mp : 1];
     const uint32_t tid = threadIdx.x, w = tid >> 6, l = tid & 63;
-    const uint32_t it = blockIdx.x, g = blockIdx.z;
-    const int c = (int)blockIdx.y;
     const uint32_t col0 = (it * 4 + w) * NT * 16;
     const i32x4* dd = reinterpret_cast<const i32x4*>(precomp + (size_t)g * PC_GROUP + PC_DD) +
                       (size_t)c * 4 * NKB * N + (size_t)(l >> 4) * N + col0 + (l & 15);
@@ -270,6 +270,51 @@ __global__ __launch_bounds__(256, 2) void k_collapse_mfma(const uint32_t* __rest
         if (q >= nq) continue;
 #pragma unroll
         for (int n = 0; n < NT; n++) collapse_store(acc[a][n], q, c, g, col0 + n * 16 + (l & 15), KS, part2, p2_stride);
+    }
+}
+
+template <int MT, int NT, bool SHARED>
+__global__ __launch_bounds__(256, 2) void k_collapse_mfma(const uint32_t* __restrict__ precomp,
+                                                          const int8_t* __restrict__ kdig,
+                                                          const uint32_t* __restrict__ KS,
+                                                          uint32_t* __restrict__ part2, size_t p2_stride,
+                                                          uint32_t nq) {
+    collapse_mfma_body<MT, NT, SHARED>(precomp, kdig, KS, part2, p2_stride, nq, blockIdx.x, (int)blockIdx.y,
+                                       blockIdx.z);
+}
+
+// Streamed pass and collapse in one launch (up to 8 queries): per group,
+// from the top, its collapse work items (CNT-wide slot tiles of both
+// limbs) and then its database-pass work items (the row slices of its
+// PNT-wide column tiles), all 256-lane workgroups of the same footprint, so
+// the two HBM streams of a group run together and groups complete in the
+// order the Horner chains consume them. Every work item, once its stores
+// are complete (one agent-scope release per workgroup), adds 1 to its
+// group's counter; a group is in place at 2 N/(64 CNT) + (N/(64 PNT)) x
+// slices.
+template <int PMT, int PNT, int CMT, int CNT>
+__global__ __launch_bounds__(256, 2) void k_pass_collapse_stream(
+    const uint8_t* __restrict__ db, const int8_t* __restrict__ dig, const uint32_t* __restrict__ S,
+    uint32_t* __restrict__ out, size_t out_qstride, uint32_t C, uint32_t kbps, const uint32_t* __restrict__ precomp,
+    const int8_t* __restrict__ kdig, const uint32_t* __restrict__ KS, uint32_t* __restrict__ part2,
+    size_t p2_stride, uint32_t nq, uint32_t G, uint32_t slice_shift, uint32_t* __restrict__ group_done) {
+    constexpr uint32_t CIT = N / (64 * CNT);  // collapse slot tiles per limb
+    constexpr uint32_t CB = 2 * CIT;          // collapse work items per group
+    constexpr uint32_t TPG = N / (64 * PNT);  // pass column tiles per group
+    const uint32_t per = CB + (TPG << slice_shift);
+    const uint32_t b = blockIdx.x, gi = b / per, r = b - gi * per, g = G - 1 - gi;
+    if (r < CB) {
+        collapse_mfma_body<CMT, CNT, false>(precomp, kdig, KS, part2, p2_stride, nq, r % CIT, (int)(r / CIT), g);
+    } else {
+        const uint32_t pr = r - CB;
+        dbpass_direct_body<PMT, PNT>(db, dig, S, out, out_qstride, C, C, nq, kbps, g * TPG + (pr >> slice_shift),
+                                     pr & ((1u << slice_shift) - 1));
+    }
+    __builtin_amdgcn_s_waitcnt(0);  // this wave's stores are complete
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __hip_atomic_fetch_add(group_done + g, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
@@ -386,6 +431,41 @@ hipError_t launch_collapse_keys(const uint32_t* keys, size_t kstride, size_t nq,
     hipLaunchKernelGGL(k_kdigits, dim3((unsigned)((nthr + 255) / 256)), dim3(256), 0, s, keys, kstride, (uint32_t)nq,
                        Mp, kdig);
     hipLaunchKernelGGL(k_ksum, dim3((unsigned)(2 * nq)), dim3(256), 0, s, keys, kstride, KS);
+    return hipGetLastError();
+}
+
+uint32_t pass_collapse_target(const DbPlan& p, size_t nq, size_t G) {
+    const uint32_t cnt = collapse_mt(nq) == 1 && G < 16 ? 1 : 2;
+    return (uint32_t)(2 * (N / (64 * cnt)) + (N / dbplan_cols(p)) * p.slices);
+}
+
+hipError_t launch_pass_collapse_stream(const uint8_t* db, const int8_t* dig, const uint32_t* S, size_t nq,
+                                       uint32_t* out, size_t out_qstride, size_t R, size_t C, const DbPlan& p,
+                                       const uint32_t* precomp, const int8_t* kdig, const uint32_t* KS,
+                                       uint32_t* part2, size_t p2_stride, uint32_t* group_done, hipStream_t s) {
+    const size_t G = C / N;
+    if (p.kind != DbPlan::MFMA || p.param > 4 || nq == 0 || nq > 8 || nq > dbplan_max_queries(p) || C % N ||
+        !group_done || p.slices == 0 || (p.slices & (p.slices - 1)) || dbplan_cols(p) != 256)
+        return hipErrorInvalidValue;
+    const uint32_t kbps = (uint32_t)(R / 16 / p.slices);
+    if (kbps % 8 || kbps == 0) return hipErrorInvalidValue;
+    uint32_t sh = 0;
+    while ((1u << sh) < p.slices) sh++;
+    const uint32_t cmt = collapse_mt(nq), cnt = cmt == 1 && G < 16 ? 1 : 2;
+    const size_t blocks = G * pass_collapse_target(p, nq, G);
+    if (blocks > 0x7fffffffu) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)blocks);
+    const uint32_t Cc = (uint32_t)C, Q = (uint32_t)nq, Gg = (uint32_t)G;
+#define PCS(PMT, CMT, CNT)                                                                                    \
+    hipLaunchKernelGGL((k_pass_collapse_stream<PMT, 4, CMT, CNT>), grid, dim3(256), 0, s, db, dig, S, out, \
+                       out_qstride, Cc, kbps, precomp, kdig, KS, part2, p2_stride, Q, Gg, sh, group_done)
+    switch (p.param * 10 + cmt) {
+        case 11: if (cnt == 1) PCS(1, 1, 1); else PCS(1, 1, 2); break;
+        case 21: if (cnt == 1) PCS(2, 1, 1); else PCS(2, 1, 2); break;
+        case 42: PCS(4, 2, 2); break;
+        default: return hipErrorInvalidValue;
+    }
+#undef PCS
     return hipGetLastError();
 }
 

@@ -15,13 +15,13 @@
 #include <string>
 
 #include "kernels.h"
+#include "dbpass_direct.h"
 
 namespace ipx {
 
 namespace {
 
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-typedef int32_t i32x4 __attribute__((ext_vector_type(4)));
 
 // a mod q for an integer-valued double 0 <= a < 2^53.
 __device__ __forceinline__ uint32_t fmod_exact(double a, double q, double qinv) {
@@ -264,23 +264,6 @@ __global__ __launch_bounds__(256) void k_bsum(const double* __restrict__ bq, uin
     if (threadIdx.x == 0) S[pr] = red64(red[0], (int)(pr & 1));
 }
 
-// Recombine one lane's accumulators of pair pr and store the canonical residue.
-__device__ __forceinline__ void store_pair(const i32x4& al, const i32x4& ah, uint32_t pr, uint32_t nq, bool corr,
-                                           const uint32_t* __restrict__ S, uint32_t* __restrict__ out,
-                                           size_t out_qstride, uint32_t slice, uint32_t OC, uint32_t col) {
-    const uint32_t q = pr >> 1;
-    const int c = (int)(pr & 1);
-    if (q >= nq) return;
-    int64_t s = corr ? (int64_t)32896 * S[pr] : 0;
-#pragma unroll
-    for (int i = 0; i < 4; i++) s += ((int64_t)ah[i] * 256 + (int64_t)al[i]) * ((int64_t)1 << (8 * i));
-    // |s| < 2^58.1 (|d_3| <= 8 bounds the top digit's term); add a multiple
-    // of q_c above that to make it non-negative
-    const uint64_t qc = c ? Q1 : Q0;
-    const uint64_t off = qc * ((1ull << 59) / qc);
-    out[(size_t)q * out_qstride + (size_t)slice * 2 * OC + (size_t)c * OC + col] =
-        red64((uint64_t)(s + (int64_t)off), c);
-}
 
 // Column tile and row slice of this workgroup: the 2-D grid's, or in a
 // streamed pass the 1-D index read as (tile counted from the top, slice).
@@ -308,10 +291,7 @@ __device__ __forceinline__ void pass_handover(const PassStream& ps, uint32_t xt)
     }
 }
 
-// Up to 2*MT queries (Mp = 16*MT digit rows): the 4 waves of a block own
-// disjoint column ranges (NT 16-column tiles each) and load their DB operands
-// straight from HBM into registers, one 64-row step ahead; every wave reads
-// all MT digit tiles (L2/L1-resident). Two waves per SIMD.
+// Up to 2*MT queries: dbpass_direct_body over the grid's (tile, slice).
 template <int MT, int NT>
 __global__ __launch_bounds__(256, 2) void k_dbpass_mfma(const uint8_t* __restrict__ db,
                                                         const int8_t* __restrict__ dig,
@@ -319,59 +299,9 @@ __global__ __launch_bounds__(256, 2) void k_dbpass_mfma(const uint8_t* __restric
                                                         uint32_t* __restrict__ out, size_t out_qstride,
                                                         uint32_t C, uint32_t OC, uint32_t nq, uint32_t kbps,
                                                         PassStream ps) {
-    constexpr uint32_t Mp = 16 * MT;
-    const uint32_t w = threadIdx.x >> 6, l = threadIdx.x & 63;
     uint32_t xt, slice;
     pass_coords(ps, xt, slice);
-    const uint32_t col0 = (xt * 4 + w) * NT * 16;
-    const uint32_t kb0 = slice * kbps;
-    const uint32_t nks = kbps / 4;
-    // lane's low-plane unit of block kb0 + l/16, column col0 + l%16; the high plane is C units later
-    const i32x4* bsrc = reinterpret_cast<const i32x4*>(db) + (size_t)(kb0 + (l >> 4)) * 2 * C + col0 + (l & 15);
-    const size_t bstep = (size_t)8 * C;  // 4 blocks x 2 planes, in 16-byte units
-    const i32x4* asrc = reinterpret_cast<const i32x4*>(dig) + (size_t)(kb0 + (l >> 4)) * Mp + (l & 15);
-    const size_t astep = (size_t)4 * Mp;
-    const i32x4 flip = {(int32_t)0x80808080, (int32_t)0x80808080, (int32_t)0x80808080, (int32_t)0x80808080};
-    i32x4 acc[MT][NT][2];
-#pragma unroll
-    for (int a = 0; a < MT; a++)
-#pragma unroll
-        for (int n = 0; n < NT; n++) acc[a][n][0] = acc[a][n][1] = i32x4{0, 0, 0, 0};
-    i32x4 v[2][NT][2];
-    i32x4 A[2][MT];
-    auto load = [&](int bi, uint32_t ks) {
-#pragma unroll
-        for (int n = 0; n < NT; n++) {
-            v[bi][n][0] = __builtin_nontemporal_load(bsrc + ks * bstep + n * 16);
-            v[bi][n][1] = __builtin_nontemporal_load(bsrc + ks * bstep + n * 16 + C);
-        }
-#pragma unroll
-        for (int a = 0; a < MT; a++) A[bi][a] = asrc[ks * astep + a * 16];
-    };
-    auto compute = [&](int bi) {
-#pragma unroll
-        for (int n = 0; n < NT; n++) {
-            const i32x4 Bl = v[bi][n][0] ^ flip, Bh = v[bi][n][1] ^ flip;
-#pragma unroll
-            for (int a = 0; a < MT; a++) {
-                acc[a][n][0] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[bi][a], Bl, acc[a][n][0], 0, 0, 0);
-                acc[a][n][1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[bi][a], Bh, acc[a][n][1], 0, 0, 0);
-            }
-        }
-    };
-    load(0, 0);
-    for (uint32_t ks = 0; ks < nks; ks += 2) {  // nks is even
-        load(1, ks + 1);
-        compute(0);
-        if (ks + 2 < nks) load(0, ks + 2);
-        compute(1);
-    }
-#pragma unroll
-    for (int a = 0; a < MT; a++)
-#pragma unroll
-        for (int n = 0; n < NT; n++)
-            store_pair(acc[a][n][0], acc[a][n][1], (uint32_t)a * 4 + (l >> 4), nq, slice == 0, S, out, out_qstride,
-                       slice, OC, col0 + n * 16 + (l & 15));
+    dbpass_direct_body<MT, NT>(db, dig, S, out, out_qstride, C, OC, nq, kbps, xt, slice);
     if (ps.group_done) pass_handover(ps, xt);
 }
 

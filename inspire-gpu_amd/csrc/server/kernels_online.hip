@@ -422,15 +422,17 @@ __global__ __launch_bounds__(1024) void k_pack_stream(PackStream ps, const uint3
             const uint32_t k = 4 * r + team;
             const bool active = k < npoly;
             const uint32_t qi = q0 + (active ? k >> 1 : 0);
+            // Df x Kn (precomp and keys, in place before the launch) now;
+            // the collapse sums after the wait, since a pass may produce them
+            // with the group (the collapse in the pass's launch)
             uint32_t sub[8];
             if (active) {
                 const uint32_t* Df = precomp + (size_t)g * PC_GROUP + PC_DF + (size_t)c * D_EFF * N;
                 const uint32_t* Kn = keys + qi * kstride + BLK + (size_t)c * N;
-                const uint32_t* p2 = part2 + qi * p2_stride + (size_t)g * 2 * N + (size_t)c * N;
 #pragma unroll
                 for (int e = 0; e < 8; e++) {
                     const uint32_t x = t + 256 * e;
-                    sub[e] = red64((uint64_t)Df[x] * Kn[x] + (uint64_t)Df[N + x] * Kn[2 * N + x] + p2[x], c);
+                    sub[e] = red64((uint64_t)Df[x] * Kn[x] + (uint64_t)Df[N + x] * Kn[2 * N + x], c);
                 }
             }
             if (r == 0) {
@@ -455,13 +457,17 @@ __global__ __launch_bounds__(1024) void k_pack_stream(PackStream ps, const uint3
                 // partials are residues (< 2^27) and at most 16 are summed:
                 // 32-bit sums, the 8 loads of a slice in flight together
                 const uint32_t* src = ps.part + qi * ps.part_qstride + (size_t)c * ps.C + (size_t)g * N + t;
+                const uint32_t* p2 = part2 + qi * p2_stride + (size_t)g * 2 * N + (size_t)c * N + t;
                 uint32_t a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
                 for (uint32_t sl = 0; sl < ps.slices; sl++) {
 #pragma unroll
                     for (int e = 0; e < 8; e++) a[e] += src[sl * sstride + 256 * e];
                 }
 #pragma unroll
-                for (int e = 0; e < 8; e++) buf[team][ls(t + 256 * e)] = red64(a[e], c);
+                for (int e = 0; e < 8; e++) {
+                    buf[team][ls(t + 256 * e)] = red64(a[e], c);
+                    sub[e] = red64((uint64_t)sub[e] + p2[256 * e], c);
+                }
             }
             __syncthreads();
             ntt_fwd_lds(buf[team], t, tw, c, active);

@@ -444,10 +444,11 @@ void GpuServerCtx::alloc_slots() {
     // INSPIRE_PIPELINE=<single>,<batched>: the pipeline shape (0 serial, 2
     // two streams, 3 three streams, 4 the collapse first, 5 three streams
     // with the collapse a chunk ahead, 6 and 7 streamed after or beside the
-    // collapse) of one-query and many-query calls
+    // collapse, 8 streamed with the collapse in the pass's launch) of
+    // one-query and many-query calls
     if (const char* pl = std::getenv("INSPIRE_PIPELINE")) {
         int a1 = -1, a2 = -1;
-        auto ok = [](int v) { return v == 0 || (v >= 2 && v <= 7); };
+        auto ok = [](int v) { return v == 0 || (v >= 2 && v <= 8); };
         if (std::sscanf(pl, "%d,%d", &a1, &a2) == 2 && ok(a1) && ok(a2)) {
             pipe_single = a1;
             pipe_batched = a2;
@@ -670,9 +671,11 @@ void GpuServerCtx::autotune_dbpass() {
                 for (uint32_t sl = 1; sl <= stream_slices_cap(C); sl *= 2) {
                     const DbPlan x = stream_plan(count, sl);
                     if (x.kind != DbPlan::MFMA || x.slices != sl) continue;
-                    // shape 7 (the collapse beside) never won at 16 GB: forced only
-                    for (const int shape : {6, 7})
-                        if (pipe_forced ? shape == forced : shape == 6) cands.push_back({sl, shape});
+                    // shape 7 (the collapse beside) never won at 16 GB: forced only;
+                    // shape 8 (the collapse in the pass's launch) up to 8 queries
+                    for (const int shape : {6, 7, 8})
+                        if (pipe_forced ? shape == forced : shape == 6 || (shape == 8 && count <= 8 && x.param <= 4))
+                            cands.push_back({sl, shape});
                 }
             auto use = [&](const Pick& p) {
                 const bool st = p.shape >= 6;
@@ -933,6 +936,9 @@ void GpuServerCtx::enqueue(const uint64_t* d_in, size_t count, uint64_t* d_out, 
         uint32_t* packed = d_packed_b + q0 * packed_stride;
         uint64_t* out = d_out + (q0 - slot0) * resp_u64s;
         DbPlan sp;
+        // shape 8 (the collapse inside the streamed pass's launch) takes the
+        // direct pass kernel: up to 8 queries, else shape 6
+        if (shape == 8 && (nq > 8 || stream_plan(nq, nq == 1 ? stream_slices1 : stream_slices).param > 4)) shape = 6;
         if (shape >= 6) {
             sp = stream_plan(nq, nq == 1 ? stream_slices1 : stream_slices);
             // a forced pair Horner cannot wait for a streamed pass: the chunked shape then
@@ -951,6 +957,7 @@ void GpuServerCtx::enqueue(const uint64_t* d_in, size_t count, uint64_t* d_out, 
             // producer runs first.
             TraceRange trs("streamed pass: %zu queries, %u row slices, shape %d", nq, sp.slices, shape);
             n_streamed.fetch_add(1, std::memory_order_relaxed);
+            const bool fused = shape == 8;
             const hipStream_t ps_s = cstream[lane], cs = shape == 7 ? ps_s : s;
             joined = true;
             uint32_t* pass_done = d_sflags[lane] + ((q0 - slot0) / kMfmaMaxQueries) * kStreamSetWords * G;
@@ -962,7 +969,14 @@ void GpuServerCtx::enqueue(const uint64_t* d_in, size_t count, uint64_t* d_out, 
                 HIPX(hipStreamWaitEvent(cs, prep_ev[lane], 0));
             }
             HIPX(launch_collapse_keys(keys, keys_stride, nq, d_kdig[lane], d_KS[lane], cs));
-            HIPX(launch_collapse(d_precomp, G, d_kdig[lane], d_KS[lane], nq, p2, part2_stride, cs, collapse_variant));
+            if (!fused)
+                HIPX(launch_collapse(d_precomp, G, d_kdig[lane], d_KS[lane], nq, p2, part2_stride, cs,
+                                     collapse_variant));
+            // The pack and the chains are released here (fused: the flags and
+            // key digits in place), before the query digits: each of their
+            // workgroups needs a whole CU, which they get while those small
+            // kernels run; released together with the pass they would wait
+            // for a CU the pass's workgroups leave wholly free, often its end.
             HIPX(hipEventRecord(coll_ev[lane][0], cs));
             dbpass_prep(q0, nq, sp, s, lane);
             uint32_t* interm = d_interm + q0 * 2 * C;
@@ -971,11 +985,15 @@ void GpuServerCtx::enqueue(const uint64_t* d_in, size_t count, uint64_t* d_out, 
             const size_t part_qstride = direct ? 2 * C : part_stride;
             // split: the top eighths of the groups in a first launch, the
             // chains released after it (they hold their CUs only from then)
-            const uint32_t split = nq == 1 ? stream_split1 : stream_split;
+            const uint32_t split = fused ? 0 : nq == 1 ? stream_split1 : stream_split;
             const size_t g_split = split ? G - G * split / 8 : 0;
             PassStream pst;
             pst.group_done = pass_done;
-            if (g_split > 0 && g_split < G) {
+            if (fused) {
+                HIPX(launch_pass_collapse_stream(db_planes(), d_dig[lane], d_S[lane], nq, part, part_qstride,
+                                                 pp.db_rows, C, sp, d_precomp, d_kdig[lane], d_KS[lane], p2,
+                                                 part2_stride, pass_done, s));
+            } else if (g_split > 0 && g_split < G) {
                 HIPX(launch_dbpass_mfma(db_planes(), d_dig[lane], d_S[lane], nq, part + g_split * N, part_qstride, C,
                                         pp.db_rows, C, g_split * N, (G - g_split) * N, sp, s, &pst));
                 HIPX(hipEventRecord(pass_ev[lane], s));
@@ -992,7 +1010,7 @@ void GpuServerCtx::enqueue(const uint64_t* d_in, size_t count, uint64_t* d_out, 
             pk.slices = sp.slices;
             pk.C = (uint32_t)C;
             pk.pass_done = pass_done;
-            pk.pass_target = (uint32_t)(N / dbplan_cols(sp) * sp.slices);
+            pk.pass_target = fused ? pass_collapse_target(sp, nq, G) : (uint32_t)(N / dbplan_cols(sp) * sp.slices);
             pk.packed_done = packed_done;
             pk.spin_limit = stream_spin_limit;
             pk.status = d_status + lane;

@@ -74,7 +74,8 @@ struct GpuServerCtx {
     // collapse beside and a chunk ahead; streamed (one full-width database
     // pass handing groups over as it runs, the chains waiting for them in
     // the kernel): 6 after the collapse of all groups, 7 with the collapse
-    // beside (2, 3, 6 and 7 timed at setup)
+    // beside, 8 with the collapse in the pass's own launch, group by group
+    // (up to 8 queries) (2, 3, 6 and 8 timed at setup)
     int pipe_single = 3, pipe_batched = 2;
     int pipe_single_chunked = 3, pipe_batched_chunked = 3;  // where a streamed shape cannot run
     bool pipe_forced = false;       // INSPIRE_PIPELINE set the shapes (not autotuned)

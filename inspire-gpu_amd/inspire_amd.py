@@ -84,6 +84,7 @@ SERVER_FUNCS = [
     ("ipir_answer_batch_device", C.c_int, [VP, VP, SZ, VP, VP]),
     ("ipir_answer_batch_packed", C.c_int, [VP, VP, SZ, VP]),
     ("ipir_answer_batch_packed_device", C.c_int, [VP, VP, SZ, VP, VP]),
+    ("ipir_device_status", C.c_int, [VP]),
     ("ipir_loop_start", VP, [VP, SZ]),
     ("ipir_loop_submit", U64, [VP, P(ipir_query_view), VP]),
     ("ipir_loop_wait", C.c_int, [VP, U64, P(C.c_double)]),

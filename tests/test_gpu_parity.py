@@ -290,7 +290,7 @@ def test_ragged_batch_over_sub_batches(medium8):
         # same streams; mixed with a chunked shape
         assert srv.lib.ipir_dbg_take_streaming(srv.h) == 0
         assert srv.lib.ipir_dbg_set_group_chunk(srv.h, 2) == 0
-        for shapes in ((6, 6), (7, 7), (6, 3), (3, 7)):
+        for shapes in ((6, 6), (7, 7), (6, 3), (3, 7), (8, 8), (8, 2)):
             assert srv.lib.ipir_dbg_set_pipeline(srv.h, *shapes) == 0
             for sl in (1, 4):
                 assert srv.lib.ipir_dbg_set_stream_slices(srv.h, 1, sl) == 0
@@ -408,12 +408,13 @@ def test_pipelined_group_chunks_equal_unchunked(medium8, groups, shape, slices):
     assert np.array_equal(got_fused, base) and np.array_equal(got1_fused[0], one[0])
 
 
-@pytest.mark.parametrize("shape", [6, 7])
+@pytest.mark.parametrize("shape", [6, 7, 8])
 def test_streamed_pass_equals_serial(medium8, shape):
     """One full-width database pass that hands each group over as soon as
-    its columns are final, the Horner chains waiting for the groups inside
-    the kernel and packing each one themselves; after the collapse of all
-    groups (6) or with the collapse beside (7); at 1, 2, 4 and 8 row slices
+    its columns are final, a pack kernel packing each group as it arrives
+    and the Horner chains waiting for it inside the kernel; after the collapse of all
+    groups (6), with the collapse beside (7) or inside the pass's launch
+    group by group (8; 20 queries then take 6); at 1 to 16 row slices
     (one: straight into interm; more: the pack sums the partials), in one
     launch or split in two (the chains released after the first). One
     query, three and twenty (the power-row pass) equal the serial answers,
@@ -462,9 +463,10 @@ def test_streamed_wait_overrun_reported(medium8):
     """The streamed chains' bounded wait, forced to give up at once (spin
     limit 0; the pass and every kernel still end normally): the blocking
     call returns 4; the asynchronous device entry point returns 0 with
-    all-ones response words (no residue has that value); the serving loop
-    reports 4 for its request; once the bound is restored the next call is
-    not blamed for the earlier failures."""
+    all-ones response words (no residue has that value) and
+    ipir_device_status then reports 4 once; the serving loop reports 4 for
+    its request; once the bound is restored the next call is not blamed
+    for the earlier failures."""
     geo, db, srv = medium8
     lib = srv.lib
     q, st = ol.make_query(geo, SEED, 777, 41)
@@ -480,8 +482,10 @@ def test_streamed_wait_overrun_reported(medium8):
         dq = ia.DeviceBuffer(q.nbytes, lib)
         dq.upload(q)
         dout = ia.DeviceBuffer(want.nbytes, lib)
+        assert lib.ipir_device_status(srv.h) == 0, "nothing to report yet"
         assert lib.ipir_answer_batch_device(srv.h, C.c_void_p(dq.p), 1, C.c_void_p(dout.p), None) == 0
-        assert lib.ipir_dev_sync() == 0
+        assert lib.ipir_device_status(srv.h) == 4, "the asynchronous call's overrun is reported"
+        assert lib.ipir_device_status(srv.h) == 0, "and then cleared"
         got = dout.download(np.zeros_like(want))
         assert np.all(got == np.uint64(2**64 - 1)), "invalid responses are all-ones"
         dq.free()
@@ -732,7 +736,7 @@ def test_device_resident_entry_matches_host_path(medium8):
     dout.free()
 
 
-@pytest.mark.parametrize("groups,shape", [(8, 0), (2, 2), (2, 3), (2, 4), (2, 5), (8, 6), (2, 7)])
+@pytest.mark.parametrize("groups,shape", [(8, 0), (2, 2), (2, 3), (2, 4), (2, 5), (8, 6), (2, 7), (8, 8)])
 def test_graph_replays_equal_eager(medium8, groups, shape):
     """Answers replayed from captured hipGraphs (serial, and pipelined over
     group chunks on two and three streams) equal the launch-by-launch ones,
