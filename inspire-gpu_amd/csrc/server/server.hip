@@ -667,7 +667,7 @@ void GpuServerCtx::autotune_dbpass() {
             // streamed shapes, at each row-slice count a streamed plan takes
             // (the slices field is then the streamed pass's)
             const int forced = count == 1 ? pipe_single : pipe_batched;
-            if (can_stream(stream))
+            if (can_stream(stream) && stream_fits(count))
                 for (uint32_t sl = 1; sl <= stream_slices_cap(C); sl *= 2) {
                     const DbPlan x = stream_plan(count, sl);
                     if (x.kind != DbPlan::MFMA || x.slices != sl) continue;
@@ -818,6 +818,10 @@ void GpuServerCtx::take_streaming() {
     stream_ok = true;
 }
 
+bool GpuServerCtx::stream_fits(size_t nq) const {
+    return nq * T + (nq + kPackQueries - 1) / kPackQueries <= kStreamMaxWaiters;
+}
+
 bool GpuServerCtx::can_stream(hipStream_t s) const {
     if (!stream_ok) return false;
     // a graph replays its branches one after another, so chains captured
@@ -941,8 +945,11 @@ void GpuServerCtx::enqueue(const uint64_t* d_in, size_t count, uint64_t* d_out, 
         if (shape == 8 && (nq > 8 || stream_plan(nq, nq == 1 ? stream_slices1 : stream_slices).param > 4)) shape = 6;
         if (shape >= 6) {
             sp = stream_plan(nq, nq == 1 ? stream_slices1 : stream_slices);
-            // a forced pair Horner cannot wait for a streamed pass: the chunked shape then
-            if (!streamable || sp.kind != DbPlan::MFMA || horner_mode == 1)
+            // a forced pair Horner cannot wait for a streamed pass, and a
+            // sub-batch whose waiting workgroups exceed kStreamMaxWaiters
+            // CUs would leave too few to the passes of 4 lanes: the chunked
+            // shape then
+            if (!streamable || sp.kind != DbPlan::MFMA || horner_mode == 1 || !stream_fits(nq))
                 shape = nchunks == 1 ? 0 : nq == 1 ? pipe_single_chunked : pipe_batched_chunked;
         }
         if (shape >= 6) {

@@ -93,6 +93,11 @@ struct GpuServerCtx {
     ipx::DbPlan stream_plan(size_t nq, uint32_t slices) const;
     // Whether an answer on s may take a streamed shape now (not under capture).
     bool can_stream(hipStream_t s) const;
+    // A streamed sub-batch of nq queries holds nq*T chain workgroups and
+    // ceil(nq/kPackQueries) pack workgroups (one CU each) while they wait:
+    // at most kStreamMaxWaiters, so 4 lanes leave 64 of 256 CUs to passes.
+    static constexpr size_t kStreamMaxWaiters = 48;
+    bool stream_fits(size_t nq) const;
     // Make this the device's streaming server (the previous one, if any,
     // then answers with chunked shapes). The caller holds this server's mu.
     void take_streaming();

@@ -459,6 +459,50 @@ def test_streamed_pass_equals_serial(medium8, shape):
         assert lib.ipir_dbg_set_pipeline(srv.h, 3, 2) == 0
 
 
+def test_streamed_answers_back_to_back(medium8):
+    """Streamed answers queued without a synchronization in between, so
+    each call's flag set is cleared and reused while the previous call's
+    pass, pack and chains may still run: six asynchronous device calls
+    alternate two query sets (1 and 5 queries, 6 and 8 shapes, 1 and 4 row
+    slices) into separate output buffers; every response equals the serial
+    answer."""
+    geo, db, srv = medium8
+    lib = srv.lib
+    rng = np.random.default_rng(808)
+    sets = [[ol.make_query(geo, SEED, int(i), 7300 + 10 * k + j)[0] for j, i in enumerate(rng.integers(0, 524288, n))]
+            for k, n in enumerate((1, 5))]
+    assert lib.ipir_dbg_take_streaming(srv.h) == 0
+    assert lib.ipir_dbg_set_pipeline(srv.h, 0, 0) == 0
+    try:
+        want = [srv.answer_batch(qs) for qs in sets]
+        dq = [ia.DeviceBuffer(np.stack(qs).nbytes, lib) for qs in sets]
+        for b, qs in zip(dq, sets):
+            b.upload(np.ascontiguousarray(np.stack(qs)))
+        for shapes, sl in (((6, 6), 1), ((8, 8), 4), ((8, 6), 2)):
+            assert lib.ipir_dbg_set_pipeline(srv.h, *shapes) == 0
+            assert lib.ipir_dbg_set_stream_slices(srv.h, 1, sl) == 0
+            assert lib.ipir_dbg_set_stream_slices(srv.h, 5, sl) == 0
+            outs = [ia.DeviceBuffer(want[k % 2].nbytes, lib) for k in range(6)]
+            n0 = lib.ipir_dbg_streamed_count(srv.h)
+            for k in range(6):
+                say(f"[back-to-back] shapes={shapes} slices={sl} call={k}")
+                assert lib.ipir_answer_batch_device(srv.h, C.c_void_p(dq[k % 2].p), len(sets[k % 2]),
+                                                    C.c_void_p(outs[k].p), None) == 0
+            assert lib.ipir_dev_sync() == 0
+            assert lib.ipir_device_status(srv.h) == 0
+            assert lib.ipir_dbg_streamed_count(srv.h) - n0 == 6
+            for k in range(6):
+                got = outs[k].download(np.zeros_like(want[k % 2]))
+                assert np.array_equal(got, want[k % 2]), f"shapes {shapes} slices {sl} call {k}"
+                outs[k].free()
+        for b in dq:
+            b.free()
+    finally:
+        assert lib.ipir_dbg_set_stream_slices(srv.h, 1, 4) == 0
+        assert lib.ipir_dbg_set_stream_slices(srv.h, 5, 4) == 0
+        assert lib.ipir_dbg_set_pipeline(srv.h, 3, 2) == 0
+
+
 def test_streamed_wait_overrun_reported(medium8):
     """The streamed chains' bounded wait, forced to give up at once (spin
     limit 0; the pass and every kernel still end normally): the blocking
