@@ -470,6 +470,21 @@ def main():
     tuning = tb.value.decode()
     log(f"[bench] tuning: {tuning}")
     achievedB = db_bytes / (dbB_ms * 1e-3) / 1e9
+
+    # the same pass as the answers run it (streamed or chunked, beside the
+    # collapse, pack and chains): HIP events around its launches on the
+    # stream they run on, over 10 answers after the timed runs
+    def in_answer(count, n=10):
+        assert lib.ipir_dbg_pass_timing(srv.h, 1, None, None) == 0
+        for _ in range(n):
+            step(count)
+        lib.ipir_dev_sync()
+        tot, npass = C.c_double(), C.c_size_t()
+        rc = lib.ipir_dbg_pass_timing(srv.h, 0, C.byref(tot), C.byref(npass))
+        return tot.value / npass.value if rc == 0 and npass.value else None
+    ia_ms1, ia_msB = in_answer(1), in_answer(B)
+    log(f"[bench] DB pass inside the answers: B=1 {ia_ms1:.3f} ms, per {min(B, 32)}-query sub-batch "
+        f"{ia_msB:.3f} ms")
     # HBM bytes per launch from the committed copy-calibrated counter runs
     # (profiles/pmc_traffic_*.md), for the same kernel on the same workload
     traffics = {}
@@ -566,6 +581,12 @@ def main():
         "verified_queries": verified,
         "roofline": {"bound": "hbm", "achieved": round(achievedB, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achievedB / HBM_PEAK_GBS, 4), "traffic": trafficB,
+                     "in_answer": None if ia_msB is None else {
+                         "ms_per_pass": round(ia_msB, 3),
+                         "achieved": round(db_bytes / (ia_msB * 1e-3) / 1e9, 1),
+                         "frac": round(db_bytes / (ia_msB * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                         "note": "the same pass inside the timed answers' shape (streamed or chunked launches "
+                                 "beside the collapse, pack and chains), HIP events on its stream, 10 answers"},
                      "traffic_profiled_kernel": traffic_kernel,
                      "kernel": f"{knameB} (database pass, batch {min(B, 32)}; {dbB_ms:.3f} ms per launch)"},
         "batched_db_pass": {"kernel": knameB, "queries": min(B, 32), "ms": round(dbB_ms, 3),
@@ -575,6 +596,7 @@ def main():
                             "int8_mfma_frac_of_5000": round(2 * int8_macs(min(B, 32), R, Cc, knameB)
                                                             / (dbB_ms * 1e-3) / 5.0e15, 4)},
         "batch1_db_pass": {"kernel": kname, "ms": round(dbpass_ms, 3), "hbm_GBs": round(achieved, 1),
+                           "in_answer_ms": None if ia_ms1 is None else round(ia_ms1, 3),
                            "frac_of_hbm_peak": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
         "tuning": tuning,
         "client": client,

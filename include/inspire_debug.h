@@ -197,6 +197,13 @@ int ipir_dbg_take_streaming(ipir_server* srv);
 int ipir_dbg_stream_parts(ipir_server* srv, size_t nq, uint32_t slices, double* out_ms);
 /* Streamed 32-query sub-batches this server has enqueued (a test counter). */
 size_t ipir_dbg_streamed_count(const ipir_server* srv);
+/* Database-pass time inside answers: on = 1 starts recording HIP events
+ * around every sub-batch's database-pass launches (the streamed launch, or
+ * each chunk's pass) on the stream they run on; on = 0 stops, waits for
+ * them and returns the summed pass time and the number of sub-batch passes
+ * (each over all columns). 1 if more than 4096 launches were marked (the
+ * rest went untimed). */
+int ipir_dbg_pass_timing(ipir_server* srv, int on, double* total_ms, size_t* passes);
 /* Per-stage device time (ms) of the last ipir_dbg_profile_batch run:
  * [unmarshal+key NTT, DB pass, collapse pass, pack finalize, Horner]. */
 int ipir_dbg_profile_batch(ipir_server* srv, const uint64_t* d_queries, size_t count, uint64_t* d_out,

@@ -391,6 +391,15 @@ int ipir_dbg_answer_taps(ipir_server* srv, const uint64_t* query, uint32_t* inte
         std::vector<uint64_t> r(c->resp_u64s);
         c->answer_host(1, r.data());
         if (resp) std::memcpy(resp, r.data(), r.size() * 8);
+        if (interm && c->last_part_slices > 1) {
+            // a streamed pass left row-slice partials (the pack sums them):
+            // the database-pass result is their sum, as the chunked passes write it
+            const size_t C = c->pp.db_cols;
+            hip_check(ipx::launch_reduce_slices(c->d_part, c->part_stride, c->last_part_slices, C, c->d_interm, 1,
+                                                0, C, c->stream),
+                      "taps: slice sums");
+            hip_check(hipStreamSynchronize(c->stream), "taps");
+        }
         if (interm && hipMemcpy(interm, c->d_interm, 2 * c->pp.db_cols * 4, hipMemcpyDeviceToHost) != hipSuccess)
             return 4;
         if (packed_b && hipMemcpy(packed_b, c->d_packed_b, c->packed_stride * 4, hipMemcpyDeviceToHost) != hipSuccess)
@@ -712,6 +721,35 @@ int ipir_dbg_stream_parts(ipir_server* srv, size_t nq, uint32_t slices, double* 
 }
 
 size_t ipir_dbg_streamed_count(const ipir_server* srv) { return srv ? srv->ctx->n_streamed.load() : 0; }
+
+int ipir_dbg_pass_timing(ipir_server* srv, int on, double* total_ms, size_t* passes) {
+    if (!srv) return 1;
+    GpuServerCtx* c = srv->ctx;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (c->loop_active) return 5;
+    if (on) {
+        c->time_pass = true;
+        c->time_pass_overflow = false;
+        c->pass_marks_used = 0;
+        c->passes_timed = 0;
+        return 0;
+    }
+    c->time_pass = false;
+    try {
+        double sum = 0;
+        for (size_t k = 0; k + 1 < c->pass_marks_used; k += 2) {
+            hip_check(hipEventSynchronize(c->pass_marks[k + 1]), "pass timing");
+            float ms = 0;
+            hip_check(hipEventElapsedTime(&ms, c->pass_marks[k], c->pass_marks[k + 1]), "pass timing");
+            sum += ms;
+        }
+        if (total_ms) *total_ms = sum;
+        if (passes) *passes = c->passes_timed;
+        return c->time_pass_overflow ? 1 : 0;
+    } catch (...) {
+        return 4;
+    }
+}
 
 int ipir_dbg_take_streaming(ipir_server* srv) {
     if (!srv) return 1;

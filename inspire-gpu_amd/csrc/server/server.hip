@@ -818,6 +818,23 @@ void GpuServerCtx::take_streaming() {
     stream_ok = true;
 }
 
+void GpuServerCtx::mark_pass(hipStream_t s, bool first) {
+    if (!time_pass) return;
+    if (pass_marks_used + 1 >= kMaxPassMarks) {  // pool full: the rest go untimed
+        time_pass_overflow = true;
+        return;
+    }
+    while (pass_marks.size() <= pass_marks_used) {
+        hipEvent_t e = nullptr;
+        HIPX(hipEventCreate(&e));
+        pass_marks.push_back(e);
+    }
+    // marks alternate start / end; a whole answer's database pass (one
+    // launch, or the launches of its chunks) counts once
+    if (first && pass_marks_used % 2 == 0) passes_timed++;
+    HIPX(hipEventRecord(pass_marks[pass_marks_used++], s));
+}
+
 bool GpuServerCtx::stream_fits(size_t nq) const {
     return nq * T + (nq + kPackQueries - 1) / kPackQueries <= kStreamMaxWaiters;
 }
@@ -931,6 +948,7 @@ void GpuServerCtx::enqueue(const uint64_t* d_in, size_t count, uint64_t* d_out, 
     // and through their waits for every collapse.
     bool joined = false;
     const bool streamable = can_stream(s);
+    last_part_slices = 0;  // the first sub-batch's result lands in interm unless it streams with slices
     for (size_t q0 = slot0; q0 < slot0 + count; q0 += kMfmaMaxQueries) {
         const size_t nq = slot0 + count - q0 < kMfmaMaxQueries ? slot0 + count - q0 : kMfmaMaxQueries;
         int shape = nq == 1 ? pipe_single : pipe_batched;
@@ -964,6 +982,7 @@ void GpuServerCtx::enqueue(const uint64_t* d_in, size_t count, uint64_t* d_out, 
             // producer runs first.
             TraceRange trs("streamed pass: %zu queries, %u row slices, shape %d", nq, sp.slices, shape);
             n_streamed.fetch_add(1, std::memory_order_relaxed);
+            if (q0 == slot0) last_part_slices = sp.slices;
             const bool fused = shape == 8;
             const hipStream_t ps_s = cstream[lane], cs = shape == 7 ? ps_s : s;
             joined = true;
@@ -996,6 +1015,7 @@ void GpuServerCtx::enqueue(const uint64_t* d_in, size_t count, uint64_t* d_out, 
             const size_t g_split = split ? G - G * split / 8 : 0;
             PassStream pst;
             pst.group_done = pass_done;
+            mark_pass(s, true);
             if (fused) {
                 HIPX(launch_pass_collapse_stream(db_planes(), d_dig[lane], d_S[lane], nq, part, part_qstride,
                                                  pp.db_rows, C, sp, d_precomp, d_kdig[lane], d_KS[lane], p2,
@@ -1010,6 +1030,7 @@ void GpuServerCtx::enqueue(const uint64_t* d_in, size_t count, uint64_t* d_out, 
                 HIPX(launch_dbpass_mfma(db_planes(), d_dig[lane], d_S[lane], nq, part, part_qstride, C, pp.db_rows,
                                         C, 0, C, sp, s, &pst));
             }
+            mark_pass(s, false);
             if (ps_s != cs) HIPX(hipStreamWaitEvent(ps_s, coll_ev[lane][0], 0));
             PackStream pk;
             pk.part = part;
@@ -1041,7 +1062,9 @@ void GpuServerCtx::enqueue(const uint64_t* d_in, size_t count, uint64_t* d_out, 
         if (shape == 0) {
             const DbPlan p = plan_for(nq, C);
             dbpass_prep(q0, nq, p, s, lane);
+            mark_pass(s, true);
             dbpass_range(q0, nq, 0, G, p, s, lane);
+            mark_pass(s, false);
             HIPX(launch_collapse_keys(keys, keys_stride, nq, d_kdig[lane], d_KS[lane], s));
             HIPX(launch_collapse(d_precomp, G, d_kdig[lane], d_KS[lane], nq, p2, part2_stride, s, collapse_variant));
             HIPX(launch_pack_finalize(d_interm + q0 * 2 * C, C, p2, part2_stride, d_precomp, 0, G, keys, keys_stride,
@@ -1086,7 +1109,9 @@ void GpuServerCtx::enqueue(const uint64_t* d_in, size_t count, uint64_t* d_out, 
             const size_t g_lo = chunks[k].first, g_hi = chunks[k].second, ng = g_hi - g_lo;
             TraceRange trk("chunk %zu: groups %zu..%zu (DB pass, collapse, pack, Horner)", k, g_lo, g_hi - 1);
             hipEvent_t dbe = chunk_ev[lane][k % kChunkEvents], ce = coll_ev[lane][k % kChunkEvents];
+            mark_pass(s, k == 0);
             dbpass_range(q0, nq, g_lo, g_hi, chunk_plan_at(nq, k, ng, p), s, lane);
+            mark_pass(s, false);
             // with the collapse on s its event covers the DB pass too (an
             // event record costs the stream a few microseconds of idle)
             if (cs != s) HIPX(hipEventRecord(dbe, s));
@@ -1345,6 +1370,7 @@ void gpu_free_server(GpuServerCtx* ctx) {
         if (ctx->hstream[l]) (void)hipStreamDestroy(ctx->hstream[l]);
         if (ctx->cstream[l]) (void)hipStreamDestroy(ctx->cstream[l]);
     }
+    for (hipEvent_t e : ctx->pass_marks) (void)hipEventDestroy(e);
     for (int l = 0; l < GpuServerCtx::kLanes; l++) {
         dfree(ctx->d_dig[l]); dfree(ctx->d_S[l]); dfree(ctx->d_kdig[l]); dfree(ctx->d_KS[l]);
     }

@@ -231,6 +231,18 @@ struct GpuServerCtx {
     // so an earlier asynchronous call's report is not taken for its own).
     void clear_status(hipStream_t s, int lane);
     std::atomic<size_t> n_streamed{0};  // streamed sub-batches enqueued (a test counter)
+    // row slices whose partials the last answer's first sub-batch left in
+    // d_part instead of a sum in d_interm (a streamed pass; 0: summed)
+    uint32_t last_part_slices = 0;
+    // Database-pass timing inside answers (a measurement aid): while
+    // time_pass is on, the launches of each sub-batch's database pass (one
+    // streamed launch, or its chunks' launches) are bracketed on their
+    // stream by events from pass_marks (start, end, start, end, ...).
+    static constexpr size_t kMaxPassMarks = 8192;
+    bool time_pass = false, time_pass_overflow = false;
+    std::vector<hipEvent_t> pass_marks;
+    size_t pass_marks_used = 0, passes_timed = 0;
+    void mark_pass(hipStream_t s, bool first);
     // Seconds of sleeps: far above any partner skew or any streamed pass's
     // time to reach a group, bounded so a chain whose partner or pass never
     // runs still ends.

@@ -130,6 +130,7 @@ SERVER_FUNCS = [
     ("ipir_dbg_set_stream_split", C.c_int, [VP, SZ, C.c_uint32]),
     ("ipir_dbg_take_streaming", C.c_int, [VP]),
     ("ipir_dbg_streamed_count", SZ, [VP]),
+    ("ipir_dbg_pass_timing", C.c_int, [VP, C.c_int, P(C.c_double), P(SZ)]),
     ("ipir_dbg_stream_parts", C.c_int, [VP, SZ, C.c_uint32, P(C.c_double)]),
     ("ipir_dbg_profile_batch", C.c_int, [VP, VP, SZ, VP, VP]),
     ("ipir_dev_alloc", VP, [SZ]),
