@@ -820,6 +820,8 @@ void GpuServerCtx::take_streaming() {
 
 void GpuServerCtx::mark_pass(hipStream_t s, bool first) {
     if (!time_pass) return;
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    if (s && hipStreamIsCapturing(s, &cap) == hipSuccess && cap != hipStreamCaptureStatusNone) return;
     if (pass_marks_used + 1 >= kMaxPassMarks) {  // pool full: the rest go untimed
         time_pass_overflow = true;
         return;
