@@ -224,11 +224,10 @@ int ipir_dbg_precomp_group(const ipir_server* srv, size_t group, uint32_t* out) 
     auto dd_at = [&](uint32_t c, uint32_t j, uint32_t p, uint32_t i) {
         const size_t k = (size_t)j * N + p;
         uint32_t v = 0;
-        for (uint32_t a = 0; a < 4; a++) {
-            const uint32_t byte = dd[((((size_t)c * 4 + a) * (DD_K / 16) + k / 16) * N + i) * 16 + k % 16];
-            v |= (a < 3 ? byte ^ 0x80u : byte) << (8 * a);
-        }
-        return v;
+        for (uint32_t a = 0; a < 3; a++)
+            v |= (dd[((((size_t)c * 3 + a) * (DD_K / 16) + k / 16) * N + i) * 16 + k % 16] ^ 0x80u) << (8 * a);
+        const uint32_t top = dd[ipx::DD_TOP_BYTES + (((size_t)c * (DD_K / 16) + k / 16) * N + i) * 8 + k % 8];
+        return v | ((k % 16 < 8 ? top & 15u : top >> 4) << 24);
     };
     uint32_t t = 1;  // 5^(N/2-2-s) mod 2N, starting at s = 0
     for (uint32_t e = 0; e < HALF - 2; e++) t = (t * 5) & (2 * N - 1);

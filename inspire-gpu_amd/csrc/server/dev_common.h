@@ -47,17 +47,23 @@ constexpr size_t REF_GROUP_WORDS = 2 * (size_t)N + 2 * RD_SZ + 2 * (size_t)D_EFF
 //   sum_s sum_j D+[s][j][i] K[j][p+(s,i)] + D-[s][j][i] K[j][p-(s,i)]
 //     = sum_{j,p} Dd[j][p][i] K[j][p]
 // with Dd[j][p][i] = D+/-[s][j][i] where p = p+/-(s, i), and 0 at the two
-// slots no step reaches. Per limb this is a K = 2N (j, p) by N (i) matrix,
-// stored as four byte planes in 16-row blocks:
-//   byte (((limb*4 + a)*(2N/16) + k/16)*N + i)*16 + k%16, k = j*N + p,
-// plane a < 3 holding byte a of the residue XOR 0x80 (read as int8 it is
-// byte - 128) and plane 3 the top byte (< 8) as is.
-constexpr size_t DD_K = 2 * (size_t)N;          // K rows per limb
-constexpr size_t DD_SZ = 2 * DD_K * N;          // words (4 plane bytes per entry)
+// slots no step reaches. Per limb this is a K = 2N (j, p) by N (i) matrix
+// of residues (< 2^27), stored as three byte planes and a nibble plane in
+// 16-row blocks, k = j*N + p:
+//   plane a < 3: byte (((limb*3 + a)*(2N/16) + k/16)*N + i)*16 + k%16
+//     holds byte a of the residue XOR 0x80 (read as int8: byte - 128);
+//   top bits (residue >> 24, < 8): byte DD_TOP_BYTES +
+//     ((limb*(2N/16) + k/16)*N + i)*8 + k%8, the low nibble for k%16 < 8
+//     and the high nibble for the row 8 further on,
+// so a lane's 8 nibble bytes widen (two masks, one shift) into the same
+// 16-byte MFMA operand as a full plane's 16 bytes: 3.5 bytes per entry.
+constexpr size_t DD_K = 2 * (size_t)N;             // K rows per limb
+constexpr size_t DD_TOP_BYTES = 2 * 3 * DD_K * N;  // the full planes of both limbs
+constexpr size_t DD_SZ = 7 * DD_K * N / 4;         // words (3.5 bytes per entry, both limbs)
 constexpr size_t PC_A = 0;
 constexpr size_t PC_DD = 2 * (size_t)N;
 constexpr size_t PC_DF = PC_DD + DD_SZ;
-constexpr size_t PC_GROUP = PC_DF + 2 * (size_t)D_EFF * N;  // 8198*N words = 67.16 MB
+constexpr size_t PC_GROUP = PC_DF + 2 * (size_t)D_EFF * N;  // 7174*N words = 58.77 MB
 
 // Twiddle tables in device memory: tw[limb*4*N + {0,1,2,3}*N + i] =
 // fwd, fwd Shoup companion, inv, inv Shoup companion.

@@ -19,6 +19,29 @@ namespace ipx {
 namespace {
 
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+
+// A lane's 16 top-bit entries of one 16-row block (8 nibble bytes) as the
+// 16-byte MFMA operand of rows 0..15: low nibbles are rows 0..7, high 8..15.
+__device__ __forceinline__ i32x4 widen_top(u32x2 t) {
+    return i32x4{(int)(t.x & 0x0f0f0f0fu), (int)(t.y & 0x0f0f0f0fu), (int)((t.x >> 4) & 0x0f0f0f0fu),
+                 (int)((t.y >> 4) & 0x0f0f0f0fu)};
+}
+
+// This lane's Dd operands of group g, limb c at the column tile starting at
+// col0: full planes (16-byte units, plane stride NKB*N) and the top nibbles
+// (8-byte units), both advancing by 4*N units per 64-row step.
+__device__ __forceinline__ const i32x4* dd_planes(const uint32_t* precomp, uint32_t g, int c, uint32_t l,
+                                                 uint32_t col0) {
+    return reinterpret_cast<const i32x4*>(precomp + (size_t)g * PC_GROUP + PC_DD) +
+           (size_t)c * 3 * (DD_K / 16) * N + (size_t)(l >> 4) * N + col0 + (l & 15);
+}
+__device__ __forceinline__ const u32x2* dd_top(const uint32_t* precomp, uint32_t g, int c, uint32_t l,
+                                              uint32_t col0) {
+    return reinterpret_cast<const u32x2*>(reinterpret_cast<const uint8_t*>(precomp + (size_t)g * PC_GROUP + PC_DD) +
+                                          DD_TOP_BYTES) +
+           (size_t)c * (DD_K / 16) * N + (size_t)(l >> 4) * N + col0 + (l & 15);
+}
 
 // e with 5^e = x mod 2N, for x = 1 mod 4 (bit by bit: 5^(2^k) = 1 + 2^(k+2)
 // mod 2^(k+3), so the k-th bit is set iff x*5^-e is not 1 mod 2^(k+3)).
@@ -47,8 +70,8 @@ __device__ __forceinline__ uint32_t inv_odd(uint32_t v) {
 // ------------------------------------------------------------ preprocess relayout
 // Block (group, limb, digit j, 16-slot tile of i): stage D+/-[.][j][i-tile]
 // (2 x 1023 steps x 16 slots) in LDS, then emit the Dd rows (j, p) of the
-// tile as the four byte planes. Sources in the reference order
-// [limb][step][digit][coeff] at Dp/Dm + group*gstride.
+// tile as the three byte planes and the nibble plane. Sources in the
+// reference order [limb][step][digit][coeff] at Dp/Dm + group*gstride.
 __global__ __launch_bounds__(1024) void k_dd_relayout(const uint32_t* __restrict__ Dp, const uint32_t* __restrict__ Dm,
                                                       size_t gstride, uint32_t* __restrict__ pc0) {
     __shared__ uint32_t T[2][NSTEPS][16];
@@ -69,7 +92,7 @@ __global__ __launch_bounds__(1024) void k_dd_relayout(const uint32_t* __restrict
     for (uint32_t x = tid; x < 16 * (N / 16); x += 1024) {
         const uint32_t il = x & 15, pb = x >> 4, i = i0 + il;
         const uint32_t vinv = inv_odd(2 * brev11(i) + 1);
-        uint32_t pl[4][4];
+        uint32_t pl[3][4], top[2] = {0, 0};
 #pragma unroll
         for (int e = 0; e < 16; e++) {
             const uint32_t p = pb * 16 + e;
@@ -77,19 +100,23 @@ __global__ __launch_bounds__(1024) void k_dd_relayout(const uint32_t* __restrict
             const uint32_t neg = (w & 3) == 3;
             const uint32_t e5 = dlog5(neg ? 2 * N - w : w);
             const uint32_t v = e5 == HALF - 1 ? 0u : T[neg][HALF - 2 - e5][il];
-            const uint32_t b[4] = {(v & 255u) ^ 128u, ((v >> 8) & 255u) ^ 128u, ((v >> 16) & 255u) ^ 128u, v >> 24};
+            const uint32_t b[3] = {(v & 255u) ^ 128u, ((v >> 8) & 255u) ^ 128u, ((v >> 16) & 255u) ^ 128u};
 #pragma unroll
-            for (int a = 0; a < 4; a++) {
+            for (int a = 0; a < 3; a++) {
                 if ((e & 3) == 0) pl[a][e >> 2] = 0;
                 pl[a][e >> 2] |= b[a] << (8 * (e & 3));
             }
+            // row e: nibble (e >= 8) of byte e % 8
+            top[(e & 7) >> 2] |= (v >> 24) << (8 * (e & 3) + 4 * (e >> 3));
         }
+        const size_t kb = (size_t)j * (N / 16) + pb;
 #pragma unroll
-        for (int a = 0; a < 4; a++) {
-            u32x4* dst = reinterpret_cast<u32x4*>(
-                dd + ((((size_t)c * 4 + a) * (DD_K / 16) + (size_t)j * (N / 16) + pb) * N + i) * 16);
+        for (int a = 0; a < 3; a++) {
+            u32x4* dst = reinterpret_cast<u32x4*>(dd + ((((size_t)c * 3 + a) * (DD_K / 16) + kb) * N + i) * 16);
             *dst = u32x4{pl[a][0], pl[a][1], pl[a][2], pl[a][3]};
         }
+        *reinterpret_cast<u32x2*>(dd + DD_TOP_BYTES + (((size_t)c * (DD_K / 16) + kb) * N + i) * 8) =
+            u32x2{top[0], top[1]};
     }
 }
 
@@ -174,8 +201,8 @@ __device__ __forceinline__ void collapse_store(const i32x4 (&acc)[4], uint32_t q
 
 // Block (4*NT*16-slot tile of i, limb, group): 4 waves of NT 16-slot tiles,
 // all M rows (MT tiles of 4 queries x 4 digits). Per 64-row K step a lane
-// loads its 16-byte operand of each Dd plane straight from HBM (a step
-// ahead). The key digits of the step (Mp x 64 bytes) are read by each wave
+// loads its 16-byte operand of each full Dd plane and its 8 top-nibble
+// bytes straight from HBM (a step ahead). The key digits of the step (Mp x 64 bytes) are read by each wave
 // from L2 (SHARED = false) or staged once per block in an LDS double
 // buffer (SHARED = true, for large MT).
 // (the body, for slot tile `it`, limb c and group g; the kernel's grid is
@@ -192,8 +219,8 @@ __device__ __forceinline__ void collapse_mfma_body(const uint32_t* __restrict__ 
     __shared__ i32x4 atile[SHARED ? 2 : 1][SHARED ? 4 * Mp : 1];
     const uint32_t tid = threadIdx.x, w = tid >> 6, l = tid & 63;
     const uint32_t col0 = (it * 4 + w) * NT * 16;
-    const i32x4* dd = reinterpret_cast<const i32x4*>(precomp + (size_t)g * PC_GROUP + PC_DD) +
-                      (size_t)c * 4 * NKB * N + (size_t)(l >> 4) * N + col0 + (l & 15);
+    const i32x4* dd = dd_planes(precomp, g, c, l, col0);
+    const u32x2* dt = dd_top(precomp, g, c, l, col0);
     const size_t pstride = (size_t)NKB * N;  // plane, in 16-byte units
     const size_t bstep = (size_t)4 * N;      // 4 blocks of 16 rows
     const i32x4* kd = reinterpret_cast<const i32x4*>(kdig) + (size_t)c * NKB * Mp;  // this limb's digits
@@ -204,15 +231,18 @@ __device__ __forceinline__ void collapse_mfma_body(const uint32_t* __restrict__ 
         for (int n = 0; n < NT; n++)
 #pragma unroll
             for (int pl = 0; pl < 4; pl++) acc[a][n][pl] = i32x4{0, 0, 0, 0};
-    i32x4 B[2][NT][4];
+    i32x4 B[2][NT][3];  // full planes; the top nibbles stay packed until their MFMA
+    u32x2 Bt[2][NT];
     i32x4 A[SHARED ? 1 : 2][SHARED ? 1 : MT];
     i32x4 apre[AU];
     auto load = [&](int bi, uint32_t ks) {
 #pragma unroll
-        for (int n = 0; n < NT; n++)
+        for (int n = 0; n < NT; n++) {
 #pragma unroll
-            for (int pl = 0; pl < 4; pl++)
+            for (int pl = 0; pl < 3; pl++)
                 B[bi][n][pl] = __builtin_nontemporal_load(dd + pl * pstride + ks * bstep + n * 16);
+            Bt[bi][n] = __builtin_nontemporal_load(dt + ks * bstep + n * 16);
+        }
         if (!SHARED) {
 #pragma unroll
             for (int a = 0; a < MT; a++) A[SHARED ? 0 : bi][SHARED ? 0 : a] = kd[((size_t)ks * 4 + (l >> 4)) * Mp + a * 16 + (l & 15)];
@@ -232,10 +262,12 @@ __device__ __forceinline__ void collapse_mfma_body(const uint32_t* __restrict__ 
             const i32x4 Aop = SHARED ? atile[SHARED ? bi : 0][SHARED ? (l >> 4) * Mp + a * 16 + (l & 15) : 0]
                                      : A[SHARED ? 0 : bi][SHARED ? 0 : a];
 #pragma unroll
-            for (int n = 0; n < NT; n++)
+            for (int n = 0; n < NT; n++) {
 #pragma unroll
-                for (int pl = 0; pl < 4; pl++)
+                for (int pl = 0; pl < 3; pl++)
                     acc[a][n][pl] = __builtin_amdgcn_mfma_i32_16x16x64_i8(Aop, B[bi][n][pl], acc[a][n][pl], 0, 0, 0);
+                acc[a][n][3] = __builtin_amdgcn_mfma_i32_16x16x64_i8(Aop, widen_top(Bt[bi][n]), acc[a][n][3], 0, 0, 0);
+            }
         }
     };
     constexpr uint32_t nks = DD_K / 64;  // even
@@ -319,8 +351,8 @@ __global__ __launch_bounds__(256, 2) void k_pass_collapse_stream(
 }
 
 // 17..32 queries (8 key-digit tiles) in the load pattern of the batched
-// database pass: every global load of a step (the Dd operands of all 4
-// planes into a register ring of DEPTH slots, the thread's share of the
+// database pass: every global load of a step (the Dd operands of the 3
+// full planes and the top nibbles into a register ring of DEPTH slots, the thread's share of the
 // step's key-digit tile into a ring of DEPTH-1 slots, stored to LDS one
 // step ahead) is issued DEPTH-1 steps before its step, so each wait covers
 // one step's loads (vector-memory loads complete in issue order); the
@@ -345,8 +377,8 @@ __global__ __launch_bounds__(64 * WAVES, 2) void k_collapse_ring(const uint32_t*
     const uint32_t g = blockIdx.z;
     const int c = (int)blockIdx.y;
     const uint32_t col0 = (blockIdx.x * WAVES + w) * NT * 16;
-    const i32x4* dd = reinterpret_cast<const i32x4*>(precomp + (size_t)g * PC_GROUP + PC_DD) +
-                      (size_t)c * 4 * NKB * N + (size_t)(l >> 4) * N + col0 + (l & 15);
+    const i32x4* dd = dd_planes(precomp, g, c, l, col0);
+    const u32x2* dt = dd_top(precomp, g, c, l, col0);
     const size_t pstride = (size_t)NKB * N;  // plane, in 16-byte units
     const size_t bstep = (size_t)4 * N;      // 4 blocks of 16 rows
     const i32x4* kd = reinterpret_cast<const i32x4*>(kdig) + (size_t)c * NKB * Mp + tid;
@@ -357,17 +389,20 @@ __global__ __launch_bounds__(64 * WAVES, 2) void k_collapse_ring(const uint32_t*
         for (int n = 0; n < NT; n++)
 #pragma unroll
             for (int pl = 0; pl < 4; pl++) acc[a][n][pl] = i32x4{0, 0, 0, 0};
-    i32x4 v[DEPTH][NT][4];
+    i32x4 v[DEPTH][NT][3];  // full planes; the top nibbles stay packed until their MFMA
+    u32x2 vt[DEPTH][NT];
     i32x4 ar[L][AU];
     auto issue = [&](int bs, int as, uint32_t step) {
         const uint32_t k = step < nks ? step : nks - 1;
 #pragma unroll
         for (int u = 0; u < (int)AU; u++) ar[as][u] = kd[(size_t)k * 4 * Mp + TH * u];
 #pragma unroll
-        for (int n = 0; n < NT; n++)
+        for (int n = 0; n < NT; n++) {
 #pragma unroll
-            for (int pl = 0; pl < 4; pl++)
+            for (int pl = 0; pl < 3; pl++)
                 v[bs][n][pl] = __builtin_nontemporal_load(dd + pl * pstride + k * bstep + n * 16);
+            vt[bs][n] = __builtin_nontemporal_load(dt + k * bstep + n * 16);
+        }
     };
     auto store_a = [&](int buf, int as) {
 #pragma unroll
@@ -378,10 +413,12 @@ __global__ __launch_bounds__(64 * WAVES, 2) void k_collapse_ring(const uint32_t*
         for (int a = 0; a < MT; a++) {
             const i32x4 Aop = atile[buf][(l >> 4) * Mp + a * 16 + (l & 15)];
 #pragma unroll
-            for (int n = 0; n < NT; n++)
+            for (int n = 0; n < NT; n++) {
 #pragma unroll
-                for (int pl = 0; pl < 4; pl++)
+                for (int pl = 0; pl < 3; pl++)
                     acc[a][n][pl] = __builtin_amdgcn_mfma_i32_16x16x64_i8(Aop, v[bs][n][pl], acc[a][n][pl], 0, 0, 0);
+                acc[a][n][3] = __builtin_amdgcn_mfma_i32_16x16x64_i8(Aop, widen_top(vt[bs][n]), acc[a][n][3], 0, 0, 0);
+            }
         }
     };
     // Step s: issue step s+L's loads, compute from slot s%DEPTH and LDS

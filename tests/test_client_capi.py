@@ -189,13 +189,14 @@ def test_product_geometry_follows_setup_rules(args, rc):
 @pytest.mark.parametrize("n,w,rows", [(2**20, 60, 32768), (2**27, 120, 32768), (2**25, 3840, 32768)])
 def test_server_footprint_arithmetic(n, w, rows):
     """ipir_server_footprint: the resident DB (2 bytes per value) and
-    precomp (8198 N words per group) are in the fixed part; a slot holds at
+    precomp (7174 N words per group: a, Dd at 3.5 bytes per entry, Df) are
+    in the fixed part; a slot holds at
     least its flat query and response staging; preprocessing peaks above
     the resident DB and precomp. Rejected geometries return 1."""
     f = ia.footprint(n, w, rows)
     rc, geo = ol.setup(n, w, rows)
     db_bytes = geo["R"] * geo["C"] * 2
-    pc_bytes = geo["G"] * 8198 * N * 4
+    pc_bytes = geo["G"] * 7174 * N * 4
     assert db_bytes + pc_bytes < f["resident_fixed"] < db_bytes + pc_bytes + 2**30
     assert f["per_slot"] >= (2 * geo["R"] + 6 * ol.BLK) * 8 + geo["T"] * 4 * N * 8
     assert f["preprocess_peak"] > db_bytes + pc_bytes
