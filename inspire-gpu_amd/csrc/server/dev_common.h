@@ -123,6 +123,28 @@ __device__ __forceinline__ uint32_t perm_slot(uint32_t t, uint32_t v /* 2BR(i)+1
     return brev11((e - 1) >> 1);
 }
 
+// ---------------------------------------------------------------- hand-over
+// Kernels that run side by side hand groups over through a counter or flag
+// in device memory, from a block on one XCD to a block on another (the
+// hardware keeps the XCDs' L2 caches coherent only through these fences).
+// Producer: every wave waits for its own stores (s_waitcnt 0), the block
+// meets at a barrier, then one lane writes the L2 back (agent-scope
+// release) and waits for that write-back before it touches the flag.
+// Consumer: the lane that saw the flag invalidates (agent-scope acquire)
+// and waits for that; after the block's barrier every wave acquires again
+// before its loads. The waits are written out: after an explicit
+// s_waitcnt the compiler drops the fence's own, which would let the flag
+// overtake the write-back and other waves' loads overtake the
+// invalidation.
+__device__ __forceinline__ void fence_release_agent_done() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __builtin_amdgcn_s_waitcnt(0);
+}
+__device__ __forceinline__ void fence_acquire_agent_done() {
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    __builtin_amdgcn_s_waitcnt(0);
+}
+
 // ---------------------------------------------------------------- LDS NTT
 // LDS slot of coefficient e: one pad word per 32 spreads the strided
 // accesses of the register-blocked phases over the banks.

@@ -345,7 +345,7 @@ __global__ __launch_bounds__(256, 2) void k_pass_collapse_stream(
     __builtin_amdgcn_s_waitcnt(0);  // this wave's stores are complete
     __syncthreads();
     if (threadIdx.x == 0) {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        fence_release_agent_done();
         __hip_atomic_fetch_add(group_done + g, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }

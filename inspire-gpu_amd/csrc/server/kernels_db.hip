@@ -285,7 +285,7 @@ __device__ __forceinline__ void pass_handover(const PassStream& ps, uint32_t xt)
     __builtin_amdgcn_s_waitcnt(0);  // this wave's stores are complete
     __syncthreads();
     if (threadIdx.x == 0) {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        fence_release_agent_done();
         __hip_atomic_fetch_add(ps.group_done + ps.group_base + xt / ps.tiles_per_group, 1u, __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
     }

@@ -203,9 +203,9 @@ __global__ __launch_bounds__(1024) void k_horner(const uint32_t* __restrict__ pa
     const Tw tw = tw_lds(twl, tlimb);
     const uint32_t inv_n = nc.inv_n[tlimb], inv_n_p = nc.inv_n_p[tlimb];
     // Beside a streamed pack: wait until this chain's packed b-part of
-    // group g is published (lane 0 spins, then one agent-scope acquire for
-    // the block, ordered before every lane's loads by the barrier). False:
-    // the wait exceeded its bound (every lane returns the same).
+    // group g is published (lane 0 spins and acquires, then every wave
+    // acquires after the barrier; the hand-over rules in dev_common.h).
+    // False: the wait exceeded its bound (every lane returns the same).
     const uint32_t* sw_done = sw.done + (size_t)(qi / sw.queries) * sw.stride;
     auto wait_packed = [&](uint32_t g) -> bool {
         if (tid == 0) {
@@ -219,10 +219,11 @@ __global__ __launch_bounds__(1024) void k_horner(const uint32_t* __restrict__ pa
                 }
                 __builtin_amdgcn_s_sleep(kSpinSleep);
             }
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            fence_acquire_agent_done();
             sw_abort = ab;
         }
         __syncthreads();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // every wave, before its loads of the group
         return !sw_abort;
     };
     constexpr bool streamed = STREAMED;
@@ -450,10 +451,11 @@ __global__ __launch_bounds__(1024) void k_pack_stream(PackStream ps, const uint3
                         }
                         __builtin_amdgcn_s_sleep(kSpinSleep);
                     }
-                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // the pass's partials, for every lane
+                    fence_acquire_agent_done();  // the pass's partials
                     abort_flag = ab;
                 }
                 __syncthreads();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // every wave, before its loads of the group
                 if (abort_flag) return;
             }
             if (active) {
@@ -485,7 +487,7 @@ __global__ __launch_bounds__(1024) void k_pack_stream(PackStream ps, const uint3
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
         if (tid == 0) {
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            fence_release_agent_done();
             __hip_atomic_store(ps.packed_done + (size_t)blk * G + g, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
