@@ -640,7 +640,8 @@ int ipir_dbg_stream_parts(ipir_server* srv, size_t nq, uint32_t slices, double* 
         const size_t C = c->pp.db_cols, G = c->G;
         uint32_t* pass_done = c->d_sflags[0];
         uint32_t* packed_done = pass_done + G;
-        const size_t nblk = (nq + ipx::kPackQueries - 1) / ipx::kPackQueries;
+        const uint32_t pq = GpuServerCtx::pack_width(nq);
+        const size_t nblk = (nq + pq - 1) / pq;
         const uint32_t target = (uint32_t)(ipx::N / ipx::dbplan_cols(p) * p.slices);
         const bool direct = p.slices == 1;
         uint32_t* part = direct ? c->d_interm : c->d_part;
@@ -680,6 +681,7 @@ int ipir_dbg_stream_parts(ipir_server* srv, size_t nq, uint32_t slices, double* 
         pk.packed_done = packed_done;
         pk.spin_limit = c->stream_spin_limit;
         pk.status = c->d_status;
+        pk.queries = pq;
         out_ms[2] = timed([&] {
             hip_check(ipx::launch_fill_u32(pass_done, G, target, s), "fill");
             hip_check(ipx::launch_fill_u32(packed_done, nblk * G, 0, s), "fill");
@@ -690,7 +692,7 @@ int ipir_dbg_stream_parts(ipir_server* srv, size_t nq, uint32_t slices, double* 
         // 3: the streamed chains alone, every group published
         ipx::StreamWait sw;
         sw.done = packed_done;
-        sw.queries = ipx::kPackQueries;
+        sw.queries = pq;
         sw.stride = (uint32_t)G;
         sw.spin_limit = c->stream_spin_limit;
         sw.status = c->d_status;

@@ -202,7 +202,9 @@ struct StreamWait {
     uint32_t* status = nullptr;
 };
 // The streamed pack beside a streamed database pass: one 1024-lane
-// workgroup per kPackQueries queries takes the groups from the top down;
+// workgroup per ps.queries queries (1, 2 or kPackQueries; its 4 teams
+// take 4 of the queries' (query, limb) polynomials per round) takes the
+// groups from the top down;
 // for each it waits until pass_done[g] reaches pass_target, then packs the
 // group's b-parts of its queries (the pass's row-slice partials summed, NTT,
 // minus the collapse sums and Df x Kn: launch_pack_finalize's output) into
@@ -221,6 +223,7 @@ struct PackStream {
     uint32_t* packed_done = nullptr;
     uint32_t spin_limit = 0;
     uint32_t* status = nullptr;
+    uint32_t queries = kPackQueries;  // queries per workgroup
 };
 hipError_t launch_pack_stream(const PackStream& ps, const uint32_t* part2, size_t part2_stride,
                               const uint32_t* precomp, const uint32_t* keys, size_t keys_stride, uint32_t* packed_b,

@@ -400,7 +400,7 @@ __global__ __launch_bounds__(1024) void k_horner(const uint32_t* __restrict__ pa
 
 // ------------------------------------------------------------ streamed pack
 // Beside a streamed database pass (launch_pack_stream): workgroup b packs
-// queries [b*kPackQueries, ...) group by group from the top; team k of
+// queries [b*ps.queries, ...) group by group from the top; team k of
 // round r takes polynomial 4r + k = (query, limb), the limb of its own
 // twiddles. The subtrahend of round 0 (collapse sums + Df x Kn, in place
 // before the launch) is loaded while lane 0 waits for the group.
@@ -413,8 +413,8 @@ __global__ __launch_bounds__(1024) void k_pack_stream(PackStream ps, const uint3
     __shared__ uint32_t twl[8 * N];
     __shared__ uint32_t abort_flag;
     const uint32_t tid = threadIdx.x, team = tid >> 8, t = tid & 255, blk = blockIdx.x;
-    const uint32_t q0 = blk * kPackQueries;
-    const uint32_t nqb = nq - q0 < kPackQueries ? nq - q0 : kPackQueries;
+    const uint32_t q0 = blk * ps.queries;
+    const uint32_t nqb = nq - q0 < ps.queries ? nq - q0 : ps.queries;
     const uint32_t npoly = 2 * nqb, rounds = (npoly + 3) / 4;
     const int c = (int)(team & 1);
     const uint32_t q = qof(c);
@@ -562,9 +562,9 @@ hipError_t launch_pack_stream(const PackStream& ps, const uint32_t* part2, size_
                               size_t packed_stride, size_t G, size_t nq, const NttConst& nc, hipStream_t s) {
     if (nq == 0 || G == 0) return hipSuccess;
     if (!ps.part || !ps.pass_done || !ps.packed_done || !ps.status || ps.slices == 0 || ps.slices > 16 ||
-        ps.C < G * N)  // at most 16 partials below 2^27 keep the 32-bit sums exact
-        return hipErrorInvalidValue;
-    const unsigned blocks = (unsigned)((nq + kPackQueries - 1) / kPackQueries);
+        ps.C < G * N || ps.queries == 0 || ps.queries > kPackQueries)  // at most 16 partials below 2^27 keep
+        return hipErrorInvalidValue;                                    // the 32-bit sums exact
+    const unsigned blocks = (unsigned)((nq + ps.queries - 1) / ps.queries);
     hipLaunchKernelGGL(k_pack_stream, dim3(blocks), dim3(1024), 0, s, ps, part2, part2_stride, precomp, keys,
                        keys_stride, packed_b, packed_stride, (uint32_t)G, (uint32_t)nq, nc);
     return hipGetLastError();

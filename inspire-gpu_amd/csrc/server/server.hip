@@ -838,7 +838,7 @@ void GpuServerCtx::mark_pass(hipStream_t s, bool first) {
 }
 
 bool GpuServerCtx::stream_fits(size_t nq) const {
-    return nq * T + (nq + kPackQueries - 1) / kPackQueries <= kStreamMaxWaiters;
+    return nq * T + (nq + pack_width(nq) - 1) / pack_width(nq) <= kStreamMaxWaiters;
 }
 
 bool GpuServerCtx::can_stream(hipStream_t s) const {
@@ -990,7 +990,8 @@ void GpuServerCtx::enqueue(const uint64_t* d_in, size_t count, uint64_t* d_out, 
             joined = true;
             uint32_t* pass_done = d_sflags[lane] + ((q0 - slot0) / kMfmaMaxQueries) * kStreamSetWords * G;
             uint32_t* packed_done = pass_done + G;
-            const size_t nblk = (nq + kPackQueries - 1) / kPackQueries;
+            const uint32_t pq = pack_width(nq);
+            const size_t nblk = (nq + pq - 1) / pq;
             HIPX(launch_fill_u32(pass_done, (1 + nblk) * G, 0, s));  // this set's last readers are done
             if (cs != s) {
                 HIPX(hipEventRecord(prep_ev[lane], s));
@@ -1044,13 +1045,14 @@ void GpuServerCtx::enqueue(const uint64_t* d_in, size_t count, uint64_t* d_out, 
             pk.packed_done = packed_done;
             pk.spin_limit = stream_spin_limit;
             pk.status = d_status + lane;
+            pk.queries = pq;
             HIPX(launch_pack_stream(pk, p2, part2_stride, d_precomp, keys, keys_stride, packed, packed_stride, G, nq,
                                     nc, ps_s));
             HIPX(hipStreamWaitEvent(hs, coll_ev[lane][0], 0));
             if (g_split > 0 && g_split < G) HIPX(hipStreamWaitEvent(hs, pass_ev[lane], 0));
             StreamWait sw;
             sw.done = packed_done;
-            sw.queries = kPackQueries;
+            sw.queries = pq;
             sw.stride = (uint32_t)G;
             sw.spin_limit = stream_spin_limit;
             sw.status = d_status + lane;

@@ -94,10 +94,15 @@ struct GpuServerCtx {
     // Whether an answer on s may take a streamed shape now (not under capture).
     bool can_stream(hipStream_t s) const;
     // A streamed sub-batch of nq queries holds nq*T chain workgroups and
-    // ceil(nq/kPackQueries) pack workgroups (one CU each) while they wait:
+    // ceil(nq/pack_width(nq)) pack workgroups (one CU each) while they wait:
     // at most kStreamMaxWaiters, so 4 lanes leave 64 of 256 CUs to passes.
     static constexpr size_t kStreamMaxWaiters = 48;
     bool stream_fits(size_t nq) const;
+    // Queries per streamed pack workgroup: 2 up to 16 queries (one round of
+    // its 4 teams per group, so the pack keeps pace with the faster passes
+    // of small batches), kPackQueries above (two rounds, fewer CUs held
+    // beside the slower 17..32-query pass).
+    static uint32_t pack_width(size_t nq) { return nq <= 16 ? 2u : ipx::kPackQueries; }
     // Make this the device's streaming server (the previous one, if any,
     // then answers with chunked shapes). The caller holds this server's mu.
     void take_streaming();
@@ -122,7 +127,9 @@ struct GpuServerCtx {
     // per lane, one set per 32-query sub-batch of a call: the streamed
     // pass's per-group completion counts [G], then the streamed pack's
     // per-(workgroup, group) publication flags [32 / kPackQueries][G]
+    // (pack_width gives at most that many workgroups: 16 queries at 2)
     static constexpr size_t kStreamSetWords = 1 + ipx::kMfmaMaxQueries / ipx::kPackQueries;
+    static_assert(16 / 2 <= ipx::kMfmaMaxQueries / ipx::kPackQueries, "pack workgroups of a 16-query sub-batch");
     uint32_t* d_sflags[kLanes] = {};
     size_t stream_sets = 0;
     // per lane: the streams the collapse and the Horner chains run on beside
