@@ -665,10 +665,15 @@ void GpuServerCtx::autotune_dbpass() {
                 }
             }
             // streamed shapes, at each row-slice count a streamed plan takes
-            // (the slices field is then the streamed pass's)
+            // (the slices field is then the streamed pass's) from 2: with one
+            // slice every workgroup streams its tile's whole column stripe, so
+            // groups complete in a few bursts and the chains wait, then trail
+            // the pass (at 16 GB 0.2-0.5 ms slower than 2..8 slices in every
+            // measurement; a setup timing that favours it is noise)
             const int forced = count == 1 ? pipe_single : pipe_batched;
+            const uint32_t sl0 = stream_slices_cap(C) >= 2 ? 2 : 1;
             if (can_stream(stream) && stream_fits(count))
-                for (uint32_t sl = 1; sl <= stream_slices_cap(C); sl *= 2) {
+                for (uint32_t sl = sl0; sl <= stream_slices_cap(C); sl *= 2) {
                     const DbPlan x = stream_plan(count, sl);
                     if (x.kind != DbPlan::MFMA || x.slices != sl) continue;
                     // shape 7 (the collapse beside) never won at 16 GB: forced only;
