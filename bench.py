@@ -548,7 +548,7 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        log("[bench] CPU baseline (oracle, 1 core, bounded sample)")
+        log("[bench] CPU baseline (oracle on the host worker threads, bounded sample)")
         cpu = cpu_baseline(geo)
 
     out = {
