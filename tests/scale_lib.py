@@ -21,8 +21,9 @@ _T0 = time.time()
 
 
 def say(*a):
-    """Progress line on stdout and, when the run collects output under
-    gpurun_out/, appended to a log there (so a long step is seen working)."""
+    """Progress line on stdout and, when the repository has a run-output
+    folder, appended to a progress log in it (so a long step is seen
+    working)."""
     msg = f"[{time.time() - _T0:7.1f} s] " + " ".join(str(x) for x in a)
     print(msg, flush=True)
     sys.stdout.flush()
