@@ -417,8 +417,10 @@ def test_streamed_pass_equals_serial(medium8, shape):
     group by group (8; 20 queries then take 6); at 1 to 16 row slices
     (one: straight into interm; more: the pack sums the partials), in one
     launch or split in two (the chains released after the first). One
-    query, three and twenty (the power-row pass) equal the serial answers,
-    host-staged and through the device entry point."""
+    query, three, sixteen and seventeen (the pack's 2 queries per workgroup
+    up to 16, filling the flag sets; 4 from 17) and twenty (the power-row
+    pass) equal the serial answers, host-staged and through the device
+    entry point."""
     geo, db, srv = medium8
     lib = srv.lib
     idxs = [int(i) for i in np.random.default_rng(606).integers(0, 524288, 20)]
@@ -428,7 +430,7 @@ def test_streamed_pass_equals_serial(medium8, shape):
     assert lib.ipir_dbg_set_stream_slices(srv.h, 1, 3) == 1  # not a power of two
     assert lib.ipir_dbg_set_pipeline(srv.h, 0, 0) == 0
     try:
-        base = {k: srv.answer_batch(qs[:k]) for k in (1, 3, 20)}
+        base = {k: srv.answer_batch(qs[:k]) for k in (1, 3, 16, 17, 20)}
         assert lib.ipir_dbg_set_pipeline(srv.h, shape, shape) == 0
         assert lib.ipir_dbg_set_stream_split(srv.h, 1, 8) == 1
         for sl, split in ((1, 0), (2, 0), (4, 0), (8, 0), (16, 0), (2, 3), (8, 4), (1, 7)):
@@ -437,10 +439,10 @@ def test_streamed_pass_equals_serial(medium8, shape):
             assert lib.ipir_dbg_set_stream_split(srv.h, 1, split) == 0
             assert lib.ipir_dbg_set_stream_split(srv.h, 20, split) == 0
             n0 = lib.ipir_dbg_streamed_count(srv.h)
-            for k in (1, 3, 20):
+            for k in (1, 3, 16, 17, 20):
                 say(f"[streamed] shape={shape} slices={sl} split={split} queries={k}")
                 assert np.array_equal(srv.answer_batch(qs[:k]), base[k]), f"slices {sl} split {split}, {k} queries"
-            assert lib.ipir_dbg_streamed_count(srv.h) == n0 + 3, "every answer streamed"
+            assert lib.ipir_dbg_streamed_count(srv.h) == n0 + 5, "every answer streamed"
         flat = np.ascontiguousarray(np.stack(qs))
         dq = ia.DeviceBuffer(flat.nbytes, lib)
         dq.upload(flat)
