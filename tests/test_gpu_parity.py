@@ -1015,7 +1015,9 @@ def test_two_ciphertext_responses_extract():
     """n_packed = 512 > D = 256: two response ciphertexts per query (the
     16 GB-and-up geometry's Horner split, reference src/gpu_protocol.cu:
     1300-1339), 4 GiB DB at db_rows 2048. Every response extracts to its
-    entry, in a batch and alone, and the two Horner kernels agree."""
+    entry, in a batch and alone, and the two Horner kernels agree; the
+    streamed shapes (two chains per query waiting on one pack workgroup)
+    give the serial answers."""
     n, w, rows = 2 ** 25, 120, 2048
     rc, geo = ol.setup(n, w, rows)
     assert geo["G"] == 512 and geo["T"] == 2
@@ -1040,6 +1042,22 @@ def test_two_ciphertext_responses_extract():
         assert srv.lib.ipir_dbg_set_horner(srv.h, 0) == 0
         one = srv.answer_batch(qs)
         assert np.array_equal(pair, batch) and np.array_equal(one, batch)
+        # streamed: each query's two chains wait for the same pack workgroup's groups
+        lib = srv.lib
+        assert lib.ipir_dbg_take_streaming(srv.h) == 0
+        assert lib.ipir_dbg_set_pipeline(srv.h, 0, 0) == 0
+        serial, serial1 = srv.answer_batch(qs), srv.answer_batch(qs[:1])
+        assert np.array_equal(serial, batch)
+        for shape in (6, 8):
+            for sl in (2, 4):
+                assert lib.ipir_dbg_set_pipeline(srv.h, shape, shape) == 0
+                assert lib.ipir_dbg_set_stream_slices(srv.h, 1, sl) == 0
+                assert lib.ipir_dbg_set_stream_slices(srv.h, 3, sl) == 0
+                n0 = lib.ipir_dbg_streamed_count(srv.h)
+                say(f"[wide] streamed shape={shape} slices={sl}")
+                assert np.array_equal(srv.answer_batch(qs), serial), f"shape {shape} slices {sl}, 3 queries"
+                assert np.array_equal(srv.answer_batch(qs[:1]), serial1), f"shape {shape} slices {sl}, 1 query"
+                assert lib.ipir_dbg_streamed_count(srv.h) == n0 + 2, "both answers streamed"
         # the packed wire path: both cts compressed on the device, query-major
         cl = ia.Client(srv.params, srv.lib)
         packed = np.stack([cl.pack_query(q) for q in qs])
