@@ -139,6 +139,9 @@ hipError_t launch_collapse_keys(const uint32_t* keys, size_t kstride, size_t nq,
 // Precomp-tensor pass in dot-product form (int8 MFMA, Dd read once) of ng
 // groups whose precomp blocks start at precomp_g0, for nq queries:
 // part2_g0 + q*p2_stride + g*2N holds [limb][N] of group g (relative).
+// A launch too small to fill the chip splits its contraction into row
+// slices whose residues add up in part2 (zeroed first on s): the words are
+// then sums of up to 8 residues (< 2^30), which every reader reduces.
 // 17..32 queries come in kCollapseVariants forms: 0 the two-step
 // prefetch kernel, 1..5 the ring kernel (depth 3, 4 in 4-wave blocks; 3, 4,
 // 5 in 8-wave blocks); fewer queries ignore the variant.

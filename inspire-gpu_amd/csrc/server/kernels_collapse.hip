@@ -179,16 +179,36 @@ __global__ __launch_bounds__(256) void k_ksum(const uint32_t* __restrict__ keys,
 //   sum_k Dd_k K_k = sum_{a,b} 256^(a+b) A_ab + 128*65793*sum_k K_k,
 // A_ab = sum_k x_a d_b over 4096 rows: |A_ab| <= 2^26, exact in i32; the
 // 16 products fold into 7 levels weighted by 256^k mod q, |sum| < 2^58.
+// Row slices (KSlice): a slice's sum over its rows, with the 128*65793 sum(K)
+// term in slice 0 only, is a residue added atomically into part2 (zeroed
+// before the launch): at most 8 residues, below 2^30; every reader of
+// part2 reduces it with its own terms.
+struct KSlice {
+    uint32_t k0 = 0;       // first 64-row step of the slice
+    uint32_t nks = DD_K / 64;  // its steps (even)
+    bool first = true;     // adds the 128*65793 sum(K) term
+    bool add = false;      // atomic add (split) or store (whole)
+};
+// The slice of a split launch's grid, z = group * ksplit + slice.
+__device__ __forceinline__ KSlice slice_of(uint32_t ksplit) {
+    KSlice k;
+    const uint32_t sl = blockIdx.z % ksplit;
+    k.nks = (uint32_t)(DD_K / 64) / ksplit;
+    k.k0 = sl * k.nks;
+    k.first = sl == 0;
+    k.add = ksplit > 1;
+    return k;
+}
 __device__ __forceinline__ void collapse_store(const i32x4 (&acc)[4], uint32_t q, int c, uint32_t g, uint32_t i,
                                                const uint32_t* __restrict__ KS, uint32_t* __restrict__ part2,
-                                               size_t p2_stride) {
+                                               size_t p2_stride, const KSlice& ks = KSlice{}) {
     const uint64_t qc = c ? Q1 : Q0;
     int64_t lv[7] = {0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
     for (int pl = 0; pl < 4; pl++)
 #pragma unroll
         for (int b = 0; b < 4; b++) lv[pl + b] += acc[pl][b];
-    int64_t s = (int64_t)8421504 * KS[q * 2 + c];  // 128*65793 * sum(K)
+    int64_t s = ks.first ? (int64_t)8421504 * KS[q * 2 + c] : 0;  // 128*65793 * sum(K)
     uint64_t wk = 1;                                // 256^k mod q_c
 #pragma unroll
     for (int k = 0; k < 7; k++) {
@@ -196,7 +216,12 @@ __device__ __forceinline__ void collapse_store(const i32x4 (&acc)[4], uint32_t q
         wk = (wk * 256) % qc;
     }
     const uint64_t off = qc * ((1ull << 59) / qc);
-    part2[(size_t)q * p2_stride + (size_t)g * 2 * N + (size_t)c * N + i] = red64((uint64_t)(s + (int64_t)off), c);
+    uint32_t* dst = part2 + (size_t)q * p2_stride + (size_t)g * 2 * N + (size_t)c * N + i;
+    const uint32_t r = red64((uint64_t)(s + (int64_t)off), c);
+    if (ks.add)
+        atomicAdd(dst, r);
+    else
+        *dst = r;
 }
 
 // Block (4*NT*16-slot tile of i, limb, group): 4 waves of NT 16-slot tiles,
@@ -212,18 +237,20 @@ __device__ __forceinline__ void collapse_mfma_body(const uint32_t* __restrict__ 
                                                    const int8_t* __restrict__ kdig,
                                                    const uint32_t* __restrict__ KS,
                                                    uint32_t* __restrict__ part2, size_t p2_stride, uint32_t nq,
-                                                   uint32_t it, int c, uint32_t g) {
+                                                   uint32_t it, int c, uint32_t g, const KSlice& ksl = KSlice{}) {
     constexpr uint32_t Mp = 16 * MT;
     constexpr uint32_t NKB = DD_K / 16;
     constexpr uint32_t AU = SHARED ? Mp * 4 / 256 : 1;  // 16-byte units of a step's digit tile per thread
     __shared__ i32x4 atile[SHARED ? 2 : 1][SHARED ? 4 * Mp : 1];
     const uint32_t tid = threadIdx.x, w = tid >> 6, l = tid & 63;
     const uint32_t col0 = (it * 4 + w) * NT * 16;
-    const i32x4* dd = dd_planes(precomp, g, c, l, col0);
-    const u32x2* dt = dd_top(precomp, g, c, l, col0);
     const size_t pstride = (size_t)NKB * N;  // plane, in 16-byte units
     const size_t bstep = (size_t)4 * N;      // 4 blocks of 16 rows
-    const i32x4* kd = reinterpret_cast<const i32x4*>(kdig) + (size_t)c * NKB * Mp;  // this limb's digits
+    // this slice's steps [k0, k0 + nks): the step index below is relative
+    const i32x4* dd = dd_planes(precomp, g, c, l, col0) + (size_t)ksl.k0 * bstep;
+    const u32x2* dt = dd_top(precomp, g, c, l, col0) + (size_t)ksl.k0 * bstep;
+    const i32x4* kd = reinterpret_cast<const i32x4*>(kdig) + (size_t)c * NKB * Mp +
+                      (size_t)ksl.k0 * 4 * Mp;  // this limb's digits
     i32x4 acc[MT][NT][4];
 #pragma unroll
     for (int a = 0; a < MT; a++)
@@ -270,7 +297,7 @@ __device__ __forceinline__ void collapse_mfma_body(const uint32_t* __restrict__ 
             }
         }
     };
-    constexpr uint32_t nks = DD_K / 64;  // even
+    const uint32_t nks = ksl.nks;  // even
     if (SHARED) {
         load_a(0);
         store_a(0);
@@ -301,7 +328,8 @@ __device__ __forceinline__ void collapse_mfma_body(const uint32_t* __restrict__ 
         const uint32_t q = a * 4 + (l >> 4);
         if (q >= nq) continue;
 #pragma unroll
-        for (int n = 0; n < NT; n++) collapse_store(acc[a][n], q, c, g, col0 + n * 16 + (l & 15), KS, part2, p2_stride);
+        for (int n = 0; n < NT; n++)
+            collapse_store(acc[a][n], q, c, g, col0 + n * 16 + (l & 15), KS, part2, p2_stride, ksl);
     }
 }
 
@@ -310,9 +338,9 @@ __global__ __launch_bounds__(256, 2) void k_collapse_mfma(const uint32_t* __rest
                                                           const int8_t* __restrict__ kdig,
                                                           const uint32_t* __restrict__ KS,
                                                           uint32_t* __restrict__ part2, size_t p2_stride,
-                                                          uint32_t nq) {
+                                                          uint32_t nq, uint32_t ksplit) {
     collapse_mfma_body<MT, NT, SHARED>(precomp, kdig, KS, part2, p2_stride, nq, blockIdx.x, (int)blockIdx.y,
-                                       blockIdx.z);
+                                       blockIdx.z / ksplit, slice_of(ksplit));
 }
 
 // Streamed pass and collapse in one launch (up to 8 queries): per group,
@@ -363,7 +391,7 @@ __global__ __launch_bounds__(64 * WAVES, 2) void k_collapse_ring(const uint32_t*
                                                                  const int8_t* __restrict__ kdig,
                                                                  const uint32_t* __restrict__ KS,
                                                                  uint32_t* __restrict__ part2, size_t p2_stride,
-                                                                 uint32_t nq) {
+                                                                 uint32_t nq, uint32_t ksplit) {
     static_assert(DEPTH >= 3, "the digit ring needs two slots");
     constexpr int L = DEPTH - 1;
     constexpr int P = DEPTH * L;
@@ -371,17 +399,19 @@ __global__ __launch_bounds__(64 * WAVES, 2) void k_collapse_ring(const uint32_t*
     constexpr uint32_t NKB = DD_K / 16;
     constexpr uint32_t TH = 64 * WAVES;
     constexpr uint32_t AU = Mp * 4 / TH;  // 16-byte units of a step's digit tile per thread
-    constexpr uint32_t nks = DD_K / 64;
+    const KSlice ksl = slice_of(ksplit);
+    const uint32_t nks = ksl.nks;
     __shared__ i32x4 atile[2][4 * Mp];
     const uint32_t tid = threadIdx.x, w = tid >> 6, l = tid & 63;
-    const uint32_t g = blockIdx.z;
+    const uint32_t g = blockIdx.z / ksplit;
     const int c = (int)blockIdx.y;
     const uint32_t col0 = (blockIdx.x * WAVES + w) * NT * 16;
-    const i32x4* dd = dd_planes(precomp, g, c, l, col0);
-    const u32x2* dt = dd_top(precomp, g, c, l, col0);
     const size_t pstride = (size_t)NKB * N;  // plane, in 16-byte units
     const size_t bstep = (size_t)4 * N;      // 4 blocks of 16 rows
-    const i32x4* kd = reinterpret_cast<const i32x4*>(kdig) + (size_t)c * NKB * Mp + tid;
+    // this slice's steps [k0, k0 + nks): the step index below is relative
+    const i32x4* dd = dd_planes(precomp, g, c, l, col0) + (size_t)ksl.k0 * bstep;
+    const u32x2* dt = dd_top(precomp, g, c, l, col0) + (size_t)ksl.k0 * bstep;
+    const i32x4* kd = reinterpret_cast<const i32x4*>(kdig) + (size_t)c * NKB * Mp + (size_t)ksl.k0 * 4 * Mp + tid;
     i32x4 acc[MT][NT][4];
 #pragma unroll
     for (int a = 0; a < MT; a++)
@@ -444,7 +474,8 @@ __global__ __launch_bounds__(64 * WAVES, 2) void k_collapse_ring(const uint32_t*
         const uint32_t q = a * 4 + (l >> 4);
         if (q >= nq) continue;
 #pragma unroll
-        for (int n = 0; n < NT; n++) collapse_store(acc[a][n], q, c, g, col0 + n * 16 + (l & 15), KS, part2, p2_stride);
+        for (int n = 0; n < NT; n++)
+            collapse_store(acc[a][n], q, c, g, col0 + n * 16 + (l & 15), KS, part2, p2_stride, ksl);
     }
 }
 
@@ -506,17 +537,53 @@ hipError_t launch_pass_collapse_stream(const uint8_t* db, const int8_t* dig, con
     return hipGetLastError();
 }
 
+namespace {
+// rows x row_words words at p (rows stride words apart) set to zero
+__global__ __launch_bounds__(256) void k_zero_rows(uint32_t* __restrict__ p, size_t stride, size_t row_words,
+                                                   size_t total) {
+    for (size_t x = (size_t)blockIdx.x * 256 + threadIdx.x; x < total; x += (size_t)gridDim.x * 256)
+        p[(x / row_words) * stride + x % row_words] = 0;
+}
+}  // namespace
+
+// Row slices of a collapse launch of `blocks` workgroups (2 limbs x tiles x
+// groups): a grid too small to fill the chip (one group, 16-column tiles:
+// 64 workgroups) splits its 4096-row contraction in up to 8 slices.
+static uint32_t collapse_ksplit(size_t blocks) {
+    uint32_t k = 1;
+    while (blocks * k < 512 && k < 8) k *= 2;
+    return k;
+}
+
 hipError_t launch_collapse(const uint32_t* precomp_g0, size_t ng, const int8_t* kdig, const uint32_t* KS, size_t nq,
                            uint32_t* part2_g0, size_t p2_stride, hipStream_t s, int variant) {
     if (nq == 0 || nq > kCollapseMaxQueries || variant < 0 || variant >= kCollapseVariants) return hipErrorInvalidValue;
     if (ng == 0) return hipSuccess;
     const uint32_t Q = (uint32_t)nq;
-#define COLLAPSE(MT, NT, SH)                                                                                    \
-    hipLaunchKernelGGL((k_collapse_mfma<MT, NT, SH>), dim3(N / (64 * NT), 2, (unsigned)ng), dim3(256), 0, s, \
-                       precomp_g0, kdig, KS, part2_g0, p2_stride, Q)
-#define RING(D, W)                                                                                            \
-    hipLaunchKernelGGL((k_collapse_ring<8, 1, D, W>), dim3(N / (16 * W), 2, (unsigned)ng), dim3(64 * W), 0, s, \
-                       precomp_g0, kdig, KS, part2_g0, p2_stride, Q)
+    // a split launch adds its slices' residues into part2, zeroed first
+    auto split = [&](unsigned gx) {
+        const uint32_t ks = collapse_ksplit((size_t)gx * 2 * ng);
+        if (ks > 1) {
+            const size_t row = ng * 2 * N, total = nq * row;
+            hipLaunchKernelGGL(k_zero_rows, dim3((unsigned)((total + 255) / 256 < 1024 ? (total + 255) / 256 : 1024)),
+                               dim3(256), 0, s, part2_g0, p2_stride, row, total);
+        }
+        return ks;
+    };
+#define COLLAPSE(MT, NT, SH)                                                                                          \
+    do {                                                                                                              \
+        const unsigned gx = N / (64 * NT);                                                                            \
+        const uint32_t ks = split(gx);                                                                                \
+        hipLaunchKernelGGL((k_collapse_mfma<MT, NT, SH>), dim3(gx, 2, (unsigned)(ng * ks)), dim3(256), 0, s,         \
+                           precomp_g0, kdig, KS, part2_g0, p2_stride, Q, ks);                                         \
+    } while (0)
+#define RING(D, W)                                                                                                    \
+    do {                                                                                                              \
+        const unsigned gx = N / (16 * W);                                                                             \
+        const uint32_t ks = split(gx);                                                                                \
+        hipLaunchKernelGGL((k_collapse_ring<8, 1, D, W>), dim3(gx, 2, (unsigned)(ng * ks)), dim3(64 * W), 0, s,      \
+                           precomp_g0, kdig, KS, part2_g0, p2_stride, Q, ks);                                         \
+    } while (0)
     switch (collapse_mt(nq)) {
         case 1:  // few groups: 16-column tiles, twice the workgroups
             if (ng < 16) COLLAPSE(1, 1, false);
