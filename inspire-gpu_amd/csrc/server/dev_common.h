@@ -130,12 +130,14 @@ __device__ __forceinline__ uint32_t perm_slot(uint32_t t, uint32_t v /* 2BR(i)+1
 // Producer: every wave waits for its own stores (s_waitcnt 0), the block
 // meets at a barrier, then one lane writes the L2 back (agent-scope
 // release) and waits for that write-back before it touches the flag.
-// Consumer: the lane that saw the flag invalidates (agent-scope acquire)
-// and waits for that; after the block's barrier every wave acquires again
-// before its loads. The waits are written out: after an explicit
-// s_waitcnt the compiler drops the fence's own, which would let the flag
-// overtake the write-back and other waves' loads overtake the
-// invalidation.
+// Consumer: the lane that saw the flag invalidates (agent-scope acquire:
+// the CU's L1 and the XCD's L2) and waits for that, then the block meets
+// at a barrier before any wave loads the group. One invalidation serves
+// the whole block, whose waves share that CU and XCD; a second one per
+// wave only flushes the caches again (1% of a 32-query answer). The waits
+// are written out: after an explicit s_waitcnt the compiler drops the
+// fence's own, which would let the flag overtake the write-back and other
+// waves' loads overtake the invalidation.
 __device__ __forceinline__ void fence_release_agent_done() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     __builtin_amdgcn_s_waitcnt(0);

@@ -203,8 +203,8 @@ __global__ __launch_bounds__(1024) void k_horner(const uint32_t* __restrict__ pa
     const Tw tw = tw_lds(twl, tlimb);
     const uint32_t inv_n = nc.inv_n[tlimb], inv_n_p = nc.inv_n_p[tlimb];
     // Beside a streamed pack: wait until this chain's packed b-part of
-    // group g is published (lane 0 spins and acquires, then every wave
-    // acquires after the barrier; the hand-over rules in dev_common.h).
+    // group g is published (lane 0 spins and acquires, then the block
+    // meets at a barrier; the hand-over rules in dev_common.h).
     // False: the wait exceeded its bound (every lane returns the same).
     const uint32_t* sw_done = sw.done + (size_t)(qi / sw.queries) * sw.stride;
     auto wait_packed = [&](uint32_t g) -> bool {
@@ -223,7 +223,6 @@ __global__ __launch_bounds__(1024) void k_horner(const uint32_t* __restrict__ pa
             sw_abort = ab;
         }
         __syncthreads();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // every wave, before its loads of the group
         return !sw_abort;
     };
     constexpr bool streamed = STREAMED;
@@ -455,7 +454,6 @@ __global__ __launch_bounds__(1024) void k_pack_stream(PackStream ps, const uint3
                     abort_flag = ab;
                 }
                 __syncthreads();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // every wave, before its loads of the group
                 if (abort_flag) return;
             }
             if (active) {
