@@ -632,8 +632,8 @@ void GpuServerCtx::autotune_dbpass() {
     // when forced. The two interact:
     // a one-slice grid needs no reduction kernel and leaves CUs that only a
     // collapse beside it uses; more slices fill the device alone but pay
-    // partials and their reduction. The best of these (or, with a single
-    // chunk, the serial answer) is then timed against the streamed shapes
+    // partials and their reduction. The best of these and the serial answer
+    // is then timed against the streamed shapes
     // at each row-slice count their full-width pass can take.
     const bool chunked_ok = G / gchunk > 1;
     if (chunked_ok || can_stream(stream)) {
@@ -647,8 +647,11 @@ void GpuServerCtx::autotune_dbpass() {
                 int shape;
             };
             std::vector<Pick> cands;
-            if (!chunked_ok && (!pipe_forced || (count == 1 ? pipe_single : pipe_batched) < 6))
-                cands.push_back({0, 0});  // one chunk: the serial answer against the streamed ones
+            const int forced = count == 1 ? pipe_single : pipe_batched;
+            // the serial answer: with one chunk the only unstreamed shape;
+            // with many it can still win where a sub-batch's Horner chains
+            // hold many of the CUs the chunked passes would run beside them
+            if (!pipe_forced || forced == 0 || (!chunked_ok && forced < 6)) cands.push_back({0, 0});
             for (uint32_t sl = 0; chunked_ok && sl <= 8; sl = sl ? 2 * sl : 1) {
                 if (sl) {
                     const size_t t = blocks * sl;
@@ -670,7 +673,6 @@ void GpuServerCtx::autotune_dbpass() {
             // groups complete in a few bursts and the chains wait, then trail
             // the pass (at 16 GB 0.2-0.5 ms slower than 2..8 slices in every
             // measurement; a setup timing that favours it is noise)
-            const int forced = count == 1 ? pipe_single : pipe_batched;
             const uint32_t sl0 = stream_slices_cap(C) >= 2 ? 2 : 1;
             if (can_stream(stream) && stream_fits(count))
                 for (uint32_t sl = sl0; sl <= stream_slices_cap(C); sl *= 2) {
@@ -956,6 +958,17 @@ void GpuServerCtx::enqueue(const uint64_t* d_in, size_t count, uint64_t* d_out, 
     bool joined = false;
     const bool streamable = can_stream(s);
     last_part_slices = 0;  // the first sub-batch's result lands in interm unless it streams with slices
+    // Serial sub-batches (shape 0) leave their Horner chains to one launch
+    // after the last of them: a sub-batch's nq x T chains each hold a CU for
+    // the whole chain, and the chains of all of them together fill the
+    // device (at 128 GB, 4 x 128 chains in two rounds over the 256 CUs
+    // instead of four rounds over 128). Contiguous slots [defer_lo, +defer_n).
+    size_t defer_lo = 0, defer_n = 0;
+    auto flush_deferred = [&]() {
+        if (defer_n)
+            horner_range(defer_n, d_out + (defer_lo - slot0) * resp_u64s, 0, G, s, defer_lo, lane, status_read);
+        defer_n = 0;
+    };
     for (size_t q0 = slot0; q0 < slot0 + count; q0 += kMfmaMaxQueries) {
         const size_t nq = slot0 + count - q0 < kMfmaMaxQueries ? slot0 + count - q0 : kMfmaMaxQueries;
         int shape = nq == 1 ? pipe_single : pipe_batched;
@@ -977,6 +990,7 @@ void GpuServerCtx::enqueue(const uint64_t* d_in, size_t count, uint64_t* d_out, 
             if (!streamable || sp.kind != DbPlan::MFMA || horner_mode == 1 || !stream_fits(nq))
                 shape = nchunks == 1 ? 0 : nq == 1 ? pipe_single_chunked : pipe_batched_chunked;
         }
+        if (shape != 0) flush_deferred();
         if (shape >= 6) {
             // Streamed: one full-width database pass on s hands each group
             // over as soon as its columns are in place (highest first); the
@@ -1078,7 +1092,8 @@ void GpuServerCtx::enqueue(const uint64_t* d_in, size_t count, uint64_t* d_out, 
             HIPX(launch_collapse(d_precomp, G, d_kdig[lane], d_KS[lane], nq, p2, part2_stride, s, collapse_variant));
             HIPX(launch_pack_finalize(d_interm + q0 * 2 * C, C, p2, part2_stride, d_precomp, 0, G, keys, keys_stride,
                                       packed, packed_stride, nq, nc, s));
-            horner_range(nq, out, 0, G, s, q0, lane, status_read);
+            if (!defer_n) defer_lo = q0;
+            defer_n += nq;
             continue;
         }
         // shape 2 keeps the collapse on s (two HBM-bound passes side by side
@@ -1152,6 +1167,7 @@ void GpuServerCtx::enqueue(const uint64_t* d_in, size_t count, uint64_t* d_out, 
             HIPX(hipStreamWaitEvent(s, prep_ev[lane], 0));
         }
     }
+    flush_deferred();
     if (joined) {
         HIPX(hipEventRecord(horner_done[lane], hs));
         HIPX(hipStreamWaitEvent(s, horner_done[lane], 0));

@@ -262,7 +262,10 @@ def test_ragged_batch_over_sub_batches(medium8):
     """A batch of 33 on a server of max_batch 40: a 32-query sub-batch then a
     one-query sub-batch, each with its own pipeline shape and Horner kernel,
     queued on the same streams, over equal chunks or with the lowest chunk
-    split; equals the serial answer and extracts."""
+    split; equals the serial answer (both sub-batches' chains in one launch
+    after them) and extracts. A serial 32-query sub-batch followed by a
+    chunked or streamed one-query one launches its chains before the
+    latter's."""
     geo, db, _ = medium8
     n, w, rows = 524288, 120, 2048
     srv = ia.Server(db, n, w, rows, max_batch=40, crs_seed=bytes(SEED))
@@ -281,7 +284,7 @@ def test_ragged_batch_over_sub_batches(medium8):
         for groups, split in ((2, 1), (4, 1), (4, 0), (4, 2), (4, 3)):
             assert srv.lib.ipir_dbg_set_group_chunk(srv.h, groups) == 0
             assert srv.lib.ipir_dbg_set_tail_split(srv.h, split) == 0
-            for shapes in ((3, 2), (2, 3), (3, 3), (4, 4), (4, 2), (5, 5), (3, 5)):
+            for shapes in ((3, 2), (2, 3), (3, 3), (4, 4), (4, 2), (5, 5), (3, 5), (3, 0)):
                 assert srv.lib.ipir_dbg_set_pipeline(srv.h, *shapes) == 0
                 got = srv.answer_batch(qs)
                 assert np.array_equal(got, serial), f"groups {groups} split {split} shapes {shapes}"
@@ -290,7 +293,7 @@ def test_ragged_batch_over_sub_batches(medium8):
         # same streams; mixed with a chunked shape
         assert srv.lib.ipir_dbg_take_streaming(srv.h) == 0
         assert srv.lib.ipir_dbg_set_group_chunk(srv.h, 2) == 0
-        for shapes in ((6, 6), (7, 7), (6, 3), (3, 7), (8, 8), (8, 2)):
+        for shapes in ((6, 6), (7, 7), (6, 3), (3, 7), (8, 8), (8, 2), (6, 0)):
             assert srv.lib.ipir_dbg_set_pipeline(srv.h, *shapes) == 0
             for sl in (1, 4):
                 assert srv.lib.ipir_dbg_set_stream_slices(srv.h, 1, sl) == 0
