@@ -123,6 +123,24 @@ __device__ __forceinline__ uint32_t perm_slot(uint32_t t, uint32_t v /* 2BR(i)+1
     return brev11((e - 1) >> 1);
 }
 
+// ---------------------------------------------------------------- block sum
+// Sum of one u64 per thread over a block of TH threads (a multiple of 64):
+// a butterfly within each wave, then thread 0 adds the TH/64 wave sums
+// (red: TH/64 words of LDS). The result is valid in thread 0 only.
+template <int TH>
+__device__ __forceinline__ uint64_t block_sum_u64(uint64_t a, uint64_t* red) {
+    static_assert(TH % 64 == 0, "whole waves");
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = a;
+    __syncthreads();
+    uint64_t s = 0;
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int w = 0; w < TH / 64; w++) s += red[w];
+    return s;
+}
+
 // ---------------------------------------------------------------- hand-over
 // Kernels that run side by side hand groups over through a counter or flag
 // in device memory, from a block on one XCD to a block on another (the

@@ -156,20 +156,17 @@ __global__ __launch_bounds__(256) void k_kdigits(const uint32_t* __restrict__ ke
 }
 
 // KS[q*2 + c] = sum_{j,p} K5_q[j][c][p] mod q_c (the plane-offset correction).
-__global__ __launch_bounds__(256) void k_ksum(const uint32_t* __restrict__ keys, size_t kstride,
-                                              uint32_t* __restrict__ KS) {
-    __shared__ uint64_t red[256];
+__global__ __launch_bounds__(1024) void k_ksum(const uint32_t* __restrict__ keys, size_t kstride,
+                                               uint32_t* __restrict__ KS) {
+    __shared__ uint64_t red[16];
     const uint32_t q = blockIdx.x >> 1, c = blockIdx.x & 1;
+    // DD_K = 2N entries: digit d's limb-c polynomial at keys + (2d + c) N
+    const uint32_t* k = keys + (size_t)q * kstride + (size_t)c * N + threadIdx.x;
     uint64_t a = 0;
-    for (uint32_t x = threadIdx.x; x < DD_K; x += 256)
-        a += keys[(size_t)q * kstride + ((size_t)(x / N) * 2 + c) * N + x % N];
-    red[threadIdx.x] = a;
-    __syncthreads();
-    for (uint32_t s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) KS[blockIdx.x] = red64(red[0], (int)c);
+#pragma unroll
+    for (int e = 0; e < 4; e++) a += k[(size_t)(e >> 1) * 2 * N + (size_t)(e & 1) * 1024];
+    const uint64_t sum = block_sum_u64<1024>(a, red);
+    if (threadIdx.x == 0) KS[blockIdx.x] = red64(sum, (int)c);
 }
 
 // Recombine one lane's 16 accumulators (4 Dd planes x 4 key digits) of
@@ -498,7 +495,7 @@ hipError_t launch_collapse_keys(const uint32_t* keys, size_t kstride, size_t nq,
     const size_t nthr = 2 * (DD_K / 16) * (Mp / 4);
     hipLaunchKernelGGL(k_kdigits, dim3((unsigned)((nthr + 255) / 256)), dim3(256), 0, s, keys, kstride, (uint32_t)nq,
                        Mp, kdig);
-    hipLaunchKernelGGL(k_ksum, dim3((unsigned)(2 * nq)), dim3(256), 0, s, keys, kstride, KS);
+    hipLaunchKernelGGL(k_ksum, dim3((unsigned)(2 * nq)), dim3(1024), 0, s, keys, kstride, KS);
     return hipGetLastError();
 }
 

@@ -244,24 +244,21 @@ __device__ __forceinline__ uint32_t pow_residue(const int32_t (&p)[8], uint32_t 
     return red64(s, c);
 }
 
-// S[pr] = sum_r b_pr[r] mod q_c, the DB-offset correction. One block per
-// pair; 8 doubles per thread per pass.
-__global__ __launch_bounds__(256) void k_bsum(const double* __restrict__ bq, uint32_t R, uint32_t* __restrict__ S) {
-    __shared__ uint64_t red[256];
+// S[pr] = sum_r b_pr[r] mod q_c, the DB-offset correction. One block of
+// 1024 lanes per pair, R a multiple of 2048: R/2048 16-byte loads per lane,
+// all in flight together (a one-query answer waits for this kernel).
+__global__ __launch_bounds__(1024) void k_bsum(const double* __restrict__ bq, uint32_t R, uint32_t* __restrict__ S) {
+    __shared__ uint64_t red[16];
     const uint32_t pr = blockIdx.x;
-    const double* b = bq + (size_t)pr * R;
+    const double2* b = reinterpret_cast<const double2*>(bq + (size_t)pr * R) + threadIdx.x;
     uint64_t a = 0;
-    for (uint32_t i = threadIdx.x * 8; i < R; i += 256 * 8) {
-#pragma unroll
-        for (int e = 0; e < 8; e++) a += (uint64_t)b[i + e];
+#pragma unroll 16
+    for (uint32_t i = 0; i < R / 2048; i++) {
+        const double2 v = b[(size_t)i * 1024];
+        a += (uint64_t)v.x + (uint64_t)v.y;
     }
-    red[threadIdx.x] = a;
-    __syncthreads();
-    for (uint32_t s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) S[pr] = red64(red[0], (int)(pr & 1));
+    const uint64_t sum = block_sum_u64<1024>(a, red);
+    if (threadIdx.x == 0) S[pr] = red64(sum, (int)(pr & 1));
 }
 
 
@@ -745,7 +742,7 @@ hipError_t launch_dbpass_f64(const uint8_t* db, const double* bq, uint32_t* out,
 
 hipError_t launch_mfma_digits(const double* bq, size_t nq, size_t R, const DbPlan& p, int8_t* dig, uint32_t* S,
                               hipStream_t s) {
-    if (p.kind != DbPlan::MFMA || nq == 0 || nq > kMfmaMaxQueries) return hipErrorInvalidValue;
+    if (p.kind != DbPlan::MFMA || nq == 0 || nq > kMfmaMaxQueries || R % 2048) return hipErrorInvalidValue;
     const uint32_t Mp = (uint32_t)dbplan_mp(p);
     if (nq > dbplan_max_queries(p)) return hipErrorInvalidValue;
     const size_t nthr = (R / 16) * (p.crt ? Mp / 7 : Mp / 4);
@@ -755,7 +752,7 @@ hipError_t launch_mfma_digits(const double* bq, size_t nq, size_t R, const DbPla
                            (uint32_t)p.pow);
     else
         hipLaunchKernelGGL(k_digits, grid, dim3(256), 0, s, bq, (uint32_t)R, (uint32_t)nq, Mp, dig);
-    hipLaunchKernelGGL(k_bsum, dim3((unsigned)(2 * nq)), dim3(256), 0, s, bq, (uint32_t)R, S);
+    hipLaunchKernelGGL(k_bsum, dim3((unsigned)(2 * nq)), dim3(1024), 0, s, bq, (uint32_t)R, S);
     return hipGetLastError();
 }
 
