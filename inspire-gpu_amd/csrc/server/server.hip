@@ -648,10 +648,13 @@ void GpuServerCtx::autotune_dbpass() {
             };
             std::vector<Pick> cands;
             const int forced = count == 1 ? pipe_single : pipe_batched;
-            // the serial answer: with one chunk the only unstreamed shape;
-            // with many it can still win where a sub-batch's Horner chains
-            // hold many of the CUs the chunked passes would run beside them
-            if (!pipe_forced || forced == 0 || (!chunked_ok && forced < 6)) cands.push_back({0, 0});
+            // the serial answer: with many chunks it can still win where a
+            // sub-batch's Horner chains hold many of the CUs the chunked
+            // passes would run beside them
+            if (!pipe_forced || forced == 0 || (!chunked_ok && forced < 6 && forced != 3)) cands.push_back({0, 0});
+            // one chunk: the collapse beside the whole pass (the two branches'
+            // small kernels and launch gaps overlap)
+            if (!chunked_ok && (!pipe_forced || forced == 3)) cands.push_back({0, 3});
             for (uint32_t sl = 0; chunked_ok && sl <= 8; sl = sl ? 2 * sl : 1) {
                 if (sl) {
                     const size_t t = blocks * sl;
@@ -972,7 +975,8 @@ void GpuServerCtx::enqueue(const uint64_t* d_in, size_t count, uint64_t* d_out, 
     for (size_t q0 = slot0; q0 < slot0 + count; q0 += kMfmaMaxQueries) {
         const size_t nq = slot0 + count - q0 < kMfmaMaxQueries ? slot0 + count - q0 : kMfmaMaxQueries;
         int shape = nq == 1 ? pipe_single : pipe_batched;
-        if (shape < 6 && nchunks == 1) shape = 0;  // one chunk: nothing to pipeline
+        // one chunk: nothing to pipeline but the collapse beside the pass (3)
+        if (shape < 6 && shape != 3 && nchunks == 1) shape = 0;
         uint32_t* p2 = d_part2 + q0 * part2_stride;
         const uint32_t* keys = d_keys + q0 * keys_stride;
         uint32_t* packed = d_packed_b + q0 * packed_stride;

@@ -279,6 +279,10 @@ def test_ragged_batch_over_sub_batches(medium8):
         assert srv.lib.ipir_dbg_set_group_chunk(srv.h, geo["G"]) == 0
         assert srv.lib.ipir_dbg_set_pipeline(srv.h, 0, 0) == 0
         serial = srv.answer_batch(qs)
+        # one chunk of all groups: the collapse beside the whole pass (3)
+        for shapes in ((3, 3), (3, 0), (0, 3)):
+            assert srv.lib.ipir_dbg_set_pipeline(srv.h, *shapes) == 0
+            assert np.array_equal(srv.answer_batch(qs), serial), f"one chunk, shapes {shapes}"
         # chunks of 2; of 4 with the lowest split 2+1+1 (bit 0), the highest
         # 1+1+2 (bit 1), both, or neither
         for groups, split in ((2, 1), (4, 1), (4, 0), (4, 2), (4, 3)):
