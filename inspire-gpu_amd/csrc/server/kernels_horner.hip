@@ -108,10 +108,13 @@ __global__ __launch_bounds__(1024) void k_horner_pair(const uint32_t* __restrict
             xw[i] = own_a[e];
             xw[N + i] = own_b[e];
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // each wave's stores complete at agent scope
+        // the hand-over rules in dev_common.h: every wave's stores complete,
+        // one release, the flag; the partner's flag, one acquire, the barrier
+        __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
         if (tid == 0) {
-            __hip_atomic_store(my_flag, k, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            fence_release_agent_done();
+            __hip_atomic_store(my_flag, k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             // a zero bound reports at the first exchange whatever the partner
             // did (a deterministic way into this failure path)
             uint32_t spins = 0;
@@ -123,9 +126,9 @@ __global__ __launch_bounds__(1024) void k_horner_pair(const uint32_t* __restrict
                 }
                 __builtin_amdgcn_s_sleep(1);
             }
+            fence_acquire_agent_done();  // the partner's values, for every wave after the barrier
         }
         __syncthreads();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // the partner's values are visible to every wave
         if (abort_flag) {
             // let the partner stop waiting for this step too
             if (tid == 0) __hip_atomic_store(my_flag, 0xffffffffu, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
