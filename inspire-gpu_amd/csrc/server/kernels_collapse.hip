@@ -582,9 +582,10 @@ hipError_t launch_collapse(const uint32_t* precomp_g0, size_t ng, const int8_t* 
                            precomp_g0, kdig, KS, part2_g0, p2_stride, Q, ks);                                         \
     } while (0)
     switch (collapse_mt(nq)) {
-        case 1:  // few groups: 16-column tiles, twice the workgroups
+        case 1:  // few groups: 16-column tiles, twice the workgroups; many: 64
             if (ng < 16) COLLAPSE(1, 1, false);
-            else COLLAPSE(1, 2, false);
+            else if (ng < 32) COLLAPSE(1, 2, false);
+            else COLLAPSE(1, 4, false);
             break;
         case 2: COLLAPSE(2, 2, false); break;
         case 4: COLLAPSE(4, 2, false); break;
