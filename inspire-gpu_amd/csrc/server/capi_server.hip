@@ -366,7 +366,7 @@ int ipir_dbg_tuning(const ipir_server* srv, char* out, size_t cap) {
          std::to_string(c->pipe_single) + "/" + std::to_string(c->pipe_batched);
     if (c->pipe_single >= 6 || c->pipe_batched >= 6) {  // streamed shapes
         auto sp = [&](size_t n, uint32_t sl, uint32_t split) {
-            const ipx::DbPlan p = c->stream_plan(n, sl);
+            const ipx::DbPlan p = c->stream_plan(n, sl, (n == 1 ? c->pipe_single : c->pipe_batched) == 8);
             return p.kind == ipx::DbPlan::MFMA
                        ? plan(p) + (split ? " split " + std::to_string(split) + "/8" : std::string())
                        : std::string("none");

@@ -56,6 +56,7 @@ struct DbPlan {
     uint32_t waves = 4;  // 8 or 16 digit tiles, shared digits: waves per block (4, or 8 = one block per CU)
     bool crt = false;    // shared digits: 7 digit rows per query (CRT value), 14 tiles (7 for power rows, 9..16 queries)
     bool pow = false;    // crt digits accumulated by power of 256 (8-wave blocks, two column tiles per wave)
+    uint32_t nt = 0;     // 1 digit tile: 16-column tiles per wave, 8 (streamed passes) or 0 = 4
 };
 // Forms of the 9..32-query pass: 0 shared digits in 4-wave blocks, 1 shared
 // DB tile (17..32 queries), 2 shared digits in 8-wave blocks, 3 and 4 the

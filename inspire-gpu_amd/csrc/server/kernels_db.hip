@@ -708,13 +708,16 @@ size_t dbplan_max_queries(const DbPlan& p) {
     return p.kind != DbPlan::MFMA ? 1 : p.crt ? dbplan_mp(p) / 7 : dbplan_mp(p) / 8;
 }
 
+// 16-column tiles per wave of an MFMA plan
+static uint32_t plan_nt(const DbPlan& p) { return p.pow ? 2 : p.nt ? p.nt : mfma_nt(p.param); }
+
 size_t dbplan_cols(const DbPlan& p) {
-    return p.kind == DbPlan::MFMA ? 16 * (p.pow ? 2 : mfma_nt(p.param)) * p.waves : 256;  // power rows: 2 tiles
+    return p.kind == DbPlan::MFMA ? 16 * plan_nt(p) * p.waves : 256;  // power rows: 2 tiles
 }
 
 std::string dbplan_kernel(const DbPlan& p) {
     if (p.kind == DbPlan::F64) return "k_dbpass_f64<" + std::to_string(p.param) + ">";
-    const std::string mt_nt = std::to_string(p.param) + ", " + std::to_string(mfma_nt(p.param));
+    const std::string mt_nt = std::to_string(p.param) + ", " + std::to_string(plan_nt(p));
     if (p.param <= 4) return "k_dbpass_mfma<" + mt_nt + ">";  // 1, 2 or 4 tiles: the direct kernel
     if (p.dbtile) return "k_dbpass_mfma_bs<16, 4, " + std::to_string(p.depth) + ">";
     // every template argument, defaults included (as profilers demangle it)
@@ -770,6 +773,7 @@ hipError_t launch_dbpass_mfma(const uint8_t* db, const int8_t* dig, const uint32
     const uint32_t kbps = (uint32_t)(R / 16) / p.slices;
     const size_t cols = dbplan_cols(p);
     if (nq > dbplan_max_queries(p) || (p.crt && p.param != 14 && !(p.pow && p.param == 7)) || kbps % 8 ||
+        (p.nt && !(p.nt == 8 && p.param == 1)) ||
         ncols % cols || col0 % cols ||
         col0 + ncols > C || ostride < ncols)
         return hipErrorInvalidValue;
@@ -799,7 +803,12 @@ hipError_t launch_dbpass_mfma(const uint8_t* db, const int8_t* dig, const uint32
     hipLaunchKernelGGL((k_dbpass_mfma_sa<MT, 2, 2, D, 8, 1>), grid, dim3(512), 0, s, d, dig, S, out, out_qstride, \
                        Cc, OC, Q, kbps, st)
     switch (p.param) {
-        case 1: MFMA(k_dbpass_mfma, 1, 4); break;
+        case 1:
+            if (p.nt == 8)
+                MFMA(k_dbpass_mfma, 1, 8);
+            else
+                MFMA(k_dbpass_mfma, 1, 4);
+            break;
         case 2: MFMA(k_dbpass_mfma, 2, 4); break;
         case 4: MFMA(k_dbpass_mfma, 4, 4); break;
         case 8:

@@ -798,11 +798,12 @@ DbPlan GpuServerCtx::chunk_plan_at(size_t nq, size_t k, size_t ng, const DbPlan&
     return ng == gchunk ? whole : chunk_plan(nq, ng * N);
 }
 
-DbPlan GpuServerCtx::stream_plan(size_t nq, uint32_t slices) const {
+DbPlan GpuServerCtx::stream_plan(size_t nq, uint32_t slices, bool fused) const {
     const size_t R = pp.db_rows, C = pp.db_cols;
     DbPlan p = plan_for(nq, C);
     if (p.kind != DbPlan::MFMA || p.dbtile)  // a kernel that cannot stream: the direct or 8-wave form
         p = nq == 1 ? dbplan_mfma(R, C, 1, plan1_target) : dbplan_form(R, C, nq, mfma_target, mfma_depth, 2);
+    if (p.param == 1 && !fused) p.nt = 8;
     // at least the slices the exactness bounds need (2^15 rows for power rows,
     // else 2^16), at most kStreamSlicesMax (the pack's 32-bit partial sums)
     uint32_t sl = slices ? slices : 1;
@@ -986,7 +987,7 @@ void GpuServerCtx::enqueue(const uint64_t* d_in, size_t count, uint64_t* d_out, 
         // direct pass kernel: up to 8 queries, else shape 6
         if (shape == 8 && (nq > 8 || stream_plan(nq, nq == 1 ? stream_slices1 : stream_slices).param > 4)) shape = 6;
         if (shape >= 6) {
-            sp = stream_plan(nq, nq == 1 ? stream_slices1 : stream_slices);
+            sp = stream_plan(nq, nq == 1 ? stream_slices1 : stream_slices, shape == 8);
             // a forced pair Horner cannot wait for a streamed pass, and a
             // sub-batch whose waiting workgroups exceed kStreamMaxWaiters
             // CUs would leave too few to the passes of 4 lanes: the chunked

@@ -90,7 +90,10 @@ struct GpuServerCtx {
     bool stream_ok = false;
     // The streamed pass's plan for nq queries (the tuned kernel, the streamed
     // slice count, at least the slices exactness needs); kind F64 if none fits.
-    ipx::DbPlan stream_plan(size_t nq, uint32_t slices) const;
+    // One- and two-query passes stream with 128-column tiles per wave (fewer,
+    // longer workgroups per group), except inside the collapse's launch
+    // (fused: the 64-column tiles its work items share).
+    ipx::DbPlan stream_plan(size_t nq, uint32_t slices, bool fused = false) const;
     // Whether an answer on s may take a streamed shape now (not under capture).
     bool can_stream(hipStream_t s) const;
     // A streamed sub-batch of nq queries holds nq*T chain workgroups and
